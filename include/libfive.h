@@ -1,0 +1,186 @@
+/*
+ * C ABI of the MI355X tape-evaluation and meshing library.
+ *
+ * The first part declares, with identical names, struct layouts and
+ * semantics, the subset of the reference C API that sits on the evaluation
+ * and meshing path (reference: libfive/include/libfive.h; implementation
+ * libfive/src/libfive.cpp).  Each declaration cites the reference
+ * declaration it replaces.  A caller of the reference library can link
+ * against this one instead for these entry points.
+ *
+ * The second part holds additive entry points with no counterpart in the
+ * reference: batched evaluation over arrays (host or device pointers),
+ * compiled tapes, mesh settings, and introspection used by tests.
+ *
+ * Every evaluation runs on the GPU.  When no GPU is usable, evaluation
+ * entry points return NaN (or a NULL mesh / non-zero status) and set the
+ * message returned by libfive_last_error(); there is no CPU fallback.
+ */
+#ifndef LIBFIVE_AMD_H
+#define LIBFIVE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* libfive.h:26-57 */
+typedef struct libfive_interval  { float lower; float upper; } libfive_interval;
+typedef struct libfive_region2   { libfive_interval X, Y; } libfive_region2;
+typedef struct libfive_region3   { libfive_interval X, Y, Z; } libfive_region3;
+typedef struct libfive_vec2      { float x, y; } libfive_vec2;
+typedef struct libfive_vec3      { float x, y, z; } libfive_vec3;
+typedef struct libfive_vec4      { float x, y, z, w; } libfive_vec4;
+typedef struct libfive_tri       { uint32_t a, b, c; } libfive_tri;
+
+/* libfive.h:99-104.  verts[0] is a dummy (0,0,0) and vert_count includes
+ * it; triangle indices are >= 1. */
+typedef struct libfive_mesh {
+    libfive_vec3* verts;
+    libfive_tri* tris;
+    uint32_t tri_count;
+    uint32_t vert_count;
+} libfive_mesh;
+
+/* libfive.h:184-188: an opaque, intrusively refcounted tree node. */
+typedef struct libfive_tree_ libfive_tree_;
+typedef struct libfive_tree_* libfive_tree;
+
+/* libfive.h:142 (impl libfive.cpp:51-56): frees a mesh returned by any
+ * render entry point. */
+void libfive_mesh_delete(libfive_mesh* m);
+
+/* libfive.h:163 / libfive.cpp:78-82: opcode lookup by Scheme-style name
+ * ("add", "nth-root"), -1 if unknown. */
+int libfive_opcode_enum(const char* op);
+/* libfive.h:168 / libfive.cpp:84-89: argument count, -1 if invalid. */
+int libfive_opcode_args(int op);
+
+/* libfive.h:196-256 / libfive.cpp:93-154: tree construction.  Returned
+ * trees are owned by the caller and released with libfive_tree_delete.
+ * Invalid opcodes, wrong arity or NULL children give NULL. */
+libfive_tree libfive_tree_x(void);
+libfive_tree libfive_tree_y(void);
+libfive_tree libfive_tree_z(void);
+libfive_tree libfive_tree_var(void);
+bool libfive_tree_is_var(libfive_tree t);
+libfive_tree libfive_tree_const(float f);
+float libfive_tree_get_const(libfive_tree t, bool* success);
+libfive_tree libfive_tree_nullary(int op);
+libfive_tree libfive_tree_unary(int op, libfive_tree a);
+libfive_tree libfive_tree_binary(int op, libfive_tree a, libfive_tree b);
+const void* libfive_tree_id(libfive_tree t);
+void libfive_tree_delete(libfive_tree ptr);
+/* libfive.h:306 / libfive.cpp:190-193 */
+libfive_tree libfive_tree_optimized(libfive_tree t);
+/* libfive.h:311-316 / libfive.cpp:216-231: prefix print, freed with
+ * libfive_free_str. */
+char* libfive_tree_print(libfive_tree t);
+void libfive_free_str(char* ptr);
+
+/* libfive.h:262 / libfive.cpp:195-199: value at one point. */
+float libfive_tree_eval_f(libfive_tree t, libfive_vec3 p);
+/* libfive.h:269 / libfive.cpp:201-207: interval bound over a box. */
+libfive_interval libfive_tree_eval_r(libfive_tree t, libfive_region3 r);
+/* libfive.h:275 / libfive.cpp:209-214: gradient at one point. */
+libfive_vec3 libfive_tree_eval_d(libfive_tree t, libfive_vec3 p);
+
+/* libfive.h:356 / libfive.cpp:341-345: dual-contouring mesh of the region
+ * with min_feature = 1/res (default max_err 1e-8).  NULL on failure. */
+libfive_mesh* libfive_tree_render_mesh(libfive_tree tree, libfive_region3 R,
+                                       float res);
+/* libfive.h:362 / libfive.cpp:347-352: the single-worker variant; on the
+ * GPU the result is the same mesh. */
+libfive_mesh* libfive_tree_render_mesh_st(libfive_tree tree,
+                                          libfive_region3 R, float res);
+
+/* ------------------------------------------------------------------------
+ * Additive entry points (no counterpart in the reference).
+ * --------------------------------------------------------------------- */
+
+/* Message for the most recent failure on this thread ("" if none). */
+const char* libfive_last_error(void);
+
+/* 1 when a GPU is usable by this process, 0 otherwise. */
+int libfive_gpu_available(void);
+
+/* Values of the tree at n points given as host arrays x[], y[], z[];
+ * writes out[n].  Returns 0 on success. */
+int libfive_tree_eval_array(libfive_tree t, const float* x, const float* y,
+                            const float* z, size_t n, float* out);
+
+/* Values and gradients at n host-array points.  Returns 0 on success. */
+int libfive_tree_eval_deriv_array(libfive_tree t, const float* x,
+                                  const float* y, const float* z, size_t n,
+                                  float* value, float* dx, float* dy,
+                                  float* dz);
+
+/* Interval bounds over n boxes (host arrays of lower/upper corners);
+ * maybe_nan may be NULL.  Returns 0 on success. */
+int libfive_tree_eval_interval_array(libfive_tree t, const float* xlo,
+                                     const float* xhi, const float* ylo,
+                                     const float* yhi, const float* zlo,
+                                     const float* zhi, size_t n,
+                                     float* lower, float* upper,
+                                     uint8_t* maybe_nan);
+
+/* A tree compiled once to a device-resident tape. */
+typedef struct libfive_tape_ libfive_tape_;
+typedef struct libfive_tape_* libfive_tape;
+
+libfive_tape libfive_tape_new(libfive_tree t);
+void libfive_tape_delete(libfive_tape t);
+/* Clause count (reference meaning: tree operations, excluding constants and
+ * inputs) and register-file size of the compiled tape. */
+uint32_t libfive_tape_clauses(libfive_tape t);
+uint32_t libfive_tape_registers(libfive_tape t);
+
+/* Evaluation with device pointers, enqueued on `stream` (a hipStream_t,
+ * NULL for the library's stream); asynchronous.  Returns 0 on success. */
+int libfive_tape_eval_values_device(libfive_tape t, const float* x,
+                                    const float* y, const float* z, size_t n,
+                                    float* out, void* stream);
+int libfive_tape_eval_derivs_device(libfive_tape t, const float* x,
+                                    const float* y, const float* z, size_t n,
+                                    float* value, float* dx, float* dy,
+                                    float* dz, void* stream);
+
+/* Compiles a tree on the host only (no GPU needed) and reports the tape's
+ * clause count (reference meaning) and register count.  Returns 0 on
+ * success. */
+int libfive_tree_tape_info(libfive_tree t, uint32_t* clauses,
+                           uint32_t* registers);
+
+/* The reference-format clause list of a tree's Deck (deck.cpp:19-94), for
+ * checking the front end: `clauses` receives 4 words (op, id, a, b) per
+ * clause, root first; constants are (id, value) pairs.  Call with NULL
+ * buffers to get the counts.  Returns 0 on success. */
+int libfive_tree_deck(libfive_tree t, uint32_t* clauses, size_t* n_clauses,
+                      uint32_t* const_ids, float* const_values,
+                      size_t* n_consts, uint32_t* xyz_ids,
+                      uint32_t* num_slots);
+
+/* Mesh settings beyond the reference's two render entry points
+ * (BRepSettings, libfive/include/libfive/render/brep/settings.hpp:27-74). */
+typedef struct libfive_mesh_settings {
+    double min_feature;     /* default 0.1 */
+    double max_err;         /* default 1e-8; < 0 disables collapse */
+    uint32_t precision;     /* 0: FP64 QEF accumulation, 1: compensated FP32 */
+    volatile const int* cancel;   /* optional; non-zero requests cancel */
+} libfive_mesh_settings;
+
+void libfive_mesh_settings_default(libfive_mesh_settings* s);
+libfive_mesh* libfive_tree_render_mesh_settings(libfive_tree tree,
+                                                libfive_region3 R,
+                                                const libfive_mesh_settings* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif  /* LIBFIVE_AMD_H */

@@ -1,0 +1,450 @@
+// extern "C" entry points (include/libfive.h).  Tree handling follows
+// libfive/src/libfive.cpp:51-231; evaluation and meshing go to the GPU.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <sstream>
+#include <string>
+
+#include "../../include/libfive.h"
+#include "eval_kernels.hpp"
+#include "interval_kernels.hpp"
+#include "mesher.hpp"
+#include "runtime.hpp"
+#include "tree.hpp"
+
+using namespace libfive;
+using namespace libfive::amd;
+
+static thread_local std::string g_last_error;
+
+static const TreeData* D(libfive_tree t) {
+    return reinterpret_cast<const TreeData*>(t);
+}
+static libfive_tree H(const TreeData* d) {
+    return reinterpret_cast<libfive_tree>(const_cast<TreeData*>(d));
+}
+
+static void set_error(const char* where, const std::exception& e) {
+    g_last_error = std::string(where) + ": " + e.what();
+    std::fprintf(stderr, "%s\n", g_last_error.c_str());
+}
+
+struct libfive_tape_ {
+    std::unique_ptr<GpuTape> t;
+};
+
+extern "C" {
+
+const char* libfive_last_error(void) { return g_last_error.c_str(); }
+
+int libfive_gpu_available(void) { return gpu_available() ? 1 : 0; }
+
+void libfive_mesh_delete(libfive_mesh* m) {
+    if (!m) return;
+    delete[] m->verts;
+    delete[] m->tris;
+    delete m;
+}
+
+int libfive_opcode_enum(const char* op) {
+    auto o = Opcode::fromScmString(op);
+    return (o == Opcode::INVALID || o == Opcode::LAST_OP) ? -1 : int(o);
+}
+
+int libfive_opcode_args(int op) {
+    return (op >= 0 && op < Opcode::LAST_OP)
+        ? Opcode::args(Opcode::Opcode(op)) : -1;
+}
+
+libfive_tree libfive_tree_x(void) { return H(Tree::X().release()); }
+libfive_tree libfive_tree_y(void) { return H(Tree::Y().release()); }
+libfive_tree libfive_tree_z(void) { return H(Tree::Z().release()); }
+libfive_tree libfive_tree_const(float f) { return H(Tree(f).release()); }
+libfive_tree libfive_tree_var(void) { return H(Tree::var().release()); }
+
+bool libfive_tree_is_var(libfive_tree t) {
+    return D(t)->op() == Opcode::VAR_FREE;
+}
+
+float libfive_tree_get_const(libfive_tree t, bool* success) {
+    if (D(t)->op() == Opcode::CONSTANT) {
+        if (success) *success = true;
+        return D(t)->value();
+    }
+    if (success) *success = false;
+    return 0;
+}
+
+static bool opcode_is_valid(int op, int expected_args) {
+    return op >= 0 && op < Opcode::LAST_OP &&
+           Opcode::args(Opcode::Opcode(op)) == expected_args;
+}
+
+libfive_tree libfive_tree_nullary(int op) {
+    return opcode_is_valid(op, 0)
+        ? H(Tree::nullary(Opcode::Opcode(op)).release()) : nullptr;
+}
+
+libfive_tree libfive_tree_unary(int op, libfive_tree a) {
+    return (opcode_is_valid(op, 1) && a)
+        ? H(Tree::unary(Opcode::Opcode(op), Tree(D(a))).release()) : nullptr;
+}
+
+libfive_tree libfive_tree_binary(int op, libfive_tree a, libfive_tree b) {
+    return (opcode_is_valid(op, 2) && a && b)
+        ? H(Tree::binary(Opcode::Opcode(op), Tree(D(a)), Tree(D(b))).release())
+        : nullptr;
+}
+
+const void* libfive_tree_id(libfive_tree t) { return t; }
+
+void libfive_tree_delete(libfive_tree ptr) {
+    if (ptr) Tree::reclaim(D(ptr));
+}
+
+libfive_tree libfive_tree_optimized(libfive_tree t) {
+    return H(Tree(D(t)).optimized().release());
+}
+
+char* libfive_tree_print(libfive_tree t) {
+    std::stringstream ss;
+    ss << Tree(D(t));
+    const auto str = ss.str();
+    auto out = static_cast<char*>(std::malloc(str.size() + 1));
+    std::memcpy(out, str.c_str(), str.size() + 1);
+    return out;
+}
+
+void libfive_free_str(char* ptr) { std::free(ptr); }
+
+////////////////////////////////////////////////////////////////////////////////
+// Point evaluation
+
+static float eval_point(libfive_tree t, libfive_vec3 p, float grad[3]) {
+    Context& c = context();
+    auto gt = compile_tree(Tree(D(t)));
+    std::lock_guard<std::mutex> lock(c.mu);
+    c.h_small[0] = p.x;
+    c.h_small[1] = p.y;
+    c.h_small[2] = p.z;
+    float* d = c.d_small;
+    hip_check(hipMemcpyAsync(d, c.h_small, 3 * sizeof(float),
+                             hipMemcpyHostToDevice, c.stream), "copy point");
+    const auto& h = gt->host;
+    if (grad) {
+        hip_check(launch_eval_derivs(gt->d.p, gt->size(), h.root_reg,
+                                     h.num_regs, d, d + 1, d + 2, d + 4, d + 5,
+                                     d + 6, d + 7, 1, c.stream),
+                  "eval_d");
+    } else {
+        hip_check(launch_eval_values(gt->d.p, gt->size(), h.root_reg,
+                                     h.num_regs, d, d + 1, d + 2, d + 4, 1,
+                                     c.stream),
+                  "eval_f");
+    }
+    hip_check(hipMemcpyAsync(c.h_small + 4, d + 4, 4 * sizeof(float),
+                             hipMemcpyDeviceToHost, c.stream), "copy result");
+    hip_check(hipStreamSynchronize(c.stream), "eval sync");
+    if (grad) {
+        grad[0] = c.h_small[5];
+        grad[1] = c.h_small[6];
+        grad[2] = c.h_small[7];
+    }
+    return c.h_small[4];
+}
+
+float libfive_tree_eval_f(libfive_tree t, libfive_vec3 p) {
+    try {
+        return eval_point(t, p, nullptr);
+    } catch (const std::exception& e) {
+        set_error("libfive_tree_eval_f", e);
+        return std::numeric_limits<float>::quiet_NaN();
+    }
+}
+
+libfive_vec3 libfive_tree_eval_d(libfive_tree t, libfive_vec3 p) {
+    try {
+        float g[3];
+        eval_point(t, p, g);
+        return {g[0], g[1], g[2]};
+    } catch (const std::exception& e) {
+        set_error("libfive_tree_eval_d", e);
+        const float n = std::numeric_limits<float>::quiet_NaN();
+        return {n, n, n};
+    }
+}
+
+libfive_interval libfive_tree_eval_r(libfive_tree t, libfive_region3 r) {
+    float lo = 0, hi = 0;
+    const int rc = libfive_tree_eval_interval_array(
+        t, &r.X.lower, &r.X.upper, &r.Y.lower, &r.Y.upper, &r.Z.lower,
+        &r.Z.upper, 1, &lo, &hi, nullptr);
+    if (rc) {
+        const float n = std::numeric_limits<float>::quiet_NaN();
+        return {n, n};
+    }
+    return {lo, hi};
+}
+
+int libfive_tree_eval_array(libfive_tree t, const float* x, const float* y,
+                            const float* z, size_t n, float* out) {
+    try {
+        Context& c = context();
+        auto gt = compile_tree(Tree(D(t)));
+        DevBuf<float> buf(4 * n);
+        std::lock_guard<std::mutex> lock(c.mu);
+        const size_t b = n * sizeof(float);
+        hip_check(hipMemcpyAsync(buf.p, x, b, hipMemcpyHostToDevice, c.stream), "copy x");
+        hip_check(hipMemcpyAsync(buf.p + n, y, b, hipMemcpyHostToDevice, c.stream), "copy y");
+        hip_check(hipMemcpyAsync(buf.p + 2 * n, z, b, hipMemcpyHostToDevice, c.stream), "copy z");
+        hip_check(launch_eval_values(gt->d.p, gt->size(), gt->host.root_reg,
+                                     gt->host.num_regs, buf.p, buf.p + n,
+                                     buf.p + 2 * n, buf.p + 3 * n, n, c.stream),
+                  "eval_values");
+        hip_check(hipMemcpyAsync(out, buf.p + 3 * n, b, hipMemcpyDeviceToHost, c.stream), "copy out");
+        hip_check(hipStreamSynchronize(c.stream), "eval_array sync");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error("libfive_tree_eval_array", e);
+        return -1;
+    }
+}
+
+int libfive_tree_eval_deriv_array(libfive_tree t, const float* x,
+                                  const float* y, const float* z, size_t n,
+                                  float* value, float* dx, float* dy,
+                                  float* dz) {
+    try {
+        Context& c = context();
+        auto gt = compile_tree(Tree(D(t)));
+        DevBuf<float> buf(7 * n);
+        std::lock_guard<std::mutex> lock(c.mu);
+        const size_t b = n * sizeof(float);
+        float* p = buf.p;
+        hip_check(hipMemcpyAsync(p, x, b, hipMemcpyHostToDevice, c.stream), "copy x");
+        hip_check(hipMemcpyAsync(p + n, y, b, hipMemcpyHostToDevice, c.stream), "copy y");
+        hip_check(hipMemcpyAsync(p + 2 * n, z, b, hipMemcpyHostToDevice, c.stream), "copy z");
+        hip_check(launch_eval_derivs(gt->d.p, gt->size(), gt->host.root_reg,
+                                     gt->host.num_regs, p, p + n, p + 2 * n,
+                                     p + 3 * n, p + 4 * n, p + 5 * n, p + 6 * n,
+                                     n, c.stream),
+                  "eval_derivs");
+        hip_check(hipMemcpyAsync(value, p + 3 * n, b, hipMemcpyDeviceToHost, c.stream), "copy v");
+        hip_check(hipMemcpyAsync(dx, p + 4 * n, b, hipMemcpyDeviceToHost, c.stream), "copy dx");
+        hip_check(hipMemcpyAsync(dy, p + 5 * n, b, hipMemcpyDeviceToHost, c.stream), "copy dy");
+        hip_check(hipMemcpyAsync(dz, p + 6 * n, b, hipMemcpyDeviceToHost, c.stream), "copy dz");
+        hip_check(hipStreamSynchronize(c.stream), "eval_deriv_array sync");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error("libfive_tree_eval_deriv_array", e);
+        return -1;
+    }
+}
+
+int libfive_tree_eval_interval_array(libfive_tree t, const float* xlo,
+                                     const float* xhi, const float* ylo,
+                                     const float* yhi, const float* zlo,
+                                     const float* zhi, size_t n,
+                                     float* lower, float* upper,
+                                     uint8_t* maybe_nan) {
+    try {
+        Context& c = context();
+        auto gt = compile_tree(Tree(D(t)));
+        DevBuf<float> buf(8 * n);
+        DevBuf<uint8_t> nanbuf(n ? n : 1);
+        std::lock_guard<std::mutex> lock(c.mu);
+        const size_t b = n * sizeof(float);
+        float* p = buf.p;
+        const float* src[6] = {xlo, xhi, ylo, yhi, zlo, zhi};
+        for (int k = 0; k < 6; ++k) {
+            hip_check(hipMemcpyAsync(p + k * n, src[k], b, hipMemcpyHostToDevice,
+                                     c.stream), "copy box");
+        }
+        hip_check(launch_eval_intervals(gt->d.p, gt->size(), gt->host.root_reg,
+                                        gt->host.num_regs, p, p + n, p + 2 * n,
+                                        p + 3 * n, p + 4 * n, p + 5 * n,
+                                        p + 6 * n, p + 7 * n, nanbuf.p, n,
+                                        c.stream),
+                  "eval_intervals");
+        hip_check(hipMemcpyAsync(lower, p + 6 * n, b, hipMemcpyDeviceToHost, c.stream), "copy lo");
+        hip_check(hipMemcpyAsync(upper, p + 7 * n, b, hipMemcpyDeviceToHost, c.stream), "copy hi");
+        if (maybe_nan) {
+            hip_check(hipMemcpyAsync(maybe_nan, nanbuf.p, n, hipMemcpyDeviceToHost,
+                                     c.stream), "copy nan");
+        }
+        hip_check(hipStreamSynchronize(c.stream), "eval_interval sync");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error("libfive_tree_eval_interval_array", e);
+        return -1;
+    }
+}
+
+////////////////////////////////////////////////////////////////////////////////
+// Compiled tapes
+
+libfive_tape libfive_tape_new(libfive_tree t) {
+    try {
+        auto out = new libfive_tape_;
+        out->t = compile_tree(Tree(D(t)));
+        return out;
+    } catch (const std::exception& e) {
+        set_error("libfive_tape_new", e);
+        return nullptr;
+    }
+}
+
+void libfive_tape_delete(libfive_tape t) { delete t; }
+
+uint32_t libfive_tape_clauses(libfive_tape t) { return t->t->host.ref_length; }
+uint32_t libfive_tape_registers(libfive_tape t) { return t->t->host.num_regs; }
+
+int libfive_tape_eval_values_device(libfive_tape t, const float* x,
+                                    const float* y, const float* z, size_t n,
+                                    float* out, void* stream) {
+    try {
+        Context& c = context();
+        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c.stream;
+        const auto& h = t->t->host;
+        hip_check(launch_eval_values(t->t->d.p, t->t->size(), h.root_reg,
+                                     h.num_regs, x, y, z, out, n, s),
+                  "eval_values");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error("libfive_tape_eval_values_device", e);
+        return -1;
+    }
+}
+
+int libfive_tape_eval_derivs_device(libfive_tape t, const float* x,
+                                    const float* y, const float* z, size_t n,
+                                    float* value, float* dx, float* dy,
+                                    float* dz, void* stream) {
+    try {
+        Context& c = context();
+        hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c.stream;
+        const auto& h = t->t->host;
+        hip_check(launch_eval_derivs(t->t->d.p, t->t->size(), h.root_reg,
+                                     h.num_regs, x, y, z, value, dx, dy, dz, n,
+                                     s),
+                  "eval_derivs");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error("libfive_tape_eval_derivs_device", e);
+        return -1;
+    }
+}
+
+int libfive_tree_tape_info(libfive_tree t, uint32_t* clauses,
+                           uint32_t* registers) {
+    try {
+        Deck deck{Tree(D(t))};
+        const DeviceTape dt = compile_deck(deck);
+        if (clauses) *clauses = dt.ref_length;
+        if (registers) *registers = dt.num_regs;
+        return 0;
+    } catch (const std::exception& e) {
+        set_error("libfive_tree_tape_info", e);
+        return -1;
+    }
+}
+
+int libfive_tree_deck(libfive_tree t, uint32_t* clauses, size_t* n_clauses,
+                      uint32_t* const_ids, float* const_values,
+                      size_t* n_consts, uint32_t* xyz_ids,
+                      uint32_t* num_slots) {
+    try {
+        Deck deck{Tree(D(t))};
+        if (clauses) {
+            for (size_t i = 0; i < deck.clauses.size(); ++i) {
+                clauses[4 * i + 0] = deck.clauses[i].op;
+                clauses[4 * i + 1] = deck.clauses[i].id;
+                clauses[4 * i + 2] = deck.clauses[i].a;
+                clauses[4 * i + 3] = deck.clauses[i].b;
+            }
+        }
+        if (n_clauses) *n_clauses = deck.clauses.size();
+        if (const_ids && const_values) {
+            for (size_t i = 0; i < deck.constants.size(); ++i) {
+                const_ids[i] = deck.constants[i].first;
+                const_values[i] = deck.constants[i].second;
+            }
+        }
+        if (n_consts) *n_consts = deck.constants.size();
+        if (xyz_ids) {
+            xyz_ids[0] = deck.X;
+            xyz_ids[1] = deck.Y;
+            xyz_ids[2] = deck.Z;
+        }
+        if (num_slots) *num_slots = uint32_t(deck.num_clauses);
+        return 0;
+    } catch (const std::exception& e) {
+        set_error("libfive_tree_deck", e);
+        return -1;
+    }
+}
+
+////////////////////////////////////////////////////////////////////////////////
+// Meshing
+
+void libfive_mesh_settings_default(libfive_mesh_settings* s) {
+    s->min_feature = 0.1;
+    s->max_err = 1e-8;
+    s->precision = 0;
+    s->cancel = nullptr;
+}
+
+static libfive_mesh* render(libfive_tree tree, libfive_region3 R,
+                            const libfive_mesh_settings& s, const char* who) {
+    try {
+        MeshSettings ms;
+        ms.min_feature = s.min_feature;
+        ms.max_err = s.max_err;
+        ms.precision = s.precision;
+        ms.cancel = s.cancel;
+        const double lo[3] = {R.X.lower, R.Y.lower, R.Z.lower};
+        const double hi[3] = {R.X.upper, R.Y.upper, R.Z.upper};
+        HostMesh m;
+        if (!render_mesh(Tree(D(tree)), lo, hi, ms, m)) {
+            std::fprintf(stderr, "libfive_tree_render_mesh: got empty mesh\n");
+            return nullptr;
+        }
+        auto out = new libfive_mesh;
+        out->vert_count = uint32_t(m.verts.size() / 3);
+        out->tri_count = uint32_t(m.tris.size() / 3);
+        out->verts = new libfive_vec3[out->vert_count];
+        out->tris = new libfive_tri[out->tri_count];
+        std::memcpy(out->verts, m.verts.data(), m.verts.size() * sizeof(float));
+        std::memcpy(out->tris, m.tris.data(), m.tris.size() * sizeof(uint32_t));
+        return out;
+    } catch (const std::exception& e) {
+        set_error(who, e);
+        return nullptr;
+    }
+}
+
+libfive_mesh* libfive_tree_render_mesh(libfive_tree tree, libfive_region3 R,
+                                       float res) {
+    libfive_mesh_settings s;
+    libfive_mesh_settings_default(&s);
+    s.min_feature = 1 / res;     // float division, as libfive.cpp:343
+    return render(tree, R, s, "libfive_tree_render_mesh");
+}
+
+libfive_mesh* libfive_tree_render_mesh_st(libfive_tree tree, libfive_region3 R,
+                                          float res) {
+    libfive_mesh_settings s;
+    libfive_mesh_settings_default(&s);
+    s.min_feature = 1 / res;
+    return render(tree, R, s, "libfive_tree_render_mesh_st");
+}
+
+libfive_mesh* libfive_tree_render_mesh_settings(libfive_tree tree,
+                                                libfive_region3 R,
+                                                const libfive_mesh_settings* s) {
+    return render(tree, R, *s, "libfive_tree_render_mesh_settings");
+}
+
+}   // extern "C"

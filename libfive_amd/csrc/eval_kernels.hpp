@@ -1,0 +1,29 @@
+// Host-side launchers for the point evaluation kernels.  All pointers are
+// device pointers; launches are asynchronous on `s`.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+#include "tape.hpp"
+
+namespace libfive {
+namespace amd {
+
+hipError_t launch_eval_values(const tape::Clause* t, uint32_t n, uint32_t root,
+                              uint32_t nregs, const float* x, const float* y,
+                              const float* z, float* out, size_t npts,
+                              hipStream_t s);
+
+hipError_t launch_eval_derivs(const tape::Clause* t, uint32_t n, uint32_t root,
+                              uint32_t nregs, const float* x, const float* y,
+                              const float* z, float* v, float* dx, float* dy,
+                              float* dz, size_t npts, hipStream_t s);
+
+// Largest register count each interpreter keeps in VGPRs.
+int max_value_regs();
+int max_deriv_regs();
+
+}   // namespace amd
+}   // namespace libfive

@@ -1,0 +1,138 @@
+"""ctypes declarations for the C ABI in include/libfive.h.
+
+Mirrors the role of the reference's Python binding module
+(libfive/bind/python/libfive/ffi.py:141-154, which declares eval_f/r/d and
+render_mesh): load the shared library and give every entry point its C
+signature.  The library is the in-tree build ``libfive_amd/libfive_amd.so``;
+importing this module never builds or falls back to anything else.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfive_amd.so")
+
+
+class libfive_interval(ctypes.Structure):
+    _fields_ = [("lower", ctypes.c_float), ("upper", ctypes.c_float)]
+
+
+class libfive_region3(ctypes.Structure):
+    _fields_ = [("X", libfive_interval), ("Y", libfive_interval),
+                ("Z", libfive_interval)]
+
+
+class libfive_vec3(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_float), ("y", ctypes.c_float),
+                ("z", ctypes.c_float)]
+
+
+class libfive_tri(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_uint32), ("b", ctypes.c_uint32),
+                ("c", ctypes.c_uint32)]
+
+
+class libfive_mesh(ctypes.Structure):
+    _fields_ = [("verts", ctypes.POINTER(libfive_vec3)),
+                ("tris", ctypes.POINTER(libfive_tri)),
+                ("tri_count", ctypes.c_uint32),
+                ("vert_count", ctypes.c_uint32)]
+
+
+class libfive_mesh_settings(ctypes.Structure):
+    _fields_ = [("min_feature", ctypes.c_double),
+                ("max_err", ctypes.c_double),
+                ("precision", ctypes.c_uint32),
+                ("cancel", ctypes.POINTER(ctypes.c_int))]
+
+
+tree_t = ctypes.c_void_p
+tape_t = ctypes.c_void_p
+fp = ctypes.POINTER(ctypes.c_float)
+u32p = ctypes.POINTER(ctypes.c_uint32)
+u8p = ctypes.POINTER(ctypes.c_uint8)
+szp = ctypes.POINTER(ctypes.c_size_t)
+
+# name: (restype, argtypes)
+SIGNATURES = {
+    "libfive_mesh_delete": (None, [ctypes.POINTER(libfive_mesh)]),
+    "libfive_opcode_enum": (ctypes.c_int, [ctypes.c_char_p]),
+    "libfive_opcode_args": (ctypes.c_int, [ctypes.c_int]),
+    "libfive_tree_x": (tree_t, []),
+    "libfive_tree_y": (tree_t, []),
+    "libfive_tree_z": (tree_t, []),
+    "libfive_tree_var": (tree_t, []),
+    "libfive_tree_is_var": (ctypes.c_bool, [tree_t]),
+    "libfive_tree_const": (tree_t, [ctypes.c_float]),
+    "libfive_tree_get_const": (ctypes.c_float,
+                               [tree_t, ctypes.POINTER(ctypes.c_bool)]),
+    "libfive_tree_nullary": (tree_t, [ctypes.c_int]),
+    "libfive_tree_unary": (tree_t, [ctypes.c_int, tree_t]),
+    "libfive_tree_binary": (tree_t, [ctypes.c_int, tree_t, tree_t]),
+    "libfive_tree_id": (ctypes.c_void_p, [tree_t]),
+    "libfive_tree_delete": (None, [tree_t]),
+    "libfive_tree_optimized": (tree_t, [tree_t]),
+    "libfive_tree_print": (ctypes.c_void_p, [tree_t]),
+    "libfive_free_str": (None, [ctypes.c_void_p]),
+    "libfive_tree_eval_f": (ctypes.c_float, [tree_t, libfive_vec3]),
+    "libfive_tree_eval_r": (libfive_interval, [tree_t, libfive_region3]),
+    "libfive_tree_eval_d": (libfive_vec3, [tree_t, libfive_vec3]),
+    "libfive_tree_render_mesh": (ctypes.POINTER(libfive_mesh),
+                                 [tree_t, libfive_region3, ctypes.c_float]),
+    "libfive_tree_render_mesh_st": (ctypes.POINTER(libfive_mesh),
+                                    [tree_t, libfive_region3, ctypes.c_float]),
+    "libfive_last_error": (ctypes.c_char_p, []),
+    "libfive_gpu_available": (ctypes.c_int, []),
+    "libfive_tree_eval_array": (ctypes.c_int,
+                                [tree_t, fp, fp, fp, ctypes.c_size_t, fp]),
+    "libfive_tree_eval_deriv_array": (ctypes.c_int,
+                                      [tree_t, fp, fp, fp, ctypes.c_size_t,
+                                       fp, fp, fp, fp]),
+    "libfive_tree_eval_interval_array": (ctypes.c_int,
+                                         [tree_t, fp, fp, fp, fp, fp, fp,
+                                          ctypes.c_size_t, fp, fp, u8p]),
+    "libfive_tape_new": (tape_t, [tree_t]),
+    "libfive_tape_delete": (None, [tape_t]),
+    "libfive_tape_clauses": (ctypes.c_uint32, [tape_t]),
+    "libfive_tape_registers": (ctypes.c_uint32, [tape_t]),
+    "libfive_tape_eval_values_device": (ctypes.c_int,
+                                        [tape_t, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_size_t, ctypes.c_void_p,
+                                         ctypes.c_void_p]),
+    "libfive_tape_eval_derivs_device": (ctypes.c_int,
+                                        [tape_t, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_size_t, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p]),
+    "libfive_tree_tape_info": (ctypes.c_int, [tree_t, u32p, u32p]),
+    "libfive_tree_deck": (ctypes.c_int, [tree_t, u32p, szp, u32p, fp, szp,
+                                         u32p, u32p]),
+    "libfive_mesh_settings_default": (None,
+                                      [ctypes.POINTER(libfive_mesh_settings)]),
+    "libfive_tree_render_mesh_settings": (
+        ctypes.POINTER(libfive_mesh),
+        [tree_t, libfive_region3, ctypes.POINTER(libfive_mesh_settings)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libfive_amd.so (raises OSError if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is missing: run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().libfive_last_error().decode()

@@ -1,0 +1,227 @@
+"""Python handle for expression trees built through the C ABI.
+
+Follows the reference's Python ``Shape`` wrapper
+(libfive/bind/python/libfive/shape.py): operator overloading builds
+trees with libfive_tree_unary / libfive_tree_binary, and evaluation and
+meshing call the C entry points.  Opcode numbers are the reference's
+(libfive/include/libfive/tree/opcode.hpp:23-61).
+"""
+import ctypes
+
+import numpy as np
+
+from . import ffi
+
+OPS = {
+    "square": 7, "sqrt": 8, "neg": 9, "sin": 10, "cos": 11, "tan": 12,
+    "asin": 13, "acos": 14, "atan": 15, "exp": 16, "add": 17, "mul": 18,
+    "min": 19, "max": 20, "sub": 21, "div": 22, "atan2": 23, "pow": 24,
+    "nth-root": 25, "mod": 26, "nanfill": 27, "abs": 28, "recip": 29,
+    "log": 30, "compare": 31, "const-var": 6,
+}
+
+
+class GpuError(RuntimeError):
+    """An evaluation or meshing call failed (the C API's last error)."""
+
+
+def _f32p(a):
+    return a.ctypes.data_as(ffi.fp)
+
+
+class Shape:
+    __slots__ = ("ptr",)
+
+    def __init__(self, ptr):
+        if not ptr:
+            raise ValueError("invalid tree (NULL)")
+        self.ptr = ptr
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                ffi.lib().libfive_tree_delete(self.ptr)
+        except Exception:
+            pass
+
+    # ---- construction ---------------------------------------------------
+    @staticmethod
+    def X():
+        return Shape(ffi.lib().libfive_tree_x())
+
+    @staticmethod
+    def Y():
+        return Shape(ffi.lib().libfive_tree_y())
+
+    @staticmethod
+    def Z():
+        return Shape(ffi.lib().libfive_tree_z())
+
+    @staticmethod
+    def var():
+        return Shape(ffi.lib().libfive_tree_var())
+
+    @staticmethod
+    def constant(v):
+        return Shape(ffi.lib().libfive_tree_const(float(v)))
+
+    @staticmethod
+    def wrap(v):
+        return v if isinstance(v, Shape) else Shape.constant(v)
+
+    # The wrapped operands are held in locals for the duration of the call:
+    # a temporary Shape would release its tree before the C call runs.
+    @staticmethod
+    def unary(op, a):
+        code = OPS[op] if isinstance(op, str) else op
+        sa = Shape.wrap(a)
+        return Shape(ffi.lib().libfive_tree_unary(code, sa.ptr))
+
+    @staticmethod
+    def binary(op, a, b):
+        code = OPS[op] if isinstance(op, str) else op
+        sa, sb = Shape.wrap(a), Shape.wrap(b)
+        return Shape(ffi.lib().libfive_tree_binary(code, sa.ptr, sb.ptr))
+
+    def __add__(self, o): return Shape.binary("add", self, o)
+    def __radd__(self, o): return Shape.binary("add", o, self)
+    def __sub__(self, o): return Shape.binary("sub", self, o)
+    def __rsub__(self, o): return Shape.binary("sub", o, self)
+    def __mul__(self, o): return Shape.binary("mul", self, o)
+    def __rmul__(self, o): return Shape.binary("mul", o, self)
+    def __truediv__(self, o): return Shape.binary("div", self, o)
+    def __rtruediv__(self, o): return Shape.binary("div", o, self)
+    def __neg__(self): return Shape.unary("neg", self)
+    def __abs__(self): return Shape.unary("abs", self)
+
+    def square(self): return Shape.unary("square", self)
+    def sqrt(self): return Shape.unary("sqrt", self)
+    def sin(self): return Shape.unary("sin", self)
+    def cos(self): return Shape.unary("cos", self)
+    def min(self, o): return Shape.binary("min", self, o)
+    def max(self, o): return Shape.binary("max", self, o)
+
+    # ---- inspection -----------------------------------------------------
+    def __str__(self):
+        L = ffi.lib()
+        p = L.libfive_tree_print(self.ptr)
+        s = ctypes.string_at(p).decode()
+        L.libfive_free_str(p)
+        return s
+
+    def optimized(self):
+        return Shape(ffi.lib().libfive_tree_optimized(self.ptr))
+
+    def deck(self):
+        """Reference-format clause list: (clauses[n,4] root first,
+        constants {id: value}, (X, Y, Z) ids, slot count)."""
+        L = ffi.lib()
+        n = ctypes.c_size_t()
+        nc = ctypes.c_size_t()
+        if L.libfive_tree_deck(self.ptr, None, ctypes.byref(n), None, None,
+                               ctypes.byref(nc), None, None):
+            raise GpuError(ffi.last_error())
+        cl = np.zeros((max(n.value, 1), 4), np.uint32)
+        ci = np.zeros(max(nc.value, 1), np.uint32)
+        cv = np.zeros(max(nc.value, 1), np.float32)
+        xyz = np.zeros(3, np.uint32)
+        ns = ctypes.c_uint32()
+        L.libfive_tree_deck(self.ptr, cl.ctypes.data_as(ffi.u32p),
+                            ctypes.byref(n), ci.ctypes.data_as(ffi.u32p),
+                            _f32p(cv), ctypes.byref(nc),
+                            xyz.ctypes.data_as(ffi.u32p), ctypes.byref(ns))
+        consts = {int(i): float(v) for i, v in zip(ci[:nc.value], cv[:nc.value])}
+        return cl[:n.value].copy(), consts, tuple(int(v) for v in xyz), ns.value
+
+    def tape_info(self):
+        """(clause count, register count) of the compiled tape (host only)."""
+        c = ctypes.c_uint32()
+        r = ctypes.c_uint32()
+        if ffi.lib().libfive_tree_tape_info(self.ptr, ctypes.byref(c),
+                                            ctypes.byref(r)):
+            raise GpuError(ffi.last_error())
+        return c.value, r.value
+
+    # ---- evaluation (GPU) -----------------------------------------------
+    def eval_f(self, x, y, z):
+        return ffi.lib().libfive_tree_eval_f(self.ptr, ffi.libfive_vec3(x, y, z))
+
+    def eval_d(self, x, y, z):
+        v = ffi.lib().libfive_tree_eval_d(self.ptr, ffi.libfive_vec3(x, y, z))
+        return (v.x, v.y, v.z)
+
+    def eval_r(self, lo, hi):
+        r = ffi.libfive_region3(ffi.libfive_interval(lo[0], hi[0]),
+                                ffi.libfive_interval(lo[1], hi[1]),
+                                ffi.libfive_interval(lo[2], hi[2]))
+        i = ffi.lib().libfive_tree_eval_r(self.ptr, r)
+        return (i.lower, i.upper)
+
+    def values(self, x, y, z):
+        x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+        out = np.empty_like(x)
+        if ffi.lib().libfive_tree_eval_array(self.ptr, _f32p(x), _f32p(y),
+                                             _f32p(z), x.size, _f32p(out)):
+            raise GpuError(ffi.last_error())
+        return out
+
+    def derivs(self, x, y, z):
+        x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+        v, dx, dy, dz = (np.empty_like(x) for _ in range(4))
+        if ffi.lib().libfive_tree_eval_deriv_array(
+                self.ptr, _f32p(x), _f32p(y), _f32p(z), x.size, _f32p(v),
+                _f32p(dx), _f32p(dy), _f32p(dz)):
+            raise GpuError(ffi.last_error())
+        return v, dx, dy, dz
+
+    def intervals(self, lo, hi):
+        """lo, hi: arrays of shape (n, 3).  Returns (lower, upper, maybe_nan)."""
+        lo = np.asarray(lo, np.float32)
+        hi = np.asarray(hi, np.float32)
+        cols = [np.ascontiguousarray(a) for a in
+                (lo[:, 0], hi[:, 0], lo[:, 1], hi[:, 1], lo[:, 2], hi[:, 2])]
+        n = lo.shape[0]
+        ol = np.empty(n, np.float32)
+        oh = np.empty(n, np.float32)
+        nan = np.empty(n, np.uint8)
+        if ffi.lib().libfive_tree_eval_interval_array(
+                self.ptr, *[_f32p(c) for c in cols], n, _f32p(ol), _f32p(oh),
+                nan.ctypes.data_as(ffi.u8p)):
+            raise GpuError(ffi.last_error())
+        return ol, oh, nan.astype(bool)
+
+    # ---- meshing (GPU) --------------------------------------------------
+    def render_mesh(self, lo, hi, res=None, min_feature=None, max_err=1e-8,
+                    precision=0):
+        L = ffi.lib()
+        R = ffi.libfive_region3(ffi.libfive_interval(lo[0], hi[0]),
+                                ffi.libfive_interval(lo[1], hi[1]),
+                                ffi.libfive_interval(lo[2], hi[2]))
+        if res is not None and min_feature is None and max_err == 1e-8 \
+                and precision == 0:
+            m = L.libfive_tree_render_mesh(self.ptr, R, float(res))
+        else:
+            s = ffi.libfive_mesh_settings()
+            L.libfive_mesh_settings_default(ctypes.byref(s))
+            if min_feature is not None:
+                s.min_feature = min_feature
+            elif res is not None:
+                s.min_feature = float(np.float32(1) / np.float32(res))
+            s.max_err = max_err
+            s.precision = precision
+            m = L.libfive_tree_render_mesh_settings(self.ptr, R,
+                                                    ctypes.byref(s))
+        if not m:
+            raise GpuError(ffi.last_error())
+        mm = m.contents
+        verts = np.ctypeslib.as_array(
+            ctypes.cast(mm.verts, ffi.fp), (mm.vert_count, 3)).copy()
+        tris = np.ctypeslib.as_array(
+            ctypes.cast(mm.tris, ffi.u32p), (mm.tri_count, 3)).copy()
+        L.libfive_mesh_delete(m)
+        return verts, tris
+
+
+X = Shape.X
+Y = Shape.Y
+Z = Shape.Z

@@ -1,0 +1,178 @@
+"""Front end (tree construction, optimiser, Deck, walk) and C ABI surface,
+checked against the reference's own expectations.  CPU only."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from libfive_amd import Shape, X, Y, Z, ffi
+from libfive_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+# libfive/test/tree.cpp:283-505 (Tree::optimized), printed forms
+@pytest.mark.parametrize("build,expected", [
+    (lambda x, y, z: (z + 2) / (2 * (z + 3)), "(/ (+ z 2) (+ (* z 2) 6))"),
+    (lambda x, y, z: -3 * x.cos() + z + 2, "(- (+ z 2) (* (cos x) 3))"),
+    (lambda x, y, z: x * 2 + y * 5 + z.cos() + 5 * z.cos(),
+     "(+ (* x 2) (* y 5) (* (cos z) 6))"),
+    (lambda x, y, z: (z - 10) / -z, "(/ (- z 10) (- z))"),
+    (lambda x, y, z: (2 * x + y) + (2 * x + y), "(+ (* y 2) (* x 4))"),
+    (lambda x, y, z: (2 * x + y) * (2 * x + y), "(square (+ y (* x 2)))"),
+    (lambda x, y, z: (x + 2 * y) + 3 * (x + 2 * y).cos(),
+     "(+ x (* y 2) (* (cos (+ x (* y 2))) 3))"),
+    (lambda x, y, z: (x + 2 * y + 7) + 3 * (x + 2 * y + 7).sin().cos(),
+     "(+ x (* y 2) (* (cos (sin (+ x (* y 2) 7))) 3) 7)"),
+    (lambda x, y, z: Shape.binary("atan2", -z / (z - 10), -z / (z - 100)),
+     "(atan2 (/ (- z) (- z 10)) (/ (- z) (- z 100)))"),
+    (lambda x, y, z: (-z) * (-z), "(square (- z))"),
+    (lambda x, y, z: -0.091 - (z - 2.7) / 0.6, "(- 4.409 (* z 1.66667))"),
+])
+def test_optimizer_prints(build, expected):
+    t = build(X(), Y(), Z())
+    assert str(t.optimized()) == expected
+
+
+def test_print_unoptimized():
+    # tree.cpp:404-409
+    z = Z()
+    assert str(-0.091 - (z - 2.7) / 0.6) == "(- -0.091 (/ (- z 2.7) 0.6))"
+    assert str(X() + 5) == "(+ x 5)"
+
+
+def n_slots(t):
+    return t.deck()[3]
+
+
+def test_optimizer_sizes():
+    # tree.cpp:375-392: min(X, Y, X, Y) collapses to 3 nodes; the version
+    # with X+1 / Y+1 to 6.
+    x, y = X(), Y()
+    t = (x.min(y)).min(x.min(y))
+    assert len(t.optimized().deck()[0]) == 1         # one MIN clause
+    xa, xb, ya, yb = x + 1, x + 1, y + 1, y + 1
+    t = xa.min(ya).min(xb.min(yb))
+    cl = t.optimized().deck()[0]
+    assert len(cl) == 3                               # 2 ADD + 1 MIN
+
+
+def test_identities():
+    # tree.cpp:640-746 identity and folding rules
+    x = X()
+    assert str(x + 0) == "x"
+    assert str(0 + x) == "x"
+    assert str(x - 0) == "x"
+    assert str(0 - x) == "(- x)"
+    assert str(x * 1) == "x"
+    assert str(1 * x) == "x"
+    assert str(x * -1) == "(- x)"
+    assert str(x * 0) == "0"
+    assert str(x * x) == "(square x)"
+    assert str(x / 1) == "x"
+    assert str(abs(abs(x))) == "(abs x)"
+    assert str(abs(x.square())) == "(square x)"
+    assert str(-(-x)) == "x"
+    assert str(x.min(x)) == "x"
+    assert str(x.max(x)) == "x"
+    assert str(x + (-Y())) == "(- x y)"
+    assert str(x - (-Y())) == "(+ x y)"
+    assert str(Shape.constant(2) + 3) == "5"
+
+
+def test_deck_layout():
+    # libfive/test/deck.cpp:19-41
+    clauses, consts, xyz, slots = (X() + 1).deck()
+    assert slots == 5
+    clauses, consts, xyz, slots = (X() + 2).deck()
+    assert xyz == (3, 4, 5)
+    clauses, consts, xyz, slots = (X() + 5).deck()
+    assert consts == {2: 5.0}
+    assert clauses.tolist() == [[17, 1, 3, 2]]
+
+
+def test_walk_order():
+    # tree.cpp:219-262: leaves first, root last; X and Y before Z.
+    t = X().min(Y()).min(Z().min(1))
+    clauses = t.deck()[0]
+    # root-first list: the root is MIN with id 1
+    assert clauses[0][0] == 19 and clauses[0][1] == 1
+    ids = [int(c[1]) for c in clauses]
+    assert ids == sorted(ids)
+
+
+def test_opcode_enum_and_args():
+    L = ffi.lib()
+    assert L.libfive_opcode_enum(b"min") == 19
+    assert L.libfive_opcode_enum(b"max") == 20
+    assert L.libfive_opcode_enum(b"VAR-X") == 2
+    assert L.libfive_opcode_enum(b"wat") == -1
+    assert L.libfive_opcode_enum(b"") == -1
+    assert L.libfive_opcode_args(17) == 2
+    assert L.libfive_opcode_args(7) == 1
+    assert L.libfive_opcode_args(2) == 0
+    assert L.libfive_opcode_args(40) == -1
+
+
+def test_tree_handles():
+    # libfive/test/api.cpp:29-40: X is a singleton, Y differs
+    L = ffi.lib()
+    a, b, c = L.libfive_tree_x(), L.libfive_tree_x(), L.libfive_tree_y()
+    assert a == b and b != c
+    for t in (a, b, c):
+        L.libfive_tree_delete(t)
+    # invalid arity / opcode give NULL (libfive.cpp:116-142)
+    x = L.libfive_tree_x()
+    assert L.libfive_tree_unary(17, x) is None
+    assert L.libfive_tree_binary(7, x, x) is None
+    assert L.libfive_tree_binary(17, x, None) is None
+    assert L.libfive_tree_nullary(99) is None
+    L.libfive_tree_delete(x)
+
+
+def test_workload_tapes():
+    info = {}
+    for name, f in [("c1", W.csg_c1), ("c2", W.random_dag),
+                    ("c3", W.sphere_union), ("c4", W.gyroid_c4)]:
+        info[name] = f().tape_info()
+    # clause counts match the Deck, registers are small enough for VGPRs
+    assert info["c3"][0] == 10999
+    assert info["c2"][0] == len(W.random_dag().deck()[0])
+    for c, r in info.values():
+        assert r <= 256
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "libfive.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(libfive_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_abi_exports_every_declared_symbol():
+    so = ffi.LIB_PATH
+    out = subprocess.run(["nm", "-D", "--defined-only", so],
+                         capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = [n for n in _declared_symbols() if n not in exported]
+    assert not missing, missing
+    L = ffi.lib()
+    for n in _declared_symbols():
+        assert hasattr(L, n)
+
+
+def test_no_gpu_fails_loudly():
+    """Without a GPU, evaluation entry points report an error instead of
+    computing on the CPU."""
+    import libfive_amd
+    if libfive_amd.gpu_available():
+        pytest.skip("GPU present")
+    t = X() + 1
+    v = t.eval_f(1.0, 2.0, 3.0)
+    assert np.isnan(v)
+    assert "GPU" in ffi.last_error()
+    with pytest.raises(libfive_amd.GpuError):
+        t.values(np.zeros(4), np.zeros(4), np.zeros(4))

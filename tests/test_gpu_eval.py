@@ -1,0 +1,163 @@
+"""GPU parity of the evaluation kernels against the oracle (and golden
+vectors), through the C ABI."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from libfive_amd import Shape, X, Y, Z, ffi
+from libfive_amd import workloads as W
+
+from helpers import BINARY, UNARY, assert_same_bits, op_shape, ulp_distance
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "ieee_ops.npz")
+
+
+@pytest.fixture(scope="module")
+def golden():
+    return np.load(GOLDEN)
+
+
+@pytest.mark.parametrize("name", BINARY + UNARY)
+def test_ieee_ops_golden(gpu, golden, name):
+    a, b = golden["a"], golden["b"]
+    got = op_shape(name).values(a, b, np.zeros_like(a))
+    assert_same_bits(got, golden[name], name)
+
+
+def test_sincos_golden(gpu, golden):
+    x = golden["trig_x"]
+    s = X().sin().values(x, x, x)
+    c = X().cos().values(x, x, x)
+    assert ulp_distance(s, golden["sin_rn"]).max() <= 1
+    assert ulp_distance(c, golden["cos_rn"]).max() <= 1
+    os_, oc = oracle.sincos(x)
+    assert_same_bits(s, os_, "sin vs oracle")
+    assert_same_bits(c, oc, "cos vs oracle")
+
+
+def test_c_api_known_answers(gpu):
+    # libfive/test/api.cpp:52-83
+    L = ffi.lib()
+    a, b = L.libfive_tree_x(), L.libfive_tree_y()
+    c = L.libfive_tree_binary(22, a, b)
+    assert L.libfive_tree_eval_f(c, ffi.libfive_vec3(1, 2, 3)) == 0.5
+    assert L.libfive_tree_eval_f(c, ffi.libfive_vec3(1, 4, 3)) == 0.25
+    assert L.libfive_tree_eval_f(c, ffi.libfive_vec3(1, -1, 3)) == -1
+    d = L.libfive_tree_binary(21, a, b)
+    r = L.libfive_tree_eval_r(d, ffi.libfive_region3(
+        ffi.libfive_interval(1, 2), ffi.libfive_interval(2, 3),
+        ffi.libfive_interval(0, 0)))
+    assert (r.lower, r.upper) == (-2, 0)
+    g = L.libfive_tree_eval_d(c, ffi.libfive_vec3(1, 2, 3))
+    assert (g.x, g.y, g.z) == (0.5, -0.25, 0.0)
+    for t in (a, b, c, d):
+        L.libfive_tree_delete(t)
+
+
+def test_known_answers_gpu(gpu):
+    assert X().eval_f(1, 2, 3) == 1.0
+    assert (X() * 2 + Y() * 3).eval_f(1, 2, 3) == 8.0
+    assert Shape.constant(3.14).eval_f(0, 0, 0) == np.float32(3.14)
+    assert Shape.binary("nth-root", X(), 3).eval_f(-0.5, 0, 0) == \
+        pytest.approx(-0.7937, rel=1e-4)
+    assert (X() + 1).eval_r((1, 1, 1), (2, 2, 2)) == (2.0, 3.0)
+    assert ((X() + Y()) * (X() - Y())).eval_r((0, 0, 0), (1, 1, 1)) == (-2.0, 2.0)
+    lo, hi = (0.1 / Y()).eval_r((-0.625, -0.3125, -0.625), (-0.3125, 0, -0.2125))
+    assert lo == -math.inf and hi == math.inf
+
+
+SHAPES = {
+    "c1_csg": lambda: W.csg_c1(),
+    "c4_gyroid": lambda: W.gyroid_c4(),
+    "c2_dag": lambda: W.random_dag(),
+    "dag_small": lambda: W.random_dag(120, seed=5),
+    "spheres50": lambda: W.sphere_union(50, seed=3),
+    "all_ops": lambda: (Shape.binary("atan2", X(), Y() + 0.5)
+                        + Shape.binary("mod", X() * 7, Y() + 2)
+                        + Shape.binary("pow", X(), 3)
+                        + Shape.binary("nth-root", Y() * 4, 3)
+                        + X().sin() * Z().cos() + abs(Y()).sqrt()
+                        + Shape.unary("exp", Z()) + Shape.unary("log", abs(X()) + 1)
+                        + Shape.unary("atan", Y()) + Shape.unary("tan", Z() * 0.5)
+                        + Shape.unary("asin", X() * 0.5)
+                        + Shape.unary("acos", Y() * 0.5)
+                        + Shape.binary("compare", X(), Y())
+                        + Shape.binary("nanfill", (X() - 0.5).sqrt(), Y())
+                        + Shape.unary("recip", Z() + 3)),
+}
+
+
+def points(n, seed, scale=1.2):
+    rng = np.random.default_rng(seed)
+    p = rng.uniform(-scale, scale, (3, n)).astype(np.float32)
+    p[:, :9] = np.array([[0, 0.5, -0.5, 1, -1, 0.25, 0, 0, 1e-20]] * 3,
+                        np.float32)
+    return p
+
+
+@pytest.mark.parametrize("name", sorted(SHAPES))
+def test_values_match_oracle(gpu, name):
+    s = SHAPES[name]()
+    p = points(20000, hash(name) & 0xffff)
+    got = s.values(*p)
+    ref = oracle.OracleDeck(s.deck()).values(*p)
+    assert_same_bits(got, ref, name)
+
+
+@pytest.mark.parametrize("name", sorted(SHAPES))
+def test_derivs_match_oracle(gpu, name):
+    s = SHAPES[name]()
+    p = points(20000, (hash(name) + 1) & 0xffff)
+    got = s.derivs(*p)
+    ref = oracle.OracleDeck(s.deck()).derivs(*p)
+    for k, nm in enumerate(("v", "dx", "dy", "dz")):
+        if name == "all_ops" and nm != "v":
+            # pow / tan / nth-root partials use double-precision libm
+            # values on both sides; compare within 4 ulp
+            assert ulp_distance(got[k], ref[k]).max() <= 4, nm
+        else:
+            assert_same_bits(got[k], ref[k], f"{name}:{nm}")
+
+
+@pytest.mark.parametrize("name", sorted(SHAPES))
+def test_intervals_match_oracle(gpu, name):
+    s = SHAPES[name]()
+    rng = np.random.default_rng(11)
+    c = rng.uniform(-1.2, 1.2, (4000, 3)).astype(np.float32)
+    w = (rng.uniform(0, 1, (4000, 3)) ** 3 * 0.6).astype(np.float32)
+    lo, hi = c - w, c + w
+    gl, gh, gn = s.intervals(lo, hi)
+    rl, rh, rn = oracle.OracleDeck(s.deck()).intervals(lo, hi)
+    assert (gn == rn).all()
+    assert_same_bits(gl, rl, name + ":lower")
+    assert_same_bits(gh, rh, name + ":upper")
+
+
+def test_grid_c2_exact_and_enclosed(gpu):
+    """C2 DAG on a 64^3 grid (the bench runs 256^3): bit-exact vs the
+    oracle, including sin/cos."""
+    s = W.random_dag()
+    x, y, z = W.grid_points(64)
+    got = s.values(x, y, z)
+    ref = oracle.OracleDeck(s.deck()).values(x, y, z, threads=8)
+    assert_same_bits(got, ref, "c2 grid")
+
+
+def test_directed_rounding_div_sqrt(gpu):
+    """Interval bounds of x / y and sqrt(x) for random point boxes equal the
+    oracle's fesetround results bit for bit."""
+    rng = np.random.default_rng(13)
+    n = 20000
+    a = (rng.standard_normal(n) * 10.0 ** rng.integers(-6, 6, n)).astype(np.float32)
+    b = (rng.standard_normal(n) * 10.0 ** rng.integers(-6, 6, n)).astype(np.float32)
+    lo = np.stack([a, b, np.zeros(n)], 1).astype(np.float32)
+    for s in (X() / Y(), abs(X()).sqrt(), X() * Y(), X() + Y(), X() - Y()):
+        gl, gh, _ = s.intervals(lo, lo)
+        rl, rh, _ = oracle.OracleDeck(s.deck()).intervals(lo, lo)
+        assert_same_bits(gl, rl, str(s))
+        assert_same_bits(gh, rh, str(s))

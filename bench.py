@@ -45,6 +45,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true",
                    help="skip the CPU baseline leg")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
+    p.add_argument("--sweep", action="store_true",
+                   help="also time the value kernel at 1, 2 and 4 points "
+                        "per lane")
     return p.parse_args()
 
 
@@ -114,7 +117,10 @@ def main():
     zd = torch.from_numpy(z).to(dev)
     out = torch.empty((4, n), dtype=torch.float32, device=dev)
     vout = torch.empty(n, dtype=torch.float32, device=dev)
-    stream = torch.cuda.current_stream()
+    # A dedicated stream (default priority): the kernels and the timing
+    # events must be on the same stream.
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
     sp = stream.cuda_stream
 
     def deriv_step():
@@ -150,6 +156,13 @@ def main():
 
     wall, kern_ms = timed(deriv_step, args.steps, args.warmup)
     v_wall, v_kern_ms = timed(value_step, args.steps, args.warmup)
+    sweep = {}
+    if args.sweep:
+        for ppl in (1, 2, 4):
+            L.libfive_set_points_per_lane(ppl)
+            w_, k_ = timed(value_step, args.steps, args.warmup)
+            sweep[str(ppl)] = round(n * T / (k_ * 1e-3) / 1e9, 2)
+        L.libfive_set_points_per_lane(0)
 
     ms_per_step = wall * 1e3 / args.steps
     value = n * T / (wall / args.steps) / 1e9
@@ -174,6 +187,7 @@ def main():
             "values_only_gpoint_op_s": round(
                 n * T / (v_wall / args.steps) / 1e9, 2),
             "values_only_kernel_ms": round(v_kern_ms, 4),
+            "values_only_by_points_per_lane": sweep or None,
         },
         "roofline": {
             "bound": "valu",

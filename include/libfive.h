@@ -150,6 +150,10 @@ int libfive_tape_eval_derivs_device(libfive_tape t, const float* x,
                                     float* value, float* dx, float* dy,
                                     float* dz, void* stream);
 
+/* Points evaluated per GPU lane by the value kernel (1, 2 or 4; 0 picks by
+ * register count).  Also read from LIBFIVE_POINTS_PER_LANE at start-up. */
+void libfive_set_points_per_lane(int p);
+
 /* Compiles a tree on the host only (no GPU needed) and reports the tape's
  * clause count (reference meaning) and register count.  Returns 0 on
  * success. */

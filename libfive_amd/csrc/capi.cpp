@@ -337,6 +337,8 @@ int libfive_tape_eval_derivs_device(libfive_tape t, const float* x,
     }
 }
 
+void libfive_set_points_per_lane(int p) { set_points_per_lane(p); }
+
 int libfive_tree_tape_info(libfive_tree t, uint32_t* clauses,
                            uint32_t* registers) {
     try {

@@ -2,6 +2,7 @@
 // compilation.
 #include "deck.hpp"
 
+#include <algorithm>
 #include <cassert>
 #include <stdexcept>
 #include <unordered_map>
@@ -180,13 +181,73 @@ DeviceTape compile_deck(const Deck& deck, const std::vector<float>* var_values) 
     };
     uint32_t extra = uint32_t(deck.num_clauses) + 1 + kFirstFreeReg;
 
+    // Evaluation order.  The reference evaluates its clause list backwards
+    // (leaves first).  Any topological order gives the same values, so the
+    // clauses are re-ordered by Sethi-Ullman labels: the operand that needs
+    // more registers is evaluated first, which minimises the register file
+    // for tree-shaped expressions.
+    std::vector<int32_t> clause_of(deck.num_clauses + 1, -1);
+    for (size_t i = 0; i < deck.clauses.size(); ++i) {
+        clause_of[deck.clauses[i].id] = int32_t(i);
+    }
+    std::vector<uint32_t> need(deck.clauses.size(), 0);
+    // Labels are computed over the reverse (leaves-first) list, where every
+    // operand precedes its users (operands have larger slot ids).
+    for (size_t k = deck.clauses.size(); k-- > 0;) {
+        const RefClause& rc = deck.clauses[k];
+        const int n = Opcode::args(Opcode::Opcode(rc.op));
+        auto child_need = [&](uint32_t s) -> uint32_t {
+            const int32_t ci = clause_of[s];
+            return ci < 0 ? 0u : need[ci];
+        };
+        if (n == 1) {
+            need[k] = std::max<uint32_t>(1, child_need(rc.a));
+        } else if (n == 2) {
+            const uint32_t na = child_need(rc.a), nb = child_need(rc.b);
+            need[k] = (na == nb) ? std::max<uint32_t>(1, na + 1)
+                                 : std::max(na, nb);
+        } else {
+            need[k] = 1;
+        }
+    }
+    std::vector<uint32_t> order;
+    order.reserve(deck.clauses.size());
+    {
+        std::vector<uint8_t> state(deck.clauses.size(), 0);   // 1 open, 2 done
+        std::vector<int32_t> stack;
+        if (!deck.clauses.empty()) stack.push_back(0);        // the root
+        while (!stack.empty()) {
+            const int32_t k = stack.back();
+            if (state[k] == 2) { stack.pop_back(); continue; }
+            const RefClause& rc = deck.clauses[k];
+            const int n = Opcode::args(Opcode::Opcode(rc.op));
+            int32_t kids[2] = {-1, -1};
+            if (n >= 1) kids[0] = clause_of[rc.a];
+            if (n >= 2) kids[1] = clause_of[rc.b];
+            if (state[k] == 0) {
+                state[k] = 1;
+                // push so that the larger-need child is evaluated first
+                int32_t first = kids[0], second = kids[1];
+                if (second >= 0 && (first < 0 || need[second] > need[first])) {
+                    std::swap(first, second);
+                }
+                if (second >= 0 && state[second] == 0) stack.push_back(second);
+                if (first >= 0 && state[first] == 0) stack.push_back(first);
+                continue;
+            }
+            state[k] = 2;
+            stack.pop_back();
+            order.push_back(uint32_t(k));
+        }
+    }
+
     DeviceTape pre;
     for (uint32_t v : deck.vars) {
         pre.clauses.push_back(make(OP_LOADI, MODE_A_IMM, prereg(v), 0, 0,
                                    cval[v], v));
     }
-    for (auto it = deck.clauses.rbegin(); it != deck.clauses.rend(); ++it) {
-        const RefClause& rc = *it;
+    for (uint32_t k : order) {
+        const RefClause& rc = deck.clauses[k];
         const int n = Opcode::args(Opcode::Opcode(rc.op));
         uint32_t m = MODE_RR, a = 0, b = 0;
         float im = 0.0f;

@@ -26,58 +26,225 @@ namespace amd {
 using namespace tape;
 namespace OC = Opcode;
 
-// Value of one clause given its operand values (b is ignored by unary
-// opcodes).
-__device__ __forceinline__ float apply_value(uint32_t o, float a, float b,
-                                             float im) {
+// Values of one clause for P points (b is ignored by unary opcodes).  The
+// common opcodes are inline; the rest go through the out-of-line scalar
+// table so that their register demand does not lower occupancy.
+template <int P>
+__device__ __forceinline__ void apply_value(uint32_t o, const float* a,
+                                            const float* b, float im,
+                                            float* out) {
+#define LF_EACH for (int p = 0; p < P; ++p)
     switch (o) {
-        case OC::OP_ADD: return a + b;
-        case OC::OP_MUL: return a * b;
-        case OC::OP_MIN: return (b < a) ? b : a;
-        case OC::OP_MAX: return (a < b) ? b : a;
-        case OC::OP_SUB: return a - b;
-        case OC::OP_NEG: return -a;
-        case OC::OP_SQUARE: return a * a;
-        case OC::OP_ABS: return fabsf(a);
-        case OC::OP_SQRT: return sqrtf(a);
-        case OC::OP_SIN: return fmath::sin_f32(a);
-        case OC::OP_COS: return fmath::cos_f32(a);
-        case OC::OP_DIV: return a / b;
-        case OP_LOADI: return im;
+        case OC::OP_ADD: LF_EACH out[p] = a[p] + b[p]; break;
+        case OC::OP_MUL: LF_EACH out[p] = a[p] * b[p]; break;
+        case OC::OP_MIN: LF_EACH out[p] = (b[p] < a[p]) ? b[p] : a[p]; break;
+        case OC::OP_MAX: LF_EACH out[p] = (a[p] < b[p]) ? b[p] : a[p]; break;
+        case OC::OP_SUB: LF_EACH out[p] = a[p] - b[p]; break;
+        case OC::OP_NEG: LF_EACH out[p] = -a[p]; break;
+        case OC::OP_SQUARE: LF_EACH out[p] = a[p] * a[p]; break;
+        case OC::OP_ABS: LF_EACH out[p] = fabsf(a[p]); break;
+        case OC::OP_SQRT: LF_EACH out[p] = sqrtf(a[p]); break;
+        case OC::OP_SIN: LF_EACH out[p] = fmath::sin_f32(a[p]); break;
+        case OC::OP_COS: LF_EACH out[p] = fmath::cos_f32(a[p]); break;
+        case OC::OP_DIV: LF_EACH out[p] = a[p] / b[p]; break;
+        case OP_LOADI: LF_EACH out[p] = im; break;
         case OP_COPY:
-        case OC::CONST_VAR: return a;
-        default: return fmath::eval_op(int(o), a, b);
+        case OC::CONST_VAR: LF_EACH out[p] = a[p]; break;
+        default: LF_EACH out[p] = fmath::eval_op(int(o), a[p], b[p]); break;
     }
+#undef LF_EACH
 }
 
-template <int NV>
+// Register access for point p (a compile-time constant after unrolling).
+#define LF_PGET(p, NV, i)                                                    \
+    ((p) == 0 ? LF_RF_GET(r0, NV, i) : (p) == 1 ? LF_RF_GET(r1, NV, i)       \
+     : (p) == 2 ? LF_RF_GET(r2, NV, i) : LF_RF_GET(r3, NV, i))
+#define LF_PSET(p, NV, i, x)                                                 \
+    do {                                                                     \
+        if ((p) == 0) LF_RF_SET(r0, NV, i, x);                               \
+        else if ((p) == 1) LF_RF_SET(r1, NV, i, x);                          \
+        else if ((p) == 2) LF_RF_SET(r2, NV, i, x);                          \
+        else LF_RF_SET(r3, NV, i, x);                                        \
+    } while (0)
+
+// One clause for P points.
+#define LF_VALUE_STEP(C)                                                     \
+    do {                                                                     \
+        const Clause c_ = (C);                                               \
+        const uint32_t o_ = tape::op(c_);                                    \
+        if (o_ != OP_NOP) {                                                  \
+            const uint32_t m_ = mode(c_);                                    \
+            const float im_ = imm(c_);                                       \
+            float a_[P], b_[P], q_[P];                                       \
+            if (m_ == MODE_A_IMM) {                                          \
+                _Pragma("unroll") for (int p = 0; p < P; ++p) a_[p] = im_;   \
+            } else {                                                         \
+                const uint32_t ia_ = ra(c_);                                 \
+                _Pragma("unroll") for (int p = 0; p < P; ++p)                \
+                    a_[p] = LF_PGET(p, NV, ia_);                             \
+            }                                                                \
+            if (!is_binary(c_)) {                                            \
+                _Pragma("unroll") for (int p = 0; p < P; ++p) b_[p] = 0.0f;  \
+            } else if (m_ == MODE_B_IMM) {                                   \
+                _Pragma("unroll") for (int p = 0; p < P; ++p) b_[p] = im_;   \
+            } else {                                                         \
+                const uint32_t ib_ = rb(c_);                                 \
+                _Pragma("unroll") for (int p = 0; p < P; ++p)                \
+                    b_[p] = LF_PGET(p, NV, ib_);                             \
+            }                                                                \
+            apply_value<P>(o_, a_, b_, im_, q_);                             \
+            const uint32_t d_ = dst(c_);                                     \
+            _Pragma("unroll") for (int p = 0; p < P; ++p)                    \
+                LF_PSET(p, NV, d_, q_[p]);                                   \
+        }                                                                    \
+    } while (0)
+
+struct Clause4 {
+    Clause c[kFetch];
+};
+
+// P points per lane: lane l of block b handles points b*256*P + l + 256*j.
+// The tape is read kFetch clauses at a time, the next batch requested
+// before the current one is interpreted.
+template <int NV, int P>
 __global__ __launch_bounds__(256)
 void k_eval_values(const Clause* __restrict__ t, uint32_t n, uint32_t root,
                    const float* __restrict__ xs, const float* __restrict__ ys,
                    const float* __restrict__ zs, float* __restrict__ out,
                    size_t npts)
 {
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= npts) return;
-    LF_RF_DECL(r);
-    LF_RF_SET(r, NV, kRegX, xs[i]);
-    LF_RF_SET(r, NV, kRegY, ys[i]);
-    LF_RF_SET(r, NV, kRegZ, zs[i]);
-    for (uint32_t k = 0; k < n; ++k) {
-        const Clause c = t[k];
-        const uint32_t o = tape::op(c);
-        const uint32_t m = mode(c);
-        const float im = imm(c);
-        const float a = (m == MODE_A_IMM) ? im : LF_RF_GET(r, NV, ra(c));
-        const float b = (nargs(o) != 2) ? 0.0f
-                      : (m == MODE_B_IMM) ? im : LF_RF_GET(r, NV, rb(c));
-        LF_RF_SET(r, NV, dst(c), apply_value(o, a, b, im));
+    const size_t base = size_t(blockIdx.x) * (256u * P) + threadIdx.x;
+    LF_RF_DECL(r0);
+    LF_RF_DECL(r1);
+    LF_RF_DECL(r2);
+    LF_RF_DECL(r3);
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const size_t i = base + 256u * p;
+        const bool ok = i < npts;
+        LF_PSET(p, NV, kRegX, ok ? xs[i] : 0.0f);
+        LF_PSET(p, NV, kRegY, ok ? ys[i] : 0.0f);
+        LF_PSET(p, NV, kRegZ, ok ? zs[i] : 0.0f);
     }
-    out[i] = LF_RF_GET(r, NV, root);
+    const Clause4* t4 = reinterpret_cast<const Clause4*>(t);
+    Clause4 cur = t4[0];
+    for (uint32_t k = 0; k < n; k += kFetch) {
+        const Clause4 nxt = t4[k / kFetch + 1];
+        LF_VALUE_STEP(cur.c[0]);
+        LF_VALUE_STEP(cur.c[1]);
+        LF_VALUE_STEP(cur.c[2]);
+        LF_VALUE_STEP(cur.c[3]);
+        cur = nxt;
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const size_t i = base + 256u * p;
+        const float v = LF_PGET(p, NV, root);
+        if (i < npts) out[i] = v;
+    }
 }
 
-// Value and partial derivatives of one clause (eval_deriv_array.cpp:119-256;
-// the value is the same as apply_value).
+// Value and partial derivatives of one clause for the opcodes outside the
+// common set (eval_deriv_array.cpp:119-256); out of line.
+__device__ __noinline__ void cold_deriv(uint32_t o, float av, float ad0,
+                                        float ad1, float ad2, float bv,
+                                        float bd0, float bd1, float bd2,
+                                        float* res) {
+    const float ad[3] = {ad0, ad1, ad2};
+    const float bd[3] = {bd0, bd1, bd2};
+    float ov;
+    float od[3];
+    switch (o) {
+        case OC::OP_ATAN2: {
+            ov = fmath::atan2_f32(av, bv);
+            const float a2 = av * av;
+            const float b2 = bv * bv;
+            const float den = a2 + b2;
+            for (int j = 0; j < 3; ++j) {
+                const float p = ad[j] * bv;
+                const float q = bd[j] * av;
+                od[j] = (p - q) / den;
+            }
+            break;
+        }
+        case OC::OP_POW: {
+            ov = fmath::pow_f32(av, bv);
+            const float f = bv * fmath::pow_f32(av, bv - 1.0f);
+            for (int j = 0; j < 3; ++j) od[j] = ad[j] * f;
+            break;
+        }
+        case OC::OP_NTH_ROOT: {
+            ov = fmath::nth_root_f32(av, bv);
+            const float f = fmath::pow_f32(av, 1.0f / bv - 1.0f) / bv;
+            for (int j = 0; j < 3; ++j) od[j] = (ad[j] == 0.0f) ? 0.0f : ad[j] * f;
+            break;
+        }
+        case OC::OP_MOD:
+            ov = fmath::mod_f32(av, bv);
+            for (int j = 0; j < 3; ++j) od[j] = ad[j];
+            break;
+        case OC::OP_NANFILL: {
+            const bool n_ = fmath::isnan_f(av);
+            ov = n_ ? bv : av;
+            for (int j = 0; j < 3; ++j) od[j] = n_ ? bd[j] : ad[j];
+            break;
+        }
+        case OC::OP_COMPARE:
+            ov = (av < bv) ? -1.0f : (av > bv) ? 1.0f : 0.0f;
+            for (int j = 0; j < 3; ++j) od[j] = 0.0f;
+            break;
+        case OC::OP_TAN: {
+            ov = fmath::tan_f32(av);
+            const float r = 1.0f / fmath::cos_f32(av);
+            const float f = r * r;
+            for (int j = 0; j < 3; ++j) od[j] = ad[j] * f;
+            break;
+        }
+        case OC::OP_ASIN: {
+            ov = fmath::asin_f32(av);
+            const float s = sqrtf(1.0f - av * av);
+            for (int j = 0; j < 3; ++j) od[j] = ad[j] / s;
+            break;
+        }
+        case OC::OP_ACOS: {
+            ov = fmath::acos_f32(av);
+            const float s = -sqrtf(1.0f - av * av);
+            for (int j = 0; j < 3; ++j) od[j] = ad[j] / s;
+            break;
+        }
+        case OC::OP_ATAN: {
+            ov = fmath::atan_f32(av);
+            const float s = av * av + 1.0f;
+            for (int j = 0; j < 3; ++j) od[j] = ad[j] / s;
+            break;
+        }
+        case OC::OP_LOG:
+            ov = fmath::log_f32(av);
+            for (int j = 0; j < 3; ++j) od[j] = ad[j] / av;
+            break;
+        case OC::OP_EXP:
+            ov = fmath::exp_f32(av);
+            for (int j = 0; j < 3; ++j) od[j] = ad[j] * ov;
+            break;
+        case OC::OP_RECIP: {
+            ov = 1.0f / av;
+            const float s = -(av * av);
+            for (int j = 0; j < 3; ++j) od[j] = ad[j] / s;
+            break;
+        }
+        default:
+            ov = av;
+            for (int j = 0; j < 3; ++j) od[j] = ad[j];
+            break;
+    }
+    res[0] = ov;
+    res[1] = od[0];
+    res[2] = od[1];
+    res[3] = od[2];
+}
+
+// Value and partial derivatives of one clause (eval_deriv_array.cpp:
+// 119-256); the value is the same as apply_value's.
 __device__ __forceinline__ void apply_deriv(uint32_t o, float av,
                                             const float ad[3], float bv,
                                             const float bd[3], float im,
@@ -121,44 +288,6 @@ __device__ __forceinline__ void apply_deriv(uint32_t o, float av,
             }
             break;
         }
-        case OC::OP_ATAN2: {
-            ov = fmath::atan2_f32(av, bv);
-            const float a2 = av * av;
-            const float b2 = bv * bv;
-            const float den = a2 + b2;
-            for (int j = 0; j < 3; ++j) {
-                const float p = ad[j] * bv;
-                const float q = bd[j] * av;
-                od[j] = (p - q) / den;
-            }
-            break;
-        }
-        case OC::OP_POW: {
-            ov = fmath::pow_f32(av, bv);
-            const float f = bv * fmath::pow_f32(av, bv - 1.0f);
-            for (int j = 0; j < 3; ++j) od[j] = ad[j] * f;
-            break;
-        }
-        case OC::OP_NTH_ROOT: {
-            ov = fmath::nth_root_f32(av, bv);
-            const float f = fmath::pow_f32(av, 1.0f / bv - 1.0f) / bv;
-            for (int j = 0; j < 3; ++j) od[j] = (ad[j] == 0.0f) ? 0.0f : ad[j] * f;
-            break;
-        }
-        case OC::OP_MOD:
-            ov = fmath::mod_f32(av, bv);
-            for (int j = 0; j < 3; ++j) od[j] = ad[j];
-            break;
-        case OC::OP_NANFILL: {
-            const bool n_ = fmath::isnan_f(av);
-            ov = n_ ? bv : av;
-            for (int j = 0; j < 3; ++j) od[j] = n_ ? bd[j] : ad[j];
-            break;
-        }
-        case OC::OP_COMPARE:
-            ov = (av < bv) ? -1.0f : (av > bv) ? 1.0f : 0.0f;
-            for (int j = 0; j < 3; ++j) od[j] = 0.0f;
-            break;
         case OC::OP_SQUARE:
             ov = av * av;
             for (int j = 0; j < 3; ++j) od[j] = (ad[j] * av) * 2.0f;
@@ -175,6 +304,10 @@ __device__ __forceinline__ void apply_deriv(uint32_t o, float av,
             ov = -av;
             for (int j = 0; j < 3; ++j) od[j] = -ad[j];
             break;
+        case OC::OP_ABS:
+            ov = fabsf(av);
+            for (int j = 0; j < 3; ++j) od[j] = (av > 0.0f) ? ad[j] : -ad[j];
+            break;
         case OC::OP_SIN: {
             float s, co;
             fmath::sin_cos_f32(av, &s, &co);
@@ -190,59 +323,63 @@ __device__ __forceinline__ void apply_deriv(uint32_t o, float av,
             for (int j = 0; j < 3; ++j) od[j] = ad[j] * ns;
             break;
         }
-        case OC::OP_TAN: {
-            ov = fmath::tan_f32(av);
-            const float r = 1.0f / fmath::cos_f32(av);
-            const float f = r * r;
-            for (int j = 0; j < 3; ++j) od[j] = ad[j] * f;
-            break;
-        }
-        case OC::OP_ASIN: {
-            ov = fmath::asin_f32(av);
-            const float s = sqrtf(1.0f - av * av);
-            for (int j = 0; j < 3; ++j) od[j] = ad[j] / s;
-            break;
-        }
-        case OC::OP_ACOS: {
-            ov = fmath::acos_f32(av);
-            const float s = -sqrtf(1.0f - av * av);
-            for (int j = 0; j < 3; ++j) od[j] = ad[j] / s;
-            break;
-        }
-        case OC::OP_ATAN: {
-            ov = fmath::atan_f32(av);
-            const float s = av * av + 1.0f;
-            for (int j = 0; j < 3; ++j) od[j] = ad[j] / s;
-            break;
-        }
-        case OC::OP_LOG:
-            ov = fmath::log_f32(av);
-            for (int j = 0; j < 3; ++j) od[j] = ad[j] / av;
-            break;
-        case OC::OP_EXP:
-            ov = fmath::exp_f32(av);
-            for (int j = 0; j < 3; ++j) od[j] = ad[j] * ov;
-            break;
-        case OC::OP_ABS:
-            ov = fabsf(av);
-            for (int j = 0; j < 3; ++j) od[j] = (av > 0.0f) ? ad[j] : -ad[j];
-            break;
-        case OC::OP_RECIP: {
-            ov = 1.0f / av;
-            const float s = -(av * av);
-            for (int j = 0; j < 3; ++j) od[j] = ad[j] / s;
-            break;
-        }
         case OP_LOADI:
             ov = im;
             for (int j = 0; j < 3; ++j) od[j] = 0.0f;
             break;
-        default:    // COPY, CONST_VAR
+        case OP_COPY:
+        case OC::CONST_VAR:
             ov = av;
             for (int j = 0; j < 3; ++j) od[j] = ad[j];
             break;
+        default: {
+            float res[4];
+            cold_deriv(o, av, ad[0], ad[1], ad[2], bv, bd[0], bd[1], bd[2], res);
+            ov = res[0];
+            od[0] = res[1];
+            od[1] = res[2];
+            od[2] = res[3];
+            break;
+        }
     }
 }
+
+#define LF_DERIV_STEP(C)                                                     \
+    do {                                                                     \
+        const Clause c_ = (C);                                               \
+        const uint32_t o_ = tape::op(c_);                                    \
+        if (o_ != OP_NOP) {                                                  \
+            const uint32_t m_ = mode(c_);                                    \
+            const float im_ = imm(c_);                                       \
+            float av_ = im_, ad_[3] = {0.0f, 0.0f, 0.0f};                    \
+            float bv_ = 0.0f, bd_[3] = {0.0f, 0.0f, 0.0f};                   \
+            if (m_ != MODE_A_IMM && o_ != OP_LOADI) {                        \
+                const uint32_t a_ = ra(c_);                                  \
+                av_ = LF_RF_GET(v, NV, a_);                                  \
+                ad_[0] = LF_RF_GET(gx, NV, a_);                              \
+                ad_[1] = LF_RF_GET(gy, NV, a_);                              \
+                ad_[2] = LF_RF_GET(gz, NV, a_);                              \
+            }                                                                \
+            if (is_binary(c_)) {                                             \
+                if (m_ == MODE_B_IMM) {                                      \
+                    bv_ = im_;                                               \
+                } else {                                                     \
+                    const uint32_t b_ = rb(c_);                              \
+                    bv_ = LF_RF_GET(v, NV, b_);                              \
+                    bd_[0] = LF_RF_GET(gx, NV, b_);                          \
+                    bd_[1] = LF_RF_GET(gy, NV, b_);                          \
+                    bd_[2] = LF_RF_GET(gz, NV, b_);                          \
+                }                                                            \
+            }                                                                \
+            float ov_, od_[3];                                               \
+            apply_deriv(o_, av_, ad_, bv_, bd_, im_, ov_, od_);              \
+            const uint32_t d_ = dst(c_);                                     \
+            LF_RF_SET(v, NV, d_, ov_);                                       \
+            LF_RF_SET(gx, NV, d_, od_[0]);                                   \
+            LF_RF_SET(gy, NV, d_, od_[1]);                                   \
+            LF_RF_SET(gz, NV, d_, od_[2]);                                   \
+        }                                                                    \
+    } while (0)
 
 template <int NV>
 __global__ __launch_bounds__(256)
@@ -253,61 +390,62 @@ void k_eval_derivs(const Clause* __restrict__ t, uint32_t n, uint32_t root,
                    float* __restrict__ odz, size_t npts)
 {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= npts) return;
+    const bool ok = i < npts;
     LF_RF_DECL(v);
     LF_RF_DECL(gx);
     LF_RF_DECL(gy);
     LF_RF_DECL(gz);
-    LF_RF_SET(v, NV, kRegX, xs[i]);
-    LF_RF_SET(v, NV, kRegY, ys[i]);
-    LF_RF_SET(v, NV, kRegZ, zs[i]);
+    LF_RF_SET(v, NV, kRegX, ok ? xs[i] : 0.0f);
+    LF_RF_SET(v, NV, kRegY, ok ? ys[i] : 0.0f);
+    LF_RF_SET(v, NV, kRegZ, ok ? zs[i] : 0.0f);
     LF_RF_SET(gx, NV, kRegX, 1.0f);
     LF_RF_SET(gy, NV, kRegY, 1.0f);
     LF_RF_SET(gz, NV, kRegZ, 1.0f);
-    for (uint32_t k = 0; k < n; ++k) {
-        const Clause c = t[k];
-        const uint32_t o = tape::op(c);
-        const uint32_t m = mode(c);
-        const float im = imm(c);
-        float av = im, ad[3] = {0.0f, 0.0f, 0.0f};
-        float bv = 0.0f, bd[3] = {0.0f, 0.0f, 0.0f};
-        if (m != MODE_A_IMM && o != OP_LOADI) {
-            const uint32_t a = ra(c);
-            av = LF_RF_GET(v, NV, a);
-            ad[0] = LF_RF_GET(gx, NV, a);
-            ad[1] = LF_RF_GET(gy, NV, a);
-            ad[2] = LF_RF_GET(gz, NV, a);
-        }
-        if (nargs(o) == 2) {
-            if (m == MODE_B_IMM) {
-                bv = im;
-            } else {
-                const uint32_t b = rb(c);
-                bv = LF_RF_GET(v, NV, b);
-                bd[0] = LF_RF_GET(gx, NV, b);
-                bd[1] = LF_RF_GET(gy, NV, b);
-                bd[2] = LF_RF_GET(gz, NV, b);
-            }
-        }
-        float ov, od[3];
-        apply_deriv(o, av, ad, bv, bd, im, ov, od);
-        const uint32_t d = dst(c);
-        LF_RF_SET(v, NV, d, ov);
-        LF_RF_SET(gx, NV, d, od[0]);
-        LF_RF_SET(gy, NV, d, od[1]);
-        LF_RF_SET(gz, NV, d, od[2]);
+    const Clause4* t4 = reinterpret_cast<const Clause4*>(t);
+    Clause4 cur = t4[0];
+    for (uint32_t k = 0; k < n; k += kFetch) {
+        const Clause4 nxt = t4[k / kFetch + 1];
+        LF_DERIV_STEP(cur.c[0]);
+        LF_DERIV_STEP(cur.c[1]);
+        LF_DERIV_STEP(cur.c[2]);
+        LF_DERIV_STEP(cur.c[3]);
+        cur = nxt;
     }
-    ov_out[i] = LF_RF_GET(v, NV, root);
-    odx[i] = LF_RF_GET(gx, NV, root);
-    ody[i] = LF_RF_GET(gy, NV, root);
-    odz[i] = LF_RF_GET(gz, NV, root);
+    if (ok) {
+        ov_out[i] = LF_RF_GET(v, NV, root);
+        odx[i] = LF_RF_GET(gx, NV, root);
+        ody[i] = LF_RF_GET(gy, NV, root);
+        odz[i] = LF_RF_GET(gz, NV, root);
+    }
 }
 
 ////////////////////////////////////////////////////////////////////////////////
 // Launchers
 
 static int banks_for(uint32_t nregs) {
-    return int((nregs + 31) / 32);
+    const uint32_t b = (nregs + 31) / 32;
+    return b <= 1 ? 1 : b <= 2 ? 2 : b <= 4 ? 4 : b <= 8 ? 8 : 0;
+}
+
+static int g_points_per_lane = 0;     // 0: choose by bank count
+
+void set_points_per_lane(int p) { g_points_per_lane = p; }
+
+static int points_for(int nv) {
+    int p = g_points_per_lane;
+    if (p <= 0) p = (nv == 1) ? 2 : 1;
+    if (nv >= 4) p = 1;
+    if (nv == 2 && p > 2) p = 2;
+    return (p >= 4) ? 4 : (p >= 2) ? 2 : 1;
+}
+
+template <int NV, int P>
+static void launch_v(const Clause* t, uint32_t n, uint32_t root,
+                     const float* x, const float* y, const float* z,
+                     float* out, size_t npts, hipStream_t s) {
+    const dim3 block(256);
+    const dim3 grid(unsigned((npts + 256u * P - 1) / (256u * P)));
+    k_eval_values<NV, P><<<grid, block, 0, s>>>(t, n, root, x, y, z, out, npts);
 }
 
 hipError_t launch_eval_values(const Clause* t, uint32_t n, uint32_t root,
@@ -316,17 +454,16 @@ hipError_t launch_eval_values(const Clause* t, uint32_t n, uint32_t root,
                               hipStream_t s)
 {
     if (npts == 0) return hipSuccess;
-    const dim3 block(256);
-    const dim3 grid(unsigned((npts + 255) / 256));
-    switch (banks_for(nregs)) {
-        case 1: k_eval_values<1><<<grid, block, 0, s>>>(t, n, root, x, y, z, out, npts); break;
-        case 2: k_eval_values<2><<<grid, block, 0, s>>>(t, n, root, x, y, z, out, npts); break;
-        case 3:
-        case 4: k_eval_values<4><<<grid, block, 0, s>>>(t, n, root, x, y, z, out, npts); break;
-        case 5: case 6: case 7:
-        case 8: k_eval_values<8><<<grid, block, 0, s>>>(t, n, root, x, y, z, out, npts); break;
-        default: return hipErrorInvalidValue;
-    }
+    const int nv = banks_for(nregs);
+    const int p = points_for(nv);
+    if (nv == 1 && p == 1) launch_v<1, 1>(t, n, root, x, y, z, out, npts, s);
+    else if (nv == 1 && p == 2) launch_v<1, 2>(t, n, root, x, y, z, out, npts, s);
+    else if (nv == 1) launch_v<1, 4>(t, n, root, x, y, z, out, npts, s);
+    else if (nv == 2 && p == 1) launch_v<2, 1>(t, n, root, x, y, z, out, npts, s);
+    else if (nv == 2) launch_v<2, 2>(t, n, root, x, y, z, out, npts, s);
+    else if (nv == 4) launch_v<4, 1>(t, n, root, x, y, z, out, npts, s);
+    else if (nv == 8) launch_v<8, 1>(t, n, root, x, y, z, out, npts, s);
+    else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
@@ -341,7 +478,6 @@ hipError_t launch_eval_derivs(const Clause* t, uint32_t n, uint32_t root,
     switch (banks_for(nregs)) {
         case 1: k_eval_derivs<1><<<grid, block, 0, s>>>(t, n, root, x, y, z, v, dx, dy, dz, npts); break;
         case 2: k_eval_derivs<2><<<grid, block, 0, s>>>(t, n, root, x, y, z, v, dx, dy, dz, npts); break;
-        case 3:
         case 4: k_eval_derivs<4><<<grid, block, 0, s>>>(t, n, root, x, y, z, v, dx, dy, dz, npts); break;
         default: return hipErrorInvalidValue;
     }

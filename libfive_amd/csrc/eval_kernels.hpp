@@ -21,6 +21,8 @@ hipError_t launch_eval_derivs(const tape::Clause* t, uint32_t n, uint32_t root,
                               const float* z, float* v, float* dx, float* dy,
                               float* dz, size_t npts, hipStream_t s);
 
+// Points per lane for the value kernel (0: automatic).
+void set_points_per_lane(int p);
 // Largest register count each interpreter keeps in VGPRs.
 int max_value_regs();
 int max_deriv_regs();

@@ -30,23 +30,40 @@ typedef float f32x32 __attribute__((ext_vector_type(32)));
     (void)P##0; (void)P##1; (void)P##2; (void)P##3;                          \
     (void)P##4; (void)P##5; (void)P##6; (void)P##7
 
-#define LF_RF_SEL2(P, a, b, hi, lo) (((hi) & 1u) ? P##b[lo] : P##a[lo])
-#define LF_RF_SEL4(P, a, b, c, d, hi, lo)                                    \
-    (((hi) & 2u) ? LF_RF_SEL2(P, c, d, hi, lo) : LF_RF_SEL2(P, a, b, hi, lo))
+// Bank selection is written as uniform branches; the empty volatile asm
+// statements keep the compiler from if-converting them into "read every
+// bank and select", which costs a VALU move per bank.
+#define LF_RF_BARRIER() __asm__ volatile("")
+
+#define LF_RF_SEL2(P, a, b, hi, lo, r)                                       \
+    do {                                                                     \
+        if ((hi) & 1u) { LF_RF_BARRIER(); r = P##b[lo]; }                    \
+        else { LF_RF_BARRIER(); r = P##a[lo]; }                              \
+    } while (0)
+#define LF_RF_SEL4(P, a, b, c, d, hi, lo, r)                                 \
+    do {                                                                     \
+        if ((hi) & 2u) LF_RF_SEL2(P, c, d, hi, lo, r);                       \
+        else LF_RF_SEL2(P, a, b, hi, lo, r);                                 \
+    } while (0)
 
 // Value of register i (uniform) in a file of NV banks (NV in {1,2,4,8}).
 #define LF_RF_GET(P, NV, i)                                                  \
     ({                                                                       \
         const uint32_t lo_ = uint32_t(i) & 31u, hi_ = uint32_t(i) >> 5;      \
-        ((NV) == 1) ? P##0[lo_]                                              \
-        : ((NV) == 2) ? LF_RF_SEL2(P, 0, 1, hi_, lo_)                        \
-        : ((NV) == 4) ? LF_RF_SEL4(P, 0, 1, 2, 3, hi_, lo_)                  \
-        : ((hi_ & 4u) ? LF_RF_SEL4(P, 4, 5, 6, 7, hi_, lo_)                  \
-                      : LF_RF_SEL4(P, 0, 1, 2, 3, hi_, lo_));                \
+        float r_;                                                            \
+        if ((NV) == 1) r_ = P##0[lo_];                                       \
+        else if ((NV) == 2) LF_RF_SEL2(P, 0, 1, hi_, lo_, r_);               \
+        else if ((NV) == 4) LF_RF_SEL4(P, 0, 1, 2, 3, hi_, lo_, r_);         \
+        else if (hi_ & 4u) LF_RF_SEL4(P, 4, 5, 6, 7, hi_, lo_, r_);          \
+        else LF_RF_SEL4(P, 0, 1, 2, 3, hi_, lo_, r_);                        \
+        r_;                                                                  \
     })
 
 #define LF_RF_PUT2(P, a, b, hi, lo, x)                                       \
-    do { if ((hi) & 1u) P##b[lo] = (x); else P##a[lo] = (x); } while (0)
+    do {                                                                     \
+        if ((hi) & 1u) { LF_RF_BARRIER(); P##b[lo] = (x); }                  \
+        else { LF_RF_BARRIER(); P##a[lo] = (x); }                            \
+    } while (0)
 #define LF_RF_PUT4(P, a, b, c, d, hi, lo, x)                                 \
     do {                                                                     \
         if ((hi) & 2u) LF_RF_PUT2(P, c, d, hi, lo, x);                       \

@@ -1,6 +1,9 @@
 #include "runtime.hpp"
 
 #include <cstdio>
+#include <cstdlib>
+
+#include "eval_kernels.hpp"
 
 namespace libfive {
 namespace amd {
@@ -11,7 +14,14 @@ void hip_check(hipError_t e, const char* what) {
     }
 }
 
+static void apply_env_settings() {
+    if (const char* p = std::getenv("LIBFIVE_POINTS_PER_LANE")) {
+        set_points_per_lane(std::atoi(p));
+    }
+}
+
 static Context* make_context() {
+    apply_env_settings();
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
         (void)hipGetLastError();
@@ -55,14 +65,16 @@ bool gpu_available() { return context_or_null() != nullptr; }
 
 GpuTape::GpuTape(DeviceTape t) : host(std::move(t)) {
     Context& c = context();
-    if (!host.clauses.empty()) {
-        d.alloc(host.clauses.size());
-        hip_check(hipMemcpyAsync(d.p, host.clauses.data(),
-                                 host.clauses.size() * sizeof(tape::Clause),
-                                 hipMemcpyHostToDevice, c.stream),
-                  "upload tape");
-        hip_check(hipStreamSynchronize(c.stream), "upload tape");
-    }
+    // Padded with NOP clauses (all-zero words) to whole fetch batches plus
+    // one batch for the interpreters' prefetch.
+    std::vector<tape::Clause> buf(tape::padded_length(size()),
+                                  tape::Clause{0, 0, 0, 0});
+    std::copy(host.clauses.begin(), host.clauses.end(), buf.begin());
+    d.alloc(buf.size());
+    hip_check(hipMemcpyAsync(d.p, buf.data(), buf.size() * sizeof(tape::Clause),
+                             hipMemcpyHostToDevice, c.stream),
+              "upload tape");
+    hip_check(hipStreamSynchronize(c.stream), "upload tape");
 }
 
 std::unique_ptr<GpuTape> compile_tree(const Tree& t) {

@@ -2,8 +2,8 @@
 //
 // A tape is a straight-line register program in evaluation order
 // (leaves first).  Each clause is 16 bytes:
-//   w0 = opcode (bits 0-7) | operand mode (bits 8-9) | destination register
-//        (bits 16-31)
+//   w0 = opcode (bits 0-7) | operand mode (bits 8-9) | binary flag (bit 10)
+//        | destination register (bits 16-31)
 //   w1 = register of operand a (bits 0-15) | register of operand b (16-31)
 //   w2 = immediate operand (float bits) when the mode says so
 //   w3 = the reference clause id (Deck slot) whose value this clause writes
@@ -37,32 +37,25 @@ enum : uint32_t {
 };
 
 // Opcodes that exist only on device tapes.  CONSTANT (1) is reused as
-// "load immediate into dst".
+// "load immediate into dst"; INVALID (0) is a no-op used to pad tapes to
+// whole fetch batches.
 enum : uint32_t {
+    OP_NOP = 0,
     OP_LOADI = 1,
     OP_COPY = 40,
 };
 
+// Clauses are fetched in batches of this many (one 64-byte scalar load);
+// device buffers hold the tape padded with NOPs to a multiple of the batch
+// plus one extra batch, so the prefetch of the next batch never reads past
+// the allocation.
+constexpr uint32_t kFetch = 4;
+inline uint32_t padded_length(uint32_t n) {
+    return (n + kFetch - 1) / kFetch * kFetch + kFetch;
+}
+
 constexpr uint32_t kRegX = 0, kRegY = 1, kRegZ = 2, kFirstFreeReg = 3;
 constexpr uint32_t kMaxRegs = 0xffff;
-
-LF_HD uint32_t op(const Clause& c) { return c.w0 & 0xffu; }
-LF_HD uint32_t mode(const Clause& c) { return (c.w0 >> 8) & 3u; }
-LF_HD uint32_t dst(const Clause& c) { return c.w0 >> 16; }
-LF_HD uint32_t ra(const Clause& c) { return c.w1 & 0xffffu; }
-LF_HD uint32_t rb(const Clause& c) { return c.w1 >> 16; }
-LF_HD float imm(const Clause& c) { return fmath::u2f(c.w2); }
-LF_HD uint32_t slot(const Clause& c) { return c.w3; }
-
-LF_HD Clause make(uint32_t op_, uint32_t mode_, uint32_t dst_, uint32_t a,
-                  uint32_t b, float im, uint32_t slot_) {
-    Clause c;
-    c.w0 = (op_ & 0xffu) | ((mode_ & 3u) << 8) | (dst_ << 16);
-    c.w1 = (a & 0xffffu) | (b << 16);
-    c.w2 = fmath::f2u(im);
-    c.w3 = slot_;
-    return c;
-}
 
 // Operand count of a device-tape opcode (Opcode::args for tree opcodes,
 // libfive/src/tree/opcode.cpp:18-65; 0 for LOADI, 1 for COPY).
@@ -79,6 +72,26 @@ LF_HD int nargs(uint32_t o) {
         default:
             return 0;
     }
+}
+
+LF_HD uint32_t op(const Clause& c) { return c.w0 & 0xffu; }
+LF_HD uint32_t mode(const Clause& c) { return (c.w0 >> 8) & 3u; }
+LF_HD bool is_binary(const Clause& c) { return (c.w0 >> 10) & 1u; }
+LF_HD uint32_t dst(const Clause& c) { return c.w0 >> 16; }
+LF_HD uint32_t ra(const Clause& c) { return c.w1 & 0xffffu; }
+LF_HD uint32_t rb(const Clause& c) { return c.w1 >> 16; }
+LF_HD float imm(const Clause& c) { return fmath::u2f(c.w2); }
+LF_HD uint32_t slot(const Clause& c) { return c.w3; }
+
+LF_HD Clause make(uint32_t op_, uint32_t mode_, uint32_t dst_, uint32_t a,
+                  uint32_t b, float im, uint32_t slot_) {
+    Clause c;
+    c.w0 = (op_ & 0xffu) | ((mode_ & 3u) << 8) |
+           (nargs(op_) == 2 ? (1u << 10) : 0u) | (dst_ << 16);
+    c.w1 = (a & 0xffffu) | (b << 16);
+    c.w2 = fmath::f2u(im);
+    c.w3 = slot_;
+    return c;
 }
 
 // Keep decision for a MIN/MAX clause (Tape::Keep, tape.hpp:38-41).

@@ -106,6 +106,7 @@ SIGNATURES = {
                                          ctypes.c_size_t, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p]),
+    "libfive_set_points_per_lane": (None, [ctypes.c_int]),
     "libfive_tree_tape_info": (ctypes.c_int, [tree_t, u32p, u32p]),
     "libfive_tree_deck": (ctypes.c_int, [tree_t, u32p, szp, u32p, fp, szp,
                                          u32p, u32p]),

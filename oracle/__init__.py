@@ -37,6 +37,24 @@ _SIGS = {
                                                _fp, _fp, _u8,
                                                ctypes.POINTER(ctypes.c_int)]),
     "oracle_sincos": (None, [_fp, ctypes.c_size_t, _fp, _fp]),
+    "oracle_render_mesh": (ctypes.c_void_p, [ctypes.c_void_p, _dp, _dp,
+                                             ctypes.c_double, ctypes.c_double]),
+    "oracle_mesh_sizes": (None, [ctypes.c_void_p] +
+                          [ctypes.POINTER(ctypes.c_size_t)] * 4),
+    "oracle_mesh_copy": (None, [ctypes.c_void_p, _fp, _u32, _u32,
+                                ctypes.POINTER(ctypes.c_uint64), _u32]),
+    "oracle_mesh_leaf_verts": (ctypes.c_size_t, [ctypes.c_void_p, _dp]),
+    "oracle_mesh_delete": (None, [ctypes.c_void_p]),
+    "oracle_corner_manifold": (ctypes.c_int, [ctypes.c_int]),
+    "oracle_marching_table": (None, [ctypes.POINTER(ctypes.c_int8)] * 3),
+    "oracle_is_inside": (ctypes.c_int, [ctypes.c_void_p, _fp]),
+    "oracle_features": (ctypes.c_size_t, [ctypes.c_void_p, _fp, _fp,
+                                          ctypes.c_size_t]),
+    "oracle_eigen3": (None, [_dp, _dp, _dp]),
+    "oracle_find_vertex": (ctypes.c_double, [_dp, _dp, ctypes.c_double, _dp,
+                                             _dp, ctypes.POINTER(ctypes.c_int)]),
+    "oracle_push_sample": (None, [_dp, _dp, _dp, ctypes.c_double]),
+    "oracle_resolution_level": (ctypes.c_int, [_dp, _dp, ctypes.c_double]),
 }
 
 _lib = None
@@ -134,6 +152,114 @@ class OracleDeck:
                                        _f(oh), nan.ctypes.data_as(_u8),
                                        ctypes.byref(ch))
         return n, bool(ch.value), (float(ol[0]), float(oh[0]), bool(nan[0]))
+
+
+    def is_inside(self, p):
+        p = _f32(p)
+        return bool(lib().oracle_is_inside(self.ptr, _f(p)))
+
+    def features(self, p):
+        p = _f32(p)
+        out = np.zeros((64, 3), np.float32)
+        n = lib().oracle_features(self.ptr, _f(p), _f(out), 64)
+        return out[:min(n, 64)]
+
+    def render_mesh(self, lo, hi, min_feature, max_err=1e-8):
+        """Dual-contouring mesh: a MeshResult (see below)."""
+        lo = np.ascontiguousarray(lo, np.float64)
+        hi = np.ascontiguousarray(hi, np.float64)
+        L = lib()
+        m = L.oracle_render_mesh(self.ptr, lo.ctypes.data_as(_dp),
+                                 hi.ctypes.data_as(_dp), min_feature, max_err)
+        try:
+            sz = [ctypes.c_size_t() for _ in range(4)]
+            L.oracle_mesh_sizes(m, *[ctypes.byref(x) for x in sz])
+            nv, nt, ns, nl = (x.value for x in sz)
+            verts = np.empty((nv, 3), np.float32)
+            tris = np.empty((nt, 3), np.uint32)
+            keys = np.empty((nv, 5), np.uint32)
+            stats = np.empty(ns, np.uint64)
+            lens = np.empty(nl, np.uint32)
+            L.oracle_mesh_copy(m, _f(verts), tris.ctypes.data_as(_u32),
+                               keys.ctypes.data_as(_u32),
+                               stats.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                               lens.ctypes.data_as(_u32))
+            nl = L.oracle_mesh_leaf_verts(m, None)
+            leaf = np.empty((nl, 3), np.float64)
+            L.oracle_mesh_leaf_verts(m, leaf.ctypes.data_as(_dp))
+        finally:
+            L.oracle_mesh_delete(m)
+        out = MeshResult(verts, tris, keys, stats, lens)
+        out.leaf_verts = leaf
+        return out
+
+
+class MeshResult:
+    """verts[n,3] (row 0 is the reserved dummy vertex), tris[m,3], keys[n,5]
+    (owning cell's lower corner x, y, z in finest-cell units, its level, its
+    patch), level statistics and the pushed tapes' clause counts."""
+
+    def __init__(self, verts, tris, keys, stats, tape_lengths):
+        self.verts, self.tris, self.keys = verts, tris, keys
+        self.root_level = int(stats[0])
+        self.ambiguous_leaves = int(stats[1])
+        self.collapsed = int(stats[2])
+        self.level_counts = stats[3:].reshape(-1, 3)   # root level first
+        self.tape_lengths = tape_lengths
+
+
+def corner_manifold(mask):
+    return bool(lib().oracle_corner_manifold(int(mask)))
+
+
+def marching_table():
+    v = np.empty((256, 4, 12, 2), np.int8)
+    e = np.empty((8, 8), np.int8)
+    p = np.empty((256, 24), np.int8)
+    i8 = ctypes.POINTER(ctypes.c_int8)
+    lib().oracle_marching_table(v.ctypes.data_as(i8), e.ctypes.data_as(i8),
+                                p.ctypes.data_as(i8))
+    return v, e, p
+
+
+def eigen3(a):
+    """Eigenvalues and eigenvectors (columns) of a symmetric 3x3 matrix by
+    the product's Jacobi routine."""
+    a = np.ascontiguousarray(a, np.float64)
+    w = np.empty(3)
+    v = np.empty((3, 3))
+    lib().oracle_eigen3(a.ctypes.data_as(_dp), w.ctypes.data_as(_dp),
+                        v.ctypes.data_as(_dp))
+    return w, v
+
+
+def find_vertex(ata, atb, btb, mass):
+    """(vertex, rank, error) of the dual-contouring vertex solve."""
+    ata = np.ascontiguousarray(ata, np.float64)
+    atb = np.ascontiguousarray(atb, np.float64)
+    mass = np.ascontiguousarray(mass, np.float64)
+    v = np.empty(3)
+    r = ctypes.c_int()
+    err = lib().oracle_find_vertex(ata.ctypes.data_as(_dp), atb.ctypes.data_as(_dp),
+                                   float(btb), mass.ctypes.data_as(_dp),
+                                   v.ctypes.data_as(_dp), ctypes.byref(r))
+    return v, r.value, err
+
+
+def push_sample(acc, pos, deriv, value):
+    """Adds one sample to a 14-double accumulator (mass x, y, z, count;
+    AtA xx, xy, xz, yy, yz, zz; AtB; BtB) in place."""
+    pos = np.ascontiguousarray(pos, np.float64)
+    deriv = np.ascontiguousarray(deriv, np.float64)
+    lib().oracle_push_sample(acc.ctypes.data_as(_dp), pos.ctypes.data_as(_dp),
+                             deriv.ctypes.data_as(_dp), float(value))
+
+
+def resolution_level(lo, hi, min_feature):
+    lo = np.ascontiguousarray(lo, np.float64)
+    hi = np.ascontiguousarray(hi, np.float64)
+    return lib().oracle_resolution_level(lo.ctypes.data_as(_dp),
+                                         hi.ctypes.data_as(_dp), min_feature)
 
 
 def sincos(x):

@@ -760,9 +760,6 @@ double eval_point_f64(const Deck& deck, const Tape& tape, double x, double y,
 
 using namespace oracle;
 
-struct oracle_deck {
-    Deck deck;
-};
 
 template <typename F>
 static void parallel_batches(size_t n, int threads, F&& f) {

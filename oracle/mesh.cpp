@@ -1,0 +1,856 @@
+// Oracle (test infrastructure only): dual-contouring mesher restatement,
+// single-threaded and recursive.
+//
+// Follows:
+//   libfive/include/libfive/render/brep/region.hpp:89-114, 165-174, 226-231
+//   libfive/src/render/brep/worker_pool.inl:36-257     build / run
+//   libfive/src/render/brep/dc/dc_tree.inl:94-813      evalInterval, evalLeaf,
+//                                                      collectChildren, findVertex
+//   libfive/src/render/brep/dc/dc_tree3.cpp:17-154     manifold tests
+//   libfive/include/libfive/render/brep/dc/intersection.hpp:32-89
+//   libfive/src/render/brep/dc/marching.cpp:115-557    marching table
+//   libfive/include/libfive/render/brep/dual.hpp:127-203   dual walk
+//   libfive/src/render/brep/dc/dc_mesher.cpp:19-169    quad emission
+//
+// Differences from the reference, all value-preserving when the shape has
+// no NaN: corner states and edge intersections are computed by each cell
+// instead of being shared with already-finished neighbours (the pruned
+// tapes give identical values inside their boxes), and the traversal order
+// is a fixed depth-first order instead of the thread pool's, so vertex
+// indices differ while the set of triangles is the same.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <set>
+
+#include "../libfive_amd/csrc/qef.hpp"
+#include "feature.hpp"
+#include "oracle.h"
+#include "oracle_impl.hpp"
+
+namespace oracle {
+
+enum State { EMPTY = 0, FILLED = 1, AMBIGUOUS = 2, UNKNOWN = 3 };
+
+////////////////////////////////////////////////////////////////////////////////
+// Marching table (marching.cpp:115-557): the 21 seed cases of Nielson's dual
+// marching cubes, closed under quarter turns about X, Y, Z.
+
+struct Marching {
+    int8_t v[256][4][12][2];
+    int8_t e[8][8];
+    int8_t p[256][24];
+};
+
+static int rot_vertex(int vert, int axis) {
+    // +90 degree rotation about `axis` on the +-1 cube corners
+    int c[3];
+    for (int i = 0; i < 3; ++i) c[i] = (vert & (1 << i)) ? 1 : -1;
+    int r[3];
+    if (axis == 0) { r[0] = c[0]; r[1] = -c[2]; r[2] = c[1]; }
+    else if (axis == 1) { r[0] = c[2]; r[1] = c[1]; r[2] = -c[0]; }
+    else { r[0] = -c[1]; r[1] = c[0]; r[2] = c[2]; }
+    int out = 0;
+    for (int i = 0; i < 3; ++i) out |= (r[i] > 0) << i;
+    return out;
+}
+
+static unsigned rot_mask(unsigned mask, int axis) {
+    unsigned out = 0;
+    for (int i = 0; i < 8; ++i) {
+        if (mask & (1u << i)) out |= 1u << rot_vertex(i, axis);
+    }
+    return out;
+}
+
+static Marching build_marching() {
+    Marching m;
+    std::memset(m.v, -1, sizeof(m.v));
+    std::memset(m.e, -1, sizeof(m.e));
+    std::memset(m.p, -1, sizeof(m.p));
+    struct Case {
+        std::vector<int> verts;
+        std::vector<std::vector<std::pair<int, int>>> patches;
+    };
+    const std::vector<Case> cases = {
+        {{0}, {{{0, 1}, {0, 2}, {0, 4}}}},
+        {{0, 1}, {{{1, 3}, {0, 2}, {0, 4}, {1, 5}}}},
+        {{0, 5}, {{{0, 1}, {0, 2}, {0, 4}}, {{5, 7}, {5, 4}, {5, 1}}}},
+        {{0, 7}, {{{0, 1}, {0, 2}, {0, 4}}, {{7, 6}, {7, 3}, {7, 5}}}},
+        {{1, 2, 3}, {{{1, 0}, {1, 5}, {3, 7}, {2, 6}, {2, 0}}}},
+        {{0, 1, 7}, {{{1, 3}, {0, 2}, {0, 4}, {1, 5}}, {{7, 6}, {7, 3}, {7, 5}}}},
+        {{1, 4, 7}, {{{1, 3}, {1, 0}, {1, 5}}, {{4, 6}, {4, 5}, {4, 0}},
+                     {{7, 6}, {7, 3}, {7, 5}}}},
+        {{0, 1, 2, 3}, {{{0, 4}, {1, 5}, {3, 7}, {2, 6}}}},
+        {{0, 2, 3, 6}, {{{0, 1}, {3, 1}, {3, 7}, {6, 7}, {6, 4}, {0, 4}}}},
+        {{0, 4, 3, 7}, {{{0, 1}, {0, 2}, {4, 6}, {4, 5}},
+                        {{3, 2}, {3, 1}, {7, 5}, {7, 6}}}},
+        {{0, 2, 3, 7}, {{{0, 1}, {3, 1}, {7, 5}, {7, 6}, {2, 6}, {0, 4}}}},
+        {{1, 2, 3, 4}, {{{4, 6}, {4, 5}, {4, 0}},
+                        {{1, 0}, {1, 5}, {3, 7}, {2, 6}, {2, 0}}}},
+        {{0, 3, 6, 5}, {{{0, 2}, {0, 4}, {0, 1}}, {{3, 7}, {3, 2}, {3, 1}},
+                        {{6, 4}, {6, 2}, {6, 7}}, {{5, 7}, {5, 1}, {5, 4}}}},
+        {{1, 3, 2, 6}, {{{1, 5}, {3, 7}, {6, 7}, {6, 4}, {2, 0}, {1, 0}}}},
+        {{0, 2, 3, 5, 6}, {{{0, 1}, {3, 1}, {3, 7}, {6, 7}, {6, 4}, {0, 4}},
+                           {{5, 7}, {5, 1}, {5, 4}}}},
+        {{2, 3, 4, 5, 6}, {{{3, 1}, {3, 7}, {6, 7}, {5, 7}, {5, 1}, {4, 0}, {2, 0}}}},
+        {{0, 4, 5, 6, 7}, {{{0, 2}, {6, 2}, {7, 3}, {5, 1}, {0, 1}}}},
+        {{1, 2, 3, 4, 5, 6}, {{{1, 0}, {4, 0}, {2, 0}}, {{6, 7}, {5, 7}, {3, 7}}}},
+        {{1, 2, 3, 4, 6, 7}, {{{1, 0}, {1, 5}, {7, 5}, {4, 5}, {4, 0}, {2, 0}}}},
+        {{2, 3, 4, 5, 6, 7}, {{{4, 0}, {5, 1}, {3, 1}, {2, 0}}}},
+        {{1, 2, 3, 4, 5, 6, 7}, {{{1, 0}, {4, 0}, {2, 0}}}},
+    };
+    for (const auto& c : cases) {
+        unsigned bitmap = 0;
+        for (int v : c.verts) bitmap |= 1u << v;
+        for (size_t p = 0; p < c.patches.size(); ++p) {
+            for (size_t e = 0; e < c.patches[p].size(); ++e) {
+                m.v[bitmap][p][e][0] = int8_t(c.patches[p][e].first);
+                m.v[bitmap][p][e][1] = int8_t(c.patches[p][e].second);
+            }
+        }
+    }
+    bool changed[256];
+    for (int i = 0; i < 256; ++i) changed[i] = m.v[i][0][0][0] != -1;
+    bool any = true;
+    while (any) {
+        any = false;
+        for (int i = 0; i < 256; ++i) {
+            if (!changed[i]) continue;
+            changed[i] = false;
+            for (int axis = 0; axis < 3; ++axis) {
+                const unsigned j = rot_mask(i, axis);
+                if (m.v[j][0][0][0] != -1) continue;
+                changed[j] = true;
+                any = true;
+                for (int p = 0; p < 4 && m.v[i][p][0][0] != -1; ++p) {
+                    for (int e = 0; e < 12 && m.v[i][p][e][0] != -1; ++e) {
+                        m.v[j][p][e][0] = int8_t(rot_vertex(m.v[i][p][e][0], axis));
+                        m.v[j][p][e][1] = int8_t(rot_vertex(m.v[i][p][e][1], axis));
+                    }
+                }
+            }
+        }
+    }
+    int next = 0;
+    for (int i = 0; i < 256; ++i) {
+        for (int p = 0; p < 4 && m.v[i][p][0][0] != -1; ++p) {
+            for (int e = 0; e < 12 && m.v[i][p][e][0] != -1; ++e) {
+                const int a = m.v[i][p][e][0], b = m.v[i][p][e][1];
+                if (m.e[a][b] == -1) m.e[a][b] = int8_t(next++);
+                m.p[i][m.e[a][b]] = int8_t(p);
+            }
+        }
+    }
+    return m;
+}
+
+static const Marching& marching() {
+    static const Marching m = build_marching();
+    return m;
+}
+
+// cornersAreManifold (dc_tree3.cpp:112-154): merge equal-signed corners
+// along cube edges until no edge joins equal signs; manifold iff at most
+// one distinct (unordered) edge class remains.
+static bool corner_safe(unsigned index) {
+    bool f[8];
+    for (int i = 0; i < 8; ++i) f[i] = (index & (1u << i)) != 0;
+    std::vector<std::pair<int, int>> edges = {
+        {0, 1}, {0, 2}, {2, 3}, {1, 3}, {4, 5}, {4, 6}, {6, 7}, {5, 7},
+        {0, 4}, {2, 6}, {1, 5}, {3, 7}};
+    while (true) {
+        bool merged = false;
+        for (const auto& e : edges) {
+            if (f[e.first] == f[e.second]) {
+                const int a = e.first, b = e.second;
+                std::vector<std::pair<int, int>> next;
+                for (auto g : edges) {
+                    if (g.first == a) g.first = b;
+                    if (g.second == a) g.second = b;
+                    if (g.first != g.second) next.push_back(g);
+                }
+                edges = next;
+                merged = true;
+                break;
+            }
+        }
+        if (!merged) break;
+    }
+    std::set<std::pair<int, int>> s;
+    for (auto e : edges) s.insert({std::min(e.first, e.second), std::max(e.first, e.second)});
+    return s.size() <= 1;
+}
+
+static bool corner_manifold(uint8_t mask) {
+    static const auto table = [] {
+        std::array<bool, 256> t;
+        for (int i = 0; i < 256; ++i) t[i] = corner_safe(i);
+        return t;
+    }();
+    return table[mask];
+}
+
+////////////////////////////////////////////////////////////////////////////////
+// Octree
+
+struct Leaf {
+    int level = 0;
+    double verts[4][3] = {};
+    int rank = 0;
+    uint8_t corner_mask = 0;
+    int vertex_count = 0;
+    bool manifold = false;
+    double mass[4] = {0, 0, 0, 0};
+    double ata[3][3] = {};
+    double atb[3] = {0, 0, 0};
+    double btb = 0;
+    uint32_t index[4] = {0, 0, 0, 0};
+};
+
+struct Cell {
+    double lo[3], hi[3];
+    int level;
+    uint32_t key[3];          // lower corner, in finest-cell units
+    State type = UNKNOWN;
+    std::unique_ptr<Cell> kids[8];
+    std::unique_ptr<Leaf> leaf;
+    bool is_branch() const { return bool(kids[0]); }
+    const Cell* child(int i) const { return is_branch() ? kids[i].get() : this; }
+    State corner_state(int i) const {
+        if (type == AMBIGUOUS) return (leaf->corner_mask & (1 << i)) ? FILLED : EMPTY;
+        return type;
+    }
+    bool manifold() const { return type == AMBIGUOUS ? leaf->manifold : true; }
+    int rank() const { return type == AMBIGUOUS ? leaf->rank : 0; }
+};
+
+struct Mesher {
+    Deck& deck;
+    IntervalEval ie;
+    DerivEval de;
+    FeatureEval fe;
+    double max_err;
+    std::vector<std::array<float, 3>> verts{{0, 0, 0}};
+    std::vector<std::array<uint32_t, 3>> tris;
+    std::vector<std::array<uint32_t, 5>> vkeys{{0, 0, 0, 0, 0}};
+    // statistics
+    std::map<int, std::array<uint64_t, 4>> level_counts;   // level -> state counts
+    std::vector<uint32_t> tape_lengths;                    // per ambiguous cell
+    uint64_t leaves_ambiguous = 0;
+    uint64_t collapsed = 0;
+
+    Mesher(Deck& d, double me) : deck(d), ie(d), de(d), fe(d), max_err(me) {}
+
+    void subdivide(Cell* c) {
+        for (int i = 0; i < 8; ++i) {
+            auto k = std::make_unique<Cell>();
+            for (int j = 0; j < 3; ++j) {
+                const double mid = (c->lo[j] + c->hi[j]) / 2;
+                const bool up = (i & (1 << j)) != 0;
+                k->lo[j] = up ? mid : c->lo[j];
+                k->hi[j] = up ? c->hi[j] : mid;
+                k->key[j] = c->key[j] + (up ? (1u << (c->level - 1)) : 0u);
+            }
+            k->level = c->level - 1;
+            c->kids[i] = std::move(k);
+        }
+    }
+
+    // returns the tape for the cell's children (null if the cell is done)
+    std::shared_ptr<Tape> eval_interval(Cell* c, const std::shared_ptr<Tape>& tape) {
+        const float lo[3] = {float(c->lo[0]), float(c->lo[1]), float(c->lo[2])};
+        const float hi[3] = {float(c->hi[0]), float(c->hi[1]), float(c->hi[2])};
+        const Iv r = ie.eval(lo, hi, *tape);
+        auto pushed = ie.push(tape);
+        // Interval::state (interval.hpp:66-77)
+        if (r.nan) c->type = AMBIGUOUS;
+        else if (r.lo > 0) c->type = EMPTY;
+        else if (r.hi < 0) c->type = FILLED;
+        else c->type = AMBIGUOUS;
+        level_counts[c->level][c->type]++;
+        if (r.nan) return tape;
+        if (c->type != AMBIGUOUS) return nullptr;
+        tape_lengths.push_back(uint32_t(pushed->t.size()));
+        return pushed;
+    }
+
+    void build(Cell* c, const std::shared_ptr<Tape>& tape) {
+        if (c->level > 0) {
+            auto next = eval_interval(c, tape);
+            if (c->type != AMBIGUOUS) return;
+            subdivide(c);
+            for (int i = 0; i < 8; ++i) build(c->kids[i].get(), next);
+            const double lo[3] = {c->lo[0], c->lo[1], c->lo[2]};
+            const double hi[3] = {c->hi[0], c->hi[1], c->hi[2]};
+            collect(c, get_base(next, lo, hi));
+        } else {
+            eval_leaf(c, tape);
+            level_counts[0][c->type]++;
+        }
+    }
+
+    bool is_inside_d(const double p[3], const std::shared_ptr<Tape>& tape) {
+        const float pf[3] = {float(p[0]), float(p[1]), float(p[2])};
+        return fe.is_inside(pf, tape);
+    }
+
+    // DCTree::evalLeaf (dc_tree.inl:131-552)
+    void eval_leaf(Cell* c, const std::shared_ptr<Tape>& tape) {
+        State corners[8];
+        float pos[8][3];
+        for (int i = 0; i < 8; ++i) {
+            for (int j = 0; j < 3; ++j) {
+                pos[i][j] = float((i & (1 << j)) ? c->hi[j] : c->lo[j]);
+            }
+            de.set(pos[i][0], pos[i][1], pos[i][2], i);
+        }
+        const float* vs = de.values(8, *tape);
+        float vals[8];
+        std::memcpy(vals, vs, sizeof(vals));
+        uint8_t ambig[8];
+        de.ambiguous(8, *tape, ambig);
+        int unambig[8], n_unambig = 0;
+        for (int i = 0; i < 8; ++i) {
+            if (vals[i] > 0 || std::isnan(vals[i])) {
+                corners[i] = EMPTY; ambig[i] = 0;
+            } else if (vals[i] < 0) {
+                corners[i] = FILLED; ambig[i] = 0;
+            } else if (!ambig[i]) {
+                unambig[n_unambig++] = i;
+            }
+        }
+        if (n_unambig) {
+            for (int k = 0; k < n_unambig; ++k) {
+                const int i = unambig[k];
+                de.set(pos[i][0], pos[i][1], pos[i][2], k);
+            }
+            de.derivs(n_unambig, *tape);
+            for (int k = 0; k < n_unambig; ++k) {
+                const bool nz = de.gslot(tape->i, 0, k) != 0 ||
+                                de.gslot(tape->i, 1, k) != 0 ||
+                                de.gslot(tape->i, 2, k) != 0;
+                corners[unambig[k]] = nz ? FILLED : EMPTY;
+            }
+        }
+        for (int i = 0; i < 8; ++i) {
+            if (ambig[i]) corners[i] = fe.is_inside(pos[i], tape) ? FILLED : EMPTY;
+        }
+        bool all_full = true, all_empty = true;
+        for (int i = 0; i < 8; ++i) {
+            all_full &= (corners[i] == FILLED);
+            all_empty &= (corners[i] == EMPTY);
+        }
+        c->type = all_empty ? EMPTY : all_full ? FILLED : AMBIGUOUS;
+        if (c->type != AMBIGUOUS) return;
+        leaves_ambiguous++;
+
+        auto leaf = std::make_unique<Leaf>();
+        uint8_t mask = 0;
+        for (int i = 0; i < 8; ++i) mask |= uint8_t((corners[i] == FILLED) << i);
+        leaf->corner_mask = mask;
+        leaf->manifold = corner_manifold(mask);
+
+        const Marching& M = marching();
+        // edge -> accumulator; an edge without any sample has no entry
+        std::map<int, std::array<double, 14>> inter;
+        auto corner_d = [&](int i, double* out) {
+            for (int j = 0; j < 3; ++j) out[j] = (i & (1 << j)) ? c->hi[j] : c->lo[j];
+        };
+        while (leaf->vertex_count < 4 &&
+               M.v[mask][leaf->vertex_count][0][0] != -1) {
+            const int pi = leaf->vertex_count;
+            int edges[12], edge_count = 0;
+            double tf[12][3], ts[12][3];
+            for (; edge_count < 12 && M.v[mask][pi][edge_count][0] != -1; ++edge_count) {
+                const int a = M.v[mask][pi][edge_count][0];
+                const int b = M.v[mask][pi][edge_count][1];
+                edges[edge_count] = M.e[a][b];
+                corner_d(a, tf[edge_count]);
+                corner_d(b, ts[edge_count]);
+            }
+            const int ec = edge_count;
+            // 4 rounds x 16 samples (dc_tree.inl:339-410)
+            for (int s = 0; s < 4; ++s) {
+                double ps[12 * 16][3];
+                for (int e = 0; e < ec; ++e) {
+                    for (int j = 0; j < 16; ++j) {
+                        const double frac = j / 15.0;
+                        const int i = j + e * 16;
+                        for (int q = 0; q < 3; ++q) {
+                            ps[i][q] = tf[e][q] * (1 - frac) + ts[e][q] * frac;
+                        }
+                        de.set(float(ps[i][0]), float(ps[i][1]), float(ps[i][2]), i);
+                    }
+                }
+                float out[12 * 16];
+                std::memcpy(out, de.values(16 * ec, *tape), sizeof(float) * 16 * ec);
+                for (int e = 0; e < ec; ++e) {
+                    for (int j = 1; j < 16; ++j) {
+                        const int i = j + e * 16;
+                        bool take = false;
+                        if (out[i] > 0) {
+                            take = true;
+                        } else if (out[i] == 0) {
+                            if (!is_inside_d(ps[i], tape)) take = true;
+                        } else if (j == 15) {
+                            take = true;
+                        }
+                        if (take) {
+                            for (int q = 0; q < 3; ++q) {
+                                tf[e][q] = ps[i - 1][q];
+                                ts[e][q] = ps[i][q];
+                            }
+                            break;
+                        }
+                    }
+                }
+            }
+            // derivatives at both bracket ends (dc_tree.inl:412-471)
+            for (int i = 0; i < ec; ++i) {
+                de.set(float(tf[i][0]), float(tf[i][1]), float(tf[i][2]), 2 * i);
+                de.set(float(ts[i][0]), float(ts[i][1]), float(ts[i][2]), 2 * i + 1);
+            }
+            de.derivs(2 * ec, *tape);
+            float ds[24][4];
+            for (int i = 0; i < 2 * ec; ++i) {
+                ds[i][0] = de.gslot(tape->i, 0, i);
+                ds[i][1] = de.gslot(tape->i, 1, i);
+                ds[i][2] = de.gslot(tape->i, 2, i);
+                ds[i][3] = de.slot(tape->i, i);
+            }
+            uint8_t amb[24];
+            de.ambiguous(2 * ec, *tape, amb);
+            for (int i = 0; i < 2 * ec; ++i) {
+                const double* p = (i & 1) ? ts[i / 2] : tf[i / 2];
+                auto acc = [&]() -> double* {
+                    auto it = inter.find(edges[i / 2]);
+                    if (it == inter.end()) {
+                        it = inter.emplace(edges[i / 2], std::array<double, 14>{}).first;
+                    }
+                    return it->second.data();
+                };
+                if (!amb[i]) {
+                    const double d[3] = {ds[i][0], ds[i][1], ds[i][2]};
+                    libfive::qef::push_sample(acc(), p, d, double(ds[i][3]));
+                } else {
+                    const float pf[3] = {float(p[0]), float(p[1]), float(p[2])};
+                    for (const auto& f : fe.features(pf, tape)) {
+                        const double d[3] = {f[0], f[1], f[2]};
+                        libfive::qef::push_sample(acc(), p, d, double(ds[i][3]));
+                    }
+                }
+            }
+            // mass point from the max-rank intersections (dc_tree.inl:481-502)
+            int max_rank = 0;
+            int ranks[12];
+            for (int i = 0; i < ec; ++i) {
+                ranks[i] = -1;
+                auto it = inter.find(edges[i]);
+                if (it == inter.end()) continue;
+                double ata[3][3];
+                libfive::qef::unpack_ata(it->second.data(), ata);
+                ranks[i] = libfive::qef::small_eigenvalue_count(ata);
+                max_rank = std::max(max_rank, ranks[i]);
+            }
+            double mass[4] = {0, 0, 0, 0};
+            for (int i = 0; i < ec; ++i) {
+                if (ranks[i] == max_rank) {
+                    const auto& a = inter.at(edges[i]);
+                    mass[0] = mass[0] + a[0] / a[3];
+                    mass[1] = mass[1] + a[1] / a[3];
+                    mass[2] = mass[2] + a[2] / a[3];
+                    mass[3] = mass[3] + 1.0;
+                }
+            }
+            double sum[14] = {};
+            for (int i = 0; i < ec; ++i) {
+                auto it = inter.find(edges[i]);
+                if (it == inter.end()) continue;
+                for (int k = 4; k < 14; ++k) sum[k] = sum[k] + it->second[k];
+            }
+            double ata[3][3];
+            libfive::qef::unpack_ata(sum, ata);
+            std::memcpy(leaf->mass, mass, sizeof(mass));
+            libfive::qef::unpack_ata(sum, leaf->ata);
+            leaf->atb[0] = sum[10]; leaf->atb[1] = sum[11]; leaf->atb[2] = sum[12];
+            leaf->btb = sum[13];
+            int rank = 0;
+            libfive::qef::find_vertex(ata, leaf->atb, leaf->btb, mass,
+                                      leaf->verts[pi], &rank);
+            leaf->rank = rank;
+            leaf->vertex_count++;
+        }
+        c->leaf = std::move(leaf);
+    }
+
+    // leafsAreManifold (dc_tree3.cpp:17-109)
+    static bool leafs_manifold(Cell* const cs[8], const State corners[8]) {
+        auto cst = [&](int c, int i) { return cs[c]->corner_state(i); };
+        enum { X = 1, Y = 2, Z = 4 };
+        auto eq2 = [&](State s, int a, int b) { return s == corners[a] || s == corners[b]; };
+        auto eq4 = [&](State s, int a, int b, int c, int d) {
+            return s == corners[a] || s == corners[b] || s == corners[c] || s == corners[d];
+        };
+        const bool edges_safe =
+            eq2(cst(0, Z), 0, Z) && eq2(cst(0, X), 0, X) && eq2(cst(0, Y), 0, Y) &&
+            eq2(cst(X, X | Y), X, X | Y) && eq2(cst(X, X | Z), X, X | Z) &&
+            eq2(cst(Y, Y | X), Y, Y | X) && eq2(cst(Y, Y | Z), Y, Y | Z) &&
+            eq2(cst(X | Y, X | Y | Z), X | Y, X | Y | Z) &&
+            eq2(cst(Z, Z | X), Z, Z | X) && eq2(cst(Z, Z | Y), Z, Z | Y) &&
+            eq2(cst(Z | X, Z | X | Y), Z | X, Z | X | Y) &&
+            eq2(cst(Z | Y, Z | Y | X), Z | Y, Z | Y | X);
+        const bool faces_safe =
+            eq4(cst(0, X | Z), 0, X, Z, X | Z) &&
+            eq4(cst(0, Y | Z), 0, Y, Z, Y | Z) &&
+            eq4(cst(0, Y | X), 0, Y, X, Y | X) &&
+            eq4(cst(X | Y | Z, X), X, X | Z, X | Y, X | Y | Z) &&
+            eq4(cst(X | Y | Z, Y), Y, Y | Z, Y | X, X | Y | Z) &&
+            eq4(cst(X | Y | Z, Z), Z, Z | Y, Z | X, X | Y | Z);
+        const State ctr = cst(0, X | Y | Z);
+        bool center_safe = false;
+        for (int i = 0; i < 8; ++i) center_safe |= (ctr == corners[i]);
+        return edges_safe && faces_safe && center_safe;
+    }
+
+    // DCTree::collectChildren (dc_tree.inl:581-735)
+    void collect(Cell* c, const std::shared_ptr<Tape>& tape) {
+        Cell* cs[8];
+        for (int i = 0; i < 8; ++i) cs[i] = c->kids[i].get();
+        for (int i = 0; i < 8; ++i) if (cs[i]->is_branch()) return;
+        bool all_empty = true, all_full = true;
+        State corners[8];
+        for (int i = 0; i < 8; ++i) {
+            corners[i] = cs[i]->corner_state(i);
+            all_empty &= (cs[i]->type == EMPTY);
+            all_full &= (cs[i]->type == FILLED);
+        }
+        if (all_empty || all_full) {
+            c->type = all_empty ? EMPTY : FILLED;
+            for (auto& k : c->kids) k.reset();
+            return;
+        }
+        uint8_t mask = 0;
+        for (int i = 0; i < 8; ++i) mask |= uint8_t((corners[i] == FILLED) << i);
+        bool manifold = corner_manifold(mask);
+        for (int i = 0; i < 8 && manifold; ++i) manifold &= cs[i]->manifold();
+        manifold = manifold && leafs_manifold(cs, corners);
+        if (!manifold) return;
+
+        auto leaf = std::make_unique<Leaf>();
+        leaf->manifold = true;
+        leaf->corner_mask = mask;
+        int rank = 0;
+        for (int i = 0; i < 8; ++i) rank = std::max(rank, cs[i]->rank());
+        leaf->rank = rank;
+        for (int i = 0; i < 8; ++i) {
+            if (cs[i]->type != AMBIGUOUS) continue;
+            const Leaf& l = *cs[i]->leaf;
+            if (l.rank == leaf->rank) {
+                for (int k = 0; k < 4; ++k) leaf->mass[k] = leaf->mass[k] + l.mass[k];
+            }
+            for (int a = 0; a < 3; ++a) {
+                for (int b = 0; b < 3; ++b) leaf->ata[a][b] = leaf->ata[a][b] + l.ata[a][b];
+                leaf->atb[a] = leaf->atb[a] + l.atb[a];
+            }
+            leaf->btb = leaf->btb + l.btb;
+        }
+        const double err = libfive::qef::find_vertex(leaf->ata, leaf->atb, leaf->btb,
+                                                     leaf->mass, leaf->verts[0], nullptr);
+        leaf->vertex_count = 1;
+        bool ok = false;
+        const double* v = leaf->verts[0];
+        const double eps = 1e-6;
+        const bool inside = v[0] >= c->lo[0] - eps && v[1] >= c->lo[1] - eps &&
+                            v[2] >= c->lo[2] - eps && v[0] <= c->hi[0] + eps &&
+                            v[1] <= c->hi[1] + eps && v[2] <= c->hi[2] + eps;
+        if (err < max_err && inside) {
+            const float vf[3] = {float(v[0]), float(v[1]), float(v[2])};
+            const double vd[3] = {vf[0], vf[1], vf[2]};
+            auto t = get_base(tape, vd, vd);
+            de.set(vf[0], vf[1], vf[2], 0);
+            const float fv = de.values(1, *t)[0];
+            if (std::fabs(fv) < max_err) ok = true;
+        }
+        if (ok) {
+            leaf->level = c->level;
+            c->type = AMBIGUOUS;
+            c->leaf = std::move(leaf);
+            for (auto& k : c->kids) k.reset();
+            collapsed++;
+        }
+    }
+
+    ////////////////////////////////////////////////////////////////////////////
+    // Dual walk (dual.hpp:127-203) and DCMesher::load (dc_mesher.cpp:19-169)
+
+    static int Q(int a) { return a == 1 ? 2 : a == 2 ? 4 : 1; }
+    static int R(int a) { return a == 1 ? 4 : a == 2 ? 1 : 2; }
+
+    uint32_t vertex_index(const Cell* t, int vi) {
+        Leaf* l = t->leaf.get();
+        if (l->index[vi] == 0) {
+            l->index[vi] = uint32_t(verts.size());
+            verts.push_back({float(l->verts[vi][0]), float(l->verts[vi][1]),
+                             float(l->verts[vi][2])});
+            vkeys.push_back({t->key[0], t->key[1], t->key[2], uint32_t(t->level),
+                             uint32_t(vi)});
+        }
+        return l->index[vi];
+    }
+
+    void load(int A, const Cell* const ts[4]) {
+        for (int i = 0; i < 4; ++i) if (ts[i]->type != AMBIGUOUS) return;
+        int index = 0;
+        for (int i = 1; i < 4; ++i) {
+            if (ts[i]->leaf->level < ts[index]->leaf->level) index = i;
+        }
+        const int q = Q(A), r = R(A);
+        const int corners[4] = {q | r, r, q, 0};
+        const State a = ts[index]->corner_state(corners[index]);
+        const State b = ts[index]->corner_state(corners[index] | A);
+        if (a == b) return;
+        const bool D = (a == FILLED);
+        const int ev[4][2] = {{q | r, q | r | A}, {r, r | A}, {q, q | A}, {0, A}};
+        const Marching& M = marching();
+        uint32_t vs[4];
+        float vp[4][3];
+        for (int i = 0; i < 4; ++i) {
+            const int es = D ? M.e[ev[i][0]][ev[i][1]] : M.e[ev[i][1]][ev[i][0]];
+            const int vi = ts[i]->leaf->level > 0 ? 0 : M.p[ts[i]->leaf->corner_mask][es];
+            vs[i] = vertex_index(ts[i], vi);
+            for (int j = 0; j < 3; ++j) vp[i][j] = float(ts[i]->leaf->verts[vi][j]);
+        }
+        if (!D) {
+            std::swap(vs[1], vs[2]);
+            for (int j = 0; j < 3; ++j) std::swap(vp[1][j], vp[2][j]);
+        }
+        float norms[4][3];
+        auto save_norm = [&](int a_, int b_, int c_) {
+            float u[3], w[3];
+            for (int j = 0; j < 3; ++j) {
+                u[j] = vp[b_][j] - vp[a_][j];
+                w[j] = vp[c_][j] - vp[a_][j];
+            }
+            float n[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2],
+                          u[0] * w[1] - u[1] * w[0]};
+            const float sq = n[0] * n[0] + (n[1] * n[1] + n[2] * n[2]);
+            if (sq > 0) {
+                const float len = std::sqrt(sq);
+                for (int j = 0; j < 3; ++j) n[j] = n[j] / len;
+            }
+            for (int j = 0; j < 3; ++j) norms[a_][j] = n[j];
+        };
+        save_norm(0, 1, 2);
+        save_norm(1, 3, 0);
+        save_norm(2, 0, 3);
+        save_norm(3, 2, 1);
+        auto dot = [](const float* x, const float* y) {
+            return x[0] * y[0] + (x[1] * y[1] + x[2] * y[2]);
+        };
+        auto push_tri = [&](uint32_t x, uint32_t y, uint32_t z) {
+            if (x != y && y != z && x != z) tris.push_back({x, y, z});
+        };
+        if (dot(norms[0], norms[3]) > dot(norms[1], norms[2])) {
+            push_tri(vs[0], vs[1], vs[2]);
+            push_tri(vs[2], vs[1], vs[3]);
+        } else {
+            push_tri(vs[0], vs[1], vs[3]);
+            push_tri(vs[0], vs[3], vs[2]);
+        }
+    }
+
+    void edge3(int A, const Cell* t0, const Cell* t1, const Cell* t2, const Cell* t3) {
+        const Cell* ts[4] = {t0, t1, t2, t3};
+        bool any = false;
+        for (auto t : ts) any |= t->is_branch();
+        if (any) {
+            const int q = Q(A), r = R(A);
+            edge3(A, t0->child(q | r), t1->child(r), t2->child(q), t3->child(0));
+            edge3(A, t0->child(q | r | A), t1->child(r | A), t2->child(q | A), t3->child(A));
+        } else {
+            load(A, ts);
+        }
+    }
+
+    void face3(int A, const Cell* t0, const Cell* t1) {
+        if (!(t0->is_branch() || t1->is_branch())) return;
+        const int q = Q(A), r = R(A);
+        for (int k : {0, q, r, q | r}) face3(A, t0->child(k | A), t1->child(k));
+        edge3(q, t0->child(A), t0->child(r | A), t1->child(0), t1->child(r));
+        edge3(q, t0->child(q | A), t0->child(q | r | A), t1->child(q), t1->child(q | r));
+        edge3(r, t0->child(A), t1->child(0), t0->child(A | q), t1->child(q));
+        edge3(r, t0->child(r | A), t1->child(r), t0->child(r | A | q), t1->child(r | q));
+    }
+
+    void work(const Cell* t) {
+        for (int A : {1, 2, 4}) {
+            const int q = Q(A), r = R(A);
+            face3(A, t->child(0), t->child(A));
+            face3(A, t->child(q), t->child(q | A));
+            face3(A, t->child(r), t->child(r | A));
+            face3(A, t->child(q | r), t->child(q | r | A));
+        }
+        for (int A : {1, 2, 4}) {
+            const int q = Q(A), r = R(A);
+            for (int a : {0, A}) {
+                edge3(A, t->child(a), t->child(q | a), t->child(r | a), t->child(q | r | a));
+            }
+        }
+    }
+
+    // every vertex of every ambiguous leaf, in depth-first order
+    void leaf_vertices(const Cell* t, std::vector<double>& out) const {
+        if (t->is_branch()) {
+            for (int i = 0; i < 8; ++i) leaf_vertices(t->kids[i].get(), out);
+        } else if (t->type == AMBIGUOUS) {
+            for (int v = 0; v < t->leaf->vertex_count; ++v) {
+                out.insert(out.end(), t->leaf->verts[v], t->leaf->verts[v] + 3);
+            }
+        }
+    }
+
+    void walk(const Cell* t) {
+        if (!t->is_branch()) return;
+        for (int i = 0; i < 8; ++i) walk(t->kids[i].get());
+        work(t);
+    }
+};
+
+// Region::withResolution (region.hpp:226-231), evaluated in float.
+int resolution_level(const double lo[3], const double hi[3], double min_feature) {
+    double md = hi[0] - lo[0];
+    md = std::min(md, hi[1] - lo[1]);
+    md = std::min(md, hi[2] - lo[2]);
+    const float r = std::log(float(md / min_feature));
+    const float m = std::fmax(0.0f, r);
+    return int(std::ceil(m / std::log(2.0f)));
+}
+
+}   // namespace oracle
+
+using namespace oracle;
+
+struct oracle_mesh {
+    std::vector<float> verts;
+    std::vector<uint32_t> tris;
+    std::vector<uint32_t> keys;
+    std::vector<uint64_t> stats;
+    std::vector<uint32_t> tape_lengths;
+    std::vector<double> leaf_verts;
+};
+
+extern "C" {
+
+oracle_mesh* oracle_render_mesh(oracle_deck* d, const double lo[3],
+                                const double hi[3], double min_feature,
+                                double max_err) {
+    Deck& deck = d->deck;
+    Mesher m(deck, max_err);
+    auto root = std::make_unique<Cell>();
+    for (int j = 0; j < 3; ++j) {
+        root->lo[j] = lo[j];
+        root->hi[j] = hi[j];
+        root->key[j] = 0;
+    }
+    root->level = resolution_level(lo, hi, min_feature);
+    m.build(root.get(), deck.tape);
+    m.walk(root.get());
+    auto out = new oracle_mesh;
+    for (const auto& v : m.verts) out->verts.insert(out->verts.end(), v.begin(), v.end());
+    for (const auto& t : m.tris) out->tris.insert(out->tris.end(), t.begin(), t.end());
+    for (const auto& k : m.vkeys) out->keys.insert(out->keys.end(), k.begin(), k.end());
+    out->stats.push_back(uint64_t(root->level));
+    out->stats.push_back(m.leaves_ambiguous);
+    out->stats.push_back(m.collapsed);
+    for (int L = root->level; L >= 0; --L) {
+        auto it = m.level_counts.find(L);
+        for (int s = 0; s < 3; ++s) {
+            out->stats.push_back(it == m.level_counts.end() ? 0 : it->second[s]);
+        }
+    }
+    out->tape_lengths = m.tape_lengths;
+    m.leaf_vertices(root.get(), out->leaf_verts);
+    return out;
+}
+
+void oracle_mesh_sizes(const oracle_mesh* m, size_t* n_verts, size_t* n_tris,
+                       size_t* n_stats, size_t* n_tape_lengths) {
+    *n_verts = m->verts.size() / 3;
+    *n_tris = m->tris.size() / 3;
+    *n_stats = m->stats.size();
+    *n_tape_lengths = m->tape_lengths.size();
+}
+
+void oracle_mesh_copy(const oracle_mesh* m, float* verts, uint32_t* tris,
+                      uint32_t* keys, uint64_t* stats, uint32_t* tape_lengths) {
+    std::memcpy(verts, m->verts.data(), m->verts.size() * sizeof(float));
+    std::memcpy(tris, m->tris.data(), m->tris.size() * sizeof(uint32_t));
+    std::memcpy(keys, m->keys.data(), m->keys.size() * sizeof(uint32_t));
+    std::memcpy(stats, m->stats.data(), m->stats.size() * sizeof(uint64_t));
+    std::memcpy(tape_lengths, m->tape_lengths.data(),
+                m->tape_lengths.size() * sizeof(uint32_t));
+}
+
+size_t oracle_mesh_leaf_verts(const oracle_mesh* m, double* out) {
+    if (out) std::memcpy(out, m->leaf_verts.data(), m->leaf_verts.size() * sizeof(double));
+    return m->leaf_verts.size() / 3;
+}
+
+void oracle_mesh_delete(oracle_mesh* m) { delete m; }
+
+int oracle_corner_manifold(int mask) { return corner_manifold(uint8_t(mask)) ? 1 : 0; }
+
+void oracle_marching_table(int8_t* v, int8_t* e, int8_t* p) {
+    const Marching& M = marching();
+    std::memcpy(v, M.v, sizeof(M.v));
+    std::memcpy(e, M.e, sizeof(M.e));
+    std::memcpy(p, M.p, sizeof(M.p));
+}
+
+int oracle_is_inside(oracle_deck* d, const float p[3]) {
+    Deck& deck = d->deck;
+    FeatureEval fe(deck);
+    return fe.is_inside(p, deck.tape) ? 1 : 0;
+}
+
+size_t oracle_features(oracle_deck* d, const float p[3], float* out, size_t cap) {
+    Deck& deck = d->deck;
+    FeatureEval fe(deck);
+    const auto fs = fe.features(p, deck.tape);
+    for (size_t i = 0; i < fs.size() && i < cap; ++i) {
+        out[3 * i] = fs[i][0];
+        out[3 * i + 1] = fs[i][1];
+        out[3 * i + 2] = fs[i][2];
+    }
+    return fs.size();
+}
+
+void oracle_eigen3(const double* a, double* w, double* v) {
+    double m[3][3], vv[3][3];
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) m[i][j] = a[3 * i + j];
+    libfive::qef::jacobi3(m, w, vv);
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) v[3 * i + j] = vv[i][j];
+}
+
+double oracle_find_vertex(const double* ata9, const double* atb,
+                          double btb, const double* mass, double* vert,
+                          int* rank) {
+    double a[3][3];
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) a[i][j] = ata9[3 * i + j];
+    return libfive::qef::find_vertex(a, atb, btb, mass, vert, rank);
+}
+
+void oracle_push_sample(double* acc, const double* pos, const double* deriv,
+                        double value) {
+    libfive::qef::push_sample(acc, pos, deriv, value);
+}
+
+int oracle_resolution_level(const double lo[3], const double hi[3], double mf) {
+    return resolution_level(lo, hi, mf);
+}
+
+}   // extern "C"

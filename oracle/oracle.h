@@ -58,6 +58,42 @@ size_t oracle_interval_push(oracle_deck* d, const float lo[3],
  * accuracy tests against float64 references). */
 void oracle_sincos(const float* x, size_t n, float* s, float* c);
 
+/* Dual-contouring mesh of the shape over [lo, hi] (Mesh::render with the
+ * DC algorithm, one worker).  The result holds the vertices (index 0 is the
+ * reserved dummy vertex), triangles, one key per vertex (x, y, z of the
+ * owning cell's lower corner in finest-cell units, its level, its patch),
+ * statistics (root level, ambiguous leaves, collapsed cells, then per level
+ * from the root down: empty, filled, ambiguous counts) and the clause
+ * count of every pushed interval tape. */
+typedef struct oracle_mesh oracle_mesh;
+oracle_mesh* oracle_render_mesh(oracle_deck* d, const double lo[3],
+                                const double hi[3], double min_feature,
+                                double max_err);
+void oracle_mesh_sizes(const oracle_mesh* m, size_t* n_verts, size_t* n_tris,
+                       size_t* n_stats, size_t* n_tape_lengths);
+void oracle_mesh_copy(const oracle_mesh* m, float* verts, uint32_t* tris,
+                      uint32_t* keys, uint64_t* stats, uint32_t* tape_lengths);
+/* All vertices of all ambiguous leaf cells (x, y, z in double); returns the
+ * count, copies when out is not NULL. */
+size_t oracle_mesh_leaf_verts(const oracle_mesh* m, double* out);
+void oracle_mesh_delete(oracle_mesh* m);
+
+/* Tables and building blocks, exposed for their own tests. */
+int oracle_corner_manifold(int mask);
+void oracle_marching_table(int8_t* v /* 256*4*12*2 */, int8_t* e /* 8*8 */,
+                           int8_t* p /* 256*24 */);
+int oracle_is_inside(oracle_deck* d, const float p[3]);
+size_t oracle_features(oracle_deck* d, const float p[3], float* out,
+                       size_t cap);
+void oracle_eigen3(const double* a, double* w, double* v);
+double oracle_find_vertex(const double* ata9, const double* atb,
+                          double btb, const double* mass, double* vert,
+                          int* rank);
+void oracle_push_sample(double* acc /* 14 */, const double* pos,
+                        const double* deriv, double value);
+int oracle_resolution_level(const double lo[3], const double hi[3],
+                            double min_feature);
+
 #ifdef __cplusplus
 }
 #endif

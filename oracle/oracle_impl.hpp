@@ -87,3 +87,7 @@ double eval_point_f64(const Deck& deck, const Tape& tape, double x, double y,
                       double z, std::vector<double>& v);
 
 }   // namespace oracle
+
+struct oracle_deck {
+    oracle::Deck deck;
+};

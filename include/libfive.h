@@ -175,6 +175,7 @@ typedef struct libfive_mesh_settings {
     double min_feature;     /* default 0.1 */
     double max_err;         /* default 1e-8; < 0 disables collapse */
     uint32_t precision;     /* 0: FP64 QEF accumulation, 1: compensated FP32 */
+    uint32_t collect_stats; /* non-zero: record per-level statistics */
     volatile const int* cancel;   /* optional; non-zero requests cancel */
 } libfive_mesh_settings;
 
@@ -182,6 +183,24 @@ void libfive_mesh_settings_default(libfive_mesh_settings* s);
 libfive_mesh* libfive_tree_render_mesh_settings(libfive_tree tree,
                                                 libfive_region3 R,
                                                 const libfive_mesh_settings* s);
+
+/* Per-vertex keys of a mesh returned by this library: 5 words per vertex
+ * (x, y, z of the owning cell's lower corner in finest-cell units, the
+ * cell's octree level, the vertex patch within the cell), starting with the
+ * dummy vertex 0.  Valid until libfive_mesh_delete; NULL for other meshes.
+ * Used to compare meshes independently of vertex numbering. */
+const uint32_t* libfive_mesh_vertex_keys(const libfive_mesh* m);
+
+/* Statistics of the last render on the calling thread made with
+ * collect_stats set: per-level cell counts (empty, filled, ambiguous; root
+ * level first) and the clause count of the tape of every ambiguous cell
+ * above the finest level (Tape::size, libfive/include/libfive/eval/
+ * tape.hpp:46).  Both return the number of entries (levels, lengths) and
+ * copy at most cap of them when out is not NULL. */
+size_t libfive_last_mesh_level_counts(uint64_t* out, size_t cap);
+size_t libfive_last_mesh_tape_lengths(uint32_t* out, size_t cap);
+/* Wall-clock milliseconds of the last render: build, collapse, emit, total. */
+void libfive_last_mesh_times(double* ms4);
 
 #ifdef __cplusplus
 }

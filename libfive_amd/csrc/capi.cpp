@@ -11,6 +11,10 @@
 #include "eval_kernels.hpp"
 #include "interval_kernels.hpp"
 #include "mesher.hpp"
+
+#include <map>
+#include <mutex>
+#include <vector>
 #include "runtime.hpp"
 #include "tree.hpp"
 
@@ -31,6 +35,10 @@ static void set_error(const char* where, const std::exception& e) {
     std::fprintf(stderr, "%s\n", g_last_error.c_str());
 }
 
+// Vertex keys of the meshes handed out, released by libfive_mesh_delete.
+static std::mutex g_keys_mu;
+static std::map<const libfive_mesh*, std::vector<uint32_t>> g_keys;
+
 struct libfive_tape_ {
     std::unique_ptr<GpuTape> t;
 };
@@ -43,9 +51,19 @@ int libfive_gpu_available(void) { return gpu_available() ? 1 : 0; }
 
 void libfive_mesh_delete(libfive_mesh* m) {
     if (!m) return;
+    {
+        std::lock_guard<std::mutex> lock(g_keys_mu);
+        g_keys.erase(m);
+    }
     delete[] m->verts;
     delete[] m->tris;
     delete m;
+}
+
+const uint32_t* libfive_mesh_vertex_keys(const libfive_mesh* m) {
+    std::lock_guard<std::mutex> lock(g_keys_mu);
+    auto it = g_keys.find(m);
+    return it == g_keys.end() ? nullptr : it->second.data();
 }
 
 int libfive_opcode_enum(const char* op) {
@@ -395,7 +413,31 @@ void libfive_mesh_settings_default(libfive_mesh_settings* s) {
     s->min_feature = 0.1;
     s->max_err = 1e-8;
     s->precision = 0;
+    s->collect_stats = 0;
     s->cancel = nullptr;
+}
+
+size_t libfive_last_mesh_level_counts(uint64_t* out, size_t cap) {
+    const auto& st = last_mesh_stats();
+    const size_t n = st.level_counts.size() / 3;
+    for (size_t i = 0; out && i < 3 * std::min(n, cap); ++i) out[i] = st.level_counts[i];
+    return n;
+}
+
+size_t libfive_last_mesh_tape_lengths(uint32_t* out, size_t cap) {
+    const auto& st = last_mesh_stats();
+    for (size_t i = 0; out && i < std::min(st.tape_lengths.size(), cap); ++i) {
+        out[i] = st.tape_lengths[i];
+    }
+    return st.tape_lengths.size();
+}
+
+void libfive_last_mesh_times(double* ms4) {
+    const auto& st = last_mesh_stats();
+    ms4[0] = st.ms_build;
+    ms4[1] = st.ms_collapse;
+    ms4[2] = st.ms_emit;
+    ms4[3] = st.ms_total;
 }
 
 static libfive_mesh* render(libfive_tree tree, libfive_region3 R,
@@ -405,6 +447,7 @@ static libfive_mesh* render(libfive_tree tree, libfive_region3 R,
         ms.min_feature = s.min_feature;
         ms.max_err = s.max_err;
         ms.precision = s.precision;
+        ms.collect_stats = s.collect_stats != 0;
         ms.cancel = s.cancel;
         const double lo[3] = {R.X.lower, R.Y.lower, R.Z.lower};
         const double hi[3] = {R.X.upper, R.Y.upper, R.Z.upper};
@@ -420,6 +463,10 @@ static libfive_mesh* render(libfive_tree tree, libfive_region3 R,
         out->tris = new libfive_tri[out->tri_count];
         std::memcpy(out->verts, m.verts.data(), m.verts.size() * sizeof(float));
         std::memcpy(out->tris, m.tris.data(), m.tris.size() * sizeof(uint32_t));
+        {
+            std::lock_guard<std::mutex> lock(g_keys_mu);
+            g_keys[out] = std::move(m.keys);
+        }
         return out;
     } catch (const std::exception& e) {
         set_error(who, e);

@@ -1,0 +1,1089 @@
+// GPU dual-contouring mesher kernels.
+//
+// Follows the reference's octree mesher (libfive/src/render/brep/
+// worker_pool.inl:36-257, dc/dc_tree.inl:94-813, dc/dc_tree3.cpp:17-154,
+// dual.hpp:127-203, dc/dc_mesher.cpp:19-169) restated as level-synchronous
+// passes:
+//   k_interval  one wave per ambiguous parent: interval-evaluate its 8
+//               children with the parent's tape (one child per lane), record
+//               MIN/MAX keep decisions with wave ballots, then prune a tape
+//               per child (Tape::push) by a backward liveness walk
+//   k_leaf      one wave per finest cell: corner states,
+//               vertex patches, the 4 x 16-sample edge search, derivatives
+//               and the FP64 QEF vertex solve (DCTree::evalLeaf)
+//   k_collapse  one wave per branch cell, bottom-up (collectChildren)
+//   k_quads     one lane per ambiguous leaf: the minimal edges it owns,
+//               located by descending the octree from the root, emitted with
+//               the reference's winding and triangulation (DCMesher::load)
+//
+// Compile with -ffp-contract=off: every rounding must match the CPU
+// restatement of the reference.
+#include <hip/hip_runtime.h>
+
+#include "mesh_kernels.hpp"
+#include "interval_ops.hpp"
+#include "mesh_tables.hpp"
+#include "point_eval.hpp"
+#include "qef.hpp"
+#include "regfile.hpp"
+
+namespace libfive {
+namespace amd {
+
+// Out-of-line derivative rules of the less common opcodes
+// (eval_deriv_array.cpp:119-256).
+__device__ __noinline__ void pe_cold_deriv(uint32_t o, float av, float ad0,
+                                           float ad1, float ad2, float bv,
+                                           float bd0, float bd1, float bd2,
+                                           float* res) {
+    namespace OC = Opcode;
+    const float ad[3] = {ad0, ad1, ad2};
+    const float bd[3] = {bd0, bd1, bd2};
+    float ov;
+    float od[3];
+    switch (o) {
+        case OC::OP_ATAN2: {
+            ov = fmath::atan2_f32(av, bv);
+            const float a2 = av * av;
+            const float b2 = bv * bv;
+            const float den = a2 + b2;
+            for (int j = 0; j < 3; ++j) {
+                const float p = ad[j] * bv;
+                const float q = bd[j] * av;
+                od[j] = (p - q) / den;
+            }
+            break;
+        }
+        case OC::OP_POW: {
+            ov = fmath::pow_f32(av, bv);
+            const float f = bv * fmath::pow_f32(av, bv - 1.0f);
+            for (int j = 0; j < 3; ++j) od[j] = ad[j] * f;
+            break;
+        }
+        case OC::OP_NTH_ROOT: {
+            ov = fmath::nth_root_f32(av, bv);
+            const float f = fmath::pow_f32(av, 1.0f / bv - 1.0f) / bv;
+            for (int j = 0; j < 3; ++j) od[j] = (ad[j] == 0.0f) ? 0.0f : ad[j] * f;
+            break;
+        }
+        case OC::OP_MOD:
+            ov = fmath::mod_f32(av, bv);
+            for (int j = 0; j < 3; ++j) od[j] = ad[j];
+            break;
+        case OC::OP_NANFILL: {
+            const bool n_ = fmath::isnan_f(av);
+            ov = n_ ? bv : av;
+            for (int j = 0; j < 3; ++j) od[j] = n_ ? bd[j] : ad[j];
+            break;
+        }
+        case OC::OP_COMPARE:
+            ov = (av < bv) ? -1.0f : (av > bv) ? 1.0f : 0.0f;
+            for (int j = 0; j < 3; ++j) od[j] = 0.0f;
+            break;
+        case OC::OP_TAN: {
+            ov = fmath::tan_f32(av);
+            const float r = 1.0f / fmath::cos_f32(av);
+            const float f = r * r;
+            for (int j = 0; j < 3; ++j) od[j] = ad[j] * f;
+            break;
+        }
+        case OC::OP_ASIN: {
+            ov = fmath::asin_f32(av);
+            const float s = sqrtf(1.0f - av * av);
+            for (int j = 0; j < 3; ++j) od[j] = ad[j] / s;
+            break;
+        }
+        case OC::OP_ACOS: {
+            ov = fmath::acos_f32(av);
+            const float s = -sqrtf(1.0f - av * av);
+            for (int j = 0; j < 3; ++j) od[j] = ad[j] / s;
+            break;
+        }
+        case OC::OP_ATAN: {
+            ov = fmath::atan_f32(av);
+            const float s = av * av + 1.0f;
+            for (int j = 0; j < 3; ++j) od[j] = ad[j] / s;
+            break;
+        }
+        case OC::OP_LOG:
+            ov = fmath::log_f32(av);
+            for (int j = 0; j < 3; ++j) od[j] = ad[j] / av;
+            break;
+        case OC::OP_EXP:
+            ov = fmath::exp_f32(av);
+            for (int j = 0; j < 3; ++j) od[j] = ad[j] * ov;
+            break;
+        case OC::OP_RECIP: {
+            ov = 1.0f / av;
+            const float s = -(av * av);
+            for (int j = 0; j < 3; ++j) od[j] = ad[j] / s;
+            break;
+        }
+        default:
+            ov = av;
+            for (int j = 0; j < 3; ++j) od[j] = ad[j];
+            break;
+    }
+    res[0] = ov;
+    res[1] = od[0];
+    res[2] = od[1];
+    res[3] = od[2];
+}
+
+namespace mesh {
+
+using namespace tape;
+namespace OC = Opcode;
+
+__constant__ int8_t c_mv[256 * 4 * 12 * 2];
+__constant__ int8_t c_me[8 * 8];
+__constant__ int8_t c_mp[256 * 24];
+__constant__ uint8_t c_manifold[256];
+
+hipError_t upload_tables(hipStream_t s) {
+    const meshing::Tables& t = meshing::tables();
+    hipError_t e;
+    if ((e = hipMemcpyToSymbolAsync(HIP_SYMBOL(c_mv), t.v, sizeof(t.v), 0,
+                                    hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbolAsync(HIP_SYMBOL(c_me), t.e, sizeof(t.e), 0,
+                                    hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+    if ((e = hipMemcpyToSymbolAsync(HIP_SYMBOL(c_mp), t.p, sizeof(t.p), 0,
+                                    hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+    return hipMemcpyToSymbolAsync(HIP_SYMBOL(c_manifold), t.manifold,
+                                  sizeof(t.manifold), 0, hipMemcpyHostToDevice, s);
+}
+
+__device__ __forceinline__ int mv(uint32_t mask, int patch, int k, int end) {
+    return c_mv[((mask * 4 + patch) * 12 + k) * 2 + end];
+}
+
+// Region of a cell: the chain of midpoint subdivisions from the root
+// (Region::subdivide, region.hpp:89-114: mid = (lower + upper) / 2).
+__device__ __forceinline__ void cell_bounds(const Params& P, const uint32_t key[3],
+                                            int level, double lo[3], double hi[3]) {
+    for (int a = 0; a < 3; ++a) {
+        double l = P.lo[a], h = P.hi[a];
+        for (int k = P.top; k > level; --k) {
+            const double mid = (l + h) / 2;
+            if ((key[a] >> (k - 1)) & 1u) l = mid;
+            else h = mid;
+        }
+        lo[a] = l;
+        hi[a] = h;
+    }
+}
+
+__device__ __forceinline__ uint64_t ballot(bool b) { return __ballot(b); }
+
+////////////////////////////////////////////////////////////////////////////////
+// Interval classification and pruning
+
+template <int NV>
+__global__ __launch_bounds__(64)
+void k_interval(Params P, int level, const uint32_t* __restrict__ groups,
+                LevelView parent, LevelView cur, uint32_t choice_words)
+{
+    HIP_DYNAMIC_SHARED(uint32_t, lds)
+    const int lane = threadIdx.x;
+    const bool at_root = (groups == nullptr);
+    uint32_t tape_id = 0, first = 0, pkey[3] = {0, 0, 0};
+    if (!at_root) {
+        const uint32_t p = groups[blockIdx.x];
+        tape_id = parent.tape[p];
+        first = uint32_t(parent.child[p]);
+        for (int a = 0; a < 3; ++a) pkey[a] = parent.key[3 * p + a];
+    }
+    tape_id = __builtin_amdgcn_readfirstlane(tape_id);
+    const uint32_t nchild = at_root ? 1u : 8u;
+    const bool active = uint32_t(lane) < nchild;
+    const uint32_t octant = active ? uint32_t(lane) : 0u;
+    const uint32_t cidx = first + octant;
+    uint32_t key[3];
+    for (int a = 0; a < 3; ++a) {
+        key[a] = pkey[a] + (at_root ? 0u : (((octant >> a) & 1u) << level));
+    }
+    double lo[3], hi[3];
+    cell_bounds(P, key, level, lo, hi);
+    float flo[3], fhi[3];
+    for (int a = 0; a < 3; ++a) {
+        flo[a] = float(lo[a]);
+        fhi[a] = float(hi[a]);
+    }
+
+    const TapeRec tr = P.tapes[tape_id];
+    const uint32_t off = __builtin_amdgcn_readfirstlane(tr.off);
+    const uint32_t n = __builtin_amdgcn_readfirstlane(tr.len);
+    const Clause* __restrict__ t = P.pool + off;
+    uint16_t* choice = reinterpret_cast<uint16_t*>(lds);
+    uint32_t* rslot = lds + choice_words;
+    const uint32_t W = (P.nregs + 31u) / 32u;
+    uint32_t* need = rslot + P.nregs;
+
+    rslot[kRegX] = P.slot_x;
+    rslot[kRegY] = P.slot_y;
+    rslot[kRegZ] = P.slot_z;
+    __syncthreads();
+
+    LF_RF_DECL(L);
+    LF_RF_DECL(U);
+    LF_RF_DECL(F);
+    {
+        const Iv ix = make_checked(flo[0], fhi[0]);
+        const Iv iy = make_checked(flo[1], fhi[1]);
+        const Iv iz = make_checked(flo[2], fhi[2]);
+        LF_RF_SET(L, NV, kRegX, ix.lo); LF_RF_SET(U, NV, kRegX, ix.hi); LF_RF_SET(F, NV, kRegX, ix.nan ? 1.0f : 0.0f);
+        LF_RF_SET(L, NV, kRegY, iy.lo); LF_RF_SET(U, NV, kRegY, iy.hi); LF_RF_SET(F, NV, kRegY, iy.nan ? 1.0f : 0.0f);
+        LF_RF_SET(L, NV, kRegZ, iz.lo); LF_RF_SET(U, NV, kRegZ, iz.hi); LF_RF_SET(F, NV, kRegZ, iz.nan ? 1.0f : 0.0f);
+    }
+    bool any_ab = false;
+    for (uint32_t k = 0; k < n; ++k) {
+        const Clause c = t[k];
+        const uint32_t o = tape::op(c);
+        const uint32_t m = mode(c);
+        const float im = imm(c);
+        Iv a, b = {0.0f, 0.0f, false};
+        if (m == MODE_A_IMM || o == OP_LOADI) {
+            a = make_checked(im, im);
+        } else {
+            const uint32_t r = ra(c);
+            a = {LF_RF_GET(L, NV, r), LF_RF_GET(U, NV, r), LF_RF_GET(F, NV, r) != 0.0f};
+        }
+        if (is_binary(c)) {
+            if (m == MODE_B_IMM) {
+                b = make_checked(im, im);
+            } else {
+                const uint32_t r = rb(c);
+                b = {LF_RF_GET(L, NV, r), LF_RF_GET(U, NV, r), LF_RF_GET(F, NV, r) != 0.0f};
+            }
+        }
+        if (o == OC::OP_MIN || o == OC::OP_MAX) {
+            // The interval keep rule (eval_interval.cpp:116-167); operands
+            // that resolve to the same reference slot keep a.
+            const bool same = (m == MODE_RR) && rslot[ra(c)] == rslot[rb(c)];
+            uint32_t keep;
+            if (same) keep = KEEP_A;
+            else if (a.lo > b.hi) keep = (o == OC::OP_MAX) ? KEEP_A : KEEP_B;
+            else if (b.lo > a.hi) keep = (o == OC::OP_MAX) ? KEEP_B : KEEP_A;
+            else keep = KEEP_BOTH;
+            const uint64_t ma = ballot(active && keep == KEEP_A);
+            const uint64_t mb = ballot(active && keep == KEEP_B);
+            choice[k] = uint16_t((ma & 0xffu) | ((mb & 0xffu) << 8));
+            any_ab |= (keep == KEEP_A || keep == KEEP_B);
+        }
+        const Iv out = iv_apply(o, a, b, im);
+        const uint32_t d = dst(c);
+        rslot[d] = (o == OP_COPY) ? rslot[ra(c)] : slot(c);
+        LF_RF_SET(L, NV, d, out.lo);
+        LF_RF_SET(U, NV, d, out.hi);
+        LF_RF_SET(F, NV, d, out.nan ? 1.0f : 0.0f);
+    }
+    const uint32_t root = P.root_reg;
+    const float rlo = LF_RF_GET(L, NV, root);
+    const float rhi = LF_RF_GET(U, NV, root);
+    const bool rnan = LF_RF_GET(F, NV, root) != 0.0f;
+    // Interval::state (interval.hpp:66-77)
+    uint8_t state;
+    if (rnan) state = AMBIGUOUS;
+    else if (rlo > 0.0f) state = EMPTY;
+    else if (rhi < 0.0f) state = FILLED;
+    else state = AMBIGUOUS;
+
+    // Tape::push for every child that may prune (tape.cpp:23-158): a
+    // backward walk keeps each clause whose result is still needed; a
+    // MIN/MAX resolved to one side becomes a copy of that side.
+    bool dopush = active && state == AMBIGUOUS && !rnan &&
+                  !(tr.flags & TAPE_TERMINAL) && any_ab;
+    uint32_t child_tape = tape_id;
+    __syncthreads();
+    if (ballot(dopush)) {
+        uint32_t* my = need + uint32_t(lane & 7) * W;
+        uint32_t base = 0;
+        if (dopush) {
+            for (uint32_t w = 0; w < W; ++w) my[w] = 0;
+            my[root >> 5] |= 1u << (root & 31u);
+            base = atomicAdd(&P.counters->pool_top, n + 8u);
+            if (base + n + 8u > P.pool_cap) {
+                atomicOr(&P.counters->error, ERR_POOL);
+                dopush = false;
+            }
+        }
+        uint32_t pos = base + n, cnt = 0, refc = 0;
+        bool changed = false, nonterm = false;
+        for (uint32_t k = n; k-- > 0;) {
+            const Clause c = t[k];
+            if (!dopush) continue;
+            const uint32_t d = dst(c);
+            if (!((my[d >> 5] >> (d & 31u)) & 1u)) continue;
+            my[d >> 5] &= ~(1u << (d & 31u));
+            const uint32_t o = tape::op(c);
+            const uint32_t m = mode(c);
+            Clause out = c;
+            bool use_a = nargs(o) >= 1 && m != MODE_A_IMM && o != OP_LOADI;
+            bool use_b = is_binary(c) && m != MODE_B_IMM;
+            if (o == OC::OP_MIN || o == OC::OP_MAX) {
+                const uint32_t ch = choice[k];
+                const bool ka = (ch >> lane) & 1u;
+                const bool kb = (ch >> (8 + lane)) & 1u;
+                if (ka || kb) {
+                    changed = true;
+                    const bool from_imm = ka ? (m == MODE_A_IMM) : (m == MODE_B_IMM);
+                    const uint32_t src = ka ? ra(c) : rb(c);
+                    if (from_imm) {
+                        out = make(OP_LOADI, MODE_A_IMM, d, 0, 0, imm(c), slot(c));
+                        use_a = use_b = false;
+                    } else {
+                        out = make(OP_COPY, MODE_RR, d, src, 0, 0.0f, slot(c));
+                        use_a = false;
+                        use_b = false;
+                        my[src >> 5] |= 1u << (src & 31u);
+                    }
+                } else {
+                    nonterm = true;
+                }
+            }
+            if (use_a) { const uint32_t r = ra(c); my[r >> 5] |= 1u << (r & 31u); }
+            if (use_b) { const uint32_t r = rb(c); my[r >> 5] |= 1u << (r & 31u); }
+            P.pool[--pos] = out;
+            ++cnt;
+            const uint32_t oo = tape::op(out);
+            if (oo != OP_COPY && oo != OP_LOADI) ++refc;
+        }
+        if (dopush && changed) {
+            for (uint32_t k = 0; k < 8u; ++k) P.pool[base + n + k] = Clause{0, 0, 0, 0};
+            const uint32_t id = atomicAdd(&P.counters->tape_count, 1u);
+            if (id >= P.tape_cap) {
+                atomicOr(&P.counters->error, ERR_TAPES);
+            } else {
+                TapeRec r;
+                r.off = pos;
+                r.len = cnt;
+                r.parent = tape_id;
+                r.flags = TAPE_INTERVAL | (nonterm ? 0u : TAPE_TERMINAL);
+                r.box[0] = flo[0]; r.box[1] = fhi[0];
+                r.box[2] = flo[1]; r.box[3] = fhi[1];
+                r.box[4] = flo[2]; r.box[5] = fhi[2];
+                r.ref_len = refc;
+                r.pad = 0;
+                P.tapes[id] = r;
+                child_tape = id;
+            }
+        }
+    }
+    if (active && state == AMBIGUOUS) {
+        atomicMax(&P.counters->max_len, P.tapes[child_tape].len);
+    }
+    if (active) {
+        cur.type[cidx] = state;
+        cur.tape[cidx] = child_tape;
+        cur.child[cidx] = -1;
+        for (int a = 0; a < 3; ++a) cur.key[3 * cidx + a] = key[a];
+    }
+}
+
+template <int NV>
+static void launch_interval_nv(const Params& p, int level, const uint32_t* groups,
+                               uint32_t ngroups, const LevelView& parent,
+                               const LevelView& cur, uint32_t choice_words,
+                               size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL(k_interval<NV>, dim3(ngroups), dim3(64), lds, s, p, level, groups, parent,
+                                                        cur, choice_words);
+}
+
+static int banks(uint32_t nregs) {
+    const uint32_t b = (nregs + 31) / 32;
+    return b <= 1 ? 1 : b <= 2 ? 2 : b <= 4 ? 4 : 0;
+}
+
+hipError_t launch_interval(const Params& p, int level, const uint32_t* groups,
+                           uint32_t ngroups, const LevelView& parent,
+                           const LevelView& cur, uint32_t max_len, hipStream_t s) {
+    if (ngroups == 0) return hipSuccess;
+    const uint32_t choice_words = (max_len + 1) / 2;
+    const uint32_t W = (p.nregs + 31) / 32;
+    const size_t lds = 4u * (choice_words + p.nregs + 8u * W);
+    if (lds > 65536) return hipErrorInvalidValue;
+    switch (banks(p.nregs)) {
+        case 1: launch_interval_nv<1>(p, level, groups, ngroups, parent, cur, choice_words, lds, s); break;
+        case 2: launch_interval_nv<2>(p, level, groups, ngroups, parent, cur, choice_words, lds, s); break;
+        case 4: launch_interval_nv<4>(p, level, groups, ngroups, parent, cur, choice_words, lds, s); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+////////////////////////////////////////////////////////////////////////////////
+// Level linking
+
+__global__ void k_flags(LevelView cur, uint32_t* flags) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cur.n) flags[i] = (cur.type[i] == AMBIGUOUS) ? 1u : 0u;
+}
+
+__global__ void k_link(LevelView cur, const uint32_t* base, uint32_t* groups) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cur.n) return;
+    if (cur.type[i] == AMBIGUOUS) {
+        cur.child[i] = int32_t(8u * base[i]);
+        groups[base[i]] = i;
+    } else {
+        cur.child[i] = -1;
+    }
+}
+
+hipError_t launch_flags(const LevelView& cur, uint32_t* flags, hipStream_t s) {
+    if (cur.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_flags, dim3((cur.n + 255) / 256), dim3(256), 0, s, cur, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_link(const LevelView& cur, const uint32_t* base,
+                       uint32_t* groups, hipStream_t s) {
+    if (cur.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_link, dim3((cur.n + 255) / 256), dim3(256), 0, s, cur, base, groups);
+    return hipGetLastError();
+}
+
+////////////////////////////////////////////////////////////////////////////////
+// Exact-zero points (FeatureEvaluator): not evaluated on this path yet; a
+// point that needs it raises ERR_FEATURE so the render fails loudly.
+
+__device__ bool feature_is_inside(const Params& P) {
+    atomicOr(&P.counters->error, ERR_FEATURE);
+    return false;
+}
+
+////////////////////////////////////////////////////////////////////////////////
+// Leaf cells
+
+template <int NV>
+__global__ __launch_bounds__(64)
+void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
+            uint32_t per_group, LevelView l1, LevelView l0)
+{
+    __shared__ double s_tf[12][3], s_ts[12][3];
+    __shared__ double s_acc[12][14];
+    __shared__ int s_rank[12];
+    __shared__ float s_v[32], s_g[32][3];
+    __shared__ uint32_t s_fl[32];
+    const int lane = threadIdx.x;
+    const uint32_t total = ngroups * per_group;
+    const uint32_t root = P.root_reg;
+
+    const uint32_t w = blockIdx.x;
+    if (w >= total) return;
+    do {
+        const uint32_t p = groups[w / per_group];
+        const uint32_t oct = w % per_group;
+        const uint32_t cidx = uint32_t(l1.child[p]) + oct;
+        uint32_t key[3];
+        for (int a = 0; a < 3; ++a) key[a] = l1.key[3 * p + a] + ((oct >> a) & 1u);
+        const uint32_t tape_id = __builtin_amdgcn_readfirstlane(l1.tape[p]);
+        const TapeRec tr = P.tapes[tape_id];
+        const uint32_t n = __builtin_amdgcn_readfirstlane(tr.len);
+        const Clause* __restrict__ t = P.pool + __builtin_amdgcn_readfirstlane(tr.off);
+        double lo[3], hi[3];
+        cell_bounds(P, key, 0, lo, hi);
+
+        // Corner states (dc_tree.inl:131-239): lane & 7 is the corner.
+        const int corner = lane & 7;
+        float cp[3];
+        for (int a = 0; a < 3; ++a) cp[a] = float(((corner >> a) & 1) ? hi[a] : lo[a]);
+        float g[3];
+        uint32_t fl;
+        const float v = pe_deriv<NV>(t, n, root, cp[0], cp[1], cp[2], g, &fl);
+        bool inside;
+        if (v > 0.0f || fmath::isnan_f(v)) inside = false;
+        else if (v < 0.0f) inside = true;
+        else if (!(fl & PT_AMBIGUOUS)) inside = (g[0] != 0.0f || g[1] != 0.0f || g[2] != 0.0f);
+        else inside = (lane < 8) ? feature_is_inside(P) : false;
+        const uint32_t mask = uint32_t(ballot(lane < 8 && inside) & 0xffu);
+        if (mask == 0u || mask == 0xffu) {
+            if (lane == 0) {
+                l0.type[cidx] = mask ? FILLED : EMPTY;
+                l0.child[cidx] = -1;
+                l0.tape[cidx] = tape_id;
+                for (int a = 0; a < 3; ++a) l0.key[3 * cidx + a] = key[a];
+            }
+            continue;
+        }
+
+        LeafRec* leaf = l0.leaf + cidx;
+        uint32_t vcount = 0, rank = 0;
+        for (int patch = 0; patch < 4; ++patch) {
+            if (mv(mask, patch, 0, 0) == -1) break;
+            int ec = 0;
+            while (ec < 12 && mv(mask, patch, ec, 0) != -1) ++ec;
+            if (lane < ec) {
+                const int a = mv(mask, patch, lane, 0), b = mv(mask, patch, lane, 1);
+                for (int q = 0; q < 3; ++q) {
+                    s_tf[lane][q] = ((a >> q) & 1) ? hi[q] : lo[q];
+                    s_ts[lane][q] = ((b >> q) & 1) ? hi[q] : lo[q];
+                }
+            }
+            __syncthreads();
+            // Edge search: 4 rounds of 16 samples (dc_tree.inl:339-410).
+            const int passes = (16 * ec + 63) / 64;
+            for (int round = 0; round < 4; ++round) {
+                for (int pass = 0; pass < passes; ++pass) {
+                    const int i = pass * 64 + lane;
+                    const int e = i >> 4, j = i & 15;
+                    const bool valid = e < ec;
+                    const double frac = j / 15.0;
+                    double ps[3];
+                    for (int q = 0; q < 3; ++q) {
+                        ps[q] = s_tf[e][q] * (1 - frac) + s_ts[e][q] * frac;
+                    }
+                    uint32_t vf;
+                    const float sv = pe_value<NV>(t, n, root, float(ps[0]), float(ps[1]),
+                                                  float(ps[2]), &vf);
+                    // First j >= 1 with v > 0, or v == 0 outside, else j = 15.
+                    const bool zero = valid && j >= 1 && sv == 0.0f;
+                    bool zero_out = false;
+                    const bool fast = zero && !(vf & PT_TIE);
+                    if (ballot(fast)) {
+                        float sg[3];
+                        uint32_t f2;
+                        pe_deriv<NV>(t, n, root, float(ps[0]), float(ps[1]), float(ps[2]), sg, &f2);
+                        const float nrm = sqrtf(sg[0] * sg[0] + (sg[1] * sg[1] + sg[2] * sg[2]));
+                        if (fast) zero_out = !(nrm > 0.0f);
+                    }
+                    if (zero && (vf & PT_TIE)) zero_out = !feature_is_inside(P);
+                    const bool take = valid && j >= 1 &&
+                        (sv > 0.0f || zero_out || (j == 15 && !(sv == 0.0f)));
+                    const uint64_t tm = ballot(take);
+                    const uint32_t seg = uint32_t(tm >> ((lane >> 4) * 16)) & 0xffffu;
+                    const int first = seg ? __builtin_ctz(seg) : -1;
+                    __syncthreads();
+                    if (valid && first >= 1) {
+                        if (j == first) {
+                            for (int q = 0; q < 3; ++q) s_ts[e][q] = ps[q];
+                        } else if (j == first - 1) {
+                            for (int q = 0; q < 3; ++q) s_tf[e][q] = ps[q];
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            // Derivatives at both bracket ends (dc_tree.inl:412-471).
+            {
+                const int e = (lane >> 1) < ec ? (lane >> 1) : 0;
+                double ep[3];
+                for (int q = 0; q < 3; ++q) ep[q] = (lane & 1) ? s_ts[e][q] : s_tf[e][q];
+                float eg[3];
+                uint32_t ef;
+                const float ev = pe_deriv<NV>(t, n, root, float(ep[0]), float(ep[1]),
+                                              float(ep[2]), eg, &ef);
+                if (lane < 2 * ec) {
+                    s_v[lane] = ev;
+                    s_g[lane][0] = eg[0];
+                    s_g[lane][1] = eg[1];
+                    s_g[lane][2] = eg[2];
+                    s_fl[lane] = ef;
+                }
+            }
+            __syncthreads();
+            if (lane < ec) {
+                double acc[14];
+                for (int q = 0; q < 14; ++q) acc[q] = 0.0;
+                bool exists = false;
+                for (int end = 0; end < 2; ++end) {
+                    const int k = 2 * lane + end;
+                    double pos[3];
+                    for (int q = 0; q < 3; ++q) pos[q] = end ? s_ts[lane][q] : s_tf[lane][q];
+                    if (!(s_fl[k] & PT_AMBIGUOUS)) {
+                        const double d[3] = {double(s_g[k][0]), double(s_g[k][1]),
+                                             double(s_g[k][2])};
+                        qef::push_sample(acc, pos, d, double(s_v[k]));
+                        exists = true;
+                    } else {
+                        feature_is_inside(P);   // features() not available
+                    }
+                }
+                int r = -1;
+                if (exists) {
+                    double ata[3][3];
+                    qef::unpack_ata(acc, ata);
+                    r = qef::small_eigenvalue_count(ata);
+                }
+                s_rank[lane] = r;
+                for (int q = 0; q < 14; ++q) s_acc[lane][q] = acc[q];
+            }
+            __syncthreads();
+            if (lane == 0) {
+                int max_rank = 0;
+                for (int e = 0; e < ec; ++e) max_rank = s_rank[e] > max_rank ? s_rank[e] : max_rank;
+                double mass[4] = {0.0, 0.0, 0.0, 0.0};
+                double sum[14];
+                for (int q = 0; q < 14; ++q) sum[q] = 0.0;
+                for (int e = 0; e < ec; ++e) {
+                    if (s_rank[e] < 0) continue;
+                    if (s_rank[e] == max_rank) {
+                        mass[0] = mass[0] + s_acc[e][0] / s_acc[e][3];
+                        mass[1] = mass[1] + s_acc[e][1] / s_acc[e][3];
+                        mass[2] = mass[2] + s_acc[e][2] / s_acc[e][3];
+                        mass[3] = mass[3] + 1.0;
+                    }
+                }
+                for (int e = 0; e < ec; ++e) {
+                    if (s_rank[e] < 0) continue;
+                    for (int q = 4; q < 14; ++q) sum[q] = sum[q] + s_acc[e][q];
+                }
+                double ata[3][3];
+                qef::unpack_ata(sum, ata);
+                int rk = 0;
+                double vert[3];
+                qef::find_vertex(ata, sum + 10, sum[13], mass, vert, &rk);
+                for (int q = 0; q < 3; ++q) leaf->vert[patch][q] = vert[q];
+                for (int q = 0; q < 4; ++q) leaf->mass[q] = mass[q];
+                for (int q = 0; q < 6; ++q) leaf->ata[q] = sum[4 + q];
+                for (int q = 0; q < 3; ++q) leaf->atb[q] = sum[10 + q];
+                leaf->btb = sum[13];
+                rank = uint32_t(rk);
+            }
+            ++vcount;
+            __syncthreads();
+        }
+        if (lane == 0) {
+            leaf->rank = rank;
+            leaf->vcount = vcount;
+            leaf->mask = mask;
+            leaf->flags = c_manifold[mask] ? 1u : 0u;
+            l0.type[cidx] = AMBIGUOUS;
+            l0.child[cidx] = -1;
+            l0.tape[cidx] = tape_id;
+            for (int a = 0; a < 3; ++a) l0.key[3 * cidx + a] = key[a];
+        }
+    } while (false);
+}
+
+hipError_t launch_leaf(const Params& p, const uint32_t* groups, uint32_t ngroups,
+                       uint32_t per_group, const LevelView& l1, const LevelView& l0,
+                       hipStream_t s) {
+    if (ngroups == 0) return hipSuccess;
+    const uint32_t grid = ngroups * per_group;
+    switch (banks(p.nregs)) {
+        case 1: hipLaunchKernelGGL(k_leaf<1>, dim3(grid), dim3(64), 0, s, p, groups, ngroups, per_group, l1, l0); break;
+        case 2: hipLaunchKernelGGL(k_leaf<2>, dim3(grid), dim3(64), 0, s, p, groups, ngroups, per_group, l1, l0); break;
+        case 4: hipLaunchKernelGGL(k_leaf<4>, dim3(grid), dim3(64), 0, s, p, groups, ngroups, per_group, l1, l0); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+////////////////////////////////////////////////////////////////////////////////
+// Collapse (collectChildren, dc_tree.inl:581-735)
+
+// Tape::getBase (tape.cpp:168-205) for a region and for a point.
+__device__ uint32_t base_for_region(const Params& P, uint32_t id, const double lo[3],
+                                    const double hi[3]) {
+    // A tape's parent always has a smaller id, so the walk ends at the
+    // base tape; the step limit only guards against corrupted records.
+    for (int step = 0; step < 64 && P.tapes[id].parent != kNoTape; ++step) {
+        const TapeRec& r = P.tapes[id];
+        if ((r.flags & TAPE_INTERVAL) &&
+            lo[0] >= double(r.box[0]) && hi[0] <= double(r.box[1]) &&
+            lo[1] >= double(r.box[2]) && hi[1] <= double(r.box[3]) &&
+            lo[2] >= double(r.box[4]) && hi[2] <= double(r.box[5])) {
+            break;
+        }
+        id = r.parent;
+    }
+    return id;
+}
+
+__device__ __forceinline__ uint8_t corner_state(const LevelView& lv, uint32_t c, int i) {
+    const uint8_t ty = lv.type[c];
+    if (ty == AMBIGUOUS) return ((lv.leaf[c].mask >> i) & 1u) ? FILLED : EMPTY;
+    return ty;
+}
+
+template <int NV>
+__global__ __launch_bounds__(64)
+void k_collapse(Params P, int level, LevelView cur, LevelView below)
+{
+    const uint32_t cell = blockIdx.x;
+    if (cell >= cur.n) return;
+    if (cur.type[cell] != AMBIGUOUS || cur.child[cell] < 0) return;
+    const uint32_t cb = uint32_t(cur.child[cell]);
+    for (int i = 0; i < 8; ++i) {
+        if (below.child[cb + i] >= 0) return;       // a branch child
+    }
+    uint8_t corners[8];
+    bool all_empty = true, all_full = true;
+    for (int i = 0; i < 8; ++i) {
+        corners[i] = corner_state(below, cb + i, i);
+        const uint8_t ty = below.type[cb + i];
+        all_empty &= (ty == EMPTY);
+        all_full &= (ty == FILLED);
+    }
+    if (all_empty || all_full) {
+        if (threadIdx.x == 0) {
+            cur.type[cell] = all_empty ? EMPTY : FILLED;
+            cur.child[cell] = -1;
+        }
+        return;
+    }
+    uint32_t mask = 0;
+    for (int i = 0; i < 8; ++i) mask |= uint32_t(corners[i] == FILLED) << i;
+    bool manifold = c_manifold[mask] != 0;
+    for (int i = 0; i < 8 && manifold; ++i) {
+        if (below.type[cb + i] == AMBIGUOUS) manifold = (below.leaf[cb + i].flags & 1u) != 0;
+    }
+    if (manifold) {
+        // leafsAreManifold (dc_tree3.cpp:17-109)
+        auto cs = [&](int c, int i) { return corner_state(below, cb + c, i); };
+        auto e2 = [&](uint8_t s, int a, int b) { return s == corners[a] || s == corners[b]; };
+        auto e4 = [&](uint8_t s, int a, int b, int c, int d) {
+            return s == corners[a] || s == corners[b] || s == corners[c] || s == corners[d];
+        };
+        const int X = 1, Y = 2, Z = 4;
+        const bool edges_safe =
+            e2(cs(0, Z), 0, Z) && e2(cs(0, X), 0, X) && e2(cs(0, Y), 0, Y) &&
+            e2(cs(X, X | Y), X, X | Y) && e2(cs(X, X | Z), X, X | Z) &&
+            e2(cs(Y, Y | X), Y, Y | X) && e2(cs(Y, Y | Z), Y, Y | Z) &&
+            e2(cs(X | Y, X | Y | Z), X | Y, X | Y | Z) &&
+            e2(cs(Z, Z | X), Z, Z | X) && e2(cs(Z, Z | Y), Z, Z | Y) &&
+            e2(cs(Z | X, Z | X | Y), Z | X, Z | X | Y) &&
+            e2(cs(Z | Y, Z | Y | X), Z | Y, Z | Y | X);
+        const bool faces_safe =
+            e4(cs(0, X | Z), 0, X, Z, X | Z) &&
+            e4(cs(0, Y | Z), 0, Y, Z, Y | Z) &&
+            e4(cs(0, Y | X), 0, Y, X, Y | X) &&
+            e4(cs(X | Y | Z, X), X, X | Z, X | Y, X | Y | Z) &&
+            e4(cs(X | Y | Z, Y), Y, Y | Z, Y | X, X | Y | Z) &&
+            e4(cs(X | Y | Z, Z), Z, Z | Y, Z | X, X | Y | Z);
+        const uint8_t ctr = cs(0, X | Y | Z);
+        bool center_safe = false;
+        for (int i = 0; i < 8; ++i) center_safe |= (ctr == corners[i]);
+        manifold = edges_safe && faces_safe && center_safe;
+    }
+    if (!manifold) return;
+
+    uint32_t rank = 0;
+    for (int i = 0; i < 8; ++i) {
+        if (below.type[cb + i] == AMBIGUOUS) {
+            const uint32_t r = below.leaf[cb + i].rank;
+            rank = r > rank ? r : rank;
+        }
+    }
+    double mass[4] = {0.0, 0.0, 0.0, 0.0};
+    double sum[14];
+    for (int q = 0; q < 14; ++q) sum[q] = 0.0;
+    for (int i = 0; i < 8; ++i) {
+        if (below.type[cb + i] != AMBIGUOUS) continue;
+        const LeafRec& l = below.leaf[cb + i];
+        if (l.rank == rank) {
+            for (int q = 0; q < 4; ++q) mass[q] = mass[q] + l.mass[q];
+        }
+        for (int q = 0; q < 6; ++q) sum[4 + q] = sum[4 + q] + l.ata[q];
+        for (int q = 0; q < 3; ++q) sum[10 + q] = sum[10 + q] + l.atb[q];
+        sum[13] = sum[13] + l.btb;
+    }
+    double ata[3][3];
+    qef::unpack_ata(sum, ata);
+    double vert[3];
+    const double err = qef::find_vertex(ata, sum + 10, sum[13], mass, vert, nullptr);
+    uint32_t key[3];
+    for (int a = 0; a < 3; ++a) key[a] = cur.key[3 * cell + a];
+    double lo[3], hi[3];
+    cell_bounds(P, key, level, lo, hi);
+    const double eps = 1e-6;
+    const bool inside = vert[0] >= lo[0] - eps && vert[1] >= lo[1] - eps &&
+                        vert[2] >= lo[2] - eps && vert[0] <= hi[0] + eps &&
+                        vert[1] <= hi[1] + eps && vert[2] <= hi[2] + eps;
+    if (!(err < P.max_err && inside)) return;
+    const float vf[3] = {float(vert[0]), float(vert[1]), float(vert[2])};
+    const double vd[3] = {vf[0], vf[1], vf[2]};
+    uint32_t tid = base_for_region(P, cur.tape[cell], lo, hi);
+    tid = __builtin_amdgcn_readfirstlane(base_for_region(P, tid, vd, vd));
+    const TapeRec tr = P.tapes[tid];
+    uint32_t fl;
+    const float fv = pe_value<NV>(P.pool + __builtin_amdgcn_readfirstlane(tr.off),
+                                  __builtin_amdgcn_readfirstlane(tr.len), P.root_reg,
+                                  vf[0], vf[1], vf[2], &fl);
+    if (!(fabsf(fv) < P.max_err)) return;
+    if (threadIdx.x == 0) {
+        LeafRec& l = cur.leaf[cell];
+        for (int q = 0; q < 3; ++q) l.vert[0][q] = vert[q];
+        for (int q = 0; q < 4; ++q) l.mass[q] = mass[q];
+        for (int q = 0; q < 6; ++q) l.ata[q] = sum[4 + q];
+        for (int q = 0; q < 3; ++q) l.atb[q] = sum[10 + q];
+        l.btb = sum[13];
+        l.rank = rank;
+        l.vcount = 1;
+        l.mask = mask;
+        l.flags = 1u | (uint32_t(level) << 8);
+        cur.child[cell] = -1;
+    }
+}
+
+hipError_t launch_collapse(const Params& p, int level, const LevelView& cur,
+                           const LevelView& below, hipStream_t s) {
+    if (cur.n == 0) return hipSuccess;
+    switch (banks(p.nregs)) {
+        case 1: hipLaunchKernelGGL(k_collapse<1>, dim3(cur.n), dim3(64), 0, s, p, level, cur, below); break;
+        case 2: hipLaunchKernelGGL(k_collapse<2>, dim3(cur.n), dim3(64), 0, s, p, level, cur, below); break;
+        case 4: hipLaunchKernelGGL(k_collapse<4>, dim3(cur.n), dim3(64), 0, s, p, level, cur, below); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+////////////////////////////////////////////////////////////////////////////////
+// Quad emission (Dual<3> walk + DCMesher::load)
+
+// Leaf cell containing a point given in doubled finest-cell units (odd
+// coordinates never lie on a cell boundary of a cell larger than the unit).
+__device__ bool locate(const Params& P, int64_t px, int64_t py, int64_t pz,
+                       int* lvl, uint32_t* idx) {
+    const int64_t lim = int64_t(2) << P.top;
+    if (px < 0 || py < 0 || pz < 0 || px >= lim || py >= lim || pz >= lim) return false;
+    int l = P.top;
+    uint32_t i = 0;
+    for (;;) {
+        const int32_t ch = P.levels[l].child[i];
+        if (ch < 0 || l == 0) break;
+        const uint32_t oct = uint32_t((px >> l) & 1) | (uint32_t((py >> l) & 1) << 1) |
+                             (uint32_t((pz >> l) & 1) << 2);
+        i = uint32_t(ch) + oct;
+        --l;
+    }
+    *lvl = l;
+    *idx = i;
+    return true;
+}
+
+__device__ __forceinline__ int axis_q(int a) { return a == 1 ? 2 : a == 2 ? 4 : 1; }
+__device__ __forceinline__ int axis_r(int a) { return a == 1 ? 4 : a == 2 ? 1 : 2; }
+__device__ __forceinline__ int bit_index(int a) { return a == 1 ? 0 : a == 2 ? 1 : 2; }
+
+__global__ void k_quads(Params P, LevelView cur, int pass, uint32_t* tri_count,
+                        uint32_t* vslot, const uint32_t* tri_off,
+                        const uint32_t* vid, uint32_t* tris)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cur.n) return;
+    if (cur.type[c] != AMBIGUOUS || cur.child[c] >= 0) {
+        if (pass == 0) tri_count[cur.cell_base + c] = 0;
+        return;
+    }
+    const int my_level = int((cur.leaf[c].flags >> 8) & 0xffu);
+    int64_t key[3];
+    for (int a = 0; a < 3; ++a) key[a] = cur.key[3 * c + a];
+    const int64_t size = int64_t(1) << my_level;
+    {
+        // Cells below a collapsed ancestor stay in their level's arrays but
+        // are no longer part of the tree: the tree leaf at this cell's
+        // centre must be the cell itself.
+        int cl;
+        uint32_t ci;
+        const bool found = locate(P, 2 * key[0] + size, 2 * key[1] + size,
+                                  2 * key[2] + size, &cl, &ci);
+        if (!found || cl != my_level || ci != c) {
+            if (pass == 0) tri_count[cur.cell_base + c] = 0;
+            return;
+        }
+    }
+    uint32_t count = 0;
+    uint32_t out = (pass == 1) ? tri_off[cur.cell_base + c] : 0u;
+    for (int A = 1; A <= 4; A <<= 1) {
+        const int Q = axis_q(A), R = axis_r(A);
+        const int ia = bit_index(A), iq = bit_index(Q), ir = bit_index(R);
+        for (int qr = 0; qr < 4; ++qr) {
+            const int q = qr & 1, r = qr >> 1;
+            const int kc = (q ? 0 : 1) | (r ? 0 : 2);
+            int lv[4];
+            uint32_t ix[4];
+            bool ok = true;
+            for (int k = 0; k < 4 && ok; ++k) {
+                if (k == kc) { lv[k] = my_level; ix[k] = c; continue; }
+                int64_t pt[3];
+                pt[ia] = 2 * key[ia] + size;
+                pt[iq] = 2 * (key[iq] + q * size) + ((k & 1) ? 1 : -1);
+                pt[ir] = 2 * (key[ir] + r * size) + ((k & 2) ? 1 : -1);
+                ok = locate(P, pt[0], pt[1], pt[2], &lv[k], &ix[k]);
+            }
+            if (!ok) continue;
+            // The segment belongs to the smallest of the four cells, and
+            // among equal ones to the first in walk order.
+            bool owner = true;
+            for (int k = 0; k < 4; ++k) {
+                if (lv[k] < my_level || (k < kc && lv[k] == my_level)) owner = false;
+            }
+            if (!owner) continue;
+            bool all_amb = true;
+            for (int k = 0; k < 4; ++k) {
+                all_amb &= (P.levels[lv[k]].type[ix[k]] == AMBIGUOUS);
+            }
+            if (!all_amb) continue;
+            const int corners4[4] = {Q | R, R, Q, 0};
+            const uint32_t m = cur.leaf[c].mask;
+            const bool sa = (m >> corners4[kc]) & 1u;
+            const bool sb = (m >> (corners4[kc] | A)) & 1u;
+            if (sa == sb) continue;
+            const bool D = sa;
+            const int ev[4][2] = {{Q | R, Q | R | A}, {R, R | A}, {Q, Q | A}, {0, A}};
+            uint32_t slots[4];
+            float vp[4][3];
+            for (int i = 0; i < 4; ++i) {
+                const int es = D ? c_me[ev[i][0] * 8 + ev[i][1]] : c_me[ev[i][1] * 8 + ev[i][0]];
+                const LevelView& L = P.levels[lv[i]];
+                const LeafRec& lf = L.leaf[ix[i]];
+                const int vi = lv[i] > 0 ? 0 : c_mp[lf.mask * 24 + es];
+                slots[i] = 4u * (L.cell_base + ix[i]) + uint32_t(vi < 0 ? 0 : vi);
+                for (int q2 = 0; q2 < 3; ++q2) vp[i][q2] = float(lf.vert[vi < 0 ? 0 : vi][q2]);
+            }
+            if (!D) {
+                uint32_t ts = slots[1]; slots[1] = slots[2]; slots[2] = ts;
+                for (int q2 = 0; q2 < 3; ++q2) {
+                    const float tv = vp[1][q2]; vp[1][q2] = vp[2][q2]; vp[2][q2] = tv;
+                }
+            }
+            float nrm[4][3];
+            const int tri_n[4][3] = {{0, 1, 2}, {1, 3, 0}, {2, 0, 3}, {3, 2, 1}};
+            for (int s = 0; s < 4; ++s) {
+                const int a0 = tri_n[s][0], b0 = tri_n[s][1], c0 = tri_n[s][2];
+                float u[3], w[3];
+                for (int q2 = 0; q2 < 3; ++q2) {
+                    u[q2] = vp[b0][q2] - vp[a0][q2];
+                    w[q2] = vp[c0][q2] - vp[a0][q2];
+                }
+                float nn[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2],
+                               u[0] * w[1] - u[1] * w[0]};
+                const float sq = nn[0] * nn[0] + (nn[1] * nn[1] + nn[2] * nn[2]);
+                if (sq > 0.0f) {
+                    const float len = sqrtf(sq);
+                    for (int q2 = 0; q2 < 3; ++q2) nn[q2] = nn[q2] / len;
+                }
+                for (int q2 = 0; q2 < 3; ++q2) nrm[a0][q2] = nn[q2];
+            }
+            const float d03 = nrm[0][0] * nrm[3][0] + (nrm[0][1] * nrm[3][1] + nrm[0][2] * nrm[3][2]);
+            const float d12 = nrm[1][0] * nrm[2][0] + (nrm[1][1] * nrm[2][1] + nrm[1][2] * nrm[2][2]);
+            uint32_t tv[2][3];
+            if (d03 > d12) {
+                tv[0][0] = slots[0]; tv[0][1] = slots[1]; tv[0][2] = slots[2];
+                tv[1][0] = slots[2]; tv[1][1] = slots[1]; tv[1][2] = slots[3];
+            } else {
+                tv[0][0] = slots[0]; tv[0][1] = slots[1]; tv[0][2] = slots[3];
+                tv[1][0] = slots[0]; tv[1][1] = slots[3]; tv[1][2] = slots[2];
+            }
+            for (int s = 0; s < 2; ++s) {
+                if (tv[s][0] == tv[s][1] || tv[s][1] == tv[s][2] || tv[s][0] == tv[s][2]) continue;
+                if (pass == 0) {
+                    for (int q2 = 0; q2 < 3; ++q2) vslot[tv[s][q2]] = 1u;
+                } else {
+                    for (int q2 = 0; q2 < 3; ++q2) tris[3 * (out + count) + q2] = vid[tv[s][q2]] + 1u;
+                }
+                ++count;
+            }
+        }
+    }
+    if (pass == 0) tri_count[cur.cell_base + c] = count;
+}
+
+hipError_t launch_quads(const Params& p, const LevelView& cur, int pass,
+                        uint32_t* tri_count, uint32_t* vslot,
+                        const uint32_t* tri_off, const uint32_t* vid,
+                        uint32_t* tris, hipStream_t s) {
+    if (cur.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_quads, dim3((cur.n + 63) / 64), dim3(64), 0, s, p, cur, pass, tri_count, vslot,
+                                                         tri_off, vid, tris);
+    return hipGetLastError();
+}
+
+__global__ void k_verts(LevelView cur, int level, const uint32_t* vslot,
+                        const uint32_t* vid, float* verts, uint32_t* keys)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 4u * cur.n) return;
+    const uint32_t c = i >> 2, vi = i & 3u;
+    const uint32_t slot = 4u * cur.cell_base + i;
+    if (!vslot[slot]) return;
+    const uint32_t id = vid[slot] + 1u;
+    const LeafRec& l = cur.leaf[c];
+    for (int q = 0; q < 3; ++q) verts[3 * id + q] = float(l.vert[vi][q]);
+    if (keys) {
+        for (int q = 0; q < 3; ++q) keys[5 * id + q] = cur.key[3 * c + q];
+        keys[5 * id + 3] = uint32_t(level);
+        keys[5 * id + 4] = vi;
+    }
+}
+
+hipError_t launch_verts(const LevelView& cur, int level, const uint32_t* vslot,
+                        const uint32_t* vid, float* verts, uint32_t* keys,
+                        hipStream_t s) {
+    if (cur.n == 0) return hipSuccess;
+    const uint32_t n = 4u * cur.n;
+    hipLaunchKernelGGL(k_verts, dim3((n + 255) / 256), dim3(256), 0, s, cur, level, vslot, vid, verts, keys);
+    return hipGetLastError();
+}
+
+// Exclusive prefix sums: each block scans 1024 values (256 threads x 4)
+// and records its total; the totals are scanned the same way and added
+// back.  No block waits on another, so every launch finishes.
+constexpr uint32_t kScanBlock = 1024;
+
+__global__ __launch_bounds__(256)
+void k_scan_blocks(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                   uint32_t* __restrict__ sums, size_t n)
+{
+    __shared__ uint32_t part[256];
+    const uint32_t t = threadIdx.x;
+    const size_t base = size_t(blockIdx.x) * kScanBlock + 4u * t;
+    uint32_t v[4], tot = 0;
+    for (int k = 0; k < 4; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : 0u;
+        tot += v[k];
+    }
+    part[t] = tot;
+    __syncthreads();
+    for (uint32_t off = 1; off < 256; off <<= 1) {
+        const uint32_t x = (t >= off) ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += x;
+        __syncthreads();
+    }
+    uint32_t run = t ? part[t - 1] : 0u;
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (t == 255) sums[blockIdx.x] = part[255];
+}
+
+__global__ __launch_bounds__(256)
+void k_scan_add(uint32_t* __restrict__ out, const uint32_t* __restrict__ offs, size_t n)
+{
+    const size_t base = size_t(blockIdx.x) * kScanBlock + 4u * threadIdx.x;
+    const uint32_t o = offs[blockIdx.x];
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < n) out[base + k] += o;
+    }
+}
+
+size_t scan_scratch_words(size_t n) {
+    size_t words = 0;
+    while (n > 1) {
+        n = (n + kScanBlock - 1) / kScanBlock;
+        words += 2 * n;
+    }
+    return words + 2;
+}
+
+hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, size_t n,
+                          uint32_t* scratch, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const size_t nb = (n + kScanBlock - 1) / kScanBlock;
+    uint32_t* sums = scratch;
+    uint32_t* offs = scratch + nb;
+    hipLaunchKernelGGL(k_scan_blocks, dim3(unsigned(nb)), dim3(256), 0, s, in, out, sums, n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || nb == 1) return e;
+    if ((e = exclusive_scan(sums, offs, nb, scratch + 2 * nb, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scan_add, dim3(unsigned(nb)), dim3(256), 0, s, out, offs, n);
+    return hipGetLastError();
+}
+
+}   // namespace mesh
+}   // namespace amd
+}   // namespace libfive

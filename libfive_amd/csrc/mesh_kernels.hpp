@@ -1,0 +1,145 @@
+// Device data layout and kernel launchers of the GPU dual-contouring mesher.
+//
+// The octree is stored level by level (level 0 = finest cells).  Each level
+// is a set of SoA arrays indexed by cell; the eight children of an
+// ambiguous cell are contiguous in the next finer level, in child order
+// (bit 0 = upper half along X, bit 1 = Y, bit 2 = Z, as Region::subdivide,
+// libfive/include/libfive/render/brep/region.hpp:89-114).  Cell bounds are
+// not stored: they are recomputed from the integer key by the same chain of
+// double-precision midpoints the reference uses.
+//
+// Pruned tapes live in one clause pool; a tape record names its clauses,
+// its parent tape and the box it was pruned for (Tape, libfive/include/
+// libfive/eval/tape.hpp:28-114).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "tape.hpp"
+
+namespace libfive {
+namespace amd {
+namespace mesh {
+
+enum : uint8_t { EMPTY = 0, FILLED = 1, AMBIGUOUS = 2 };
+
+constexpr uint32_t kNoTape = 0xffffffffu;
+
+struct TapeRec {
+    uint32_t off;        // first clause in the pool
+    uint32_t len;        // clause count (padding excluded)
+    uint32_t parent;     // parent tape id, kNoTape for the base tape
+    uint32_t flags;      // TAPE_* bits
+    float box[6];        // xlo, xhi, ylo, yhi, zlo, zhi of an interval tape
+    uint32_t ref_len;    // clauses that are not COPY / LOADI
+    uint32_t pad;
+};
+enum : uint32_t { TAPE_INTERVAL = 1u, TAPE_TERMINAL = 2u };
+
+// Per-cell dual-contouring data (DCLeaf, libfive/include/libfive/render/
+// brep/dc/dc_tree.hpp:36-100); the QEF sums are those of the last vertex
+// solved.
+struct LeafRec {
+    double vert[4][3];
+    double mass[4];
+    double ata[6];       // xx, xy, xz, yy, yz, zz
+    double atb[3];
+    double btb;
+    uint32_t rank;
+    uint32_t vcount;
+    uint32_t mask;       // corner mask (bit i = corner i inside)
+    uint32_t flags;      // bit 0 manifold; bits 8-15 level of the vertex
+};
+
+struct LevelView {
+    uint32_t* key;       // x, y, z of the lower corner in finest-cell units
+    uint8_t* type;
+    int32_t* child;      // first child in the finer level, -1 if not a branch
+    uint32_t* tape;      // tape of the cell's children (or of its leaf eval)
+    LeafRec* leaf;       // valid when the cell is an ambiguous leaf
+    uint32_t n;
+    uint32_t cell_base;  // first global cell number of this level
+};
+
+struct Counters {
+    uint32_t pool_top;
+    uint32_t tape_count;
+    uint32_t unused;
+    uint32_t error;
+    uint32_t max_len;    // longest tape of an ambiguous cell of the last level
+    uint32_t pad[3];
+};
+enum : uint32_t {
+    ERR_POOL = 1u,       // clause pool exhausted
+    ERR_TAPES = 2u,      // tape records exhausted
+    ERR_FEATURE = 4u,    // feature evaluation needed but capacity exceeded
+    ERR_LOCATE = 8u,     // point location inconsistency
+};
+
+struct Params {
+    double lo[3], hi[3];
+    int top;                     // level of the root cell
+    double max_err;
+    tape::Clause* pool;
+    uint32_t pool_cap;
+    TapeRec* tapes;
+    uint32_t tape_cap;
+    uint32_t nregs;
+    uint32_t root_reg;
+    uint32_t slot_x, slot_y, slot_z;
+    Counters* counters;
+    const LevelView* levels;     // device copy, indexed by level
+    uint8_t* feature_scratch;
+    uint32_t feature_scratch_bytes;
+};
+
+hipError_t upload_tables(hipStream_t s);
+
+// Interval classification and pruning of the cells of one level: one wave
+// per parent (8 children), or the root alone when groups is null.
+hipError_t launch_interval(const Params& p, int level, const uint32_t* groups,
+                           uint32_t ngroups, const LevelView& parent,
+                           const LevelView& cur, uint32_t max_len,
+                           hipStream_t s);
+
+// Links an evaluated level to the next: child offsets and the list of
+// ambiguous cells (8 children each).  base = exclusive scan of flags.
+hipError_t launch_flags(const LevelView& cur, uint32_t* flags, hipStream_t s);
+hipError_t launch_link(const LevelView& cur, const uint32_t* base,
+                       uint32_t* groups, hipStream_t s);
+
+// Leaf cells (level 0): corner states, vertex patches, edge search, QEF.
+// groups lists cells of l1 whose per_group (8, or 1 for a root leaf)
+// children in l0 are evaluated, one wave per child.
+hipError_t launch_leaf(const Params& p, const uint32_t* groups,
+                       uint32_t ngroups, uint32_t per_group,
+                       const LevelView& l1, const LevelView& l0,
+                       hipStream_t s);
+
+// Bottom-up collapse of one level.
+hipError_t launch_collapse(const Params& p, int level, const LevelView& cur,
+                           const LevelView& below, hipStream_t s);
+
+// Quad emission over the ambiguous leaves of one level.  Pass 0 counts
+// triangles per cell and marks used vertex slots; pass 1 writes them.
+hipError_t launch_quads(const Params& p, const LevelView& cur, int pass,
+                        uint32_t* tri_count, uint32_t* vslot,
+                        const uint32_t* tri_off, const uint32_t* vid,
+                        uint32_t* tris, hipStream_t s);
+// Vertex positions (and keys: x, y, z, level, patch) of the used slots.
+hipError_t launch_verts(const LevelView& cur, int level, const uint32_t* vslot,
+                        const uint32_t* vid, float* verts, uint32_t* keys,
+                        hipStream_t s);
+
+// Exclusive prefix sum of n uint32 values (device-wide), using
+// scan_scratch_words(n) words of device scratch.
+size_t scan_scratch_words(size_t n);
+hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, size_t n,
+                          uint32_t* scratch, hipStream_t s);
+
+}   // namespace mesh
+}   // namespace amd
+}   // namespace libfive

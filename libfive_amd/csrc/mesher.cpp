@@ -1,17 +1,391 @@
+// GPU dual-contouring mesher: host orchestration of the level-synchronous
+// passes in mesh_kernels.hip.
+//
+//   build     root interval pass, then per level: link the ambiguous cells
+//             (prefix sum), interval-evaluate and prune their children; the
+//             finest level goes to the leaf kernel
+//             (WorkerPool::build/run, worker_pool.inl:36-257)
+//   collapse  levels 1 .. root, bottom-up (collectChildren,
+//             dc_tree.inl:581-735)
+//   emit      quads per level (count, prefix sums, write), vertex gather,
+//             copy to the host (Dual<3>::walk + DCMesher,
+//             dual.hpp:127-203, dc_mesher.cpp:19-169)
 #include "mesher.hpp"
 
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <thread>
+
+#include "deck.hpp"
+#include "mesh_kernels.hpp"
 #include "runtime.hpp"
 
 namespace libfive {
 namespace amd {
 
+using namespace mesh;
+
 static thread_local MeshStats g_stats;
 
 const MeshStats& last_mesh_stats() { return g_stats; }
 
-bool render_mesh(const Tree&, const double*, const double*,
-                 const MeshSettings&, HostMesh&) {
-    throw Error("render_mesh: the GPU mesher is not built yet");
+int resolution_level(const double lo[3], const double hi[3], double min_feature) {
+    double md = hi[0] - lo[0];
+    md = std::min(md, hi[1] - lo[1]);
+    md = std::min(md, hi[2] - lo[2]);
+    const float r = logf(float(md / min_feature));
+    const float m = fmaxf(0.0f, r);
+    return int(std::ceil(m / logf(2.0f)));
+}
+
+namespace {
+
+struct LevelStore {
+    DevBuf<uint32_t> key;
+    DevBuf<uint8_t> type;
+    DevBuf<int32_t> child;
+    DevBuf<uint32_t> tape;
+    DevBuf<LeafRec> leaf;
+    uint32_t n = 0;
+
+    void alloc(uint32_t count) {
+        n = count;
+        key.alloc(3 * size_t(count));
+        type.alloc(count);
+        child.alloc(count);
+        tape.alloc(count);
+        leaf.alloc(count);
+    }
+    LevelView view(uint32_t base) {
+        LevelView v;
+        v.key = key.p;
+        v.type = type.p;
+        v.child = child.p;
+        v.tape = tape.p;
+        v.leaf = leaf.p;
+        v.n = n;
+        v.cell_base = base;
+        return v;
+    }
+};
+
+void wait(hipStream_t s, const char* what);
+
+template <typename T>
+void grow(DevBuf<T>& b, size_t used, size_t need, hipStream_t s) {
+    if (need <= b.n) return;
+    DevBuf<T> nb(std::max(need, 2 * b.n));
+    if (used) {
+        hip_check(hipMemcpyAsync(nb.p, b.p, used * sizeof(T),
+                                 hipMemcpyDeviceToDevice, s), "grow copy");
+    }
+    wait(s, "buffer growth");
+    std::swap(b.p, nb.p);
+    std::swap(b.n, nb.n);
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(
+        std::chrono::steady_clock::now() - t0).count();
+}
+
+// Waits for the stream with a time limit, so that a GPU step that does not
+// finish is reported by name instead of blocking the caller forever.
+void wait(hipStream_t s, const char* what) {
+    static const bool trace = std::getenv("LIBFIVE_MESH_TRACE") != nullptr;
+    if (trace) {
+        std::fprintf(stderr, "render_mesh: waiting for %s\n", what);
+        std::fflush(stderr);
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        const hipError_t e = hipStreamQuery(s);
+        if (e == hipSuccess) return;
+        if (e != hipErrorNotReady) hip_check(e, what);
+        if (ms_since(t0) > 60000.0) {
+            throw Error(std::string("render_mesh: GPU step '") + what +
+                        "' did not finish within 60 s");
+        }
+        std::this_thread::sleep_for(std::chrono::microseconds(50));
+    }
+}
+
+void check_errors(uint32_t err) {
+    if (err & ERR_FEATURE) {
+        throw Error("render_mesh: a sample point lies exactly on a min/max "
+                    "feature (value 0 with tied branches), which needs the "
+                    "feature evaluator");
+    }
+    if (err & (ERR_POOL | ERR_TAPES)) {
+        throw Error("render_mesh: tape storage exhausted");
+    }
+    if (err) throw Error("render_mesh: internal error " + std::to_string(err));
+}
+
+}   // namespace
+
+bool render_mesh(const Tree& tree, const double lo[3], const double hi[3],
+                 const MeshSettings& settings, HostMesh& out) {
+    const auto t_start = std::chrono::steady_clock::now();
+    Context& ctx = context();
+    std::lock_guard<std::mutex> lock(ctx.mu);
+    hipStream_t s = ctx.stream;
+    g_stats = MeshStats();
+
+    static bool tables_ready = false;
+    if (!tables_ready) {
+        hip_check(upload_tables(s), "mesh tables");
+        tables_ready = true;
+    }
+
+    const Deck deck(tree);
+    const DeviceTape base = compile_deck(deck, nullptr);
+    const uint32_t nregs = base.num_regs;
+    if (nregs > 128) {
+        throw Error("render_mesh: the tape needs " + std::to_string(nregs) +
+                    " registers; the mesher supports 128");
+    }
+    const int top = resolution_level(lo, hi, settings.min_feature);
+    if (top > 20) throw Error("render_mesh: resolution too fine (more than 20 levels)");
+    g_stats.top = top;
+
+    // Clause pool and tape records; the base tape is record 0.
+    const uint32_t nbase = uint32_t(base.clauses.size());
+    const uint32_t base_padded = tape::padded_length(nbase);
+    DevBuf<tape::Clause> pool(std::max<size_t>(size_t(base_padded) * 64, 1u << 20));
+    {
+        std::vector<tape::Clause> h(base_padded, tape::Clause{0, 0, 0, 0});
+        std::copy(base.clauses.begin(), base.clauses.end(), h.begin());
+        hip_check(hipMemcpyAsync(pool.p, h.data(), h.size() * sizeof(tape::Clause),
+                                 hipMemcpyHostToDevice, s), "upload tape");
+    }
+    DevBuf<TapeRec> tapes(1u << 16);
+    {
+        TapeRec r{};
+        r.off = 0;
+        r.len = nbase;
+        r.parent = kNoTape;
+        r.flags = 0;
+        r.ref_len = base.ref_length;
+        hip_check(hipMemcpyAsync(tapes.p, &r, sizeof(r), hipMemcpyHostToDevice, s),
+                  "upload tape record");
+    }
+    DevBuf<Counters> counters(1);
+    Counters hc{};
+    hc.pool_top = base_padded;
+    hc.tape_count = 1;
+    hip_check(hipMemcpyAsync(counters.p, &hc, sizeof(hc), hipMemcpyHostToDevice, s),
+              "counters");
+
+    Params P{};
+    for (int a = 0; a < 3; ++a) {
+        P.lo[a] = lo[a];
+        P.hi[a] = hi[a];
+    }
+    P.top = top;
+    P.max_err = settings.max_err;
+    P.nregs = std::max<uint32_t>(nregs, tape::kFirstFreeReg);
+    P.root_reg = base.root_reg;
+    P.slot_x = deck.X;
+    P.slot_y = deck.Y;
+    P.slot_z = deck.Z;
+    P.counters = counters.p;
+    auto refresh = [&]() {
+        P.pool = pool.p;
+        P.pool_cap = uint32_t(std::min<size_t>(pool.n, 0xffffffffu));
+        P.tapes = tapes.p;
+        P.tape_cap = uint32_t(tapes.n);
+    };
+    refresh();
+
+    std::vector<LevelStore> levels(top + 1);
+    std::vector<DevBuf<uint32_t>> groups(top + 2);
+    // Per-level scan buffers stay allocated until the render ends, so that
+    // no device memory is released while kernels are in flight.
+    std::vector<DevBuf<uint32_t>> flags(top + 1), basei(top + 1), scan_tmp(top + 1);
+
+    // ---- build ------------------------------------------------------------
+    levels[top].alloc(1);
+    if (top == 0) {
+        // The root is a leaf: evaluate it directly with the base tape.
+        LevelStore virt;
+        virt.alloc(1);
+        const uint32_t zero_key[3] = {0, 0, 0};
+        const int32_t child0 = 0;
+        const uint32_t tape0 = 0;
+        hip_check(hipMemcpyAsync(virt.key.p, zero_key, sizeof(zero_key), hipMemcpyHostToDevice, s), "root");
+        hip_check(hipMemcpyAsync(virt.child.p, &child0, sizeof(child0), hipMemcpyHostToDevice, s), "root");
+        hip_check(hipMemcpyAsync(virt.tape.p, &tape0, sizeof(tape0), hipMemcpyHostToDevice, s), "root");
+        groups[1].alloc(1);
+        hip_check(hipMemsetAsync(groups[1].p, 0, sizeof(uint32_t), s), "root");
+        hip_check(launch_leaf(P, groups[1].p, 1, 1, virt.view(0), levels[0].view(0), s),
+                  "leaf kernel");
+        hip_check(hipMemcpyAsync(&hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost, s), "counters");
+        wait(s, "leaf kernel");
+        check_errors(hc.error);
+    } else {
+        hip_check(hipMemsetAsync(&counters.p->max_len, 0, sizeof(uint32_t), s), "max_len");
+        hip_check(launch_interval(P, top, nullptr, 1, levels[top].view(0),
+                                  levels[top].view(0), nbase, s), "interval kernel");
+        for (int L = top; L >= 1; --L) {
+            LevelStore& cur = levels[L];
+            flags[L].alloc(cur.n);
+            basei[L].alloc(cur.n);
+            scan_tmp[L].alloc(scan_scratch_words(cur.n));
+            groups[L].alloc(cur.n);
+            hip_check(launch_flags(cur.view(0), flags[L].p, s), "flags");
+            hip_check(exclusive_scan(flags[L].p, basei[L].p, cur.n, scan_tmp[L].p, s), "scan");
+            hip_check(launch_link(cur.view(0), basei[L].p, groups[L].p, s), "link");
+            uint32_t last[2] = {0, 0};
+            hip_check(hipMemcpyAsync(&last[0], basei[L].p + cur.n - 1, 4, hipMemcpyDeviceToHost, s), "count");
+            hip_check(hipMemcpyAsync(&last[1], flags[L].p + cur.n - 1, 4, hipMemcpyDeviceToHost, s), "count");
+            hip_check(hipMemcpyAsync(&hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost, s), "counters");
+            wait(s, L == top ? "root interval pass" : "interval pass");
+            check_errors(hc.error);
+            const uint64_t namb = uint64_t(last[0]) + last[1];
+            if (namb > cur.n) throw Error("render_mesh: inconsistent cell count");
+            if (namb == 0) break;
+            if (namb * 8u > 0xffffffffull / 4) throw Error("render_mesh: too many cells");
+            levels[L - 1].alloc(uint32_t(namb * 8u));
+            if (L - 1 > 0) {
+                const uint32_t max_len = hc.max_len;
+                if (max_len == 0 || max_len > hc.pool_top) {
+                    throw Error("render_mesh: inconsistent tape length");
+                }
+                grow(pool, hc.pool_top, size_t(hc.pool_top) + size_t(namb) * 8u * (max_len + 8u), s);
+                grow(tapes, hc.tape_count, size_t(hc.tape_count) + size_t(namb) * 8u, s);
+                refresh();
+                hip_check(hipMemsetAsync(&counters.p->max_len, 0, sizeof(uint32_t), s), "max_len");
+                hip_check(launch_interval(P, L - 1, groups[L].p, uint32_t(namb), cur.view(0),
+                                          levels[L - 1].view(0), max_len, s),
+                          "interval kernel");
+            } else {
+                hip_check(launch_leaf(P, groups[1].p, uint32_t(namb), 8, cur.view(0),
+                                      levels[0].view(0), s), "leaf kernel");
+                hip_check(hipMemcpyAsync(&hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost, s), "counters");
+                wait(s, "leaf kernel");
+                check_errors(hc.error);
+            }
+            if (settings.cancel && *settings.cancel) {
+                wait(s, "cancel");
+                return false;
+            }
+        }
+    }
+
+    // Global cell numbering (level 0 first).
+    std::vector<uint32_t> cell_base(top + 1, 0);
+    uint64_t total_cells = 0;
+    for (int L = 0; L <= top; ++L) {
+        cell_base[L] = uint32_t(total_cells);
+        total_cells += levels[L].n;
+    }
+    if (total_cells * 4 > 0xffffffffull) throw Error("render_mesh: too many cells");
+
+    if (settings.collect_stats) {
+        // Per-level state counts and pruned tape lengths, before collapse.
+        std::vector<TapeRec> trs(hc.tape_count);
+        hip_check(hipMemcpyAsync(trs.data(), tapes.p, trs.size() * sizeof(TapeRec),
+                                 hipMemcpyDeviceToHost, s), "stats");
+        for (int L = top; L >= 0; --L) {
+            std::vector<uint8_t> ty(levels[L].n);
+            std::vector<uint32_t> tp(levels[L].n);
+            if (levels[L].n) {
+                hip_check(hipMemcpyAsync(ty.data(), levels[L].type.p, ty.size(),
+                                         hipMemcpyDeviceToHost, s), "stats");
+                hip_check(hipMemcpyAsync(tp.data(), levels[L].tape.p, tp.size() * 4,
+                                         hipMemcpyDeviceToHost, s), "stats");
+            }
+            wait(s, "statistics copy");
+            uint64_t cnt[3] = {0, 0, 0};
+            for (size_t i = 0; i < ty.size(); ++i) {
+                if (ty[i] > AMBIGUOUS || tp[i] >= trs.size()) {
+                    throw Error("render_mesh: inconsistent cell record");
+                }
+                cnt[ty[i]]++;
+                if (L > 0 && ty[i] == AMBIGUOUS) {
+                    g_stats.tape_lengths.push_back(trs[tp[i]].ref_len);
+                }
+            }
+            for (int k = 0; k < 3; ++k) g_stats.level_counts.push_back(cnt[k]);
+            if (L == 0) g_stats.leaves = cnt[AMBIGUOUS];
+        }
+    }
+    g_stats.ms_build = ms_since(t_start);
+
+    // ---- collapse -----------------------------------------------------------
+    const auto t_collapse = std::chrono::steady_clock::now();
+    std::vector<LevelView> views(top + 1);
+    for (int L = 0; L <= top; ++L) views[L] = levels[L].view(cell_base[L]);
+    for (int L = 1; L <= top; ++L) {
+        if (levels[L].n && levels[L - 1].n) {
+            hip_check(launch_collapse(P, L, views[L], views[L - 1], s), "collapse kernel");
+        }
+    }
+    wait(s, "collapse");
+    g_stats.ms_collapse = ms_since(t_collapse);
+
+    // ---- emit -----------------------------------------------------------------
+    const auto t_emit = std::chrono::steady_clock::now();
+    DevBuf<LevelView> d_views(top + 1);
+    hip_check(hipMemcpyAsync(d_views.p, views.data(), views.size() * sizeof(LevelView),
+                             hipMemcpyHostToDevice, s), "views");
+    P.levels = d_views.p;
+    DevBuf<uint32_t> tri_count(total_cells), tri_off(total_cells);
+    DevBuf<uint32_t> vslot(4 * total_cells), vid(4 * total_cells);
+    DevBuf<uint32_t> scan_a(scan_scratch_words(total_cells));
+    DevBuf<uint32_t> scan_b(scan_scratch_words(4 * total_cells));
+    hip_check(hipMemsetAsync(vslot.p, 0, vslot.n * sizeof(uint32_t), s), "vslot");
+    for (int L = 0; L <= top; ++L) {
+        hip_check(launch_quads(P, views[L], 0, tri_count.p, vslot.p, nullptr, nullptr,
+                               nullptr, s), "quad count");
+    }
+    hip_check(exclusive_scan(tri_count.p, tri_off.p, total_cells, scan_a.p, s), "scan");
+    hip_check(exclusive_scan(vslot.p, vid.p, vslot.n, scan_b.p, s), "scan");
+    uint32_t tail[4] = {0, 0, 0, 0};
+    hip_check(hipMemcpyAsync(&tail[0], tri_off.p + total_cells - 1, 4, hipMemcpyDeviceToHost, s), "tail");
+    hip_check(hipMemcpyAsync(&tail[1], tri_count.p + total_cells - 1, 4, hipMemcpyDeviceToHost, s), "tail");
+    hip_check(hipMemcpyAsync(&tail[2], vid.p + vslot.n - 1, 4, hipMemcpyDeviceToHost, s), "tail");
+    hip_check(hipMemcpyAsync(&tail[3], vslot.p + vslot.n - 1, 4, hipMemcpyDeviceToHost, s), "tail");
+    hip_check(hipMemcpyAsync(&hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost, s), "counters");
+    wait(s, "quad count");
+    check_errors(hc.error);
+    const uint64_t ntris64 = uint64_t(tail[0]) + tail[1];
+    const uint64_t nverts64 = uint64_t(tail[2]) + tail[3] + 1;    // with the dummy vertex
+    if (ntris64 > 24 * total_cells || nverts64 > 4 * total_cells + 1) {
+        throw Error("render_mesh: inconsistent mesh size");
+    }
+    const uint32_t ntris = uint32_t(ntris64), nverts = uint32_t(nverts64);
+    DevBuf<uint32_t> d_tris(3 * size_t(std::max<uint32_t>(ntris, 1)));
+    DevBuf<float> d_verts(3 * size_t(nverts));
+    DevBuf<uint32_t> d_keys(5 * size_t(nverts));
+    hip_check(hipMemsetAsync(d_verts.p, 0, 3 * sizeof(float), s), "dummy");
+    hip_check(hipMemsetAsync(d_keys.p, 0, 5 * sizeof(uint32_t), s), "dummy");
+    for (int L = 0; L <= top; ++L) {
+        hip_check(launch_quads(P, views[L], 1, tri_count.p, vslot.p, tri_off.p, vid.p,
+                               d_tris.p, s), "quad write");
+        hip_check(launch_verts(views[L], L, vslot.p, vid.p, d_verts.p, d_keys.p, s),
+                  "vertex gather");
+    }
+    out.verts.resize(3 * size_t(nverts));
+    out.tris.resize(3 * size_t(ntris));
+    out.keys.resize(5 * size_t(nverts));
+    hip_check(hipMemcpyAsync(out.verts.data(), d_verts.p, out.verts.size() * sizeof(float),
+                             hipMemcpyDeviceToHost, s), "copy verts");
+    if (ntris) {
+        hip_check(hipMemcpyAsync(out.tris.data(), d_tris.p, out.tris.size() * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, s), "copy tris");
+    }
+    hip_check(hipMemcpyAsync(out.keys.data(), d_keys.p, out.keys.size() * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, s), "copy keys");
+    wait(s, "mesh copy");
+    g_stats.ms_emit = ms_since(t_emit);
+    g_stats.ms_total = ms_since(t_start);
+    return true;
 }
 
 }   // namespace amd

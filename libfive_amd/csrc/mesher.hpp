@@ -1,5 +1,6 @@
 // Dual-contouring mesher entry (Mesh::render, libfive/src/render/brep/
-// mesh.cpp:35-125, DC branch 53-72).
+// mesh.cpp:35-125, DC branch 53-72): builds the octree, collapses it and
+// emits triangles on the GPU, then copies the mesh to the host.
 #pragma once
 
 #include <cstdint>
@@ -14,24 +15,27 @@ struct MeshSettings {
     double min_feature = 0.1;     // settings.hpp:34-43 defaults
     double max_err = 1e-8;
     uint32_t precision = 0;       // 0: FP64 QEF sums, 1: compensated FP32
+    bool collect_stats = false;   // per-level cell counts and tape lengths
     volatile const int* cancel = nullptr;
 };
 
 // Flat output: verts as xyz floats with the dummy vertex 0 first, tris as
-// index triples.
+// index triples, keys as 5 words per vertex (cell x, y, z in finest-cell
+// units, cell level, vertex patch).
 struct HostMesh {
     std::vector<float> verts;
     std::vector<uint32_t> tris;
+    std::vector<uint32_t> keys;
 };
 
-// Per-phase statistics of the last render on this thread.
+// Statistics of the last render on this thread.
 struct MeshStats {
-    uint32_t levels = 0;
-    std::vector<uint32_t> cells_per_level;        // cells evaluated per level
-    std::vector<uint32_t> ambiguous_per_level;
-    uint32_t leaves = 0;                          // ambiguous leaf cells
-    uint32_t verts = 0, tris = 0;
-    double ms_total = 0;
+    int top = 0;                                  // root level
+    std::vector<uint64_t> level_counts;           // root first: empty, filled, ambiguous
+    std::vector<uint32_t> tape_lengths;           // per ambiguous cell above level 0
+    uint64_t leaves = 0;                          // ambiguous leaf cells
+    uint64_t collapsed = 0;
+    double ms_build = 0, ms_collapse = 0, ms_emit = 0, ms_total = 0;
 };
 
 // Returns false when cancelled (the C API then returns NULL).
@@ -39,6 +43,9 @@ bool render_mesh(const Tree& t, const double lo[3], const double hi[3],
                  const MeshSettings& s, HostMesh& out);
 
 const MeshStats& last_mesh_stats();
+
+// Region::withResolution (region.hpp:226-231), evaluated in float.
+int resolution_level(const double lo[3], const double hi[3], double min_feature);
 
 }   // namespace amd
 }   // namespace libfive

@@ -1,0 +1,278 @@
+// Wave-level tape interpreters for the mesher: every lane evaluates its own
+// point with the same (wave-uniform) tape, as the point kernels do, and
+// additionally reports, per lane, whether some MIN/MAX clause saw
+//   - equal operands (bit 0: ArrayEvaluator::getAmbiguous,
+//     libfive/src/eval/eval_array.cpp:198-217), or
+//   - operands of which neither is strictly greater, NaN included (bit 1:
+//     the clauses a SPECIALIZED push keeps, eval_array.cpp:121-170).
+// Device-only header; include from .hip files compiled with
+// -ffp-contract=off.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fmath.hpp"
+#include "opcode.hpp"
+#include "regfile.hpp"
+#include "tape.hpp"
+
+namespace libfive {
+namespace amd {
+
+enum : uint32_t { PT_AMBIGUOUS = 1u, PT_TIE = 2u };
+
+struct Clause4x {
+    tape::Clause c[tape::kFetch];
+};
+
+__device__ __forceinline__ float pe_value_op(uint32_t o, float a, float b,
+                                             float im) {
+    namespace OC = Opcode;
+    switch (o) {
+        case OC::OP_ADD: return a + b;
+        case OC::OP_MUL: return a * b;
+        case OC::OP_MIN: return (b < a) ? b : a;
+        case OC::OP_MAX: return (a < b) ? b : a;
+        case OC::OP_SUB: return a - b;
+        case OC::OP_NEG: return -a;
+        case OC::OP_SQUARE: return a * a;
+        case OC::OP_ABS: return fabsf(a);
+        case OC::OP_SQRT: return sqrtf(a);
+        case OC::OP_SIN: return fmath::sin_f32(a);
+        case OC::OP_COS: return fmath::cos_f32(a);
+        case OC::OP_DIV: return a / b;
+        case tape::OP_LOADI: return im;
+        case tape::OP_COPY:
+        case OC::CONST_VAR: return a;
+        default: return fmath::eval_op(int(o), a, b);
+    }
+}
+
+__device__ __forceinline__ uint32_t pe_minmax_flags(uint32_t o, float a,
+                                                    float b) {
+    if (o != Opcode::OP_MIN && o != Opcode::OP_MAX) return 0u;
+    return ((a == b) ? PT_AMBIGUOUS : 0u) | ((!(a > b) && !(b > a)) ? PT_TIE : 0u);
+}
+
+#define LF_PE_VALUE_STEP(C)                                                  \
+    do {                                                                     \
+        const tape::Clause c_ = (C);                                         \
+        const uint32_t o_ = tape::op(c_);                                    \
+        if (o_ != tape::OP_NOP) {                                            \
+            const uint32_t m_ = tape::mode(c_);                              \
+            const float im_ = tape::imm(c_);                                 \
+            const float a_ = (m_ == tape::MODE_A_IMM || o_ == tape::OP_LOADI) \
+                ? im_ : LF_RF_GET(r, NV, tape::ra(c_));                      \
+            float b_ = 0.0f;                                                 \
+            if (tape::is_binary(c_)) {                                       \
+                b_ = (m_ == tape::MODE_B_IMM) ? im_                          \
+                                              : LF_RF_GET(r, NV, tape::rb(c_)); \
+            }                                                                \
+            flags |= pe_minmax_flags(o_, a_, b_);                            \
+            LF_RF_SET(r, NV, tape::dst(c_), pe_value_op(o_, a_, b_, im_));   \
+        }                                                                    \
+    } while (0)
+
+// Value of the tape at (x, y, z); *flags gets PT_* bits.
+template <int NV>
+__device__ __forceinline__ float pe_value(const tape::Clause* __restrict__ t,
+                                          uint32_t n, uint32_t root, float x,
+                                          float y, float z, uint32_t* out_flags)
+{
+    LF_RF_DECL(r);
+    LF_RF_SET(r, NV, tape::kRegX, x);
+    LF_RF_SET(r, NV, tape::kRegY, y);
+    LF_RF_SET(r, NV, tape::kRegZ, z);
+    uint32_t flags = 0;
+    const Clause4x* t4 = reinterpret_cast<const Clause4x*>(t);
+    for (uint32_t k = 0; k < n; k += tape::kFetch) {
+        const Clause4x cur = t4[k / tape::kFetch];
+        LF_PE_VALUE_STEP(cur.c[0]);
+        LF_PE_VALUE_STEP(cur.c[1]);
+        LF_PE_VALUE_STEP(cur.c[2]);
+        LF_PE_VALUE_STEP(cur.c[3]);
+    }
+    *out_flags = flags;
+    return LF_RF_GET(r, NV, root);
+}
+
+// Value and partial derivatives of one clause (the rules of
+// DerivArrayEvaluator, libfive/src/eval/eval_deriv_array.cpp:119-256).
+__device__ __noinline__ void pe_cold_deriv(uint32_t o, float av, float ad0,
+                                           float ad1, float ad2, float bv,
+                                           float bd0, float bd1, float bd2,
+                                           float* res);
+
+__device__ __forceinline__ void pe_deriv_op(uint32_t o, float av,
+                                            const float ad[3], float bv,
+                                            const float bd[3], float im,
+                                            float& ov, float od[3]) {
+    namespace OC = Opcode;
+    switch (o) {
+        case OC::OP_ADD:
+            ov = av + bv;
+            for (int j = 0; j < 3; ++j) od[j] = ad[j] + bd[j];
+            break;
+        case OC::OP_SUB:
+            ov = av - bv;
+            for (int j = 0; j < 3; ++j) od[j] = ad[j] - bd[j];
+            break;
+        case OC::OP_MUL:
+            ov = av * bv;
+            for (int j = 0; j < 3; ++j) {
+                const float p = bd[j] * av;
+                const float q = ad[j] * bv;
+                od[j] = p + q;
+            }
+            break;
+        case OC::OP_MIN: {
+            ov = (bv < av) ? bv : av;
+            const bool s = av < bv;
+            for (int j = 0; j < 3; ++j) od[j] = s ? ad[j] : bd[j];
+            break;
+        }
+        case OC::OP_MAX: {
+            ov = (av < bv) ? bv : av;
+            const bool s = av < bv;
+            for (int j = 0; j < 3; ++j) od[j] = s ? bd[j] : ad[j];
+            break;
+        }
+        case OC::OP_DIV: {
+            ov = av / bv;
+            const float b2 = bv * bv;
+            for (int j = 0; j < 3; ++j) {
+                const float p = ad[j] * bv;
+                const float q = bd[j] * av;
+                od[j] = (p - q) / b2;
+            }
+            break;
+        }
+        case OC::OP_SQUARE:
+            ov = av * av;
+            for (int j = 0; j < 3; ++j) od[j] = (ad[j] * av) * 2.0f;
+            break;
+        case OC::OP_SQRT: {
+            ov = sqrtf(av);
+            const float den = 2.0f * ov;
+            for (int j = 0; j < 3; ++j) {
+                od[j] = (av < 0.0f || ad[j] == 0.0f) ? 0.0f : ad[j] / den;
+            }
+            break;
+        }
+        case OC::OP_NEG:
+            ov = -av;
+            for (int j = 0; j < 3; ++j) od[j] = -ad[j];
+            break;
+        case OC::OP_ABS:
+            ov = fabsf(av);
+            for (int j = 0; j < 3; ++j) od[j] = (av > 0.0f) ? ad[j] : -ad[j];
+            break;
+        case OC::OP_SIN: {
+            float s, co;
+            fmath::sin_cos_f32(av, &s, &co);
+            ov = s;
+            for (int j = 0; j < 3; ++j) od[j] = ad[j] * co;
+            break;
+        }
+        case OC::OP_COS: {
+            float s, co;
+            fmath::sin_cos_f32(av, &s, &co);
+            ov = co;
+            const float ns = -s;
+            for (int j = 0; j < 3; ++j) od[j] = ad[j] * ns;
+            break;
+        }
+        case tape::OP_LOADI:
+            ov = im;
+            for (int j = 0; j < 3; ++j) od[j] = 0.0f;
+            break;
+        case tape::OP_COPY:
+        case OC::CONST_VAR:
+            ov = av;
+            for (int j = 0; j < 3; ++j) od[j] = ad[j];
+            break;
+        default: {
+            float res[4];
+            pe_cold_deriv(o, av, ad[0], ad[1], ad[2], bv, bd[0], bd[1], bd[2], res);
+            ov = res[0];
+            od[0] = res[1];
+            od[1] = res[2];
+            od[2] = res[3];
+            break;
+        }
+    }
+}
+
+#define LF_PE_DERIV_STEP(C)                                                  \
+    do {                                                                     \
+        const tape::Clause c_ = (C);                                         \
+        const uint32_t o_ = tape::op(c_);                                    \
+        if (o_ != tape::OP_NOP) {                                            \
+            const uint32_t m_ = tape::mode(c_);                              \
+            const float im_ = tape::imm(c_);                                 \
+            float av_ = im_, ad_[3] = {0.0f, 0.0f, 0.0f};                    \
+            float bv_ = 0.0f, bd_[3] = {0.0f, 0.0f, 0.0f};                   \
+            if (m_ != tape::MODE_A_IMM && o_ != tape::OP_LOADI) {            \
+                const uint32_t a_ = tape::ra(c_);                            \
+                av_ = LF_RF_GET(v, NV, a_);                                  \
+                ad_[0] = LF_RF_GET(gx, NV, a_);                              \
+                ad_[1] = LF_RF_GET(gy, NV, a_);                              \
+                ad_[2] = LF_RF_GET(gz, NV, a_);                              \
+            }                                                                \
+            if (tape::is_binary(c_)) {                                       \
+                if (m_ == tape::MODE_B_IMM) {                                \
+                    bv_ = im_;                                               \
+                } else {                                                     \
+                    const uint32_t b_ = tape::rb(c_);                        \
+                    bv_ = LF_RF_GET(v, NV, b_);                              \
+                    bd_[0] = LF_RF_GET(gx, NV, b_);                          \
+                    bd_[1] = LF_RF_GET(gy, NV, b_);                          \
+                    bd_[2] = LF_RF_GET(gz, NV, b_);                          \
+                }                                                            \
+            }                                                                \
+            flags |= pe_minmax_flags(o_, av_, bv_);                          \
+            float ov_, od_[3];                                               \
+            pe_deriv_op(o_, av_, ad_, bv_, bd_, im_, ov_, od_);              \
+            const uint32_t d_ = tape::dst(c_);                               \
+            LF_RF_SET(v, NV, d_, ov_);                                       \
+            LF_RF_SET(gx, NV, d_, od_[0]);                                   \
+            LF_RF_SET(gy, NV, d_, od_[1]);                                   \
+            LF_RF_SET(gz, NV, d_, od_[2]);                                   \
+        }                                                                    \
+    } while (0)
+
+// Value and gradient of the tape at (x, y, z).
+template <int NV>
+__device__ __forceinline__ float pe_deriv(const tape::Clause* __restrict__ t,
+                                          uint32_t n, uint32_t root, float x,
+                                          float y, float z, float g[3],
+                                          uint32_t* out_flags)
+{
+    LF_RF_DECL(v);
+    LF_RF_DECL(gx);
+    LF_RF_DECL(gy);
+    LF_RF_DECL(gz);
+    LF_RF_SET(v, NV, tape::kRegX, x);
+    LF_RF_SET(v, NV, tape::kRegY, y);
+    LF_RF_SET(v, NV, tape::kRegZ, z);
+    LF_RF_SET(gx, NV, tape::kRegX, 1.0f);
+    LF_RF_SET(gy, NV, tape::kRegY, 1.0f);
+    LF_RF_SET(gz, NV, tape::kRegZ, 1.0f);
+    uint32_t flags = 0;
+    const Clause4x* t4 = reinterpret_cast<const Clause4x*>(t);
+    for (uint32_t k = 0; k < n; k += tape::kFetch) {
+        const Clause4x cur = t4[k / tape::kFetch];
+        LF_PE_DERIV_STEP(cur.c[0]);
+        LF_PE_DERIV_STEP(cur.c[1]);
+        LF_PE_DERIV_STEP(cur.c[2]);
+        LF_PE_DERIV_STEP(cur.c[3]);
+    }
+    *out_flags = flags;
+    g[0] = LF_RF_GET(gx, NV, root);
+    g[1] = LF_RF_GET(gy, NV, root);
+    g[2] = LF_RF_GET(gz, NV, root);
+    return LF_RF_GET(v, NV, root);
+}
+
+}   // namespace amd
+}   // namespace libfive

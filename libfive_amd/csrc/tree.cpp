@@ -234,7 +234,7 @@ Tree Tree::unary(Opcode::Opcode op, const Tree& lhs) {
 
 Tree Tree::binary(Opcode::Opcode op, const Tree& lhs, const Tree& rhs) {
     using namespace Opcode;
-    if (Opcode::args(op) != 2) return invalid();
+    if (::libfive::Opcode::args(op) != 2) return invalid();
     const bool lc = is_const(lhs), rc = is_const(rhs);
     if (lc && rc) {
         return Tree(fold_constant(op, lhs->value(), rhs->value()));

@@ -43,6 +43,7 @@ class libfive_mesh_settings(ctypes.Structure):
     _fields_ = [("min_feature", ctypes.c_double),
                 ("max_err", ctypes.c_double),
                 ("precision", ctypes.c_uint32),
+                ("collect_stats", ctypes.c_uint32),
                 ("cancel", ctypes.POINTER(ctypes.c_int))]
 
 
@@ -110,6 +111,12 @@ SIGNATURES = {
     "libfive_tree_tape_info": (ctypes.c_int, [tree_t, u32p, u32p]),
     "libfive_tree_deck": (ctypes.c_int, [tree_t, u32p, szp, u32p, fp, szp,
                                          u32p, u32p]),
+    "libfive_mesh_vertex_keys": (u32p, [ctypes.POINTER(libfive_mesh)]),
+    "libfive_last_mesh_level_counts": (ctypes.c_size_t,
+                                       [ctypes.POINTER(ctypes.c_uint64),
+                                        ctypes.c_size_t]),
+    "libfive_last_mesh_tape_lengths": (ctypes.c_size_t, [u32p, ctypes.c_size_t]),
+    "libfive_last_mesh_times": (None, [ctypes.POINTER(ctypes.c_double)]),
     "libfive_mesh_settings_default": (None,
                                       [ctypes.POINTER(libfive_mesh_settings)]),
     "libfive_tree_render_mesh_settings": (

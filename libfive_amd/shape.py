@@ -192,13 +192,16 @@ class Shape:
 
     # ---- meshing (GPU) --------------------------------------------------
     def render_mesh(self, lo, hi, res=None, min_feature=None, max_err=1e-8,
-                    precision=0):
+                    precision=0, details=False):
+        """Dual-contouring mesh over the box [lo, hi].  Returns (verts,
+        tris); with details=True a MeshResult that also holds per-vertex
+        keys, per-level cell counts, pruned tape lengths and timings."""
         L = ffi.lib()
         R = ffi.libfive_region3(ffi.libfive_interval(lo[0], hi[0]),
                                 ffi.libfive_interval(lo[1], hi[1]),
                                 ffi.libfive_interval(lo[2], hi[2]))
         if res is not None and min_feature is None and max_err == 1e-8 \
-                and precision == 0:
+                and precision == 0 and not details:
             m = L.libfive_tree_render_mesh(self.ptr, R, float(res))
         else:
             s = ffi.libfive_mesh_settings()
@@ -209,17 +212,50 @@ class Shape:
                 s.min_feature = float(np.float32(1) / np.float32(res))
             s.max_err = max_err
             s.precision = precision
+            s.collect_stats = 1 if details else 0
             m = L.libfive_tree_render_mesh_settings(self.ptr, R,
                                                     ctypes.byref(s))
         if not m:
             raise GpuError(ffi.last_error())
-        mm = m.contents
-        verts = np.ctypeslib.as_array(
-            ctypes.cast(mm.verts, ffi.fp), (mm.vert_count, 3)).copy()
-        tris = np.ctypeslib.as_array(
-            ctypes.cast(mm.tris, ffi.u32p), (mm.tri_count, 3)).copy()
-        L.libfive_mesh_delete(m)
-        return verts, tris
+        try:
+            mm = m.contents
+            verts = np.ctypeslib.as_array(
+                ctypes.cast(mm.verts, ffi.fp), (mm.vert_count, 3)).copy()
+            if mm.tri_count:
+                tris = np.ctypeslib.as_array(
+                    ctypes.cast(mm.tris, ffi.u32p), (mm.tri_count, 3)).copy()
+            else:
+                tris = np.zeros((0, 3), np.uint32)
+            if not details:
+                return verts, tris
+            kp = L.libfive_mesh_vertex_keys(m)
+            keys = np.ctypeslib.as_array(kp, (mm.vert_count, 5)).copy()
+        finally:
+            L.libfive_mesh_delete(m)
+        n = L.libfive_last_mesh_level_counts(None, 0)
+        counts = np.zeros(3 * max(n, 1), np.uint64)
+        L.libfive_last_mesh_level_counts(
+            counts.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), n)
+        nt = L.libfive_last_mesh_tape_lengths(None, 0)
+        lens = np.zeros(max(nt, 1), np.uint32)
+        L.libfive_last_mesh_tape_lengths(lens.ctypes.data_as(ffi.u32p), nt)
+        ms = (ctypes.c_double * 4)()
+        L.libfive_last_mesh_times(ms)
+        return MeshResult(verts, tris, keys, counts[:3 * n].reshape(-1, 3),
+                          lens[:nt], list(ms))
+
+
+class MeshResult:
+    """A mesh with its vertex keys (owning cell x, y, z in finest-cell
+    units, level, patch), per-level cell counts (root level first: empty,
+    filled, ambiguous), pruned tape lengths and the render's timings in ms
+    (build, collapse, emit, total)."""
+
+    def __init__(self, verts, tris, keys, level_counts, tape_lengths, ms):
+        self.verts, self.tris, self.keys = verts, tris, keys
+        self.level_counts = level_counts
+        self.tape_lengths = tape_lengths
+        self.ms = ms
 
 
 X = Shape.X

@@ -199,6 +199,11 @@ const uint32_t* libfive_mesh_vertex_keys(const libfive_mesh* m);
  * copy at most cap of them when out is not NULL. */
 size_t libfive_last_mesh_level_counts(uint64_t* out, size_t cap);
 size_t libfive_last_mesh_tape_lengths(uint32_t* out, size_t cap);
+/* Every cell of the last render (with collect_stats) whose state was
+ * evaluated, before collapse: 5 words per cell (x, y, z of the lower corner
+ * in finest-cell units, level, state: 0 empty, 1 filled, 2 ambiguous).
+ * Returns the count; copies at most cap cells when out is not NULL. */
+size_t libfive_last_mesh_cells(uint32_t* out, size_t cap);
 /* Wall-clock milliseconds of the last render: build, collapse, emit, total. */
 void libfive_last_mesh_times(double* ms4);
 

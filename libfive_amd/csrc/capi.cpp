@@ -432,6 +432,13 @@ size_t libfive_last_mesh_tape_lengths(uint32_t* out, size_t cap) {
     return st.tape_lengths.size();
 }
 
+size_t libfive_last_mesh_cells(uint32_t* out, size_t cap) {
+    const auto& st = last_mesh_stats();
+    const size_t n = st.cells.size() / 5;
+    for (size_t i = 0; out && i < 5 * std::min(n, cap); ++i) out[i] = st.cells[i];
+    return n;
+}
+
 void libfive_last_mesh_times(double* ms4) {
     const auto& st = last_mesh_stats();
     ms4[0] = st.ms_build;

@@ -24,6 +24,7 @@
 #include "interval_ops.hpp"
 #include "mesh_tables.hpp"
 #include "point_eval.hpp"
+#include "feature_eval.hpp"
 #include "qef.hpp"
 #include "regfile.hpp"
 
@@ -272,7 +273,10 @@ void k_interval(Params P, int level, const uint32_t* __restrict__ groups,
         }
         const Iv out = iv_apply(o, a, b, im);
         const uint32_t d = dst(c);
-        rslot[d] = (o == OP_COPY) ? rslot[ra(c)] : slot(c);
+        const uint32_t rs = (o == OP_COPY) ? rslot[ra(c)] : slot(c);
+        __syncthreads();
+        rslot[d] = rs;
+        __syncthreads();
         LF_RF_SET(L, NV, d, out.lo);
         LF_RF_SET(U, NV, d, out.hi);
         LF_RF_SET(F, NV, d, out.nan ? 1.0f : 0.0f);
@@ -444,33 +448,35 @@ hipError_t launch_link(const LevelView& cur, const uint32_t* base,
 }
 
 ////////////////////////////////////////////////////////////////////////////////
-// Exact-zero points (FeatureEvaluator): not evaluated on this path yet; a
-// point that needs it raises ERR_FEATURE so the render fails loudly.
-
-__device__ bool feature_is_inside(const Params& P) {
-    atomicOr(&P.counters->error, ERR_FEATURE);
-    return false;
-}
-
-////////////////////////////////////////////////////////////////////////////////
 // Leaf cells
 
-template <int NV>
+// FEAT = false: a cell that needs the exact-zero feature evaluation is
+// appended to `deferred` and left untouched.  FEAT = true: the cells listed
+// in `work` are evaluated with that path, each wave with its own workspace.
+template <int NV, bool FEAT>
 __global__ __launch_bounds__(64)
 void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
-            uint32_t per_group, LevelView l1, LevelView l0)
+            uint32_t per_group, LevelView l1, LevelView l0,
+            const uint32_t* __restrict__ work, uint32_t* __restrict__ deferred,
+            uint8_t* __restrict__ ws_base, size_t ws_bytes)
 {
     __shared__ double s_tf[12][3], s_ts[12][3];
     __shared__ double s_acc[12][14];
     __shared__ int s_rank[12];
     __shared__ float s_v[32], s_g[32][3];
     __shared__ uint32_t s_fl[32];
+    __shared__ float s_feat[FEAT ? 24 : 1][feat::kMaxOut][3];
+    __shared__ int s_nfeat[FEAT ? 24 : 1];
     const int lane = threadIdx.x;
     const uint32_t total = ngroups * per_group;
     const uint32_t root = P.root_reg;
+    uint32_t* err = &P.counters->error;
 
-    const uint32_t w = blockIdx.x;
+    const uint32_t w = FEAT ? work[blockIdx.x] : blockIdx.x;
     if (w >= total) return;
+    auto defer = [&]() {
+        if (lane == 0) deferred[atomicAdd(&P.counters->deferred, 1u)] = w;
+    };
     do {
         const uint32_t p = groups[w / per_group];
         const uint32_t oct = w % per_group;
@@ -483,6 +489,8 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
         const Clause* __restrict__ t = P.pool + __builtin_amdgcn_readfirstlane(tr.off);
         double lo[3], hi[3];
         cell_bounds(P, key, 0, lo, hi);
+        feat::Workspace W{};
+        if (FEAT) W = feat::carve(ws_base + size_t(blockIdx.x) * ws_bytes, n, P.nregs);
 
         // Corner states (dc_tree.inl:131-239): lane & 7 is the corner.
         const int corner = lane & 7;
@@ -495,7 +503,27 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
         if (v > 0.0f || fmath::isnan_f(v)) inside = false;
         else if (v < 0.0f) inside = true;
         else if (!(fl & PT_AMBIGUOUS)) inside = (g[0] != 0.0f || g[1] != 0.0f || g[2] != 0.0f);
-        else inside = (lane < 8) ? feature_is_inside(P) : false;
+        else inside = false;
+        {
+            // Ambiguous zeros: FeatureEvaluator::isInside, one point at a
+            // time on the whole wave.
+            uint64_t need = ballot(lane < 8 && v == 0.0f && (fl & PT_AMBIGUOUS));
+            if (need) {
+                if (!FEAT) { defer(); return; }
+                while (need) {
+                    const int l = __builtin_ctzll(need);
+                    need &= need - 1;
+                    const float q[3] = {__shfl(cp[0], l), __shfl(cp[1], l), __shfl(cp[2], l)};
+                    int r = 0;
+                    if (lane == 0) {
+                        r = feat::is_inside(t, n, P.nregs, root, P.slot_x, P.slot_y, P.slot_z,
+                                            q, W, err) ? 1 : 0;
+                    }
+                    r = __shfl(r, 0);
+                    if (lane == l) inside = r != 0;
+                }
+            }
+        }
         const uint32_t mask = uint32_t(ballot(lane < 8 && inside) & 0xffu);
         if (mask == 0u || mask == 0xffu) {
             if (lane == 0) {
@@ -547,7 +575,26 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
                         const float nrm = sqrtf(sg[0] * sg[0] + (sg[1] * sg[1] + sg[2] * sg[2]));
                         if (fast) zero_out = !(nrm > 0.0f);
                     }
-                    if (zero && (vf & PT_TIE)) zero_out = !feature_is_inside(P);
+                    {
+                        uint64_t need = ballot(zero && (vf & PT_TIE));
+                        if (need) {
+                            if (!FEAT) { defer(); return; }
+                            const float fp[3] = {float(ps[0]), float(ps[1]), float(ps[2])};
+                            while (need) {
+                                const int l = __builtin_ctzll(need);
+                                need &= need - 1;
+                                const float q[3] = {__shfl(fp[0], l), __shfl(fp[1], l),
+                                                    __shfl(fp[2], l)};
+                                int r = 0;
+                                if (lane == 0) {
+                                    r = feat::is_inside(t, n, P.nregs, root, P.slot_x, P.slot_y,
+                                                        P.slot_z, q, W, err) ? 1 : 0;
+                                }
+                                r = __shfl(r, 0);
+                                if (lane == l) zero_out = (r == 0);
+                            }
+                        }
+                    }
                     const bool take = valid && j >= 1 &&
                         (sv > 0.0f || zero_out || (j == 15 && !(sv == 0.0f)));
                     const uint64_t tm = ballot(take);
@@ -580,6 +627,29 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
                     s_g[lane][2] = eg[2];
                     s_fl[lane] = ef;
                 }
+                // Ambiguous ends contribute every distinct gradient of the
+                // point (FeatureEvaluator::features).
+                uint64_t need = ballot(lane < 2 * ec && (ef & PT_AMBIGUOUS));
+                if (need) {
+                    if (!FEAT) { defer(); return; }
+                    const float fp[3] = {float(ep[0]), float(ep[1]), float(ep[2])};
+                    while (need) {
+                        const int l = __builtin_ctzll(need);
+                        need &= need - 1;
+                        const float q[3] = {__shfl(fp[0], l), __shfl(fp[1], l), __shfl(fp[2], l)};
+                        if (lane == 0) {
+                            float fo[feat::kMaxOut][3];
+                            const int nf = feat::features(t, n, P.nregs, root, P.slot_x,
+                                                          P.slot_y, P.slot_z, q, W, fo, err);
+                            s_nfeat[l] = nf;
+                            for (int f = 0; f < nf; ++f) {
+                                s_feat[l][f][0] = fo[f][0];
+                                s_feat[l][f][1] = fo[f][1];
+                                s_feat[l][f][2] = fo[f][2];
+                            }
+                        }
+                    }
+                }
             }
             __syncthreads();
             if (lane < ec) {
@@ -595,8 +665,13 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
                                              double(s_g[k][2])};
                         qef::push_sample(acc, pos, d, double(s_v[k]));
                         exists = true;
-                    } else {
-                        feature_is_inside(P);   // features() not available
+                    } else if (FEAT) {
+                        for (int f = 0; f < s_nfeat[k]; ++f) {
+                            const double d[3] = {double(s_feat[k][f][0]), double(s_feat[k][f][1]),
+                                                 double(s_feat[k][f][2])};
+                            qef::push_sample(acc, pos, d, double(s_v[k]));
+                            exists = true;
+                        }
                     }
                 }
                 int r = -1;
@@ -656,18 +731,45 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
     } while (false);
 }
 
+#define LF_LEAF_LAUNCH(NV, FEAT, grid)                                             \
+    hipLaunchKernelGGL((k_leaf<NV, FEAT>), dim3(grid), dim3(64), 0, s, p, groups,    \
+                       ngroups, per_group, l1, l0, work, deferred, ws, ws_bytes)
+
 hipError_t launch_leaf(const Params& p, const uint32_t* groups, uint32_t ngroups,
                        uint32_t per_group, const LevelView& l1, const LevelView& l0,
-                       hipStream_t s) {
+                       uint32_t* deferred, hipStream_t s) {
     if (ngroups == 0) return hipSuccess;
     const uint32_t grid = ngroups * per_group;
+    const uint32_t* work = nullptr;
+    uint8_t* ws = nullptr;
+    const size_t ws_bytes = 0;
     switch (banks(p.nregs)) {
-        case 1: hipLaunchKernelGGL(k_leaf<1>, dim3(grid), dim3(64), 0, s, p, groups, ngroups, per_group, l1, l0); break;
-        case 2: hipLaunchKernelGGL(k_leaf<2>, dim3(grid), dim3(64), 0, s, p, groups, ngroups, per_group, l1, l0); break;
-        case 4: hipLaunchKernelGGL(k_leaf<4>, dim3(grid), dim3(64), 0, s, p, groups, ngroups, per_group, l1, l0); break;
+        case 1: LF_LEAF_LAUNCH(1, false, grid); break;
+        case 2: LF_LEAF_LAUNCH(2, false, grid); break;
+        case 4: LF_LEAF_LAUNCH(4, false, grid); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_leaf_features(const Params& p, const uint32_t* groups,
+                                uint32_t ngroups, uint32_t per_group,
+                                const LevelView& l1, const LevelView& l0,
+                                const uint32_t* work, uint32_t nwork, uint8_t* ws,
+                                size_t ws_bytes, hipStream_t s) {
+    if (nwork == 0) return hipSuccess;
+    uint32_t* deferred = nullptr;
+    switch (banks(p.nregs)) {
+        case 1: LF_LEAF_LAUNCH(1, true, nwork); break;
+        case 2: LF_LEAF_LAUNCH(2, true, nwork); break;
+        case 4: LF_LEAF_LAUNCH(4, true, nwork); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+size_t leaf_feature_workspace(uint32_t max_len, uint32_t nregs) {
+    return (feat::workspace_bytes(max_len, nregs) + 255) & ~size_t(255);
 }
 
 ////////////////////////////////////////////////////////////////////////////////

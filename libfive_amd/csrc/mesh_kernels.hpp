@@ -67,7 +67,7 @@ struct LevelView {
 struct Counters {
     uint32_t pool_top;
     uint32_t tape_count;
-    uint32_t unused;
+    uint32_t deferred;   // leaf cells left for the feature pass
     uint32_t error;
     uint32_t max_len;    // longest tape of an ambiguous cell of the last level
     uint32_t pad[3];
@@ -75,7 +75,7 @@ struct Counters {
 enum : uint32_t {
     ERR_POOL = 1u,       // clause pool exhausted
     ERR_TAPES = 2u,      // tape records exhausted
-    ERR_FEATURE = 4u,    // feature evaluation needed but capacity exceeded
+    ERR_FEATURE = 4u,    // a feature evaluation exceeded its fixed capacity
     ERR_LOCATE = 8u,     // point location inconsistency
 };
 
@@ -92,8 +92,6 @@ struct Params {
     uint32_t slot_x, slot_y, slot_z;
     Counters* counters;
     const LevelView* levels;     // device copy, indexed by level
-    uint8_t* feature_scratch;
-    uint32_t feature_scratch_bytes;
 };
 
 hipError_t upload_tables(hipStream_t s);
@@ -113,11 +111,20 @@ hipError_t launch_link(const LevelView& cur, const uint32_t* base,
 
 // Leaf cells (level 0): corner states, vertex patches, edge search, QEF.
 // groups lists cells of l1 whose per_group (8, or 1 for a root leaf)
-// children in l0 are evaluated, one wave per child.
+// children in l0 are evaluated, one wave per child.  Cells that need the
+// exact-zero feature evaluation are listed in `deferred` (counted in
+// Counters::deferred) and evaluated by launch_leaf_features, each with
+// leaf_feature_workspace() bytes of device workspace.
 hipError_t launch_leaf(const Params& p, const uint32_t* groups,
                        uint32_t ngroups, uint32_t per_group,
                        const LevelView& l1, const LevelView& l0,
-                       hipStream_t s);
+                       uint32_t* deferred, hipStream_t s);
+hipError_t launch_leaf_features(const Params& p, const uint32_t* groups,
+                                uint32_t ngroups, uint32_t per_group,
+                                const LevelView& l1, const LevelView& l0,
+                                const uint32_t* work, uint32_t nwork,
+                                uint8_t* ws, size_t ws_bytes, hipStream_t s);
+size_t leaf_feature_workspace(uint32_t max_len, uint32_t nregs);
 
 // Bottom-up collapse of one level.
 hipError_t launch_collapse(const Params& p, int level, const LevelView& cur,

@@ -116,14 +116,47 @@ void wait(hipStream_t s, const char* what) {
 
 void check_errors(uint32_t err) {
     if (err & ERR_FEATURE) {
-        throw Error("render_mesh: a sample point lies exactly on a min/max "
-                    "feature (value 0 with tied branches), which needs the "
-                    "feature evaluator");
+        throw Error("render_mesh: a point on a min/max feature has more "
+                    "features than the GPU feature evaluation holds");
     }
     if (err & (ERR_POOL | ERR_TAPES)) {
         throw Error("render_mesh: tape storage exhausted");
     }
     if (err) throw Error("render_mesh: internal error " + std::to_string(err));
+}
+
+// Evaluates the leaf cells of `groups` (DCTree::evalLeaf): one pass for
+// all cells, then the cells that met an exact zero on a tied MIN/MAX again
+// with the feature evaluation, in batches that bound the workspace.
+void run_leaves(const Params& P, const uint32_t* groups, uint32_t ngroups,
+                uint32_t per_group, const LevelView& l1, const LevelView& l0,
+                uint32_t max_len, DevBuf<Counters>& counters, Counters& hc,
+                hipStream_t s) {
+    const uint32_t cells = ngroups * per_group;
+    DevBuf<uint32_t> deferred(cells);
+    hip_check(hipMemsetAsync(&counters.p->deferred, 0, sizeof(uint32_t), s), "deferred");
+    hip_check(launch_leaf(P, groups, ngroups, per_group, l1, l0, deferred.p, s),
+              "leaf kernel");
+    hip_check(hipMemcpyAsync(&hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost, s),
+              "counters");
+    wait(s, "leaf kernel");
+    check_errors(hc.error);
+    const uint32_t nd = hc.deferred;
+    if (nd == 0) return;
+    if (nd > cells) throw Error("render_mesh: inconsistent deferred cell count");
+    const size_t ws_bytes = leaf_feature_workspace(max_len, P.nregs);
+    const size_t batch = std::max<size_t>(1, std::min<size_t>(nd, (size_t(1) << 30) / ws_bytes));
+    DevBuf<uint8_t> ws(batch * ws_bytes);
+    for (size_t first = 0; first < nd; first += batch) {
+        const uint32_t count = uint32_t(std::min<size_t>(batch, nd - first));
+        hip_check(launch_leaf_features(P, groups, ngroups, per_group, l1, l0,
+                                       deferred.p + first, count, ws.p, ws_bytes, s),
+                  "leaf feature kernel");
+        hip_check(hipMemcpyAsync(&hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost, s),
+                  "counters");
+        wait(s, "leaf feature kernel");
+        check_errors(hc.error);
+    }
 }
 
 }   // namespace
@@ -222,11 +255,8 @@ bool render_mesh(const Tree& tree, const double lo[3], const double hi[3],
         hip_check(hipMemcpyAsync(virt.tape.p, &tape0, sizeof(tape0), hipMemcpyHostToDevice, s), "root");
         groups[1].alloc(1);
         hip_check(hipMemsetAsync(groups[1].p, 0, sizeof(uint32_t), s), "root");
-        hip_check(launch_leaf(P, groups[1].p, 1, 1, virt.view(0), levels[0].view(0), s),
-                  "leaf kernel");
-        hip_check(hipMemcpyAsync(&hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost, s), "counters");
-        wait(s, "leaf kernel");
-        check_errors(hc.error);
+        run_leaves(P, groups[1].p, 1, 1, virt.view(0), levels[0].view(0), nbase,
+                   counters, hc, s);
     } else {
         hip_check(hipMemsetAsync(&counters.p->max_len, 0, sizeof(uint32_t), s), "max_len");
         hip_check(launch_interval(P, top, nullptr, 1, levels[top].view(0),
@@ -264,11 +294,9 @@ bool render_mesh(const Tree& tree, const double lo[3], const double hi[3],
                                           levels[L - 1].view(0), max_len, s),
                           "interval kernel");
             } else {
-                hip_check(launch_leaf(P, groups[1].p, uint32_t(namb), 8, cur.view(0),
-                                      levels[0].view(0), s), "leaf kernel");
-                hip_check(hipMemcpyAsync(&hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost, s), "counters");
-                wait(s, "leaf kernel");
-                check_errors(hc.error);
+                run_leaves(P, groups[1].p, uint32_t(namb), 8, cur.view(0),
+                           levels[0].view(0), std::max<uint32_t>(hc.max_len, 1),
+                           counters, hc, s);
             }
             if (settings.cancel && *settings.cancel) {
                 wait(s, "cancel");
@@ -293,11 +321,13 @@ bool render_mesh(const Tree& tree, const double lo[3], const double hi[3],
                                  hipMemcpyDeviceToHost, s), "stats");
         for (int L = top; L >= 0; --L) {
             std::vector<uint8_t> ty(levels[L].n);
-            std::vector<uint32_t> tp(levels[L].n);
+            std::vector<uint32_t> tp(levels[L].n), ky(3 * size_t(levels[L].n));
             if (levels[L].n) {
                 hip_check(hipMemcpyAsync(ty.data(), levels[L].type.p, ty.size(),
                                          hipMemcpyDeviceToHost, s), "stats");
                 hip_check(hipMemcpyAsync(tp.data(), levels[L].tape.p, tp.size() * 4,
+                                         hipMemcpyDeviceToHost, s), "stats");
+                hip_check(hipMemcpyAsync(ky.data(), levels[L].key.p, ky.size() * 4,
                                          hipMemcpyDeviceToHost, s), "stats");
             }
             wait(s, "statistics copy");
@@ -307,6 +337,9 @@ bool render_mesh(const Tree& tree, const double lo[3], const double hi[3],
                     throw Error("render_mesh: inconsistent cell record");
                 }
                 cnt[ty[i]]++;
+                g_stats.cells.insert(g_stats.cells.end(),
+                                     {ky[3 * i], ky[3 * i + 1], ky[3 * i + 2], uint32_t(L),
+                                      uint32_t(ty[i])});
                 if (L > 0 && ty[i] == AMBIGUOUS) {
                     g_stats.tape_lengths.push_back(trs[tp[i]].ref_len);
                 }

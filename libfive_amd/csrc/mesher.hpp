@@ -33,6 +33,7 @@ struct MeshStats {
     int top = 0;                                  // root level
     std::vector<uint64_t> level_counts;           // root first: empty, filled, ambiguous
     std::vector<uint32_t> tape_lengths;           // per ambiguous cell above level 0
+    std::vector<uint32_t> cells;                  // x, y, z, level, state per cell
     uint64_t leaves = 0;                          // ambiguous leaf cells
     uint64_t collapsed = 0;
     double ms_build = 0, ms_collapse = 0, ms_emit = 0, ms_total = 0;

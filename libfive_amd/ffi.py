@@ -117,6 +117,7 @@ SIGNATURES = {
                                         ctypes.c_size_t]),
     "libfive_last_mesh_tape_lengths": (ctypes.c_size_t, [u32p, ctypes.c_size_t]),
     "libfive_last_mesh_times": (None, [ctypes.POINTER(ctypes.c_double)]),
+    "libfive_last_mesh_cells": (ctypes.c_size_t, [u32p, ctypes.c_size_t]),
     "libfive_mesh_settings_default": (None,
                                       [ctypes.POINTER(libfive_mesh_settings)]),
     "libfive_tree_render_mesh_settings": (

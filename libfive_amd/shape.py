@@ -241,8 +241,13 @@ class Shape:
         L.libfive_last_mesh_tape_lengths(lens.ctypes.data_as(ffi.u32p), nt)
         ms = (ctypes.c_double * 4)()
         L.libfive_last_mesh_times(ms)
-        return MeshResult(verts, tris, keys, counts[:3 * n].reshape(-1, 3),
-                          lens[:nt], list(ms))
+        nc = L.libfive_last_mesh_cells(None, 0)
+        cells = np.zeros((max(nc, 1), 5), np.uint32)
+        L.libfive_last_mesh_cells(cells.ctypes.data_as(ffi.u32p), nc)
+        out = MeshResult(verts, tris, keys, counts[:3 * n].reshape(-1, 3),
+                         lens[:nt], list(ms))
+        out.cells = cells[:nc]
+        return out
 
 
 class MeshResult:

@@ -39,11 +39,15 @@ _SIGS = {
     "oracle_sincos": (None, [_fp, ctypes.c_size_t, _fp, _fp]),
     "oracle_render_mesh": (ctypes.c_void_p, [ctypes.c_void_p, _dp, _dp,
                                              ctypes.c_double, ctypes.c_double]),
+    "oracle_render_mesh_mt": (ctypes.c_void_p, [ctypes.c_void_p, _dp, _dp,
+                                                ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_int]),
     "oracle_mesh_sizes": (None, [ctypes.c_void_p] +
                           [ctypes.POINTER(ctypes.c_size_t)] * 4),
     "oracle_mesh_copy": (None, [ctypes.c_void_p, _fp, _u32, _u32,
                                 ctypes.POINTER(ctypes.c_uint64), _u32]),
     "oracle_mesh_leaf_verts": (ctypes.c_size_t, [ctypes.c_void_p, _dp]),
+    "oracle_mesh_cells": (ctypes.c_size_t, [ctypes.c_void_p, _u32]),
     "oracle_mesh_delete": (None, [ctypes.c_void_p]),
     "oracle_corner_manifold": (ctypes.c_int, [ctypes.c_int]),
     "oracle_marching_table": (None, [ctypes.POINTER(ctypes.c_int8)] * 3),
@@ -164,13 +168,14 @@ class OracleDeck:
         n = lib().oracle_features(self.ptr, _f(p), _f(out), 64)
         return out[:min(n, 64)]
 
-    def render_mesh(self, lo, hi, min_feature, max_err=1e-8):
+    def render_mesh(self, lo, hi, min_feature, max_err=1e-8, threads=1):
         """Dual-contouring mesh: a MeshResult (see below)."""
         lo = np.ascontiguousarray(lo, np.float64)
         hi = np.ascontiguousarray(hi, np.float64)
         L = lib()
-        m = L.oracle_render_mesh(self.ptr, lo.ctypes.data_as(_dp),
-                                 hi.ctypes.data_as(_dp), min_feature, max_err)
+        m = L.oracle_render_mesh_mt(self.ptr, lo.ctypes.data_as(_dp),
+                                    hi.ctypes.data_as(_dp), min_feature, max_err,
+                                    int(threads))
         try:
             sz = [ctypes.c_size_t() for _ in range(4)]
             L.oracle_mesh_sizes(m, *[ctypes.byref(x) for x in sz])
@@ -187,10 +192,14 @@ class OracleDeck:
             nl = L.oracle_mesh_leaf_verts(m, None)
             leaf = np.empty((nl, 3), np.float64)
             L.oracle_mesh_leaf_verts(m, leaf.ctypes.data_as(_dp))
+            nc = L.oracle_mesh_cells(m, None)
+            cells = np.empty((nc, 5), np.uint32)
+            L.oracle_mesh_cells(m, cells.ctypes.data_as(_u32))
         finally:
             L.oracle_mesh_delete(m)
         out = MeshResult(verts, tris, keys, stats, lens)
         out.leaf_verts = leaf
+        out.cells = cells
         return out
 
 

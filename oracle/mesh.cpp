@@ -19,6 +19,8 @@
 // is a fixed depth-first order instead of the thread pool's, so vertex
 // indices differ while the set of triangles is the same.
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <array>
 #include <cmath>
 #include <cstring>
@@ -242,6 +244,7 @@ struct Mesher {
     std::vector<uint32_t> tape_lengths;                    // per ambiguous cell
     uint64_t leaves_ambiguous = 0;
     uint64_t collapsed = 0;
+    std::vector<uint32_t> cells;   // x, y, z, level, state of every evaluated cell
 
     Mesher(Deck& d, double me) : deck(d), ie(d), de(d), fe(d), max_err(me) {}
 
@@ -272,6 +275,8 @@ struct Mesher {
         else if (r.hi < 0) c->type = FILLED;
         else c->type = AMBIGUOUS;
         level_counts[c->level][c->type]++;
+        cells.insert(cells.end(), {c->key[0], c->key[1], c->key[2], uint32_t(c->level),
+                                   uint32_t(c->type)});
         if (r.nan) return tape;
         if (c->type != AMBIGUOUS) return nullptr;
         tape_lengths.push_back(uint32_t(pushed->t.size()));
@@ -290,6 +295,8 @@ struct Mesher {
         } else {
             eval_leaf(c, tape);
             level_counts[0][c->type]++;
+            cells.insert(cells.end(), {c->key[0], c->key[1], c->key[2], 0u,
+                                       uint32_t(c->type)});
         }
     }
 
@@ -741,13 +748,39 @@ struct oracle_mesh {
     std::vector<uint64_t> stats;
     std::vector<uint32_t> tape_lengths;
     std::vector<double> leaf_verts;
+    std::vector<uint32_t> cells;
 };
 
 extern "C" {
 
-oracle_mesh* oracle_render_mesh(oracle_deck* d, const double lo[3],
-                                const double hi[3], double min_feature,
-                                double max_err) {
+// Parallel build: the top levels are expanded on the calling thread, the
+// subtrees below them are built by worker threads (each with its own Deck
+// scratch and evaluators), then the top levels are collapsed bottom-up and
+// the dual walk runs on the calling thread.  The tree is the same as the
+// sequential build's.
+struct Expansion {
+    std::vector<std::pair<Cell*, std::shared_ptr<Tape>>> tasks;
+    std::vector<std::pair<Cell*, std::shared_ptr<Tape>>> collects;   // post-order
+};
+
+static void expand(Mesher& m, Cell* c, const std::shared_ptr<Tape>& tape, int depth,
+                   Expansion& ex) {
+    if (depth == 0 || c->level == 0) {
+        ex.tasks.push_back({c, tape});
+        return;
+    }
+    auto next = m.eval_interval(c, tape);
+    if (c->type != AMBIGUOUS) return;
+    m.subdivide(c);
+    for (int i = 0; i < 8; ++i) expand(m, c->kids[i].get(), next, depth - 1, ex);
+    const double clo[3] = {c->lo[0], c->lo[1], c->lo[2]};
+    const double chi[3] = {c->hi[0], c->hi[1], c->hi[2]};
+    ex.collects.push_back({c, get_base(next, clo, chi)});
+}
+
+oracle_mesh* oracle_render_mesh_mt(oracle_deck* d, const double lo[3],
+                                   const double hi[3], double min_feature,
+                                   double max_err, int threads) {
     Deck& deck = d->deck;
     Mesher m(deck, max_err);
     auto root = std::make_unique<Cell>();
@@ -757,7 +790,41 @@ oracle_mesh* oracle_render_mesh(oracle_deck* d, const double lo[3],
         root->key[j] = 0;
     }
     root->level = resolution_level(lo, hi, min_feature);
-    m.build(root.get(), deck.tape);
+    if (threads <= 1) {
+        m.build(root.get(), deck.tape);
+    } else {
+        Expansion ex;
+        expand(m, root.get(), deck.tape, 3, ex);
+        std::atomic<size_t> next_task{0};
+        std::vector<std::unique_ptr<Deck>> decks;
+        std::vector<std::unique_ptr<Mesher>> workers;
+        for (int t = 0; t < threads; ++t) {
+            decks.push_back(std::make_unique<Deck>(deck));
+            workers.push_back(std::make_unique<Mesher>(*decks.back(), max_err));
+        }
+        std::vector<std::thread> pool;
+        for (int t = 0; t < threads; ++t) {
+            pool.emplace_back([&, t] {
+                for (;;) {
+                    const size_t i = next_task.fetch_add(1);
+                    if (i >= ex.tasks.size()) break;
+                    workers[t]->build(ex.tasks[i].first, ex.tasks[i].second);
+                }
+            });
+        }
+        for (auto& th : pool) th.join();
+        for (auto& c : ex.collects) m.collect(c.first, c.second);
+        for (auto& w : workers) {
+            for (auto& lc : w->level_counts) {
+                for (int k = 0; k < 4; ++k) m.level_counts[lc.first][k] += lc.second[k];
+            }
+            m.tape_lengths.insert(m.tape_lengths.end(), w->tape_lengths.begin(),
+                                  w->tape_lengths.end());
+            m.leaves_ambiguous += w->leaves_ambiguous;
+            m.collapsed += w->collapsed;
+            m.cells.insert(m.cells.end(), w->cells.begin(), w->cells.end());
+        }
+    }
     m.walk(root.get());
     auto out = new oracle_mesh;
     for (const auto& v : m.verts) out->verts.insert(out->verts.end(), v.begin(), v.end());
@@ -774,7 +841,14 @@ oracle_mesh* oracle_render_mesh(oracle_deck* d, const double lo[3],
     }
     out->tape_lengths = m.tape_lengths;
     m.leaf_vertices(root.get(), out->leaf_verts);
+    out->cells = std::move(m.cells);
     return out;
+}
+
+oracle_mesh* oracle_render_mesh(oracle_deck* d, const double lo[3],
+                                const double hi[3], double min_feature,
+                                double max_err) {
+    return oracle_render_mesh_mt(d, lo, hi, min_feature, max_err, 1);
 }
 
 void oracle_mesh_sizes(const oracle_mesh* m, size_t* n_verts, size_t* n_tris,
@@ -798,6 +872,11 @@ void oracle_mesh_copy(const oracle_mesh* m, float* verts, uint32_t* tris,
 size_t oracle_mesh_leaf_verts(const oracle_mesh* m, double* out) {
     if (out) std::memcpy(out, m->leaf_verts.data(), m->leaf_verts.size() * sizeof(double));
     return m->leaf_verts.size() / 3;
+}
+
+size_t oracle_mesh_cells(const oracle_mesh* m, uint32_t* out) {
+    if (out) std::memcpy(out, m->cells.data(), m->cells.size() * sizeof(uint32_t));
+    return m->cells.size() / 5;
 }
 
 void oracle_mesh_delete(oracle_mesh* m) { delete m; }

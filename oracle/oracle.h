@@ -69,6 +69,11 @@ typedef struct oracle_mesh oracle_mesh;
 oracle_mesh* oracle_render_mesh(oracle_deck* d, const double lo[3],
                                 const double hi[3], double min_feature,
                                 double max_err);
+/* The same mesh built by `threads` worker threads (the top three octree
+ * levels on the calling thread, the subtrees below them in parallel). */
+oracle_mesh* oracle_render_mesh_mt(oracle_deck* d, const double lo[3],
+                                   const double hi[3], double min_feature,
+                                   double max_err, int threads);
 void oracle_mesh_sizes(const oracle_mesh* m, size_t* n_verts, size_t* n_tris,
                        size_t* n_stats, size_t* n_tape_lengths);
 void oracle_mesh_copy(const oracle_mesh* m, float* verts, uint32_t* tris,
@@ -76,6 +81,10 @@ void oracle_mesh_copy(const oracle_mesh* m, float* verts, uint32_t* tris,
 /* All vertices of all ambiguous leaf cells (x, y, z in double); returns the
  * count, copies when out is not NULL. */
 size_t oracle_mesh_leaf_verts(const oracle_mesh* m, double* out);
+/* Every cell whose state was evaluated (interval or leaf evaluation), as
+ * x, y, z (finest-cell units), level, state (0 empty, 1 filled, 2
+ * ambiguous); returns the count, copies when out is not NULL. */
+size_t oracle_mesh_cells(const oracle_mesh* m, uint32_t* out);
 void oracle_mesh_delete(oracle_mesh* m);
 
 /* Tables and building blocks, exposed for their own tests. */

@@ -34,6 +34,10 @@ def compare(shape, lo, hi, mf, max_err=1e-8):
     o = oracle.OracleDeck(shape.deck()).render_mesh(lo3, hi3, mf, max_err)
     assert g.level_counts.shape == o.level_counts.shape
     assert (g.level_counts == o.level_counts).all()
+    # identical interval / leaf classification, cell by cell
+    gc = {tuple(c[:4]): int(c[4]) for c in g.cells.tolist()}
+    oc = {tuple(c[:4]): int(c[4]) for c in o.cells.tolist()}
+    assert gc == oc
     assert sorted(g.tape_lengths.tolist()) == sorted(o.tape_lengths.tolist())
     assert g.verts.shape == o.verts.shape
     assert g.tris.shape == o.tris.shape
@@ -75,10 +79,73 @@ def test_sphere_union_matches_oracle(gpu):
     compare(W.sphere_union(n=60), -1.2, 1.2, 2.4 / 64)
 
 
-@pytest.mark.skip(reason="needs the exact-zero feature evaluation on the GPU")
 def test_sliced_box_matches_oracle(gpu):
+    """Box faces on the grid: exact zeros on tied MIN/MAX go through the
+    feature evaluation."""
     s = W.box((0, 0, 0), (1, 1, 1)).max(X() + Y() + Z() - 1.3)
     compare(s, -2.0, 2.0, 0.1)
+
+
+@pytest.mark.parametrize("res", [8, 16, 32])
+def test_c1_csg_matches_oracle(gpu, res):
+    """The CPU-runnable reference configuration (sphere union box minus a
+    cylinder over [-2, 2]^3) at resolution 16 and neighbours."""
+    g = compare(W.csg_c1(), -2.0, 2.0, float(np.float32(1) / np.float32(res)))
+    check_edge_pairs(g.tris)
+
+
+def cube15():
+    x, y, z = X(), Y(), Z()
+    return ((-(x + 1.5)).max(x - 1.5).max((-(y + 1.5)).max(y - 1.5))
+            .max((-(z + 1.5)).max(z - 1.5)))
+
+
+def test_cube_face_count(gpu):
+    """libfive/test/mesh.cpp:70-89 on the GPU."""
+    g = compare(cube15(), -3.0, 3.0, 0.15)
+    assert g.tris.shape[0] == 12 and g.verts.shape[0] == 9
+
+
+def test_rectangular_prism_face_count(gpu):
+    """libfive/test/mesh.cpp:91-102 (its region is not a cube)."""
+    x, y, z = X(), Y(), Z()
+    t = (-x).max(x - 4).max((-y).max(y - 1)).max((-z).max(z - 0.25))
+    verts, tris = t.render_mesh([-1, -1, -1], [5, 2, 1.25], min_feature=0.125)
+    assert verts.shape[0] == 9 and tris.shape[0] == 12
+
+
+def test_box_corner_features_match_oracle(gpu):
+    """Boxes sharing corners and edges on the grid: many-way ties."""
+    B = W.box
+    s = (B((-1, -1, -1), (0, 0, 0)).min(B((0, 0, 0), (1, 1, 1)))
+         .min(B((-.5, -.5, -.5), (.5, .5, .5)).max(-B((0, -1, -1), (1, 0, 1)))))
+    compare(s, -2.0, 2.0, 0.125)
+    t = B((-1, -1, -1), (1, 1, 1)).max(-(B((-.5, -2, -.5), (.5, 2, .5)).max(-W.sphere(0.25))))
+    compare(t, -2.0, 2.0, 0.125)
+
+
+@pytest.mark.parametrize("i", range(4))
+def test_random_csg_matches_oracle(gpu, i):
+    """Seeded random CSG models of the many-model workload."""
+    models = W.csg_batch(4, seed=W.SEED_BASE + 40)
+    shape, _ = models[i]
+    compare(shape, -1.0, 1.0, 2.0 / 32)
+
+
+@pytest.mark.parametrize("case", ["sphere_cutout", "nested_rings", "sphere_on_box"])
+def test_feature_shapes_match_oracle(gpu, case):
+    x, y, z = X(), Y(), Z()
+    if case == "sphere_cutout":
+        compare(W.sphere(1.0).max(-W.circle(0.5)), -5.0, 5.0,
+                float(np.float32(1) / np.float32(9)))
+    elif case == "nested_rings":
+        compare((-z).max(W.circle(1.0).max(-W.circle(0.5))).max(z - 1),
+                -2.0, 2.0, 0.2)
+    else:
+        g = compare(W.sphere(0.7, (0, 0, 0.1)).min(W.box((-1, -1, -1), (1, 1, 0.1))),
+                    -10.0, 10.0, 0.25)
+        t = g.tris
+        assert ((t[:, 0] != t[:, 1]) & (t[:, 0] != t[:, 2]) & (t[:, 1] != t[:, 2])).all()
 
 
 def test_c_api_unit_sphere(gpu):

@@ -1,6 +1,6 @@
-"""Benchmark: tape evaluation throughput on one MI355X.
+"""Benchmark: tape evaluation throughput and meshing time on one MI355X.
 
-Workload (the array-evaluation configuration): the seeded random
+Headline (value): the array-evaluation workload, the seeded random
 expression of 1000 operations over {add, sub, mul, min, max, neg, abs,
 square, sqrt(abs(.)), sin, cos} with leaves x, y, z and constants
 U(-2, 2) (libfive_amd.workloads.random_dag), evaluated with values and
@@ -9,10 +9,17 @@ grid over [-1, 1]^3.  A step is one pass over the grid; inputs and outputs
 are resident in HBM.  The metric is point-operations per second: grid
 points x tape clauses / time.
 
+Meshing (the "mesh" object): milliseconds to mesh the thickened gyroid
+intersected with the unit sphere over [-1.2, 1.2]^3 at 512 cells per axis
+through libfive_tree_render_mesh (tree in, libfive_mesh out, host copies
+included), plus the CSG configuration at resolution 16 and the 1000-sphere
+union's octree to depth 8 with its pruned-tape statistics.
+
 Prints one JSON line.  --gpus above 1 is rejected: this build measures one
 GPU (multi-GPU sharding is described in DESIGN.md, not run).
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -45,6 +52,15 @@ def parse():
     p.add_argument("--no-cpu", action="store_true",
                    help="skip the CPU baseline leg")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
+    p.add_argument("--no-mesh", action="store_true",
+                   help="skip the meshing measurements")
+    p.add_argument("--mesh-res", type=int, default=512)
+    p.add_argument("--mesh-steps", type=int, default=5)
+    p.add_argument("--cpu-mesh-res", type=int, default=256,
+                   help="gyroid resolution for the CPU meshing baseline")
+    p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                   help="only run a plain device-to-device copy of MIB MiB "
+                        "three times (calibrates the HBM byte counters)")
     p.add_argument("--sweep", action="store_true",
                    help="also time the value kernel at 1, 2 and 4 points "
                         "per lane")
@@ -83,6 +99,119 @@ def cpu_baseline(shape, target_seconds):
     }
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` as recorded in the committed
+    rocprofv3 summary (FETCH_SIZE and WRITE_SIZE runs scaled by a plain
+    copy's counts), or None when no summary has it."""
+    path = os.path.join(ROOT, "profiles", "lf_derivs_random_dag_256cube.md")
+    try:
+        for line in open(path):
+            if line.startswith(f"traffic_bytes_per_dispatch {kernel}:"):
+                return int(float(line.split(":")[1]))
+    except OSError:
+        pass
+    return None
+
+
+def render_ms(shape, lo, hi, res, steps, warmup=1):
+    """Wall-clock ms of libfive_tree_render_mesh (median of steps)."""
+    from libfive_amd import ffi
+    L = ffi.lib()
+    R = ffi.libfive_region3(ffi.libfive_interval(lo, hi),
+                            ffi.libfive_interval(lo, hi),
+                            ffi.libfive_interval(lo, hi))
+    times, counts = [], None
+    for i in range(warmup + steps):
+        t0 = time.perf_counter()
+        m = L.libfive_tree_render_mesh(shape.ptr, R, float(res))
+        dt = time.perf_counter() - t0
+        if not m:
+            raise RuntimeError(ffi.last_error())
+        counts = (m.contents.vert_count, m.contents.tri_count)
+        L.libfive_mesh_delete(m)
+        if i >= warmup:
+            times.append(dt * 1e3)
+    return float(np.median(times)), counts
+
+
+def mesh_measurements(args):
+    from libfive_amd import workloads as W
+    import oracle
+    res = args.mesh_res
+    gy = W.gyroid_c4()
+    lo, hi = W.C4_REGION[0][0], W.C4_REGION[1][0]
+    ms, (nv, nt) = render_ms(gy, lo, hi, res / (hi - lo), args.mesh_steps)
+    d = gy.render_mesh([lo] * 3, [hi] * 3, res=res / (hi - lo), details=True)
+    out = {
+        "workload": f"thickened gyroid (k=10) x unit sphere, [-1.2,1.2]^3, "
+                    f"{res}^3 cells, FP64 QEF",
+        "ms": round(ms, 2),
+        "verts": int(nv) - 1, "tris": int(nt),
+        "phase_ms": {k: round(v, 2) for k, v in
+                     zip(("build", "collapse", "emit", "total"), d.ms)},
+        "ambiguous_leaves": int(d.level_counts[-1][2]),
+    }
+    # CSG configuration at resolution 16 (64^3 finest cells)
+    c1 = W.csg_c1()
+    c1_ms, (c1v, c1t) = render_ms(c1, -2.0, 2.0, W.C1_RES, 5)
+    out["csg_res16"] = {"ms": round(c1_ms, 2), "verts": int(c1v) - 1,
+                        "tris": int(c1t)}
+    # 1000-sphere union, octree to depth 8 with pruned tapes
+    su = W.sphere_union()
+    s_lo, s_hi = W.C3_REGION[0][0], W.C3_REGION[1][0]
+    t0 = time.perf_counter()
+    sm = su.render_mesh([s_lo] * 3, [s_hi] * 3,
+                        min_feature=(s_hi - s_lo) / 2 ** W.C3_DEPTH, details=True)
+    su_ms = (time.perf_counter() - t0) * 1e3
+    lens = np.asarray(sm.tape_lengths)
+    hist, edges = np.histogram(lens, bins=[0, 10, 20, 50, 100, 200, 500, 1000,
+                                           2000, 5000, 20000])
+    out["sphere_union_depth8"] = {
+        "ms": round(su_ms, 2), "tape_clauses": int(su.deck()[0].shape[0]),
+        "cells_per_level": sm.level_counts.tolist(),
+        "pruned_tape_len_hist": {f"{int(a)}-{int(b)}": int(c)
+                                 for a, b, c in zip(edges[:-1], edges[1:], hist)},
+        "tris": int(sm.tris.shape[0]),
+    }
+    # the same union meshed at 512 cells per axis
+    su_ms512, (suv, sut) = render_ms(su, s_lo, s_hi, res / (s_hi - s_lo),
+                                     max(1, args.mesh_steps // 2))
+    out["sphere_union_512"] = {"ms": round(su_ms512, 2),
+                               "verts": int(suv) - 1, "tris": int(sut)}
+    if not args.no_cpu:
+        # The oracle mesher (CPU restatement of the reference's DC mesher)
+        # on the same gyroid at a lower resolution, with the GPU time there.
+        cres = args.cpu_mesh_res
+        threads = max(1, min(16, os.cpu_count() or 1))
+        od = oracle.OracleDeck(gy.deck())
+        mf = float(np.float32(1) / np.float32(cres / (hi - lo)))
+        lo32, hi32 = float(np.float32(lo)), float(np.float32(hi))
+        t0 = time.perf_counter()
+        om = od.render_mesh([lo32] * 3, [hi32] * 3, mf, threads=threads)
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        gpu_ms, _ = render_ms(gy, lo, hi, cres / (hi - lo), 3)
+        # vertex error of the GPU mesh against the FP64 error oracle (the
+        # same octree algorithm with double evaluation), vertices matched
+        # by owning cell and patch
+        gm = gy.render_mesh([lo] * 3, [hi] * 3, res=cres / (hi - lo),
+                            details=True)
+        r64 = od.render_mesh([lo32] * 3, [hi32] * 3, mf, threads=threads,
+                             fp64=True)
+        err, matched, nverts = oracle.vertex_error(gm.verts, gm.keys, r64)
+        out["vertex_error_vs_fp64"] = {
+            "max_abs": err, "matched": matched, "verts": nverts,
+            "workload": f"same gyroid at {cres}^3",
+        }
+        out["cpu_baseline"] = {
+            "value": round(cpu_ms, 1), "unit": "ms", "cores": threads,
+            "kind": "port",
+            "sample": f"same gyroid at {cres}^3 ({om.tris.shape[0]} tris); "
+                      f"GPU at {cres}^3: {gpu_ms:.1f} ms",
+            "gpu_ms_same_sample": round(gpu_ms, 2),
+        }
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -91,6 +220,16 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_probe:
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n * 4,
+                          "copy_probe_bytes_written": n * 4}))
+        return
     import libfive_amd
     from libfive_amd import ffi
     from libfive_amd import workloads as W
@@ -154,8 +293,11 @@ def main():
         kern = [a.elapsed_time(b) for a, b in evs]
         return wall, float(np.mean(kern))
 
-    wall, kern_ms = timed(deriv_step, args.steps, args.warmup)
-    v_wall, v_kern_ms = timed(value_step, args.steps, args.warmup)
+    # the tape interpreter (the path every tape can take, including the
+    # per-cell pruned tapes of the mesher)
+    L.libfive_tape_use_specialized(tape, 0)
+    i_wall, i_kern_ms = timed(deriv_step, args.steps, args.warmup)
+    iv_wall, iv_kern_ms = timed(value_step, args.steps, args.warmup)
     sweep = {}
     if args.sweep:
         for ppl in (1, 2, 4):
@@ -163,12 +305,20 @@ def main():
             w_, k_ = timed(value_step, args.steps, args.warmup)
             sweep[str(ppl)] = round(n * T / (k_ * 1e-3) / 1e9, 2)
         L.libfive_set_points_per_lane(0)
+    # the tape specialised into straight-line kernels (compiled once per
+    # tape, before the timed region, like the reference's Deck build)
+    compile_ms = ctypes.c_double()
+    if L.libfive_tape_specialize(tape, ctypes.byref(compile_ms)):
+        raise SystemExit("specialisation failed: " + ffi.last_error())
+    wall, kern_ms = timed(deriv_step, args.steps, args.warmup)
+    v_wall, v_kern_ms = timed(value_step, args.steps, args.warmup)
 
     ms_per_step = wall * 1e3 / args.steps
     value = n * T / (wall / args.steps) / 1e9
     achieved = n * dflops / (kern_ms * 1e-3) / 1e12
     result = {
-        "metric": "Gpoint-op/s tape eval (values + partial derivatives)",
+        "metric": ("Gpoint-op/s tape eval + ms to mesh @512^3, batched vs "
+                   "plain GPU at 1e-5 vtx err"),
         "value": round(value, 2),
         "unit": "Gpoint-op/s",
         "n_gpus": 1,
@@ -184,10 +334,17 @@ def main():
             "workload": "random 1000-op DAG, DerivArrayEvaluator pass, "
                         "256^3 grid on [-1,1]^3",
             "points": n, "tape_clauses": T, "registers": int(regs),
+            "path": "specialised straight-line kernels (hipRTC, gfx950)",
+            "specialise_compile_ms": round(compile_ms.value, 1),
             "values_only_gpoint_op_s": round(
                 n * T / (v_wall / args.steps) / 1e9, 2),
             "values_only_kernel_ms": round(v_kern_ms, 4),
-            "values_only_by_points_per_lane": sweep or None,
+            "interpreter_gpoint_op_s": round(
+                n * T / (i_wall / args.steps) / 1e9, 2),
+            "interpreter_kernel_ms": round(i_kern_ms, 4),
+            "interpreter_values_only_gpoint_op_s": round(
+                n * T / (iv_wall / args.steps) / 1e9, 2),
+            "interpreter_by_points_per_lane": sweep or None,
         },
         "roofline": {
             "bound": "valu",
@@ -195,8 +352,8 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": None,
-            "kernel": "k_eval_derivs",
+            "traffic": profiled_traffic("lf_derivs"),
+            "kernel": "lf_derivs",
             "kernel_ms": round(kern_ms, 4),
             "flops_per_point": dflops,
         },
@@ -204,6 +361,8 @@ def main():
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(shape, args.cpu_seconds)
     L.libfive_tape_delete(tape)
+    if not args.no_mesh:
+        result["mesh"] = mesh_measurements(args)
     print(json.dumps(result))
 
 

@@ -150,6 +150,29 @@ int libfive_tape_eval_derivs_device(libfive_tape t, const float* x,
                                     float* value, float* dx, float* dy,
                                     float* dz, void* stream);
 
+/* The same with host pointers (copies in, evaluates, copies out; blocks). */
+int libfive_tape_eval_values(libfive_tape t, const float* x, const float* y,
+                             const float* z, size_t n, float* out);
+int libfive_tape_eval_derivs(libfive_tape t, const float* x, const float* y,
+                             const float* z, size_t n, float* value, float* dx,
+                             float* dy, float* dz);
+
+/* Tape specialisation (no counterpart in the reference, whose evaluators
+ * interpret the tape on the CPU: libfive/src/eval/eval_array.cpp:107-118).
+ * Compiles the tape into straight-line value and derivative kernels with
+ * hipRTC (about a second for a thousand clauses) and loads them; after
+ * that, libfive_tape_eval_values_device / _derivs_device run the
+ * specialised kernels, which give the same bits as the interpreter.
+ * compile_ms (may be NULL) receives the compile and load time.  Returns 0
+ * on success; on failure the interpreter stays in use. */
+int libfive_tape_specialize(libfive_tape t, double* compile_ms);
+/* Chooses the specialised kernels (on != 0, if compiled) or the
+ * interpreter for the *_device evaluation calls of this tape. */
+void libfive_tape_use_specialized(libfive_tape t, int on);
+/* Compiles a tree's specialised kernels on the host without loading them
+ * (no GPU needed); returns the code object size in bytes, or 0 on error. */
+size_t libfive_tree_specialized_code_size(libfive_tree t);
+
 /* Points evaluated per GPU lane by the value kernel (1, 2 or 4; 0 picks by
  * register count).  Also read from LIBFIVE_POINTS_PER_LANE at start-up. */
 void libfive_set_points_per_lane(int p);

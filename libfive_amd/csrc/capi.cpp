@@ -11,6 +11,7 @@
 #include "eval_kernels.hpp"
 #include "interval_kernels.hpp"
 #include "mesher.hpp"
+#include "tape_jit.hpp"
 
 #include <map>
 #include <mutex>
@@ -41,6 +42,8 @@ static std::map<const libfive_mesh*, std::vector<uint32_t>> g_keys;
 
 struct libfive_tape_ {
     std::unique_ptr<GpuTape> t;
+    std::unique_ptr<SpecializedTape> spec;
+    bool use_spec = false;
 };
 
 extern "C" {
@@ -326,6 +329,11 @@ int libfive_tape_eval_values_device(libfive_tape t, const float* x,
         Context& c = context();
         hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c.stream;
         const auto& h = t->t->host;
+        if (t->spec && t->use_spec) {
+            hip_check(launch_specialized_values(*t->spec, x, y, z, out, n, s),
+                      "eval_values (specialised)");
+            return 0;
+        }
         hip_check(launch_eval_values(t->t->d.p, t->t->size(), h.root_reg,
                                      h.num_regs, x, y, z, out, n, s),
                   "eval_values");
@@ -344,6 +352,12 @@ int libfive_tape_eval_derivs_device(libfive_tape t, const float* x,
         Context& c = context();
         hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c.stream;
         const auto& h = t->t->host;
+        if (t->spec && t->use_spec) {
+            hip_check(launch_specialized_derivs(*t->spec, x, y, z, value, dx, dy,
+                                                dz, n, s),
+                      "eval_derivs (specialised)");
+            return 0;
+        }
         hip_check(launch_eval_derivs(t->t->d.p, t->t->size(), h.root_reg,
                                      h.num_regs, x, y, z, value, dx, dy, dz, n,
                                      s),
@@ -352,6 +366,86 @@ int libfive_tape_eval_derivs_device(libfive_tape t, const float* x,
     } catch (const std::exception& e) {
         set_error("libfive_tape_eval_derivs_device", e);
         return -1;
+    }
+}
+
+int libfive_tape_eval_values(libfive_tape t, const float* x, const float* y,
+                             const float* z, size_t n, float* out) {
+    try {
+        Context& c = context();
+        DevBuf<float> buf(4 * n);
+        std::lock_guard<std::mutex> lock(c.mu);
+        const size_t b = n * sizeof(float);
+        float* p = buf.p;
+        hip_check(hipMemcpyAsync(p, x, b, hipMemcpyHostToDevice, c.stream), "copy x");
+        hip_check(hipMemcpyAsync(p + n, y, b, hipMemcpyHostToDevice, c.stream), "copy y");
+        hip_check(hipMemcpyAsync(p + 2 * n, z, b, hipMemcpyHostToDevice, c.stream), "copy z");
+        if (libfive_tape_eval_values_device(t, p, p + n, p + 2 * n, n, p + 3 * n,
+                                            c.stream)) {
+            return -1;
+        }
+        hip_check(hipMemcpyAsync(out, p + 3 * n, b, hipMemcpyDeviceToHost, c.stream), "copy out");
+        hip_check(hipStreamSynchronize(c.stream), "tape eval sync");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error("libfive_tape_eval_values", e);
+        return -1;
+    }
+}
+
+int libfive_tape_eval_derivs(libfive_tape t, const float* x, const float* y,
+                             const float* z, size_t n, float* value, float* dx,
+                             float* dy, float* dz) {
+    try {
+        Context& c = context();
+        DevBuf<float> buf(7 * n);
+        std::lock_guard<std::mutex> lock(c.mu);
+        const size_t b = n * sizeof(float);
+        float* p = buf.p;
+        hip_check(hipMemcpyAsync(p, x, b, hipMemcpyHostToDevice, c.stream), "copy x");
+        hip_check(hipMemcpyAsync(p + n, y, b, hipMemcpyHostToDevice, c.stream), "copy y");
+        hip_check(hipMemcpyAsync(p + 2 * n, z, b, hipMemcpyHostToDevice, c.stream), "copy z");
+        if (libfive_tape_eval_derivs_device(t, p, p + n, p + 2 * n, n, p + 3 * n,
+                                            p + 4 * n, p + 5 * n, p + 6 * n,
+                                            c.stream)) {
+            return -1;
+        }
+        float* outs[4] = {value, dx, dy, dz};
+        for (int k = 0; k < 4; ++k) {
+            hip_check(hipMemcpyAsync(outs[k], p + (3 + k) * n, b,
+                                     hipMemcpyDeviceToHost, c.stream), "copy out");
+        }
+        hip_check(hipStreamSynchronize(c.stream), "tape eval sync");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error("libfive_tape_eval_derivs", e);
+        return -1;
+    }
+}
+
+int libfive_tape_specialize(libfive_tape t, double* compile_ms) {
+    try {
+        if (!t->spec) t->spec = specialize(t->t->host);
+        t->use_spec = true;
+        if (compile_ms) *compile_ms = t->spec->compile_ms;
+        return 0;
+    } catch (const std::exception& e) {
+        set_error("libfive_tape_specialize", e);
+        return -1;
+    }
+}
+
+void libfive_tape_use_specialized(libfive_tape t, int on) {
+    t->use_spec = (on != 0);
+}
+
+size_t libfive_tree_specialized_code_size(libfive_tree t) {
+    try {
+        Deck deck{Tree(D(t))};
+        return compile_specialized(compile_deck(deck)).size();
+    } catch (const std::exception& e) {
+        set_error("libfive_tree_specialized_code_size", e);
+        return 0;
     }
 }
 

@@ -23,10 +23,22 @@
 // fused multiply-adds are written explicitly with ::fma().
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <cstdint>
 #include <cmath>
+#endif
 
-#if defined(__HIPCC__) || defined(__HIP__)
+#if defined(__HIPCC_RTC__)
+// Run-time compilation: hipRTC supplies the fixed-width integer types.
+using __hip_internal::int32_t;
+using __hip_internal::int64_t;
+using __hip_internal::uint8_t;
+using __hip_internal::uint16_t;
+using __hip_internal::uint32_t;
+using __hip_internal::uint64_t;
+#define LF_HD __host__ __device__ inline __attribute__((always_inline))
+#define LF_HD_COLD __host__ __device__ inline __attribute__((noinline))
+#elif defined(__HIPCC__) || defined(__HIP__)
 #include <hip/hip_runtime.h>
 #define LF_HD __host__ __device__ inline __attribute__((always_inline))
 // Rarely executed helpers are kept out of line on the device so that their
@@ -80,7 +92,7 @@ LF_HD float succ_f(float f) {
 // Bits of 2/pi after the binary point, most significant first (computed
 // from Machin's formula with integer arithmetic; regenerated and checked by
 // the tests).
-#if defined(__HIPCC__) || defined(__HIP__)
+#if defined(__HIPCC__) || defined(__HIP__) || defined(__HIPCC_RTC__)
 __device__ __constant__
 #endif
 static const uint32_t kTwoOverPi[10] = {

@@ -292,75 +292,146 @@ void k_interval(Params P, int level, const uint32_t* __restrict__ groups,
     else if (rhi < 0.0f) state = FILLED;
     else state = AMBIGUOUS;
 
-    // Tape::push for every child that may prune (tape.cpp:23-158): a
-    // backward walk keeps each clause whose result is still needed; a
-    // MIN/MAX resolved to one side becomes a copy of that side.
+    // Tape::push for every child that may prune (tape.cpp:23-158), in two
+    // walks per child lane.  The first (root first) marks the clauses whose
+    // result is still needed, a MIN/MAX resolved to one side needing only
+    // that side.  The second (leaves first) writes the kept clauses with
+    // their operands renamed through the resolved clauses, as the
+    // reference resolves its remap chains (tape.cpp:124-146): a resolved
+    // clause or a copy makes its destination an alias of the register that
+    // holds the value and writes nothing.  Because registers are reused,
+    // an alias is written out as a copy just before its register would be
+    // overwritten; an alias always names a register that holds its own
+    // value, so copies never chain.
     bool dopush = active && state == AMBIGUOUS && !rnan &&
                   !(tr.flags & TAPE_TERMINAL) && any_ab;
     uint32_t child_tape = tape_id;
     __syncthreads();
     if (ballot(dopush)) {
         uint32_t* my = need + uint32_t(lane & 7) * W;
+        uint8_t* const bytes = reinterpret_cast<uint8_t*>(need + 8u * W);
+        uint8_t* ren = bytes + uint32_t(lane & 7) * P.nregs;
+        uint8_t* cnt = ren + 8u * P.nregs;
+        uint8_t* lv = bytes + 16u * P.nregs;     // shared by the child lanes
         uint32_t base = 0;
         if (dopush) {
             for (uint32_t w = 0; w < W; ++w) my[w] = 0;
             my[root >> 5] |= 1u << (root & 31u);
-            base = atomicAdd(&P.counters->pool_top, n + 8u);
-            if (base + n + 8u > P.pool_cap) {
+            base = atomicAdd(&P.counters->pool_top, n + 9u);
+            if (base + n + 9u > P.pool_cap) {
                 atomicOr(&P.counters->error, ERR_POOL);
                 dopush = false;
             }
         }
-        uint32_t pos = base + n, cnt = 0, refc = 0;
         bool changed = false, nonterm = false;
         for (uint32_t k = n; k-- > 0;) {
             const Clause c = t[k];
-            if (!dopush) continue;
+            bool live = false;
             const uint32_t d = dst(c);
-            if (!((my[d >> 5] >> (d & 31u)) & 1u)) continue;
-            my[d >> 5] &= ~(1u << (d & 31u));
-            const uint32_t o = tape::op(c);
-            const uint32_t m = mode(c);
-            Clause out = c;
-            bool use_a = nargs(o) >= 1 && m != MODE_A_IMM && o != OP_LOADI;
-            bool use_b = is_binary(c) && m != MODE_B_IMM;
-            if (o == OC::OP_MIN || o == OC::OP_MAX) {
-                const uint32_t ch = choice[k];
-                const bool ka = (ch >> lane) & 1u;
-                const bool kb = (ch >> (8 + lane)) & 1u;
-                if (ka || kb) {
-                    changed = true;
-                    const bool from_imm = ka ? (m == MODE_A_IMM) : (m == MODE_B_IMM);
-                    const uint32_t src = ka ? ra(c) : rb(c);
-                    if (from_imm) {
-                        out = make(OP_LOADI, MODE_A_IMM, d, 0, 0, imm(c), slot(c));
-                        use_a = use_b = false;
+            if (dopush && ((my[d >> 5] >> (d & 31u)) & 1u)) {
+                live = true;
+                my[d >> 5] &= ~(1u << (d & 31u));
+                const uint32_t o = tape::op(c);
+                const uint32_t m = mode(c);
+                bool use_a = nargs(o) >= 1 && m != MODE_A_IMM && o != OP_LOADI;
+                bool use_b = is_binary(c) && m != MODE_B_IMM;
+                if (o == OC::OP_MIN || o == OC::OP_MAX) {
+                    const uint32_t ch = choice[k];
+                    const bool ka = (ch >> lane) & 1u;
+                    const bool kb = (ch >> (8 + lane)) & 1u;
+                    if (ka || kb) {
+                        changed = true;
+                        use_a = use_a && ka;
+                        use_b = use_b && kb;
                     } else {
-                        out = make(OP_COPY, MODE_RR, d, src, 0, 0.0f, slot(c));
-                        use_a = false;
-                        use_b = false;
-                        my[src >> 5] |= 1u << (src & 31u);
+                        nonterm = true;
                     }
-                } else {
-                    nonterm = true;
+                }
+                if (use_a) { const uint32_t r = ra(c); my[r >> 5] |= 1u << (r & 31u); }
+                if (use_b) { const uint32_t r = rb(c); my[r >> 5] |= 1u << (r & 31u); }
+            }
+            const uint64_t lm = ballot(live);
+            if (lane == 0) lv[k] = uint8_t(lm & 0xffu);
+        }
+        __syncthreads();
+        uint32_t pos = base, refc = 0;
+        const bool emit = dopush && changed;
+        if (emit) {
+            for (uint32_t r = 0; r < P.nregs; ++r) {
+                ren[r] = uint8_t(r);
+                cnt[r] = 0;
+            }
+        }
+        // Before register d is written: aliases of d become copies of it,
+        // and d stops being an alias itself.
+        auto will_write = [&](uint32_t d) {
+            for (uint32_t y = 0; cnt[d] != 0 && y < P.nregs; ++y) {
+                if (y != d && ren[y] == d) {
+                    P.pool[pos++] = make(OP_COPY, MODE_RR, y, d, 0, 0.0f, 0);
+                    ren[y] = uint8_t(y);
+                    cnt[d]--;
                 }
             }
-            if (use_a) { const uint32_t r = ra(c); my[r >> 5] |= 1u << (r & 31u); }
-            if (use_b) { const uint32_t r = rb(c); my[r >> 5] |= 1u << (r & 31u); }
-            P.pool[--pos] = out;
-            ++cnt;
-            const uint32_t oo = tape::op(out);
-            if (oo != OP_COPY && oo != OP_LOADI) ++refc;
+            if (ren[d] != d) {
+                cnt[ren[d]]--;
+                ren[d] = uint8_t(d);
+            }
+        };
+        for (uint32_t k = 0; k < n; ++k) {
+            if (!emit || !((lv[k] >> lane) & 1u)) continue;
+            const Clause c = t[k];
+            const uint32_t o = tape::op(c);
+            const uint32_t m = mode(c);
+            const uint32_t d = dst(c);
+            bool alias = (o == OP_COPY);
+            bool ka = true;
+            if (o == OC::OP_MIN || o == OC::OP_MAX) {
+                const uint32_t ch = choice[k];
+                ka = (ch >> lane) & 1u;
+                alias = ka || ((ch >> (8 + lane)) & 1u);
+            }
+            if (alias) {
+                const bool from_imm = (o != OP_COPY) &&
+                                      (ka ? (m == MODE_A_IMM) : (m == MODE_B_IMM));
+                if (from_imm) {
+                    will_write(d);
+                    P.pool[pos++] = make(OP_LOADI, MODE_A_IMM, d, 0, 0, imm(c), slot(c));
+                    continue;
+                }
+                const uint32_t r = ren[ka ? ra(c) : rb(c)];
+                if (r == d) continue;            // d already holds the value
+                if (cnt[d] != 0) {               // d's register is aliased: copy
+                    will_write(d);
+                    P.pool[pos++] = make(OP_COPY, MODE_RR, d, r, 0, 0.0f, slot(c));
+                } else {
+                    if (ren[d] != d) cnt[ren[d]]--;
+                    ren[d] = uint8_t(r);
+                    cnt[r]++;
+                }
+                continue;
+            }
+            const bool use_a = nargs(o) >= 1 && m != MODE_A_IMM && o != OP_LOADI;
+            const bool use_b = is_binary(c) && m != MODE_B_IMM;
+            const uint32_t a2 = use_a ? ren[ra(c)] : ra(c);
+            const uint32_t b2 = use_b ? ren[rb(c)] : rb(c);
+            will_write(d);
+            Clause out = c;
+            out.w1 = (a2 & 0xffffu) | (b2 << 16);
+            P.pool[pos++] = out;
+            if (o != OP_LOADI) ++refc;
         }
-        if (dopush && changed) {
-            for (uint32_t k = 0; k < 8u; ++k) P.pool[base + n + k] = Clause{0, 0, 0, 0};
+        if (emit && ren[root] != root) {
+            P.pool[pos++] = make(OP_COPY, MODE_RR, root, ren[root], 0, 0.0f, 0);
+        }
+        if (emit) {
+            for (uint32_t k = 0; k < 8u; ++k) P.pool[pos + k] = Clause{0, 0, 0, 0};
             const uint32_t id = atomicAdd(&P.counters->tape_count, 1u);
             if (id >= P.tape_cap) {
                 atomicOr(&P.counters->error, ERR_TAPES);
             } else {
                 TapeRec r;
-                r.off = pos;
-                r.len = cnt;
+                r.off = base;
+                r.len = pos - base;
                 r.parent = tape_id;
                 r.flags = TAPE_INTERVAL | (nonterm ? 0u : TAPE_TERMINAL);
                 r.box[0] = flo[0]; r.box[1] = fhi[0];
@@ -404,7 +475,9 @@ hipError_t launch_interval(const Params& p, int level, const uint32_t* groups,
     if (ngroups == 0) return hipSuccess;
     const uint32_t choice_words = (max_len + 1) / 2;
     const uint32_t W = (p.nregs + 31) / 32;
-    const size_t lds = 4u * (choice_words + p.nregs + 8u * W);
+    // choice (uint16 per clause), rslot, need bits, then per child lane the
+    // alias and alias-count bytes, then one live byte per clause
+    const size_t lds = 4u * (choice_words + p.nregs + 8u * W) + 16u * p.nregs + max_len;
     if (lds > 65536) return hipErrorInvalidValue;
     switch (banks(p.nregs)) {
         case 1: launch_interval_nv<1>(p, level, groups, ngroups, parent, cur, choice_words, lds, s); break;

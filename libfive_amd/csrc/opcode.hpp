@@ -6,8 +6,10 @@
 // Argument counts follow libfive/src/tree/opcode.cpp:18-65, names and
 // commutativity follow libfive/src/tree/opcode.cpp:80-273.
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <cstdint>
 #include <string>
+#endif
 
 namespace libfive {
 namespace Opcode {
@@ -49,6 +51,7 @@ enum Opcode : int {
     LAST_OP = 33,
 };
 
+#ifndef __HIPCC_RTC__
 // Number of tree children for an opcode, or -1 for INVALID / LAST_OP.
 int args(Opcode op);
 
@@ -65,6 +68,7 @@ std::string toOpString(Opcode op);
 
 bool isCommutative(Opcode op);
 bool isIdempotent(Opcode op);
+#endif
 
 }   // namespace Opcode
 }   // namespace libfive

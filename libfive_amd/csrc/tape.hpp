@@ -18,7 +18,9 @@
 // state fits in VGPRs.
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <cstdint>
+#endif
 
 #include "fmath.hpp"
 

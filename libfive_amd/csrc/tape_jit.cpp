@@ -1,0 +1,220 @@
+// Tape specialisation with hipRTC (see tape_jit.hpp).
+#include "tape_jit.hpp"
+
+#include <hip/hiprtc.h>
+
+#include <chrono>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "runtime.hpp"
+#include "tape.hpp"
+
+// kJitHeaderNames / kJitHeaderSources: the text of fmath.hpp, opcode.hpp,
+// tape.hpp and clause_ops.hpp, generated at build time.
+#include "jit_headers.inc"
+
+namespace libfive {
+namespace amd {
+
+using namespace tape;
+
+SpecializedTape::~SpecializedTape() {
+    if (module) (void)hipModuleUnload(module);
+}
+
+namespace {
+
+std::string fimm(float f) {
+    char buf[48];
+    std::snprintf(buf, sizeof(buf), "fmath::u2f(0x%08xu)", fmath::f2u(f));
+    return buf;
+}
+
+// Register -> expression of its current value (and of its partials).
+struct Names {
+    std::vector<std::string> v;
+    std::vector<std::string> g[3];
+    explicit Names(uint32_t nregs) : v(nregs, "0.0f") {
+        for (auto& gi : g) gi.assign(nregs, "0.0f");
+        v[kRegX] = "px";
+        v[kRegY] = "py";
+        v[kRegZ] = "pz";
+        g[0][kRegX] = "1.0f";
+        g[1][kRegY] = "1.0f";
+        g[2][kRegZ] = "1.0f";
+    }
+};
+
+// Operands of a clause exactly as the interpreter loop forms them
+// (LF_VALUE_STEP and LF_DERIV_STEP in eval_kernels.hip).
+void value_body(const DeviceTape& t, std::string& out) {
+    Names n(t.num_regs);
+    char buf[256];
+    size_t k = 0;
+    for (const Clause& c : t.clauses) {
+        const uint32_t o = op(c);
+        if (o == OP_NOP) continue;
+        const uint32_t m = mode(c);
+        const std::string im = fimm(imm(c));
+        const std::string a = (m == MODE_A_IMM) ? im : n.v[ra(c)];
+        const std::string b = !is_binary(c) ? std::string("0.0f")
+                              : (m == MODE_B_IMM) ? im : n.v[rb(c)];
+        std::snprintf(buf, sizeof(buf),
+                      "    float v%zu; { const float a_ = %s, b_ = %s; "
+                      "apply_value<1>(%uu, &a_, &b_, %s, &v%zu); }\n",
+                      k, a.c_str(), b.c_str(), o, im.c_str(), k);
+        out += buf;
+        n.v[dst(c)] = "v" + std::to_string(k);
+        ++k;
+    }
+    out += "    out[i] = " + n.v[t.root_reg] + ";\n";
+}
+
+void deriv_body(const DeviceTape& t, std::string& out) {
+    Names n(t.num_regs);
+    char buf[512];
+    size_t k = 0;
+    for (const Clause& c : t.clauses) {
+        const uint32_t o = op(c);
+        if (o == OP_NOP) continue;
+        const uint32_t m = mode(c);
+        const std::string im = fimm(imm(c));
+        std::string av = im, ad[3] = {"0.0f", "0.0f", "0.0f"};
+        std::string bv = "0.0f", bd[3] = {"0.0f", "0.0f", "0.0f"};
+        if (m != MODE_A_IMM && o != OP_LOADI) {
+            av = n.v[ra(c)];
+            for (int j = 0; j < 3; ++j) ad[j] = n.g[j][ra(c)];
+        }
+        if (is_binary(c)) {
+            if (m == MODE_B_IMM) {
+                bv = im;
+            } else {
+                bv = n.v[rb(c)];
+                for (int j = 0; j < 3; ++j) bd[j] = n.g[j][rb(c)];
+            }
+        }
+        std::snprintf(buf, sizeof(buf),
+                      "    float v%zu, d%zu[3]; { const float ad_[3] = {%s, %s, %s}; "
+                      "const float bd_[3] = {%s, %s, %s}; "
+                      "apply_deriv(%uu, %s, ad_, %s, bd_, %s, v%zu, d%zu); }\n",
+                      k, k, ad[0].c_str(), ad[1].c_str(), ad[2].c_str(),
+                      bd[0].c_str(), bd[1].c_str(), bd[2].c_str(), o,
+                      av.c_str(), bv.c_str(), im.c_str(), k, k);
+        out += buf;
+        const uint32_t d = dst(c);
+        n.v[d] = "v" + std::to_string(k);
+        for (int j = 0; j < 3; ++j) {
+            n.g[j][d] = "d" + std::to_string(k) + "[" + std::to_string(j) + "]";
+        }
+        ++k;
+    }
+    const uint32_t r = t.root_reg;
+    out += "    ov[i] = " + n.v[r] + ";\n";
+    out += "    odx[i] = " + n.g[0][r] + ";\n";
+    out += "    ody[i] = " + n.g[1][r] + ";\n";
+    out += "    odz[i] = " + n.g[2][r] + ";\n";
+}
+
+}   // namespace
+
+std::string specialized_source(const DeviceTape& t) {
+    std::string s =
+        "#include \"clause_ops.hpp\"\n"
+        "using namespace libfive;\n"
+        "using namespace libfive::amd;\n"
+        "extern \"C\" __global__ __launch_bounds__(256)\n"
+        "void lf_values(const float* __restrict__ xs, const float* __restrict__ ys,\n"
+        "               const float* __restrict__ zs, float* __restrict__ out,\n"
+        "               unsigned long long n) {\n"
+        "    const unsigned long long i = (unsigned long long)blockIdx.x * 256ull + threadIdx.x;\n"
+        "    if (i >= n) return;\n"
+        "    const float px = xs[i], py = ys[i], pz = zs[i];\n";
+    value_body(t, s);
+    s += "}\n\n"
+         "extern \"C\" __global__ __launch_bounds__(256)\n"
+         "void lf_derivs(const float* __restrict__ xs, const float* __restrict__ ys,\n"
+         "               const float* __restrict__ zs, float* __restrict__ ov,\n"
+         "               float* __restrict__ odx, float* __restrict__ ody,\n"
+         "               float* __restrict__ odz, unsigned long long n) {\n"
+         "    const unsigned long long i = (unsigned long long)blockIdx.x * 256ull + threadIdx.x;\n"
+         "    if (i >= n) return;\n"
+         "    const float px = xs[i], py = ys[i], pz = zs[i];\n";
+    deriv_body(t, s);
+    s += "}\n";
+    return s;
+}
+
+static void rtc_check(hiprtcResult r, const char* what) {
+    if (r != HIPRTC_SUCCESS) {
+        throw Error(std::string(what) + ": " + hiprtcGetErrorString(r));
+    }
+}
+
+std::vector<char> compile_specialized(const DeviceTape& t) {
+    const std::string src = specialized_source(t);
+    hiprtcProgram prog = nullptr;
+    rtc_check(hiprtcCreateProgram(&prog, src.c_str(), "lf_tape.hip",
+                                  int(sizeof(kJitHeaderNames) / sizeof(kJitHeaderNames[0])),
+                                  kJitHeaderSources, kJitHeaderNames),
+              "hiprtcCreateProgram");
+    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17",
+                          "-ffp-contract=off", "-Xclang", "-target-feature",
+                          "-Xclang", "-packed-fp32-ops"};
+    const hiprtcResult cr = hiprtcCompileProgram(prog, int(sizeof(opts) / sizeof(opts[0])), opts);
+    if (cr != HIPRTC_SUCCESS) {
+        size_t ls = 0;
+        hiprtcGetProgramLogSize(prog, &ls);
+        std::string log(ls, '\0');
+        if (ls) hiprtcGetProgramLog(prog, &log[0]);
+        hiprtcDestroyProgram(&prog);
+        throw Error("tape specialisation failed to compile: " + log.substr(0, 2000));
+    }
+    size_t code_size = 0;
+    rtc_check(hiprtcGetCodeSize(prog, &code_size), "hiprtcGetCodeSize");
+    std::vector<char> code(code_size);
+    rtc_check(hiprtcGetCode(prog, code.data()), "hiprtcGetCode");
+    hiprtcDestroyProgram(&prog);
+    return code;
+}
+
+std::unique_ptr<SpecializedTape> specialize(const DeviceTape& t) {
+    context();     // device selected and initialised
+    const auto t0 = std::chrono::steady_clock::now();
+    const std::vector<char> code = compile_specialized(t);
+    auto k = std::make_unique<SpecializedTape>();
+    hip_check(hipModuleLoadData(&k->module, code.data()), "hipModuleLoadData");
+    hip_check(hipModuleGetFunction(&k->values, k->module, "lf_values"),
+              "hipModuleGetFunction(lf_values)");
+    hip_check(hipModuleGetFunction(&k->derivs, k->module, "lf_derivs"),
+              "hipModuleGetFunction(lf_derivs)");
+    k->code_bytes = code.size();
+    k->compile_ms = std::chrono::duration<double, std::milli>(
+                        std::chrono::steady_clock::now() - t0).count();
+    return k;
+}
+
+hipError_t launch_specialized_values(const SpecializedTape& k, const float* x,
+                                     const float* y, const float* z,
+                                     float* out, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    unsigned long long nn = n;
+    void* args[] = {&x, &y, &z, &out, &nn};
+    const unsigned grid = unsigned((n + 255) / 256);
+    return hipModuleLaunchKernel(k.values, grid, 1, 1, 256, 1, 1, 0, s, args, nullptr);
+}
+
+hipError_t launch_specialized_derivs(const SpecializedTape& k, const float* x,
+                                     const float* y, const float* z, float* v,
+                                     float* dx, float* dy, float* dz, size_t n,
+                                     hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    unsigned long long nn = n;
+    void* args[] = {&x, &y, &z, &v, &dx, &dy, &dz, &nn};
+    const unsigned grid = unsigned((n + 255) / 256);
+    return hipModuleLaunchKernel(k.derivs, grid, 1, 1, 256, 1, 1, 0, s, args, nullptr);
+}
+
+}   // namespace amd
+}   // namespace libfive

@@ -107,6 +107,15 @@ SIGNATURES = {
                                          ctypes.c_size_t, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p]),
+    "libfive_tape_eval_values": (ctypes.c_int, [tape_t, fp, fp, fp,
+                                                ctypes.c_size_t, fp]),
+    "libfive_tape_eval_derivs": (ctypes.c_int, [tape_t, fp, fp, fp,
+                                                ctypes.c_size_t, fp, fp, fp,
+                                                fp]),
+    "libfive_tape_specialize": (ctypes.c_int,
+                                [tape_t, ctypes.POINTER(ctypes.c_double)]),
+    "libfive_tape_use_specialized": (None, [tape_t, ctypes.c_int]),
+    "libfive_tree_specialized_code_size": (ctypes.c_size_t, [tree_t]),
     "libfive_set_points_per_lane": (None, [ctypes.c_int]),
     "libfive_tree_tape_info": (ctypes.c_int, [tree_t, u32p, u32p]),
     "libfive_tree_deck": (ctypes.c_int, [tree_t, u32p, szp, u32p, fp, szp,

@@ -42,6 +42,10 @@ _SIGS = {
     "oracle_render_mesh_mt": (ctypes.c_void_p, [ctypes.c_void_p, _dp, _dp,
                                                 ctypes.c_double, ctypes.c_double,
                                                 ctypes.c_int]),
+    "oracle_render_mesh_ex": (ctypes.c_void_p, [ctypes.c_void_p, _dp, _dp,
+                                                ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_int, ctypes.c_int]),
+    "oracle_mesh_verts64": (ctypes.c_size_t, [ctypes.c_void_p, _dp]),
     "oracle_mesh_sizes": (None, [ctypes.c_void_p] +
                           [ctypes.POINTER(ctypes.c_size_t)] * 4),
     "oracle_mesh_copy": (None, [ctypes.c_void_p, _fp, _u32, _u32,
@@ -168,14 +172,17 @@ class OracleDeck:
         n = lib().oracle_features(self.ptr, _f(p), _f(out), 64)
         return out[:min(n, 64)]
 
-    def render_mesh(self, lo, hi, min_feature, max_err=1e-8, threads=1):
-        """Dual-contouring mesh: a MeshResult (see below)."""
+    def render_mesh(self, lo, hi, min_feature, max_err=1e-8, threads=1,
+                    fp64=False):
+        """Dual-contouring mesh: a MeshResult (see below).  fp64=True gives
+        the FP64 error oracle (double evaluation of values, partials and
+        the collapse test); its double vertices are in .verts64."""
         lo = np.ascontiguousarray(lo, np.float64)
         hi = np.ascontiguousarray(hi, np.float64)
         L = lib()
-        m = L.oracle_render_mesh_mt(self.ptr, lo.ctypes.data_as(_dp),
+        m = L.oracle_render_mesh_ex(self.ptr, lo.ctypes.data_as(_dp),
                                     hi.ctypes.data_as(_dp), min_feature, max_err,
-                                    int(threads))
+                                    int(threads), 1 if fp64 else 0)
         try:
             sz = [ctypes.c_size_t() for _ in range(4)]
             L.oracle_mesh_sizes(m, *[ctypes.byref(x) for x in sz])
@@ -195,11 +202,14 @@ class OracleDeck:
             nc = L.oracle_mesh_cells(m, None)
             cells = np.empty((nc, 5), np.uint32)
             L.oracle_mesh_cells(m, cells.ctypes.data_as(_u32))
+            v64 = np.empty((L.oracle_mesh_verts64(m, None), 3), np.float64)
+            L.oracle_mesh_verts64(m, v64.ctypes.data_as(_dp))
         finally:
             L.oracle_mesh_delete(m)
         out = MeshResult(verts, tris, keys, stats, lens)
         out.leaf_verts = leaf
         out.cells = cells
+        out.verts64 = v64
         return out
 
 
@@ -277,3 +287,25 @@ def sincos(x):
     c = np.empty_like(x)
     lib().oracle_sincos(_f(x), x.size, _f(s), _f(c))
     return s, c
+
+
+def _pack_keys(keys):
+    k = keys.astype(np.uint64)
+    return (k[:, 0] | (k[:, 1] << np.uint64(18)) | (k[:, 2] << np.uint64(36))
+            | (k[:, 3] << np.uint64(54)) | (k[:, 4] << np.uint64(59)))
+
+
+def vertex_error(verts, keys, ref):
+    """Vertex error of a mesh against the FP64 error oracle's mesh `ref`:
+    vertices are matched by key (owning cell and patch); returns
+    (max |v - v64| over matched vertices, matched count, vertex count)."""
+    a = _pack_keys(np.asarray(keys)[1:])
+    b = _pack_keys(np.asarray(ref.keys)[1:])
+    order = np.argsort(b)
+    bs = b[order]
+    pos = np.clip(np.searchsorted(bs, a), 0, max(len(bs) - 1, 0))
+    hit = (bs[pos] == a) if len(bs) else np.zeros(len(a), bool)
+    j = order[pos[hit]] + 1
+    d = np.abs(np.asarray(verts)[1:][hit].astype(np.float64) - ref.verts64[j])
+    err = float(d.max()) if d.size else 0.0
+    return err, int(hit.sum()), len(a)

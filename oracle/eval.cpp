@@ -753,6 +753,68 @@ double eval_point_f64(const Deck& deck, const Tape& tape, double x, double y,
     return v[tape.i];
 }
 
+// Forward-mode partials in double, with the rules of
+// DerivArrayEvaluator (libfive/src/eval/eval_deriv_array.cpp:119-256).
+static void deriv_op_f64(uint32_t op, double av, const double* ad, double bv,
+                         const double* bd, double ov, double* od) {
+    for (int j = 0; j < 3; ++j) {
+        double r;
+        switch (op) {
+            case 17: r = ad[j] + bd[j]; break;
+            case 21: r = ad[j] - bd[j]; break;
+            case 18: r = bd[j] * av + ad[j] * bv; break;
+            case 19: r = (av < bv) ? ad[j] : bd[j]; break;
+            case 20: r = (av < bv) ? bd[j] : ad[j]; break;
+            case 22: r = (ad[j] * bv - bd[j] * av) / (bv * bv); break;
+            case 23: r = (ad[j] * bv - bd[j] * av) / (av * av + bv * bv); break;
+            case 24: r = ad[j] * (bv * std::pow(av, bv - 1)); break;
+            case 25: r = (ad[j] == 0) ? 0.0 : ad[j] * (std::pow(av, 1.0 / bv - 1) / bv); break;
+            case 26: r = ad[j]; break;
+            case 27: r = std::isnan(av) ? bd[j] : ad[j]; break;
+            case 31: r = 0; break;
+            case 7: r = ad[j] * av * 2; break;
+            case 8: r = (av < 0 || ad[j] == 0) ? 0.0 : ad[j] / (2 * ov); break;
+            case 9: r = -ad[j]; break;
+            case 10: r = ad[j] * std::cos(av); break;
+            case 11: r = ad[j] * -std::sin(av); break;
+            case 12: r = ad[j] / (std::cos(av) * std::cos(av)); break;
+            case 13: r = ad[j] / std::sqrt(1 - av * av); break;
+            case 14: r = ad[j] / -std::sqrt(1 - av * av); break;
+            case 15: r = ad[j] / (av * av + 1); break;
+            case 16: r = ad[j] * ov; break;
+            case 28: r = (av > 0) ? ad[j] : -ad[j]; break;
+            case 29: r = ad[j] / -(av * av); break;
+            case 30: r = ad[j] / av; break;
+            default: r = ad[j]; break;
+        }
+        od[j] = r;
+    }
+}
+
+double eval_deriv_f64(const Deck& deck, const Tape& tape, double x, double y,
+                      double z, std::vector<double>& v, double grad[3]) {
+    const size_t n = deck.num_slots + 1;
+    v.assign(4 * n, 0.0);
+    double* g = v.data() + n;
+    for (auto& c : deck.constants) v[c.first] = c.second;
+    v[deck.X] = x;
+    v[deck.Y] = y;
+    v[deck.Z] = z;
+    g[3 * deck.X] = 1;
+    g[3 * deck.Y + 1] = 1;
+    g[3 * deck.Z + 2] = 1;
+    for (auto it = tape.t.rbegin(); it != tape.t.rend(); ++it) {
+        const double av = v[it->a], bv = v[it->b];
+        const double ov = value_op_f64(it->op, av, bv);
+        double od[3];
+        deriv_op_f64(it->op, av, g + 3 * it->a, bv, g + 3 * it->b, ov, od);
+        v[it->id] = ov;
+        for (int j = 0; j < 3; ++j) g[3 * it->id + j] = od[j];
+    }
+    for (int j = 0; j < 3; ++j) grad[j] = g[3 * tape.i + j];
+    return v[tape.i];
+}
+
 }   // namespace oracle
 
 ////////////////////////////////////////////////////////////////////////////////

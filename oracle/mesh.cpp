@@ -1,5 +1,6 @@
 // Oracle (test infrastructure only): dual-contouring mesher restatement,
-// single-threaded and recursive.
+// recursive, optionally with worker threads below the top octree levels,
+// and with an FP64 error-oracle mode (double evaluation).
 //
 // Follows:
 //   libfive/include/libfive/render/brep/region.hpp:89-114, 165-174, 226-231
@@ -236,7 +237,14 @@ struct Mesher {
     DerivEval de;
     FeatureEval fe;
     double max_err;
+    // FP64 error oracle: values, partials and the |f(v)| collapse test are
+    // evaluated in double on the unpruned tape at double positions; points
+    // where the double value is exactly zero fall back to the float path
+    // (ambiguity, isInside and features stay in float).
+    bool fp64 = false;
+    std::vector<double> sc;
     std::vector<std::array<float, 3>> verts{{0, 0, 0}};
+    std::vector<std::array<double, 3>> verts64{{0, 0, 0}};
     std::vector<std::array<uint32_t, 3>> tris;
     std::vector<std::array<uint32_t, 5>> vkeys{{0, 0, 0, 0, 0}};
     // statistics
@@ -246,7 +254,15 @@ struct Mesher {
     uint64_t collapsed = 0;
     std::vector<uint32_t> cells;   // x, y, z, level, state of every evaluated cell
 
-    Mesher(Deck& d, double me) : deck(d), ie(d), de(d), fe(d), max_err(me) {}
+    Mesher(Deck& d, double me, bool f64 = false)
+        : deck(d), ie(d), de(d), fe(d), max_err(me), fp64(f64) {}
+
+    double f64(const double* p) {
+        return eval_point_f64(deck, *deck.tape, p[0], p[1], p[2], sc);
+    }
+    double d64(const double* p, double g[3]) {
+        return eval_deriv_f64(deck, *deck.tape, p[0], p[1], p[2], sc, g);
+    }
 
     void subdivide(Cell* c) {
         for (int i = 0; i < 8; ++i) {
@@ -346,6 +362,15 @@ struct Mesher {
         for (int i = 0; i < 8; ++i) {
             if (ambig[i]) corners[i] = fe.is_inside(pos[i], tape) ? FILLED : EMPTY;
         }
+        if (fp64) {
+            for (int i = 0; i < 8; ++i) {
+                double pd[3];
+                for (int j = 0; j < 3; ++j) pd[j] = (i & (1 << j)) ? c->hi[j] : c->lo[j];
+                const double v = f64(pd);
+                if (v > 0 || std::isnan(v)) corners[i] = EMPTY;
+                else if (v < 0) corners[i] = FILLED;
+            }
+        }
         bool all_full = true, all_empty = true;
         for (int i = 0; i < 8; ++i) {
             all_full &= (corners[i] == FILLED);
@@ -393,8 +418,13 @@ struct Mesher {
                         de.set(float(ps[i][0]), float(ps[i][1]), float(ps[i][2]), i);
                     }
                 }
-                float out[12 * 16];
-                std::memcpy(out, de.values(16 * ec, *tape), sizeof(float) * 16 * ec);
+                double out[12 * 16];
+                if (fp64) {
+                    for (int i = 0; i < 16 * ec; ++i) out[i] = f64(ps[i]);
+                } else {
+                    const float* o = de.values(16 * ec, *tape);
+                    for (int i = 0; i < 16 * ec; ++i) out[i] = o[i];
+                }
                 for (int e = 0; e < ec; ++e) {
                     for (int j = 1; j < 16; ++j) {
                         const int i = j + e * 16;
@@ -422,12 +452,13 @@ struct Mesher {
                 de.set(float(ts[i][0]), float(ts[i][1]), float(ts[i][2]), 2 * i + 1);
             }
             de.derivs(2 * ec, *tape);
-            float ds[24][4];
+            double ds[24][4];
             for (int i = 0; i < 2 * ec; ++i) {
                 ds[i][0] = de.gslot(tape->i, 0, i);
                 ds[i][1] = de.gslot(tape->i, 1, i);
                 ds[i][2] = de.gslot(tape->i, 2, i);
                 ds[i][3] = de.slot(tape->i, i);
+                if (fp64) ds[i][3] = d64((i & 1) ? ts[i / 2] : tf[i / 2], ds[i]);
             }
             uint8_t amb[24];
             de.ambiguous(2 * ec, *tape, amb);
@@ -579,7 +610,7 @@ struct Mesher {
             const double vd[3] = {vf[0], vf[1], vf[2]};
             auto t = get_base(tape, vd, vd);
             de.set(vf[0], vf[1], vf[2], 0);
-            const float fv = de.values(1, *t)[0];
+            const double fv = fp64 ? f64(v) : double(de.values(1, *t)[0]);
             if (std::fabs(fv) < max_err) ok = true;
         }
         if (ok) {
@@ -603,6 +634,7 @@ struct Mesher {
             l->index[vi] = uint32_t(verts.size());
             verts.push_back({float(l->verts[vi][0]), float(l->verts[vi][1]),
                              float(l->verts[vi][2])});
+            verts64.push_back({l->verts[vi][0], l->verts[vi][1], l->verts[vi][2]});
             vkeys.push_back({t->key[0], t->key[1], t->key[2], uint32_t(t->level),
                              uint32_t(vi)});
         }
@@ -749,6 +781,7 @@ struct oracle_mesh {
     std::vector<uint32_t> tape_lengths;
     std::vector<double> leaf_verts;
     std::vector<uint32_t> cells;
+    std::vector<double> verts64;
 };
 
 extern "C" {
@@ -778,11 +811,11 @@ static void expand(Mesher& m, Cell* c, const std::shared_ptr<Tape>& tape, int de
     ex.collects.push_back({c, get_base(next, clo, chi)});
 }
 
-oracle_mesh* oracle_render_mesh_mt(oracle_deck* d, const double lo[3],
+oracle_mesh* oracle_render_mesh_ex(oracle_deck* d, const double lo[3],
                                    const double hi[3], double min_feature,
-                                   double max_err, int threads) {
+                                   double max_err, int threads, int fp64) {
     Deck& deck = d->deck;
-    Mesher m(deck, max_err);
+    Mesher m(deck, max_err, fp64 != 0);
     auto root = std::make_unique<Cell>();
     for (int j = 0; j < 3; ++j) {
         root->lo[j] = lo[j];
@@ -800,7 +833,8 @@ oracle_mesh* oracle_render_mesh_mt(oracle_deck* d, const double lo[3],
         std::vector<std::unique_ptr<Mesher>> workers;
         for (int t = 0; t < threads; ++t) {
             decks.push_back(std::make_unique<Deck>(deck));
-            workers.push_back(std::make_unique<Mesher>(*decks.back(), max_err));
+            workers.push_back(std::make_unique<Mesher>(*decks.back(), max_err,
+                                                       fp64 != 0));
         }
         std::vector<std::thread> pool;
         for (int t = 0; t < threads; ++t) {
@@ -842,7 +876,14 @@ oracle_mesh* oracle_render_mesh_mt(oracle_deck* d, const double lo[3],
     out->tape_lengths = m.tape_lengths;
     m.leaf_vertices(root.get(), out->leaf_verts);
     out->cells = std::move(m.cells);
+    for (const auto& v : m.verts64) out->verts64.insert(out->verts64.end(), v.begin(), v.end());
     return out;
+}
+
+oracle_mesh* oracle_render_mesh_mt(oracle_deck* d, const double lo[3],
+                                   const double hi[3], double min_feature,
+                                   double max_err, int threads) {
+    return oracle_render_mesh_ex(d, lo, hi, min_feature, max_err, threads, 0);
 }
 
 oracle_mesh* oracle_render_mesh(oracle_deck* d, const double lo[3],
@@ -872,6 +913,11 @@ void oracle_mesh_copy(const oracle_mesh* m, float* verts, uint32_t* tris,
 size_t oracle_mesh_leaf_verts(const oracle_mesh* m, double* out) {
     if (out) std::memcpy(out, m->leaf_verts.data(), m->leaf_verts.size() * sizeof(double));
     return m->leaf_verts.size() / 3;
+}
+
+size_t oracle_mesh_verts64(const oracle_mesh* m, double* out) {
+    if (out) std::memcpy(out, m->verts64.data(), m->verts64.size() * sizeof(double));
+    return m->verts64.size() / 3;
 }
 
 size_t oracle_mesh_cells(const oracle_mesh* m, uint32_t* out) {

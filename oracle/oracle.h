@@ -74,6 +74,16 @@ oracle_mesh* oracle_render_mesh(oracle_deck* d, const double lo[3],
 oracle_mesh* oracle_render_mesh_mt(oracle_deck* d, const double lo[3],
                                    const double hi[3], double min_feature,
                                    double max_err, int threads);
+/* With fp64 != 0: the FP64 error oracle.  The same octree algorithm, but
+ * field values, partial derivatives and the collapse test's |f(v)| are
+ * evaluated in double on the unpruned tape at double positions (points
+ * whose double value is exactly zero use the float ambiguity and feature
+ * path).  Its vertices in double come from oracle_mesh_verts64. */
+oracle_mesh* oracle_render_mesh_ex(oracle_deck* d, const double lo[3],
+                                   const double hi[3], double min_feature,
+                                   double max_err, int threads, int fp64);
+/* Emitted vertices in double (same order and count as the float ones). */
+size_t oracle_mesh_verts64(const oracle_mesh* m, double* out);
 void oracle_mesh_sizes(const oracle_mesh* m, size_t* n_verts, size_t* n_tris,
                        size_t* n_stats, size_t* n_tape_lengths);
 void oracle_mesh_copy(const oracle_mesh* m, float* verts, uint32_t* tris,

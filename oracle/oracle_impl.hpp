@@ -85,6 +85,9 @@ struct IntervalEval {
 
 double eval_point_f64(const Deck& deck, const Tape& tape, double x, double y,
                       double z, std::vector<double>& v);
+// Value and gradient in double (scratch v is resized as needed).
+double eval_deriv_f64(const Deck& deck, const Tape& tape, double x, double y,
+                      double z, std::vector<double>& v, double grad[3]);
 
 }   // namespace oracle
 

@@ -176,3 +176,14 @@ def test_no_gpu_fails_loudly():
     assert "GPU" in ffi.last_error()
     with pytest.raises(libfive_amd.GpuError):
         t.values(np.zeros(4), np.zeros(4), np.zeros(4))
+
+
+def test_specialized_kernels_compile_on_host():
+    """The straight-line kernels generated from a tape compile for gfx950
+    with hipRTC without a GPU (the GPU tests check their results)."""
+    assert ffi.lib().libfive_tree_specialized_code_size(W.csg_c1().ptr) > 0, \
+        ffi.last_error()
+    s = Shape.binary("atan2", X(), Y()) + Shape.unary("exp", Z()) + \
+        Shape.binary("mod", X(), 0.5) + X().sin() * Y().cos()
+    assert ffi.lib().libfive_tree_specialized_code_size(s.ptr) > 0, \
+        ffi.last_error()

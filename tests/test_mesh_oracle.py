@@ -349,3 +349,34 @@ def test_mesh_is_deterministic_and_keys_unique():
     assert (a.verts == b.verts).all() and (a.tris == b.tris).all()
     keys = {tuple(k) for k in a.keys[1:].tolist()}
     assert len(keys) == a.verts.shape[0] - 1
+
+
+def test_fp64_error_oracle_plane_exact():
+    """FP64 error oracle on a linear field: every vertex lies on the plane
+    to double precision (a dropped term or wrong partial would move it),
+    and the float mesher's vertices are within float rounding of it."""
+    s = X() + Y() * 0.5 + Z() * 0.25 - 0.3
+    d = oracle.OracleDeck(s.deck())
+    lo, hi = [-1.0] * 3, [1.0] * 3
+    # max_err < 0 keeps every leaf (a plane would collapse to the root)
+    r = d.render_mesh(lo, hi, 0.1, max_err=-1.0, fp64=True)
+    v = r.verts64[1:]
+    assert v.shape[0] > 100
+    f = v[:, 0] + v[:, 1] * 0.5 + v[:, 2] * 0.25 - np.float64(np.float32(0.3))
+    assert np.abs(f).max() < 1e-12
+    o = d.render_mesh(lo, hi, 0.1, max_err=-1.0)
+    err, matched, n = oracle.vertex_error(o.verts, o.keys, r)
+    assert matched == n and err < 1e-6
+
+
+def test_fp64_error_oracle_gyroid_within_1e5():
+    """Float (plain-form) mesh of the thickened gyroid against the FP64
+    error oracle: every vertex matched by key, within 1e-5."""
+    s = W.gyroid_c4()
+    d = oracle.OracleDeck(s.deck())
+    lo = [float(np.float32(-1.2))] * 3
+    hi = [float(np.float32(1.2))] * 3
+    o = d.render_mesh(lo, hi, 2.4 / 32, threads=4)
+    r = d.render_mesh(lo, hi, 2.4 / 32, threads=4, fp64=True)
+    err, matched, n = oracle.vertex_error(o.verts, o.keys, r)
+    assert matched == n and err < 1e-5

@@ -56,6 +56,8 @@ def parse():
                    help="skip the meshing measurements")
     p.add_argument("--mesh-res", type=int, default=512)
     p.add_argument("--mesh-steps", type=int, default=5)
+    p.add_argument("--batch-models", type=int, default=256,
+                   help="models in the many-model meshing leg (0: skip)")
     p.add_argument("--cpu-mesh-res", type=int, default=256,
                    help="gyroid resolution for the CPU meshing baseline")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -178,6 +180,41 @@ def mesh_measurements(args):
                                      max(1, args.mesh_steps // 2))
     out["sphere_union_512"] = {"ms": round(su_ms512, 2),
                                "verts": int(suv) - 1, "tris": int(sut)}
+    # Many independent models (256 seeded random CSG trees, 10-2000 ops,
+    # resolutions 32..256 over [-1,1]^3): one libfive_tree_render_mesh
+    # call per model (plain) against one batched call that runs 8 models
+    # at a time on their own streams.
+    if args.batch_models:
+        from libfive_amd import ffi, render_mesh_batch
+        L = ffi.lib()
+        models = W.csg_batch(args.batch_models)
+        shapes = [m for m, _ in models]
+        res = [r / 2.0 for _, r in models]
+        regions = [((-1.0,) * 3, (1.0,) * 3)] * len(shapes)
+        R = ffi.libfive_region3(ffi.libfive_interval(-1, 1),
+                                ffi.libfive_interval(-1, 1),
+                                ffi.libfive_interval(-1, 1))
+        render_mesh_batch(shapes[:8], regions[:8], res[:8], keep=False)
+        t0 = time.perf_counter()
+        tris_plain = 0
+        for sh, r in zip(shapes, res):
+            m = L.libfive_tree_render_mesh(sh.ptr, R, float(r))
+            if not m:
+                raise RuntimeError(ffi.last_error())
+            tris_plain += m.contents.tri_count
+            L.libfive_mesh_delete(m)
+        plain_ms = (time.perf_counter() - t0) * 1e3
+        t0 = time.perf_counter()
+        counts = render_mesh_batch(shapes, regions, res, streams=8, keep=False)
+        batch_ms = (time.perf_counter() - t0) * 1e3
+        tris_batch = sum(int(t) for _, t in counts)
+        out["many_models"] = {
+            "models": len(shapes), "plain_ms": round(plain_ms, 1),
+            "batched_ms": round(batch_ms, 1),
+            "batched_over_plain": round(plain_ms / batch_ms, 2),
+            "tris": tris_batch, "same_tri_count": tris_batch == tris_plain,
+        }
+
     if not args.no_cpu:
         # The oracle mesher (CPU restatement of the reference's DC mesher)
         # on the same gyroid at a lower resolution, with the GPU time there.

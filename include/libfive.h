@@ -157,6 +157,19 @@ int libfive_tape_eval_derivs(libfive_tape t, const float* x, const float* y,
                              const float* z, size_t n, float* value, float* dx,
                              float* dy, float* dz);
 
+/* Many independent meshes in one call (the many-model configuration; no
+ * counterpart in the reference, whose callers loop over
+ * libfive_tree_render_mesh, libfive.cpp:341-345).  Model i is meshed over
+ * regions[i] at res[i] exactly as libfive_tree_render_mesh would (same
+ * output), `streams` models at a time on their own GPU streams (<= 0: 8),
+ * largest resolution first.  out[i] receives the mesh or NULL; returns the
+ * number of models that failed (the first error is in
+ * libfive_last_error()). */
+int libfive_trees_render_mesh_batch(const libfive_tree* trees,
+                                    const libfive_region3* regions,
+                                    const float* res, size_t n, int streams,
+                                    libfive_mesh** out);
+
 /* Tape specialisation (no counterpart in the reference, whose evaluators
  * interpret the tape on the CPU: libfive/src/eval/eval_array.cpp:107-118).
  * Compiles the tape into straight-line value and derivative kernels with

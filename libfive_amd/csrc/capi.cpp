@@ -1,5 +1,8 @@
 // extern "C" entry points (include/libfive.h).  Tree handling follows
 // libfive/src/libfive.cpp:51-231; evaluation and meshing go to the GPU.
+#include <algorithm>
+#include <atomic>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -541,6 +544,8 @@ void libfive_last_mesh_times(double* ms4) {
     ms4[3] = st.ms_total;
 }
 
+static libfive_mesh* to_c_mesh(HostMesh& m);
+
 static libfive_mesh* render(libfive_tree tree, libfive_region3 R,
                             const libfive_mesh_settings& s, const char* who) {
     try {
@@ -557,21 +562,94 @@ static libfive_mesh* render(libfive_tree tree, libfive_region3 R,
             std::fprintf(stderr, "libfive_tree_render_mesh: got empty mesh\n");
             return nullptr;
         }
-        auto out = new libfive_mesh;
-        out->vert_count = uint32_t(m.verts.size() / 3);
-        out->tri_count = uint32_t(m.tris.size() / 3);
-        out->verts = new libfive_vec3[out->vert_count];
-        out->tris = new libfive_tri[out->tri_count];
-        std::memcpy(out->verts, m.verts.data(), m.verts.size() * sizeof(float));
-        std::memcpy(out->tris, m.tris.data(), m.tris.size() * sizeof(uint32_t));
-        {
-            std::lock_guard<std::mutex> lock(g_keys_mu);
-            g_keys[out] = std::move(m.keys);
-        }
-        return out;
+        return to_c_mesh(m);
     } catch (const std::exception& e) {
         set_error(who, e);
         return nullptr;
+    }
+}
+
+static libfive_mesh* to_c_mesh(HostMesh& m) {
+    auto out = new libfive_mesh;
+    out->vert_count = uint32_t(m.verts.size() / 3);
+    out->tri_count = uint32_t(m.tris.size() / 3);
+    out->verts = new libfive_vec3[out->vert_count];
+    out->tris = new libfive_tri[out->tri_count];
+    std::memcpy(out->verts, m.verts.data(), m.verts.size() * sizeof(float));
+    std::memcpy(out->tris, m.tris.data(), m.tris.size() * sizeof(uint32_t));
+    std::lock_guard<std::mutex> lock(g_keys_mu);
+    g_keys[out] = std::move(m.keys);
+    return out;
+}
+
+int libfive_trees_render_mesh_batch(const libfive_tree* trees,
+                                    const libfive_region3* regions,
+                                    const float* res, size_t n, int streams,
+                                    libfive_mesh** out) {
+    for (size_t i = 0; i < n; ++i) out[i] = nullptr;
+    if (n == 0) return 0;
+    std::vector<hipStream_t> ss;
+    try {
+        context();
+        const int k = int(std::max<size_t>(1, std::min<size_t>(
+            n, size_t(streams > 0 ? streams : 8))));
+        for (int i = 0; i < k; ++i) {
+            hipStream_t st = nullptr;
+            hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking),
+                      "batch stream");
+            ss.push_back(st);
+        }
+        // Largest models first (cells grow with res^3) so that the last
+        // models to start are the short ones.
+        std::vector<size_t> order(n);
+        for (size_t i = 0; i < n; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+            return res[a] > res[b];
+        });
+        std::atomic<size_t> next{0};
+        std::atomic<int> failed{0};
+        std::mutex err_mu;
+        std::string first_error;
+        auto worker = [&](int w) {
+            for (;;) {
+                const size_t j = next.fetch_add(1);
+                if (j >= n) break;
+                const size_t i = order[j];
+                try {
+                    MeshSettings ms;
+                    ms.min_feature = 1 / res[i];
+                    const libfive_region3& R = regions[i];
+                    const double lo[3] = {R.X.lower, R.Y.lower, R.Z.lower};
+                    const double hi[3] = {R.X.upper, R.Y.upper, R.Z.upper};
+                    HostMesh m;
+                    if (render_mesh_on(Tree(D(trees[i])), lo, hi, ms, m, ss[w])) {
+                        out[i] = to_c_mesh(m);
+                    } else {
+                        failed++;
+                    }
+                } catch (const std::exception& e) {
+                    failed++;
+                    std::lock_guard<std::mutex> lock(err_mu);
+                    if (first_error.empty()) {
+                        first_error = "model " + std::to_string(i) + ": " + e.what();
+                    }
+                }
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int w = 0; w < k; ++w) pool.emplace_back(worker, w);
+        for (auto& th : pool) th.join();
+        for (auto st : ss) (void)hipStreamDestroy(st);
+        ss.clear();
+        if (failed) {
+            set_error("libfive_trees_render_mesh_batch",
+                      Error(first_error.empty() ? "cancelled" : first_error));
+        }
+        return failed;
+    } catch (const std::exception& e) {
+        for (auto st : ss) (void)hipStreamDestroy(st);
+        set_error("libfive_trees_render_mesh_batch", e);
+        return int(n);
     }
 }
 

@@ -569,9 +569,14 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
         const int corner = lane & 7;
         float cp[3];
         for (int a = 0; a < 3; ++a) cp[a] = float(((corner >> a) & 1) ? hi[a] : lo[a]);
-        float g[3];
+        // Values first; partials only when a corner is an unambiguous zero.
+        float g[3] = {0.0f, 0.0f, 0.0f};
         uint32_t fl;
-        const float v = pe_deriv<NV>(t, n, root, cp[0], cp[1], cp[2], g, &fl);
+        const float v = pe_value<NV>(t, n, root, cp[0], cp[1], cp[2], &fl);
+        if (ballot(lane < 8 && v == 0.0f && !(fl & PT_AMBIGUOUS))) {
+            uint32_t fl2;
+            (void)pe_deriv<NV>(t, n, root, cp[0], cp[1], cp[2], g, &fl2);
+        }
         bool inside;
         if (v > 0.0f || fmath::isnan_f(v)) inside = false;
         else if (v < 0.0f) inside = true;

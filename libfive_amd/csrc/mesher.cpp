@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <thread>
 
 #include "deck.hpp"
@@ -163,17 +164,27 @@ void run_leaves(const Params& P, const uint32_t* groups, uint32_t ngroups,
 
 bool render_mesh(const Tree& tree, const double lo[3], const double hi[3],
                  const MeshSettings& settings, HostMesh& out) {
-    const auto t_start = std::chrono::steady_clock::now();
     Context& ctx = context();
     std::lock_guard<std::mutex> lock(ctx.mu);
-    hipStream_t s = ctx.stream;
-    g_stats = MeshStats();
+    return render_mesh_on(tree, lo, hi, settings, out, ctx.stream);
+}
 
-    static bool tables_ready = false;
-    if (!tables_ready) {
+// The lookup tables live in __constant__ memory: uploaded once per process
+// and completed before any render uses them.
+static void ensure_tables() {
+    static std::once_flag once;
+    std::call_once(once, [] {
+        hipStream_t s = context().stream;
         hip_check(upload_tables(s), "mesh tables");
-        tables_ready = true;
-    }
+        hip_check(hipStreamSynchronize(s), "mesh tables");
+    });
+}
+
+bool render_mesh_on(const Tree& tree, const double lo[3], const double hi[3],
+                    const MeshSettings& settings, HostMesh& out, hipStream_t s) {
+    const auto t_start = std::chrono::steady_clock::now();
+    g_stats = MeshStats();
+    ensure_tables();
 
     const Deck deck(tree);
     const DeviceTape base = compile_deck(deck, nullptr);

@@ -6,6 +6,8 @@
 #include <cstdint>
 #include <vector>
 
+#include <hip/hip_runtime.h>
+
 #include "tree.hpp"
 
 namespace libfive {
@@ -39,9 +41,15 @@ struct MeshStats {
     double ms_build = 0, ms_collapse = 0, ms_emit = 0, ms_total = 0;
 };
 
-// Returns false when cancelled (the C API then returns NULL).
+// Returns false when cancelled (the C API then returns NULL).  Runs on the
+// library stream, one render at a time.
 bool render_mesh(const Tree& t, const double lo[3], const double hi[3],
                  const MeshSettings& s, HostMesh& out);
+
+// The same on the caller's stream, without the library lock: renders on
+// different streams (from different host threads) run concurrently.
+bool render_mesh_on(const Tree& t, const double lo[3], const double hi[3],
+                    const MeshSettings& s, HostMesh& out, hipStream_t stream);
 
 const MeshStats& last_mesh_stats();
 

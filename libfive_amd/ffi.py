@@ -112,6 +112,10 @@ SIGNATURES = {
     "libfive_tape_eval_derivs": (ctypes.c_int, [tape_t, fp, fp, fp,
                                                 ctypes.c_size_t, fp, fp, fp,
                                                 fp]),
+    "libfive_trees_render_mesh_batch": (
+        ctypes.c_int, [ctypes.POINTER(tree_t), ctypes.POINTER(libfive_region3),
+                       fp, ctypes.c_size_t, ctypes.c_int,
+                       ctypes.POINTER(ctypes.POINTER(libfive_mesh))]),
     "libfive_tape_specialize": (ctypes.c_int,
                                 [tape_t, ctypes.POINTER(ctypes.c_double)]),
     "libfive_tape_use_specialized": (None, [tape_t, ctypes.c_int]),

@@ -250,6 +250,47 @@ class Shape:
         return out
 
 
+def _mesh_arrays(m):
+    mm = m.contents
+    verts = np.ctypeslib.as_array(
+        ctypes.cast(mm.verts, ffi.fp), (mm.vert_count, 3)).copy()
+    if mm.tri_count:
+        tris = np.ctypeslib.as_array(
+            ctypes.cast(mm.tris, ffi.u32p), (mm.tri_count, 3)).copy()
+    else:
+        tris = np.zeros((0, 3), np.uint32)
+    return verts, tris
+
+
+def render_mesh_batch(shapes, regions, res, streams=8, keep=True):
+    """Meshes many shapes in one call (libfive_trees_render_mesh_batch).
+    regions[i] = (lo, hi) 3-vectors, res[i] = cells per unit.  Returns a
+    list of (verts, tris), or of (vert_count, tri_count) with keep=False."""
+    L = ffi.lib()
+    n = len(shapes)
+    trees = (ffi.tree_t * n)(*[s.ptr for s in shapes])
+    R = (ffi.libfive_region3 * n)()
+    for i, (lo, hi) in enumerate(regions):
+        R[i] = ffi.libfive_region3(ffi.libfive_interval(lo[0], hi[0]),
+                                   ffi.libfive_interval(lo[1], hi[1]),
+                                   ffi.libfive_interval(lo[2], hi[2]))
+    rr = np.ascontiguousarray(res, np.float32)
+    out = (ctypes.POINTER(ffi.libfive_mesh) * n)()
+    failed = L.libfive_trees_render_mesh_batch(
+        trees, R, rr.ctypes.data_as(ffi.fp), n, int(streams), out)
+    try:
+        if failed:
+            raise GpuError(ffi.last_error())
+        if keep:
+            return [_mesh_arrays(out[i]) for i in range(n)]
+        return [(out[i].contents.vert_count, out[i].contents.tri_count)
+                for i in range(n)]
+    finally:
+        for i in range(n):
+            if out[i]:
+                L.libfive_mesh_delete(out[i])
+
+
 class MeshResult:
     """A mesh with its vertex keys (owning cell x, y, z in finest-cell
     units, level, patch), per-level cell counts (root level first: empty,

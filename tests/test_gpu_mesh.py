@@ -165,3 +165,20 @@ def test_empty_region_gives_dummy_vertex_only(gpu):
 def test_root_leaf(gpu):
     """min_feature larger than the region: the root is evaluated as a leaf."""
     compare(W.sphere(0.5), -1.0, 1.0, 5.0)
+
+
+def test_batch_matches_single_renders(gpu):
+    """libfive_trees_render_mesh_batch gives, model by model, the meshes of
+    libfive_tree_render_mesh (same vertex and triangle arrays)."""
+    from libfive_amd import render_mesh_batch
+    models = W.csg_batch(24, seed=W.SEED_BASE + 41)
+    shapes = [m for m, _ in models]
+    res = [r / 2.0 for _, r in models]       # cells per unit over [-1, 1]
+    regions = [((-1, -1, -1), (1, 1, 1))] * len(shapes)
+    print("batch of", len(shapes), "models, res", sorted(set(res)), flush=True)
+    got = render_mesh_batch(shapes, regions, res, streams=6)
+    for s, r, (v, t) in zip(shapes, res, got):
+        v1, t1 = s.render_mesh([-1] * 3, [1] * 3, res=r)
+        assert v.shape == v1.shape and t.shape == t1.shape
+        assert (v.view(np.uint32) == v1.view(np.uint32)).all()
+        assert (t == t1).all()

@@ -5,6 +5,7 @@
 
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -48,7 +49,12 @@ struct Names {
 };
 
 // Operands of a clause exactly as the interpreter loop forms them
-// (LF_VALUE_STEP and LF_DERIV_STEP in eval_kernels.hip).
+// (LF_VALUE_STEP and LF_DERIV_STEP in eval_kernels.hip).  After each clause
+// an empty asm statement takes its results as in-out register operands:
+// the clause is computed there, in tape order, instead of being sunk
+// towards its uses (which keeps all of its inputs alive; on the
+// 809-clause benchmark tape the derivative kernel otherwise needs all 512
+// VGPRs and spills, and with the barriers it needs 168).
 void value_body(const DeviceTape& t, std::string& out) {
     Names n(t.num_regs);
     char buf[256];
@@ -63,8 +69,9 @@ void value_body(const DeviceTape& t, std::string& out) {
                               : (m == MODE_B_IMM) ? im : n.v[rb(c)];
         std::snprintf(buf, sizeof(buf),
                       "    float v%zu; { const float a_ = %s, b_ = %s; "
-                      "apply_value<1>(%uu, &a_, &b_, %s, &v%zu); }\n",
-                      k, a.c_str(), b.c_str(), o, im.c_str(), k);
+                      "apply_value<1>(%uu, &a_, &b_, %s, &v%zu); }\n"
+                      "    asm volatile(\"\" : \"+v\"(v%zu));\n",
+                      k, a.c_str(), b.c_str(), o, im.c_str(), k, k);
         out += buf;
         n.v[dst(c)] = "v" + std::to_string(k);
         ++k;
@@ -98,10 +105,12 @@ void deriv_body(const DeviceTape& t, std::string& out) {
         std::snprintf(buf, sizeof(buf),
                       "    float v%zu, d%zu[3]; { const float ad_[3] = {%s, %s, %s}; "
                       "const float bd_[3] = {%s, %s, %s}; "
-                      "apply_deriv(%uu, %s, ad_, %s, bd_, %s, v%zu, d%zu); }\n",
+                      "apply_deriv(%uu, %s, ad_, %s, bd_, %s, v%zu, d%zu); }\n"
+                      "    asm volatile(\"\" : \"+v\"(v%zu), \"+v\"(d%zu[0]), "
+                      "\"+v\"(d%zu[1]), \"+v\"(d%zu[2]));\n",
                       k, k, ad[0].c_str(), ad[1].c_str(), ad[2].c_str(),
                       bd[0].c_str(), bd[1].c_str(), bd[2].c_str(), o,
-                      av.c_str(), bv.c_str(), im.c_str(), k, k);
+                      av.c_str(), bv.c_str(), im.c_str(), k, k, k, k, k, k);
         out += buf;
         const uint32_t d = dst(c);
         n.v[d] = "v" + std::to_string(k);
@@ -154,6 +163,13 @@ static void rtc_check(hiprtcResult r, const char* what) {
 
 std::vector<char> compile_specialized(const DeviceTape& t) {
     const std::string src = specialized_source(t);
+    // LIBFIVE_AMD_JIT_DUMP=<path>: write the generated source there.
+    if (const char* dump = std::getenv("LIBFIVE_AMD_JIT_DUMP")) {
+        if (FILE* f = std::fopen(dump, "w")) {
+            std::fwrite(src.data(), 1, src.size(), f);
+            std::fclose(f);
+        }
+    }
     hiprtcProgram prog = nullptr;
     rtc_check(hiprtcCreateProgram(&prog, src.c_str(), "lf_tape.hip",
                                   int(sizeof(kJitHeaderNames) / sizeof(kJitHeaderNames[0])),

@@ -181,7 +181,8 @@ def test_no_gpu_fails_loudly():
 def test_specialized_kernels_compile_on_host():
     """The straight-line kernels generated from a tape compile for gfx950
     with hipRTC without a GPU (the GPU tests check their results)."""
-    assert ffi.lib().libfive_tree_specialized_code_size(W.csg_c1().ptr) > 0, \
+    c1 = W.csg_c1()
+    assert ffi.lib().libfive_tree_specialized_code_size(c1.ptr) > 0, \
         ffi.last_error()
     s = Shape.binary("atan2", X(), Y()) + Shape.unary("exp", Z()) + \
         Shape.binary("mod", X(), 0.5) + X().sin() * Y().cos()

@@ -575,7 +575,7 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
         const float v = pe_value<NV>(t, n, root, cp[0], cp[1], cp[2], &fl);
         if (ballot(lane < 8 && v == 0.0f && !(fl & PT_AMBIGUOUS))) {
             uint32_t fl2;
-            (void)pe_deriv<NV>(t, n, root, cp[0], cp[1], cp[2], g, &fl2);
+            (void)pe_grad3<NV>(t, n, root, cp[0], cp[1], cp[2], g, &fl2);
         }
         bool inside;
         if (v > 0.0f || fmath::isnan_f(v)) inside = false;
@@ -649,7 +649,7 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
                     if (ballot(fast)) {
                         float sg[3];
                         uint32_t f2;
-                        pe_deriv<NV>(t, n, root, float(ps[0]), float(ps[1]), float(ps[2]), sg, &f2);
+                        pe_grad3<NV>(t, n, root, float(ps[0]), float(ps[1]), float(ps[2]), sg, &f2);
                         const float nrm = sqrtf(sg[0] * sg[0] + (sg[1] * sg[1] + sg[2] * sg[2]));
                         if (fast) zero_out = !(nrm > 0.0f);
                     }
@@ -696,7 +696,7 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
                 for (int q = 0; q < 3; ++q) ep[q] = (lane & 1) ? s_ts[e][q] : s_tf[e][q];
                 float eg[3];
                 uint32_t ef;
-                const float ev = pe_deriv<NV>(t, n, root, float(ep[0]), float(ep[1]),
+                const float ev = pe_grad3<NV>(t, n, root, float(ep[0]), float(ep[1]),
                                               float(ep[2]), eg, &ef);
                 if (lane < 2 * ec) {
                     s_v[lane] = ev;

@@ -274,5 +274,87 @@ __device__ __forceinline__ float pe_deriv(const tape::Clause* __restrict__ t,
     return LF_RF_GET(v, NV, root);
 }
 
+// Value and one partial derivative (axis 0, 1, 2 for x, y, z): the same
+// rules with one gradient component carried, so the register file is two
+// banks instead of four; the component is bit-identical to pe_deriv's.
+#define LF_PE_DERIV1_STEP(C)                                                 \
+    do {                                                                     \
+        const tape::Clause c_ = (C);                                         \
+        const uint32_t o_ = tape::op(c_);                                    \
+        if (o_ != tape::OP_NOP) {                                            \
+            const uint32_t m_ = tape::mode(c_);                              \
+            const float im_ = tape::imm(c_);                                 \
+            float av_ = im_, ad_[3] = {0.0f, 0.0f, 0.0f};                    \
+            float bv_ = 0.0f, bd_[3] = {0.0f, 0.0f, 0.0f};                   \
+            if (m_ != tape::MODE_A_IMM && o_ != tape::OP_LOADI) {            \
+                const uint32_t a_ = tape::ra(c_);                            \
+                av_ = LF_RF_GET(v, NV, a_);                                  \
+                ad_[0] = LF_RF_GET(gd, NV, a_);                              \
+            }                                                                \
+            if (tape::is_binary(c_)) {                                       \
+                if (m_ == tape::MODE_B_IMM) {                                \
+                    bv_ = im_;                                               \
+                } else {                                                     \
+                    const uint32_t b_ = tape::rb(c_);                        \
+                    bv_ = LF_RF_GET(v, NV, b_);                              \
+                    bd_[0] = LF_RF_GET(gd, NV, b_);                          \
+                }                                                            \
+            }                                                                \
+            flags |= pe_minmax_flags(o_, av_, bv_);                          \
+            float ov_, od_[3];                                               \
+            pe_deriv_op(o_, av_, ad_, bv_, bd_, im_, ov_, od_);              \
+            const uint32_t d_ = tape::dst(c_);                               \
+            LF_RF_SET(v, NV, d_, ov_);                                       \
+            LF_RF_SET(gd, NV, d_, od_[0]);                                   \
+        }                                                                    \
+    } while (0)
+
+template <int NV>
+__device__ __forceinline__ float pe_deriv1(const tape::Clause* __restrict__ t,
+                                           uint32_t n, uint32_t root, float x,
+                                           float y, float z, int axis,
+                                           float* g, uint32_t* out_flags)
+{
+    LF_RF_DECL(v);
+    LF_RF_DECL(gd);
+    LF_RF_SET(v, NV, tape::kRegX, x);
+    LF_RF_SET(v, NV, tape::kRegY, y);
+    LF_RF_SET(v, NV, tape::kRegZ, z);
+    LF_RF_SET(gd, NV, tape::kRegX, axis == 0 ? 1.0f : 0.0f);
+    LF_RF_SET(gd, NV, tape::kRegY, axis == 1 ? 1.0f : 0.0f);
+    LF_RF_SET(gd, NV, tape::kRegZ, axis == 2 ? 1.0f : 0.0f);
+    uint32_t flags = 0;
+    const Clause4x* t4 = reinterpret_cast<const Clause4x*>(t);
+    for (uint32_t k = 0; k < n; k += tape::kFetch) {
+        const Clause4x cur = t4[k / tape::kFetch];
+        LF_PE_DERIV1_STEP(cur.c[0]);
+        LF_PE_DERIV1_STEP(cur.c[1]);
+        LF_PE_DERIV1_STEP(cur.c[2]);
+        LF_PE_DERIV1_STEP(cur.c[3]);
+    }
+    *out_flags = flags;
+    *g = LF_RF_GET(gd, NV, root);
+    return LF_RF_GET(v, NV, root);
+}
+
+// Value and gradient as three single-component passes (pe_deriv's bits).
+template <int NV>
+__device__ __forceinline__ float pe_grad3(const tape::Clause* __restrict__ t,
+                                          uint32_t n, uint32_t root, float x,
+                                          float y, float z, float g[3],
+                                          uint32_t* out_flags)
+{
+    float v = 0.0f;
+#pragma unroll 1
+    for (int axis = 0; axis < 3; ++axis) {
+        float ga;
+        v = pe_deriv1<NV>(t, n, root, x, y, z, axis, &ga, out_flags);
+        if (axis == 0) g[0] = ga;
+        else if (axis == 1) g[1] = ga;
+        else g[2] = ga;
+    }
+    return v;
+}
+
 }   // namespace amd
 }   // namespace libfive

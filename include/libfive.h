@@ -223,8 +223,9 @@ libfive_mesh* libfive_tree_render_mesh_settings(libfive_tree tree,
 /* Per-vertex keys of a mesh returned by this library: 5 words per vertex
  * (x, y, z of the owning cell's lower corner in finest-cell units, the
  * cell's octree level, the vertex patch within the cell), starting with the
- * dummy vertex 0.  Valid until libfive_mesh_delete; NULL for other meshes.
- * Used to compare meshes independently of vertex numbering. */
+ * dummy vertex 0.  Kept for meshes rendered with collect_stats set; valid
+ * until libfive_mesh_delete; NULL for other meshes.  Used to compare meshes
+ * independently of vertex numbering. */
 const uint32_t* libfive_mesh_vertex_keys(const libfive_mesh* m);
 
 /* Statistics of the last render on the calling thread made with

@@ -544,7 +544,42 @@ void libfive_last_mesh_times(double* ms4) {
     ms4[3] = st.ms_total;
 }
 
-static libfive_mesh* to_c_mesh(HostMesh& m);
+// Renders into a new libfive_mesh whose arrays (new[], as
+// libfive_mesh_delete expects) receive the copy from the device directly.
+// Returns NULL when cancelled.
+static libfive_mesh* render_to_c(const Tree& tree, const double lo[3],
+                                 const double hi[3], const MeshSettings& ms,
+                                 bool keys, hipStream_t* stream) {
+    std::unique_ptr<libfive_mesh> out(new libfive_mesh{nullptr, nullptr, 0, 0});
+    struct Arrays {
+        libfive_mesh* m;
+        ~Arrays() {
+            if (m) {
+                delete[] m->verts;
+                delete[] m->tris;
+            }
+        }
+    } arrays{out.get()};
+    HostMesh hm;
+    hm.want_keys = keys;
+    hm.alloc = [&](size_t nv, size_t nt, float** v, uint32_t** t) {
+        out->verts = new libfive_vec3[nv];
+        out->tris = new libfive_tri[nt ? nt : 1];
+        out->vert_count = uint32_t(nv);
+        out->tri_count = uint32_t(nt);
+        *v = &out->verts[0].x;
+        *t = &out->tris[0].a;
+    };
+    const bool done = stream ? render_mesh_on(tree, lo, hi, ms, hm, *stream)
+                             : render_mesh(tree, lo, hi, ms, hm);
+    if (!done) return nullptr;
+    arrays.m = nullptr;
+    if (keys) {
+        std::lock_guard<std::mutex> lock(g_keys_mu);
+        g_keys[out.get()] = std::move(hm.keys);
+    }
+    return out.release();
+}
 
 static libfive_mesh* render(libfive_tree tree, libfive_region3 R,
                             const libfive_mesh_settings& s, const char* who) {
@@ -557,29 +592,14 @@ static libfive_mesh* render(libfive_tree tree, libfive_region3 R,
         ms.cancel = s.cancel;
         const double lo[3] = {R.X.lower, R.Y.lower, R.Z.lower};
         const double hi[3] = {R.X.upper, R.Y.upper, R.Z.upper};
-        HostMesh m;
-        if (!render_mesh(Tree(D(tree)), lo, hi, ms, m)) {
-            std::fprintf(stderr, "libfive_tree_render_mesh: got empty mesh\n");
-            return nullptr;
-        }
-        return to_c_mesh(m);
+        libfive_mesh* m = render_to_c(Tree(D(tree)), lo, hi, ms,
+                                      s.collect_stats != 0, nullptr);
+        if (!m) std::fprintf(stderr, "libfive_tree_render_mesh: got empty mesh\n");
+        return m;
     } catch (const std::exception& e) {
         set_error(who, e);
         return nullptr;
     }
-}
-
-static libfive_mesh* to_c_mesh(HostMesh& m) {
-    auto out = new libfive_mesh;
-    out->vert_count = uint32_t(m.verts.size() / 3);
-    out->tri_count = uint32_t(m.tris.size() / 3);
-    out->verts = new libfive_vec3[out->vert_count];
-    out->tris = new libfive_tri[out->tri_count];
-    std::memcpy(out->verts, m.verts.data(), m.verts.size() * sizeof(float));
-    std::memcpy(out->tris, m.tris.data(), m.tris.size() * sizeof(uint32_t));
-    std::lock_guard<std::mutex> lock(g_keys_mu);
-    g_keys[out] = std::move(m.keys);
-    return out;
 }
 
 int libfive_trees_render_mesh_batch(const libfive_tree* trees,
@@ -621,12 +641,8 @@ int libfive_trees_render_mesh_batch(const libfive_tree* trees,
                     const libfive_region3& R = regions[i];
                     const double lo[3] = {R.X.lower, R.Y.lower, R.Z.lower};
                     const double hi[3] = {R.X.upper, R.Y.upper, R.Z.upper};
-                    HostMesh m;
-                    if (render_mesh_on(Tree(D(trees[i])), lo, hi, ms, m, ss[w])) {
-                        out[i] = to_c_mesh(m);
-                    } else {
-                        failed++;
-                    }
+                    out[i] = render_to_c(Tree(D(trees[i])), lo, hi, ms, false, &ss[w]);
+                    if (!out[i]) failed++;
                 } catch (const std::exception& e) {
                     failed++;
                     std::lock_guard<std::mutex> lock(err_mu);

@@ -689,32 +689,43 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
                     __syncthreads();
                 }
             }
-            // Derivatives at both bracket ends (dc_tree.inl:412-471).
+            // Derivatives at both bracket ends (dc_tree.inl:412-471): one
+            // lane per (axis, end), one partial per lane, so that the 2*ec
+            // ends of all three axes share a pass.
             {
-                const int e = (lane >> 1) < ec ? (lane >> 1) : 0;
-                double ep[3];
-                for (int q = 0; q < 3; ++q) ep[q] = (lane & 1) ? s_ts[e][q] : s_tf[e][q];
-                float eg[3];
-                uint32_t ef;
-                const float ev = pe_grad3<NV>(t, n, root, float(ep[0]), float(ep[1]),
-                                              float(ep[2]), eg, &ef);
-                if (lane < 2 * ec) {
-                    s_v[lane] = ev;
-                    s_g[lane][0] = eg[0];
-                    s_g[lane][1] = eg[1];
-                    s_g[lane][2] = eg[2];
-                    s_fl[lane] = ef;
+                const int m = 2 * ec;
+                for (int base = 0; base < 3 * m; base += 64) {
+                    const int i = base + lane;
+                    const int axis = i / m, k = i - axis * m;
+                    const bool valid = axis < 3;
+                    const int e = valid ? (k >> 1) : 0;
+                    const bool up = valid && (k & 1);
+                    float ep[3];
+                    for (int q = 0; q < 3; ++q) ep[q] = float(up ? s_ts[e][q] : s_tf[e][q]);
+                    float ga;
+                    uint32_t ef;
+                    const float ev = pe_deriv1<NV>(t, n, root, ep[0], ep[1], ep[2],
+                                                   valid ? axis : 0, &ga, &ef);
+                    if (valid) {
+                        s_g[k][axis] = ga;
+                        if (axis == 0) {
+                            s_v[k] = ev;
+                            s_fl[k] = ef;
+                        }
+                    }
                 }
+                __syncthreads();
                 // Ambiguous ends contribute every distinct gradient of the
                 // point (FeatureEvaluator::features).
-                uint64_t need = ballot(lane < 2 * ec && (ef & PT_AMBIGUOUS));
+                uint64_t need = ballot(lane < m && (s_fl[lane < m ? lane : 0] & PT_AMBIGUOUS));
                 if (need) {
                     if (!FEAT) { defer(); return; }
-                    const float fp[3] = {float(ep[0]), float(ep[1]), float(ep[2])};
                     while (need) {
                         const int l = __builtin_ctzll(need);
                         need &= need - 1;
-                        const float q[3] = {__shfl(fp[0], l), __shfl(fp[1], l), __shfl(fp[2], l)};
+                        const int e = l >> 1;
+                        float q[3];
+                        for (int c = 0; c < 3; ++c) q[c] = float((l & 1) ? s_ts[e][c] : s_tf[e][c]);
                         if (lane == 0) {
                             float fo[feat::kMaxOut][3];
                             const int nf = feat::features(t, n, P.nregs, root, P.slot_x,

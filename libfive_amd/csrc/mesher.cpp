@@ -46,11 +46,11 @@ int resolution_level(const double lo[3], const double hi[3], double min_feature)
 namespace {
 
 struct LevelStore {
-    DevBuf<uint32_t> key;
-    DevBuf<uint8_t> type;
-    DevBuf<int32_t> child;
-    DevBuf<uint32_t> tape;
-    DevBuf<LeafRec> leaf;
+    PoolBuf<uint32_t> key;
+    PoolBuf<uint8_t> type;
+    PoolBuf<int32_t> child;
+    PoolBuf<uint32_t> tape;
+    PoolBuf<LeafRec> leaf;
     uint32_t n = 0;
 
     void alloc(uint32_t count) {
@@ -77,16 +77,15 @@ struct LevelStore {
 void wait(hipStream_t s, const char* what);
 
 template <typename T>
-void grow(DevBuf<T>& b, size_t used, size_t need, hipStream_t s) {
+void grow(PoolBuf<T>& b, size_t used, size_t need, hipStream_t s) {
     if (need <= b.n) return;
-    DevBuf<T> nb(std::max(need, 2 * b.n));
+    PoolBuf<T> nb(std::max(need, 2 * b.n));
     if (used) {
         hip_check(hipMemcpyAsync(nb.p, b.p, used * sizeof(T),
                                  hipMemcpyDeviceToDevice, s), "grow copy");
     }
     wait(s, "buffer growth");
-    std::swap(b.p, nb.p);
-    std::swap(b.n, nb.n);
+    b.swap(nb);
 }
 
 double ms_since(std::chrono::steady_clock::time_point t0) {
@@ -131,10 +130,10 @@ void check_errors(uint32_t err) {
 // with the feature evaluation, in batches that bound the workspace.
 void run_leaves(const Params& P, const uint32_t* groups, uint32_t ngroups,
                 uint32_t per_group, const LevelView& l1, const LevelView& l0,
-                uint32_t max_len, DevBuf<Counters>& counters, Counters& hc,
+                uint32_t max_len, PoolBuf<Counters>& counters, Counters& hc,
                 hipStream_t s) {
     const uint32_t cells = ngroups * per_group;
-    DevBuf<uint32_t> deferred(cells);
+    PoolBuf<uint32_t> deferred(cells);
     hip_check(hipMemsetAsync(&counters.p->deferred, 0, sizeof(uint32_t), s), "deferred");
     hip_check(launch_leaf(P, groups, ngroups, per_group, l1, l0, deferred.p, s),
               "leaf kernel");
@@ -147,7 +146,7 @@ void run_leaves(const Params& P, const uint32_t* groups, uint32_t ngroups,
     if (nd > cells) throw Error("render_mesh: inconsistent deferred cell count");
     const size_t ws_bytes = leaf_feature_workspace(max_len, P.nregs);
     const size_t batch = std::max<size_t>(1, std::min<size_t>(nd, (size_t(1) << 30) / ws_bytes));
-    DevBuf<uint8_t> ws(batch * ws_bytes);
+    PoolBuf<uint8_t> ws(batch * ws_bytes);
     for (size_t first = 0; first < nd; first += batch) {
         const uint32_t count = uint32_t(std::min<size_t>(batch, nd - first));
         hip_check(launch_leaf_features(P, groups, ngroups, per_group, l1, l0,
@@ -180,8 +179,24 @@ static void ensure_tables() {
     });
 }
 
+static bool render_body(const Tree& tree, const double lo[3], const double hi[3],
+                        const MeshSettings& settings, HostMesh& out, hipStream_t s);
+
 bool render_mesh_on(const Tree& tree, const double lo[3], const double hi[3],
                     const MeshSettings& settings, HostMesh& out, hipStream_t s) {
+    pool_defer_begin();
+    try {
+        const bool r = render_body(tree, lo, hi, settings, out, s);
+        pool_defer_end(s);
+        return r;
+    } catch (...) {
+        pool_defer_end(s);
+        throw;
+    }
+}
+
+static bool render_body(const Tree& tree, const double lo[3], const double hi[3],
+                        const MeshSettings& settings, HostMesh& out, hipStream_t s) {
     const auto t_start = std::chrono::steady_clock::now();
     g_stats = MeshStats();
     ensure_tables();
@@ -200,14 +215,14 @@ bool render_mesh_on(const Tree& tree, const double lo[3], const double hi[3],
     // Clause pool and tape records; the base tape is record 0.
     const uint32_t nbase = uint32_t(base.clauses.size());
     const uint32_t base_padded = tape::padded_length(nbase);
-    DevBuf<tape::Clause> pool(std::max<size_t>(size_t(base_padded) * 64, 1u << 20));
+    PoolBuf<tape::Clause> pool(std::max<size_t>(size_t(base_padded) * 64, 1u << 20));
     {
         std::vector<tape::Clause> h(base_padded, tape::Clause{0, 0, 0, 0});
         std::copy(base.clauses.begin(), base.clauses.end(), h.begin());
         hip_check(hipMemcpyAsync(pool.p, h.data(), h.size() * sizeof(tape::Clause),
                                  hipMemcpyHostToDevice, s), "upload tape");
     }
-    DevBuf<TapeRec> tapes(1u << 16);
+    PoolBuf<TapeRec> tapes(1u << 16);
     {
         TapeRec r{};
         r.off = 0;
@@ -218,7 +233,7 @@ bool render_mesh_on(const Tree& tree, const double lo[3], const double hi[3],
         hip_check(hipMemcpyAsync(tapes.p, &r, sizeof(r), hipMemcpyHostToDevice, s),
                   "upload tape record");
     }
-    DevBuf<Counters> counters(1);
+    PoolBuf<Counters> counters(1);
     Counters hc{};
     hc.pool_top = base_padded;
     hc.tape_count = 1;
@@ -247,10 +262,10 @@ bool render_mesh_on(const Tree& tree, const double lo[3], const double hi[3],
     refresh();
 
     std::vector<LevelStore> levels(top + 1);
-    std::vector<DevBuf<uint32_t>> groups(top + 2);
+    std::vector<PoolBuf<uint32_t>> groups(top + 2);
     // Per-level scan buffers stay allocated until the render ends, so that
     // no device memory is released while kernels are in flight.
-    std::vector<DevBuf<uint32_t>> flags(top + 1), basei(top + 1), scan_tmp(top + 1);
+    std::vector<PoolBuf<uint32_t>> flags(top + 1), basei(top + 1), scan_tmp(top + 1);
 
     // ---- build ------------------------------------------------------------
     levels[top].alloc(1);
@@ -375,14 +390,14 @@ bool render_mesh_on(const Tree& tree, const double lo[3], const double hi[3],
 
     // ---- emit -----------------------------------------------------------------
     const auto t_emit = std::chrono::steady_clock::now();
-    DevBuf<LevelView> d_views(top + 1);
+    PoolBuf<LevelView> d_views(top + 1);
     hip_check(hipMemcpyAsync(d_views.p, views.data(), views.size() * sizeof(LevelView),
                              hipMemcpyHostToDevice, s), "views");
     P.levels = d_views.p;
-    DevBuf<uint32_t> tri_count(total_cells), tri_off(total_cells);
-    DevBuf<uint32_t> vslot(4 * total_cells), vid(4 * total_cells);
-    DevBuf<uint32_t> scan_a(scan_scratch_words(total_cells));
-    DevBuf<uint32_t> scan_b(scan_scratch_words(4 * total_cells));
+    PoolBuf<uint32_t> tri_count(total_cells), tri_off(total_cells);
+    PoolBuf<uint32_t> vslot(4 * total_cells), vid(4 * total_cells);
+    PoolBuf<uint32_t> scan_a(scan_scratch_words(total_cells));
+    PoolBuf<uint32_t> scan_b(scan_scratch_words(4 * total_cells));
     hip_check(hipMemsetAsync(vslot.p, 0, vslot.n * sizeof(uint32_t), s), "vslot");
     for (int L = 0; L <= top; ++L) {
         hip_check(launch_quads(P, views[L], 0, tri_count.p, vslot.p, nullptr, nullptr,
@@ -404,28 +419,40 @@ bool render_mesh_on(const Tree& tree, const double lo[3], const double hi[3],
         throw Error("render_mesh: inconsistent mesh size");
     }
     const uint32_t ntris = uint32_t(ntris64), nverts = uint32_t(nverts64);
-    DevBuf<uint32_t> d_tris(3 * size_t(std::max<uint32_t>(ntris, 1)));
-    DevBuf<float> d_verts(3 * size_t(nverts));
-    DevBuf<uint32_t> d_keys(5 * size_t(nverts));
+    PoolBuf<uint32_t> d_tris(3 * size_t(std::max<uint32_t>(ntris, 1)));
+    PoolBuf<float> d_verts(3 * size_t(nverts));
+    PoolBuf<uint32_t> d_keys(out.want_keys ? 5 * size_t(nverts) : 0);
     hip_check(hipMemsetAsync(d_verts.p, 0, 3 * sizeof(float), s), "dummy");
-    hip_check(hipMemsetAsync(d_keys.p, 0, 5 * sizeof(uint32_t), s), "dummy");
+    if (out.want_keys) {
+        hip_check(hipMemsetAsync(d_keys.p, 0, 5 * sizeof(uint32_t), s), "dummy");
+    }
     for (int L = 0; L <= top; ++L) {
         hip_check(launch_quads(P, views[L], 1, tri_count.p, vslot.p, tri_off.p, vid.p,
                                d_tris.p, s), "quad write");
         hip_check(launch_verts(views[L], L, vslot.p, vid.p, d_verts.p, d_keys.p, s),
                   "vertex gather");
     }
-    out.verts.resize(3 * size_t(nverts));
-    out.tris.resize(3 * size_t(ntris));
-    out.keys.resize(5 * size_t(nverts));
-    hip_check(hipMemcpyAsync(out.verts.data(), d_verts.p, out.verts.size() * sizeof(float),
+    float* vdst = nullptr;
+    uint32_t* tdst = nullptr;
+    if (out.alloc) {
+        out.alloc(nverts, ntris, &vdst, &tdst);
+    } else {
+        out.verts.resize(3 * size_t(nverts));
+        out.tris.resize(3 * size_t(ntris));
+        vdst = out.verts.data();
+        tdst = out.tris.data();
+    }
+    hip_check(hipMemcpyAsync(vdst, d_verts.p, 3 * size_t(nverts) * sizeof(float),
                              hipMemcpyDeviceToHost, s), "copy verts");
     if (ntris) {
-        hip_check(hipMemcpyAsync(out.tris.data(), d_tris.p, out.tris.size() * sizeof(uint32_t),
+        hip_check(hipMemcpyAsync(tdst, d_tris.p, 3 * size_t(ntris) * sizeof(uint32_t),
                                  hipMemcpyDeviceToHost, s), "copy tris");
     }
-    hip_check(hipMemcpyAsync(out.keys.data(), d_keys.p, out.keys.size() * sizeof(uint32_t),
-                             hipMemcpyDeviceToHost, s), "copy keys");
+    if (out.want_keys) {
+        out.keys.resize(5 * size_t(nverts));
+        hip_check(hipMemcpyAsync(out.keys.data(), d_keys.p, out.keys.size() * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, s), "copy keys");
+    }
     wait(s, "mesh copy");
     g_stats.ms_emit = ms_since(t_emit);
     g_stats.ms_total = ms_since(t_start);

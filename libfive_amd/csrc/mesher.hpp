@@ -4,6 +4,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 #include <hip/hip_runtime.h>
@@ -28,6 +29,12 @@ struct HostMesh {
     std::vector<float> verts;
     std::vector<uint32_t> tris;
     std::vector<uint32_t> keys;
+    // When set, called once with the vertex count (dummy included) and the
+    // triangle count; it returns the host buffers (3 floats per vertex,
+    // 3 indices per triangle) the mesh is copied into, and verts / tris
+    // stay empty.
+    std::function<void(size_t, size_t, float**, uint32_t**)> alloc;
+    bool want_keys = true;          // fill keys
 };
 
 // Statistics of the last render on this thread.

@@ -2,11 +2,78 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <map>
+#include <mutex>
+#include <vector>
 
 #include "eval_kernels.hpp"
 
 namespace libfive {
 namespace amd {
+
+namespace {
+std::mutex g_pool_mu;
+std::multimap<size_t, void*> g_pool;     // size class -> free block
+size_t g_pool_bytes = 0;                 // bytes held in g_pool
+}
+
+void* pool_alloc(size_t bytes, size_t* granted) {
+    size_t c = 4096;
+    while (c < bytes) c <<= 1;
+    *granted = c;
+    {
+        std::lock_guard<std::mutex> lock(g_pool_mu);
+        auto it = g_pool.find(c);
+        if (it != g_pool.end()) {
+            void* p = it->second;
+            g_pool.erase(it);
+            g_pool_bytes -= c;
+            return p;
+        }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, c);
+    if (e == hipErrorOutOfMemory) {
+        // Give the cached blocks back and try once more.
+        (void)hipGetLastError();
+        std::lock_guard<std::mutex> lock(g_pool_mu);
+        for (auto& kv : g_pool) (void)hipFree(kv.second);
+        g_pool.clear();
+        g_pool_bytes = 0;
+        e = hipMalloc(&p, c);
+    }
+    hip_check(e, "hipMalloc");
+    return p;
+}
+
+namespace {
+thread_local bool t_defer = false;
+thread_local std::vector<std::pair<void*, size_t>> t_pending;
+}
+
+void pool_defer_begin() { t_defer = true; }
+
+void pool_defer_end(hipStream_t s) {
+    (void)hipStreamSynchronize(s);
+    t_defer = false;
+    std::vector<std::pair<void*, size_t>> pending;
+    pending.swap(t_pending);
+    for (auto& b : pending) pool_free(b.first, b.second);
+}
+
+void pool_free(void* p, size_t granted) {
+    if (t_defer) {
+        t_pending.emplace_back(p, granted);
+        return;
+    }
+    std::lock_guard<std::mutex> lock(g_pool_mu);
+    if (g_pool_bytes + granted > kPoolKeep) {
+        (void)hipFree(p);
+        return;
+    }
+    g_pool.emplace(granted, p);
+    g_pool_bytes += granted;
+}
 
 void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess) {

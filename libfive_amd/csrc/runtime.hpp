@@ -7,6 +7,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <utility>
 
 #include "deck.hpp"
 #include "tape.hpp"
@@ -58,6 +59,50 @@ struct DevBuf {
         if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
+    }
+};
+
+// Device memory cache for the mesher's per-render buffers: a released
+// block is kept and handed to a later request of the same size class
+// (powers of two from 4 KiB), so that repeated renders do not pay for
+// hipMalloc / hipFree.  Blocks are returned only after the render's stream
+// has finished with them.  At most kPoolKeep bytes stay cached.
+void* pool_alloc(size_t bytes, size_t* granted);
+void pool_free(void* p, size_t granted);
+
+// Between pool_defer_begin() and pool_defer_end(s) on one thread, released
+// blocks are held back; pool_defer_end waits for stream s and then returns
+// them to the cache, so that no block is reused while a kernel of that
+// stream may still use it (also when the render ends by an exception).
+void pool_defer_begin();
+void pool_defer_end(hipStream_t s);
+constexpr size_t kPoolKeep = size_t(48) << 30;
+
+template <typename T>
+struct PoolBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    size_t granted = 0;
+    PoolBuf() = default;
+    explicit PoolBuf(size_t count) { alloc(count); }
+    PoolBuf(const PoolBuf&) = delete;
+    PoolBuf& operator=(const PoolBuf&) = delete;
+    ~PoolBuf() { reset(); }
+    void alloc(size_t count) {
+        reset();
+        if (count) p = static_cast<T*>(pool_alloc(count * sizeof(T), &granted));
+        n = count;
+    }
+    void reset() {
+        if (p) pool_free(p, granted);
+        p = nullptr;
+        n = 0;
+        granted = 0;
+    }
+    void swap(PoolBuf& o) {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+        std::swap(granted, o.granted);
     }
 };
 

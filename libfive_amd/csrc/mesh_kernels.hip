@@ -179,6 +179,22 @@ __device__ __forceinline__ uint64_t ballot(bool b) { return __ballot(b); }
 ////////////////////////////////////////////////////////////////////////////////
 // Interval classification and pruning
 
+// Synchronises the lanes of one wavefront for LDS exchanges (the interval
+// kernel runs one wave per workgroup, so this is all the workgroup needs).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Tapes are walked four clauses per (scalar) load, the next batch loaded
+// before the current one is used.  Every tape in the pool is followed by
+// at least 7 clauses of padding or other tapes, and batch b covers clauses
+// [4b, 4b + 4).
+struct Batch4 {
+    Clause c[4];
+};
+
 template <int NV>
 __global__ __launch_bounds__(64)
 void k_interval(Params P, int level, const uint32_t* __restrict__ groups,
@@ -220,9 +236,13 @@ void k_interval(Params P, int level, const uint32_t* __restrict__ groups,
     const uint32_t W = (P.nregs + 31u) / 32u;
     uint32_t* need = rslot + P.nregs;
 
-    rslot[kRegX] = P.slot_x;
-    rslot[kRegY] = P.slot_y;
-    rslot[kRegZ] = P.slot_z;
+    // The reference slot each register's value resolves to (the same in
+    // every lane), kept as a register bank so that the a == b test of the
+    // keep rule needs no LDS traffic.
+    LF_RF_DECL(RS);
+    LF_RF_SET(RS, NV, kRegX, fmath::u2f(P.slot_x));
+    LF_RF_SET(RS, NV, kRegY, fmath::u2f(P.slot_y));
+    LF_RF_SET(RS, NV, kRegZ, fmath::u2f(P.slot_z));
     __syncthreads();
 
     LF_RF_DECL(L);
@@ -237,8 +257,16 @@ void k_interval(Params P, int level, const uint32_t* __restrict__ groups,
         LF_RF_SET(L, NV, kRegZ, iz.lo); LF_RF_SET(U, NV, kRegZ, iz.hi); LF_RF_SET(F, NV, kRegZ, iz.nan ? 1.0f : 0.0f);
     }
     bool any_ab = false;
-    for (uint32_t k = 0; k < n; ++k) {
-        const Clause c = t[k];
+    const Batch4* tb = reinterpret_cast<const Batch4*>(t);
+    const uint32_t nb = (n + 3u) / 4u;
+    Batch4 cur4 = tb[0];
+    for (uint32_t b = 0; b < nb; ++b) {
+      const Batch4 nxt4 = tb[b + 1];
+#pragma unroll
+      for (uint32_t q = 0; q < 4; ++q) {
+        const uint32_t k = 4u * b + q;
+        if (k >= n) break;
+        const Clause c = cur4.c[q];
         const uint32_t o = tape::op(c);
         const uint32_t m = mode(c);
         const float im = imm(c);
@@ -260,7 +288,8 @@ void k_interval(Params P, int level, const uint32_t* __restrict__ groups,
         if (o == OC::OP_MIN || o == OC::OP_MAX) {
             // The interval keep rule (eval_interval.cpp:116-167); operands
             // that resolve to the same reference slot keep a.
-            const bool same = (m == MODE_RR) && rslot[ra(c)] == rslot[rb(c)];
+            const bool same = (m == MODE_RR) &&
+                fmath::f2u(LF_RF_GET(RS, NV, ra(c))) == fmath::f2u(LF_RF_GET(RS, NV, rb(c)));
             uint32_t keep;
             if (same) keep = KEEP_A;
             else if (a.lo > b.hi) keep = (o == OC::OP_MAX) ? KEEP_A : KEEP_B;
@@ -273,13 +302,13 @@ void k_interval(Params P, int level, const uint32_t* __restrict__ groups,
         }
         const Iv out = iv_apply(o, a, b, im);
         const uint32_t d = dst(c);
-        const uint32_t rs = (o == OP_COPY) ? rslot[ra(c)] : slot(c);
-        __syncthreads();
-        rslot[d] = rs;
-        __syncthreads();
+        const float rs = (o == OP_COPY) ? LF_RF_GET(RS, NV, ra(c)) : fmath::u2f(slot(c));
+        LF_RF_SET(RS, NV, d, rs);
         LF_RF_SET(L, NV, d, out.lo);
         LF_RF_SET(U, NV, d, out.hi);
         LF_RF_SET(F, NV, d, out.nan ? 1.0f : 0.0f);
+      }
+      cur4 = nxt4;
     }
     const uint32_t root = P.root_reg;
     const float rlo = LF_RF_GET(L, NV, root);
@@ -324,8 +353,14 @@ void k_interval(Params P, int level, const uint32_t* __restrict__ groups,
             }
         }
         bool changed = false, nonterm = false;
-        for (uint32_t k = n; k-- > 0;) {
-            const Clause c = t[k];
+        Batch4 bcur = tb[nb - 1];
+        for (uint32_t b = nb; b-- > 0;) {
+          const Batch4 bprv = tb[b > 0 ? b - 1 : 0];
+#pragma unroll
+          for (int q = 3; q >= 0; --q) {
+            const uint32_t k = 4u * b + uint32_t(q);
+            if (k >= n) continue;
+            const Clause c = bcur.c[q];
             bool live = false;
             const uint32_t d = dst(c);
             if (dopush && ((my[d >> 5] >> (d & 31u)) & 1u)) {
@@ -352,8 +387,10 @@ void k_interval(Params P, int level, const uint32_t* __restrict__ groups,
             }
             const uint64_t lm = ballot(live);
             if (lane == 0) lv[k] = uint8_t(lm & 0xffu);
+          }
+          bcur = bprv;
         }
-        __syncthreads();
+        wave_sync();
         uint32_t pos = base, refc = 0;
         const bool emit = dopush && changed;
         if (emit) {
@@ -377,9 +414,15 @@ void k_interval(Params P, int level, const uint32_t* __restrict__ groups,
                 ren[d] = uint8_t(d);
             }
         };
-        for (uint32_t k = 0; k < n; ++k) {
+        Batch4 ecur = tb[0];
+        for (uint32_t b = 0; b < nb; ++b) {
+          const Batch4 enxt = tb[b + 1];
+#pragma unroll
+          for (uint32_t qq = 0; qq < 4; ++qq) {
+            const uint32_t k = 4u * b + qq;
+            if (k >= n) break;
             if (!emit || !((lv[k] >> lane) & 1u)) continue;
-            const Clause c = t[k];
+            const Clause c = ecur.c[qq];
             const uint32_t o = tape::op(c);
             const uint32_t m = mode(c);
             const uint32_t d = dst(c);
@@ -419,6 +462,8 @@ void k_interval(Params P, int level, const uint32_t* __restrict__ groups,
             out.w1 = (a2 & 0xffffu) | (b2 << 16);
             P.pool[pos++] = out;
             if (o != OP_LOADI) ++refc;
+          }
+          ecur = enxt;
         }
         if (emit && ren[root] != root) {
             P.pool[pos++] = make(OP_COPY, MODE_RR, root, ren[root], 0, 0.0f, 0);

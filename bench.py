@@ -158,13 +158,24 @@ def mesh_measurements(args):
     c1_ms, (c1v, c1t) = render_ms(c1, -2.0, 2.0, W.C1_RES, 5)
     out["csg_res16"] = {"ms": round(c1_ms, 2), "verts": int(c1v) - 1,
                         "tris": int(c1t)}
+    if not args.no_cpu:
+        # the reference's CPU-runnable configuration on 8 host threads
+        od1 = oracle.OracleDeck(c1.deck())
+        mf1 = float(np.float32(1) / np.float32(W.C1_RES))
+        ts = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            od1.render_mesh([-2.0] * 3, [2.0] * 3, mf1, threads=8)
+            ts.append((time.perf_counter() - t0) * 1e3)
+        out["csg_res16"]["cpu_baseline"] = {
+            "value": round(float(np.median(ts)), 2), "unit": "ms", "cores": 8,
+            "kind": "port", "sample": "the whole configuration, median of 5"}
     # 1000-sphere union, octree to depth 8 with pruned tapes
     su = W.sphere_union()
     s_lo, s_hi = W.C3_REGION[0][0], W.C3_REGION[1][0]
-    t0 = time.perf_counter()
+    su_ms, _ = render_ms(su, s_lo, s_hi, 2 ** W.C3_DEPTH / (s_hi - s_lo), 3)
     sm = su.render_mesh([s_lo] * 3, [s_hi] * 3,
                         min_feature=(s_hi - s_lo) / 2 ** W.C3_DEPTH, details=True)
-    su_ms = (time.perf_counter() - t0) * 1e3
     lens = np.asarray(sm.tape_lengths)
     hist, edges = np.histogram(lens, bins=[0, 10, 20, 50, 100, 200, 500, 1000,
                                            2000, 5000, 20000])
@@ -175,6 +186,21 @@ def mesh_measurements(args):
                                  for a, b, c in zip(edges[:-1], edges[1:], hist)},
         "tris": int(sm.tris.shape[0]),
     }
+    if not args.no_cpu:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        ods = oracle.OracleDeck(su.deck())
+        lo32, hi32 = float(np.float32(s_lo)), float(np.float32(s_hi))
+        mfs = float(np.float32(1) / np.float32(2 ** W.C3_DEPTH / (s_hi - s_lo)))
+        t0 = time.perf_counter()
+        osm = ods.render_mesh([lo32] * 3, [hi32] * 3, mfs, threads=threads)
+        cpu_su = (time.perf_counter() - t0) * 1e3
+        out["sphere_union_depth8"]["cpu_baseline"] = {
+            "value": round(cpu_su, 1), "unit": "ms", "cores": threads,
+            "kind": "port", "sample": "the whole configuration, one run",
+            "same_level_counts": bool(np.array_equal(
+                np.asarray(osm.level_counts, np.uint64),
+                np.asarray(sm.level_counts, np.uint64))),
+        }
     # the same union meshed at 512 cells per axis
     su_ms512, (suv, sut) = render_ms(su, s_lo, s_hi, res / (s_hi - s_lo),
                                      max(1, args.mesh_steps // 2))

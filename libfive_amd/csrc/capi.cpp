@@ -588,6 +588,10 @@ static libfive_mesh* render(libfive_tree tree, libfive_region3 R,
         ms.min_feature = s.min_feature;
         ms.max_err = s.max_err;
         ms.precision = s.precision;
+        if (s.precision != 0) {
+            throw Error("precision " + std::to_string(s.precision) +
+                        " is not available (only 0, FP64 QEF accumulation)");
+        }
         ms.collect_stats = s.collect_stats != 0;
         ms.cancel = s.cancel;
         const double lo[3] = {R.X.lower, R.Y.lower, R.Z.lower};

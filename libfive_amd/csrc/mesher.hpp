@@ -17,7 +17,7 @@ namespace amd {
 struct MeshSettings {
     double min_feature = 0.1;     // settings.hpp:34-43 defaults
     double max_err = 1e-8;
-    uint32_t precision = 0;       // 0: FP64 QEF sums, 1: compensated FP32
+    uint32_t precision = 0;       // QEF accumulation: only 0 (FP64) is built
     bool collect_stats = false;   // per-level cell counts and tape lengths
     volatile const int* cancel = nullptr;
 };

@@ -442,10 +442,21 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
         vdst = out.verts.data();
         tdst = out.tris.data();
     }
-    hip_check(hipMemcpyAsync(vdst, d_verts.p, 3 * size_t(nverts) * sizeof(float),
-                             hipMemcpyDeviceToHost, s), "copy verts");
+    // Device -> pinned staging at full link speed, then host threads copy
+    // into the caller's (pageable) arrays.
+    const size_t vbytes = 3 * size_t(nverts) * sizeof(float);
+    const size_t tbytes = 3 * size_t(ntris) * sizeof(uint32_t);
+    size_t granted = 0;
+    char* stage = static_cast<char*>(pinned_alloc(vbytes + tbytes, &granted));
+    struct StageRelease {
+        char* p;
+        size_t n;
+        ~StageRelease() { pinned_free(p, n); }
+    } stage_release{stage, granted};
+    hip_check(hipMemcpyAsync(stage, d_verts.p, vbytes, hipMemcpyDeviceToHost, s),
+              "copy verts");
     if (ntris) {
-        hip_check(hipMemcpyAsync(tdst, d_tris.p, 3 * size_t(ntris) * sizeof(uint32_t),
+        hip_check(hipMemcpyAsync(stage + vbytes, d_tris.p, tbytes,
                                  hipMemcpyDeviceToHost, s), "copy tris");
     }
     if (out.want_keys) {
@@ -454,6 +465,8 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
                                  hipMemcpyDeviceToHost, s), "copy keys");
     }
     wait(s, "mesh copy");
+    parallel_memcpy(vdst, stage, vbytes);
+    if (ntris) parallel_memcpy(tdst, stage + vbytes, tbytes);
     g_stats.ms_emit = ms_since(t_emit);
     g_stats.ms_total = ms_since(t_start);
     return true;

@@ -2,6 +2,9 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <algorithm>
+#include <thread>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -48,7 +51,12 @@ void* pool_alloc(size_t bytes, size_t* granted) {
 
 namespace {
 thread_local bool t_defer = false;
-thread_local std::vector<std::pair<void*, size_t>> t_pending;
+struct Pending {
+    void* p;
+    size_t n;
+    bool pinned;
+};
+thread_local std::vector<Pending> t_pending;
 }
 
 void pool_defer_begin() { t_defer = true; }
@@ -56,14 +64,17 @@ void pool_defer_begin() { t_defer = true; }
 void pool_defer_end(hipStream_t s) {
     (void)hipStreamSynchronize(s);
     t_defer = false;
-    std::vector<std::pair<void*, size_t>> pending;
+    std::vector<Pending> pending;
     pending.swap(t_pending);
-    for (auto& b : pending) pool_free(b.first, b.second);
+    for (auto& b : pending) {
+        if (b.pinned) pinned_free(b.p, b.n);
+        else pool_free(b.p, b.n);
+    }
 }
 
 void pool_free(void* p, size_t granted) {
     if (t_defer) {
-        t_pending.emplace_back(p, granted);
+        t_pending.push_back({p, granted, false});
         return;
     }
     std::lock_guard<std::mutex> lock(g_pool_mu);
@@ -73,6 +84,59 @@ void pool_free(void* p, size_t granted) {
     }
     g_pool.emplace(granted, p);
     g_pool_bytes += granted;
+}
+
+namespace {
+std::mutex g_pinned_mu;
+std::multimap<size_t, void*> g_pinned;
+}
+
+void* pinned_alloc(size_t bytes, size_t* granted) {
+    size_t c = size_t(1) << 20;
+    while (c < bytes) c <<= 1;
+    *granted = c;
+    {
+        std::lock_guard<std::mutex> lock(g_pinned_mu);
+        auto it = g_pinned.find(c);
+        if (it != g_pinned.end()) {
+            void* p = it->second;
+            g_pinned.erase(it);
+            return p;
+        }
+    }
+    void* p = nullptr;
+    hip_check(hipHostMalloc(&p, c, hipHostMallocDefault), "hipHostMalloc");
+    return p;
+}
+
+void pinned_free(void* p, size_t granted) {
+    if (t_defer) {
+        t_pending.push_back({p, granted, true});
+        return;
+    }
+    std::lock_guard<std::mutex> lock(g_pinned_mu);
+    g_pinned.emplace(granted, p);
+}
+
+void parallel_memcpy(void* dst, const void* src, size_t bytes) {
+    const size_t chunk = size_t(8) << 20;
+    const size_t parts = std::min<size_t>(8, (bytes + chunk - 1) / chunk);
+    if (parts <= 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t per = (bytes + parts - 1) / parts;
+    std::vector<std::thread> ts;
+    for (size_t i = 1; i < parts; ++i) {
+        const size_t off = i * per;
+        const size_t len = std::min(per, bytes - off);
+        ts.emplace_back([=] {
+            std::memcpy(static_cast<char*>(dst) + off,
+                        static_cast<const char*>(src) + off, len);
+        });
+    }
+    std::memcpy(dst, src, std::min(per, bytes));
+    for (auto& t : ts) t.join();
 }
 
 void hip_check(hipError_t e, const char* what) {

@@ -106,6 +106,13 @@ struct PoolBuf {
     }
 };
 
+// Pinned host staging blocks (hipHostMalloc), cached like pool_alloc's.
+void* pinned_alloc(size_t bytes, size_t* granted);
+void pinned_free(void* p, size_t granted);
+
+// memcpy split over host threads for large copies.
+void parallel_memcpy(void* dst, const void* src, size_t bytes);
+
 // A compiled tape resident in device memory.
 struct GpuTape {
     DeviceTape host;

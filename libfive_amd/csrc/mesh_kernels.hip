@@ -571,11 +571,12 @@ hipError_t launch_link(const LevelView& cur, const uint32_t* base,
 // FEAT = false: a cell that needs the exact-zero feature evaluation is
 // appended to `deferred` and left untouched.  FEAT = true: the cells listed
 // in `work` are evaluated with that path, each wave with its own workspace.
-// At least 4 waves per SIMD: the compiler keeps the kernel within 128
-// VGPRs, spilling rarely used values, which measured faster than 2 waves
-// with no spills (gyroid 512^3: 169 -> 152 ms per render).
+// With a one-bank register file, at least 4 waves per SIMD: the compiler
+// keeps the kernel within 128 VGPRs, spilling rarely used values, which
+// measured faster than 2 waves with no spills (gyroid 512^3: 169 -> 152 ms
+// per render).  Larger files keep the compiler's choice.
 template <int NV, bool FEAT>
-__global__ __launch_bounds__(64, 4)
+__global__ __launch_bounds__(64, NV == 1 ? 4 : 1)
 void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
             uint32_t per_group, LevelView l1, LevelView l0,
             const uint32_t* __restrict__ work, uint32_t* __restrict__ deferred,

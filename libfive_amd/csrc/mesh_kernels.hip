@@ -675,7 +675,7 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
                     s_ts[lane][q] = ((b >> q) & 1) ? hi[q] : lo[q];
                 }
             }
-            __syncthreads();
+            wave_sync();
             // Edge search: 4 rounds of 16 samples (dc_tree.inl:339-410).
             const int passes = (16 * ec + 63) / 64;
             for (int round = 0; round < 4; ++round) {
@@ -727,7 +727,7 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
                     const uint64_t tm = ballot(take);
                     const uint32_t seg = uint32_t(tm >> ((lane >> 4) * 16)) & 0xffffu;
                     const int first = seg ? __builtin_ctz(seg) : -1;
-                    __syncthreads();
+                    wave_sync();
                     if (valid && first >= 1) {
                         if (j == first) {
                             for (int q = 0; q < 3; ++q) s_ts[e][q] = ps[q];
@@ -735,7 +735,7 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
                             for (int q = 0; q < 3; ++q) s_tf[e][q] = ps[q];
                         }
                     }
-                    __syncthreads();
+                    wave_sync();
                 }
             }
             // Derivatives at both bracket ends (dc_tree.inl:412-471): one
@@ -763,7 +763,7 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
                         }
                     }
                 }
-                __syncthreads();
+                wave_sync();
                 // Ambiguous ends contribute every distinct gradient of the
                 // point (FeatureEvaluator::features).
                 uint64_t need = ballot(lane < m && (s_fl[lane < m ? lane : 0] & PT_AMBIGUOUS));
@@ -789,7 +789,7 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
                     }
                 }
             }
-            __syncthreads();
+            wave_sync();
             if (lane < ec) {
                 double acc[14];
                 for (int q = 0; q < 14; ++q) acc[q] = 0.0;
@@ -821,7 +821,7 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
                 s_rank[lane] = r;
                 for (int q = 0; q < 14; ++q) s_acc[lane][q] = acc[q];
             }
-            __syncthreads();
+            wave_sync();
             if (lane == 0) {
                 int max_rank = 0;
                 for (int e = 0; e < ec; ++e) max_rank = s_rank[e] > max_rank ? s_rank[e] : max_rank;
@@ -854,7 +854,7 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
                 rank = uint32_t(rk);
             }
             ++vcount;
-            __syncthreads();
+            wave_sync();
         }
         if (lane == 0) {
             leaf->rank = rank;

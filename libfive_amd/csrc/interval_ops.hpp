@@ -36,12 +36,6 @@ struct Iv {
 #define LF_DEV __device__ __forceinline__
 
 namespace rnd {
-LF_DEV float add_dn(float a, float b) { return __ocml_add_rtn_f32(a, b); }
-LF_DEV float add_up(float a, float b) { return __ocml_add_rtp_f32(a, b); }
-LF_DEV float sub_dn(float a, float b) { return __ocml_sub_rtn_f32(a, b); }
-LF_DEV float sub_up(float a, float b) { return __ocml_sub_rtp_f32(a, b); }
-LF_DEV float mul_dn(float a, float b) { return __ocml_mul_rtn_f32(a, b); }
-LF_DEV float mul_up(float a, float b) { return __ocml_mul_rtp_f32(a, b); }
 // Quotients and square roots of floats are computed correctly rounded in
 // double precision and then rounded to float in the wanted direction.  This
 // equals directed rounding of the exact result: for float operands, a
@@ -56,6 +50,12 @@ LF_DEV float dbl_up(double v) {
     const float f = float(v);
     return (double(f) < v) ? fmath::succ_f(f) : f;
 }
+LF_DEV float add_dn(float a, float b) { return __ocml_add_rtn_f32(a, b); }
+LF_DEV float add_up(float a, float b) { return __ocml_add_rtp_f32(a, b); }
+LF_DEV float sub_dn(float a, float b) { return __ocml_sub_rtn_f32(a, b); }
+LF_DEV float sub_up(float a, float b) { return __ocml_sub_rtp_f32(a, b); }
+LF_DEV float mul_dn(float a, float b) { return __ocml_mul_rtn_f32(a, b); }
+LF_DEV float mul_up(float a, float b) { return __ocml_mul_rtp_f32(a, b); }
 LF_DEV float div_dn(float a, float b) { return dbl_dn(double(a) / double(b)); }
 LF_DEV float div_up(float a, float b) { return dbl_up(double(a) / double(b)); }
 LF_DEV float sqrt_dn(float a) { return dbl_dn(::sqrt(double(a))); }
@@ -574,13 +574,11 @@ LF_DEV Iv iv_mod(Iv a, Iv b) {
 }
 
 // One interval clause.
-LF_DEV Iv iv_apply(uint32_t o, Iv a, Iv b, float im) {
+// Opcodes outside the common set, out of line: the interval kernels are
+// latency-bound walks over one tape, and keeping every rule inline makes
+// the loop body large enough to miss in the instruction cache.
+static __device__ __noinline__ Iv iv_apply_cold(uint32_t o, Iv a, Iv b) {
     switch (o) {
-        case 17: return iv_add(a, b);
-        case 18: return iv_mul(a, b);
-        case 19: return iv_min(a, b);
-        case 20: return iv_max(a, b);
-        case 21: return iv_sub(a, b);
         case 22: return iv_div(a, b);
         case 23: return iv_atan2(a, b);
         case 24: return iv_pow(a, b);
@@ -588,9 +586,6 @@ LF_DEV Iv iv_apply(uint32_t o, Iv a, Iv b, float im) {
         case 26: return iv_mod(a, b);
         case 27: return iv_nanfill(a, b);
         case 31: return iv_compare(a, b);
-        case 7: return iv_square(a);
-        case 8: return iv_sqrt(a);
-        case 9: return iv_neg(a);
         case 10: return iv_sin(a);
         case 11: return iv_cos(a);
         case 12: return iv_tan(a);
@@ -598,11 +593,27 @@ LF_DEV Iv iv_apply(uint32_t o, Iv a, Iv b, float im) {
         case 14: return iv_acos(a);
         case 15: return iv_atan(a);
         case 16: return iv_exp(a);
-        case 28: return iv_abs(a);
         case 29: return iv_recip(a);
         case 30: return iv_log(a);
+        default: return a;
+    }
+}
+
+LF_DEV Iv iv_apply(uint32_t o, Iv a, Iv b, float im) {
+    switch (o) {
+        case 17: return iv_add(a, b);
+        case 18: return iv_mul(a, b);
+        case 19: return iv_min(a, b);
+        case 20: return iv_max(a, b);
+        case 21: return iv_sub(a, b);
+        case 7: return iv_square(a);
+        case 8: return iv_sqrt(a);
+        case 9: return iv_neg(a);
+        case 28: return iv_abs(a);
         case 1: return {im, im, fmath::isnan_f(im)};     // LOADI
-        default: return a;                                // COPY, CONST_VAR
+        case 6:
+        case 40: return a;                                // CONST_VAR, COPY
+        default: return iv_apply_cold(o, a, b);
     }
 }
 

@@ -50,15 +50,35 @@ struct Names {
 
 // Operands of a clause exactly as the interpreter loop forms them
 // (LF_VALUE_STEP and LF_DERIV_STEP in eval_kernels.hip).  After each clause
-// an empty asm statement takes its results as in-out register operands:
+// an empty asm statement takes its results as in-out register operands
+// (or after each group of clauses, see pin_group):
 // the clause is computed there, in tape order, instead of being sunk
 // towards its uses (which keeps all of its inputs alive; on the
 // 809-clause benchmark tape the derivative kernel otherwise needs all 512
 // VGPRs and spills, and with the barriers it needs 168).
+// Clauses per pinning statement (LIBFIVE_AMD_JIT_GROUP, default 1): the
+// results of a group are pinned together after its last clause.
+static size_t pin_group() {
+    const char* e = std::getenv("LIBFIVE_AMD_JIT_GROUP");
+    const long g = e ? std::atol(e) : 1;
+    return size_t(g < 1 ? 1 : g > 4 ? 4 : g);
+}
+
+static void pin(std::string& out, const std::vector<std::string>& vars) {
+    if (vars.empty()) return;
+    out += "    asm volatile(\"\" :";
+    for (size_t i = 0; i < vars.size(); ++i) {
+        out += (i ? ", \"+v\"(" : " \"+v\"(") + vars[i] + ")";
+    }
+    out += ");\n";
+}
+
 void value_body(const DeviceTape& t, std::string& out) {
     Names n(t.num_regs);
     char buf[256];
     size_t k = 0;
+    const size_t group = pin_group();
+    std::vector<std::string> pending;
     for (const Clause& c : t.clauses) {
         const uint32_t o = op(c);
         if (o == OP_NOP) continue;
@@ -69,13 +89,18 @@ void value_body(const DeviceTape& t, std::string& out) {
                               : (m == MODE_B_IMM) ? im : n.v[rb(c)];
         std::snprintf(buf, sizeof(buf),
                       "    float v%zu; { const float a_ = %s, b_ = %s; "
-                      "apply_value<1>(%uu, &a_, &b_, %s, &v%zu); }\n"
-                      "    asm volatile(\"\" : \"+v\"(v%zu));\n",
-                      k, a.c_str(), b.c_str(), o, im.c_str(), k, k);
+                      "apply_value<1>(%uu, &a_, &b_, %s, &v%zu); }\n",
+                      k, a.c_str(), b.c_str(), o, im.c_str(), k);
         out += buf;
+        pending.push_back("v" + std::to_string(k));
+        if (pending.size() >= group) {
+            pin(out, pending);
+            pending.clear();
+        }
         n.v[dst(c)] = "v" + std::to_string(k);
         ++k;
     }
+    pin(out, pending);
     out += "    out[i] = " + n.v[t.root_reg] + ";\n";
 }
 
@@ -83,6 +108,8 @@ void deriv_body(const DeviceTape& t, std::string& out) {
     Names n(t.num_regs);
     char buf[512];
     size_t k = 0;
+    const size_t group = pin_group();
+    std::vector<std::string> pending;
     for (const Clause& c : t.clauses) {
         const uint32_t o = op(c);
         if (o == OP_NOP) continue;
@@ -105,13 +132,19 @@ void deriv_body(const DeviceTape& t, std::string& out) {
         std::snprintf(buf, sizeof(buf),
                       "    float v%zu, d%zu[3]; { const float ad_[3] = {%s, %s, %s}; "
                       "const float bd_[3] = {%s, %s, %s}; "
-                      "apply_deriv(%uu, %s, ad_, %s, bd_, %s, v%zu, d%zu); }\n"
-                      "    asm volatile(\"\" : \"+v\"(v%zu), \"+v\"(d%zu[0]), "
-                      "\"+v\"(d%zu[1]), \"+v\"(d%zu[2]));\n",
+                      "apply_deriv(%uu, %s, ad_, %s, bd_, %s, v%zu, d%zu); }\n",
                       k, k, ad[0].c_str(), ad[1].c_str(), ad[2].c_str(),
                       bd[0].c_str(), bd[1].c_str(), bd[2].c_str(), o,
-                      av.c_str(), bv.c_str(), im.c_str(), k, k, k, k, k, k);
+                      av.c_str(), bv.c_str(), im.c_str(), k, k);
         out += buf;
+        const std::string ks = std::to_string(k);
+        for (const char* sfx : {"", "[0]", "[1]", "[2]"}) {
+            pending.push_back((*sfx ? "d" : "v") + ks + sfx);
+        }
+        if (pending.size() >= 4 * group) {
+            pin(out, pending);
+            pending.clear();
+        }
         const uint32_t d = dst(c);
         n.v[d] = "v" + std::to_string(k);
         for (int j = 0; j < 3; ++j) {
@@ -119,6 +152,7 @@ void deriv_body(const DeviceTape& t, std::string& out) {
         }
         ++k;
     }
+    pin(out, pending);
     const uint32_t r = t.root_reg;
     out += "    ov[i] = " + n.v[r] + ";\n";
     out += "    odx[i] = " + n.g[0][r] + ";\n";

@@ -47,6 +47,15 @@ typedef struct libfive_mesh {
     uint32_t vert_count;
 } libfive_mesh;
 
+/* libfive.h:106-118: an indexed mesh with the vertex indices of each
+ * triangle followed by -1 (coord_index_count / 4 triangles). */
+typedef struct libfive_mesh_coords {
+    libfive_vec3* verts;
+    uint32_t vert_count;
+    int32_t* coord_indices;
+    uint32_t coord_index_count;
+} libfive_mesh_coords;
+
 /* libfive.h:184-188: an opaque, intrusively refcounted tree node. */
 typedef struct libfive_tree_ libfive_tree_;
 typedef struct libfive_tree_* libfive_tree;
@@ -156,6 +165,21 @@ int libfive_tape_eval_values(libfive_tape t, const float* x, const float* y,
 int libfive_tape_eval_derivs(libfive_tape t, const float* x, const float* y,
                              const float* z, size_t n, float* value, float* dx,
                              float* dy, float* dz);
+
+/* Mesh output forms (libfive.h:369-410, libfive.cpp:354-440): the same
+ * mesh as libfive_tree_render_mesh, as -1-separated coordinate indices, or
+ * written as binary STL (80-byte header, triangle count, per triangle a
+ * zero normal, three vertices and a zero attribute word;
+ * mesh.cpp:139-198).  save_meshes takes a NULL-terminated tree array and
+ * uses max_err = 10^-quality. */
+libfive_mesh_coords* libfive_tree_render_mesh_coords(libfive_tree tree,
+                                                     libfive_region3 R,
+                                                     float res);
+void libfive_mesh_coords_delete(libfive_mesh_coords* m);
+bool libfive_tree_save_mesh(libfive_tree tree, libfive_region3 R, float res,
+                            const char* f);
+bool libfive_tree_save_meshes(libfive_tree trees[], libfive_region3 R,
+                              float res, float quality, const char* f);
 
 /* Many independent meshes in one call (the many-model configuration; no
  * counterpart in the reference, whose callers loop over

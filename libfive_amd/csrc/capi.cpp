@@ -689,6 +689,104 @@ libfive_mesh* libfive_tree_render_mesh_st(libfive_tree tree, libfive_region3 R,
     return render(tree, R, s, "libfive_tree_render_mesh_st");
 }
 
+libfive_mesh_coords* libfive_tree_render_mesh_coords(libfive_tree tree,
+                                                     libfive_region3 R,
+                                                     float res) {
+    libfive_mesh* m = libfive_tree_render_mesh(tree, R, res);
+    if (!m) return nullptr;
+    auto out = new libfive_mesh_coords;
+    out->vert_count = m->vert_count;
+    out->verts = m->verts;                    // new[]-allocated, moved over
+    out->coord_index_count = 4 * m->tri_count;
+    out->coord_indices = new int32_t[out->coord_index_count ? out->coord_index_count : 1];
+    for (uint32_t i = 0; i < m->tri_count; ++i) {
+        out->coord_indices[4 * i] = int32_t(m->tris[i].a);
+        out->coord_indices[4 * i + 1] = int32_t(m->tris[i].b);
+        out->coord_indices[4 * i + 2] = int32_t(m->tris[i].c);
+        out->coord_indices[4 * i + 3] = -1;
+    }
+    m->verts = nullptr;
+    libfive_mesh_delete(m);
+    return out;
+}
+
+void libfive_mesh_coords_delete(libfive_mesh_coords* m) {
+    if (!m) return;
+    delete[] m->verts;
+    delete[] m->coord_indices;
+    delete m;
+}
+
+// Binary STL of several meshes (Mesh::saveSTL, mesh.cpp:139-198).
+static bool save_stl(const char* f, const std::vector<libfive_mesh*>& meshes) {
+    const std::string name(f ? f : "");
+    if (name.size() < 4 || (name.compare(name.size() - 4, 4, ".stl") != 0 &&
+                            name.compare(name.size() - 4, 4, ".STL") != 0)) {
+        std::fprintf(stderr, "libfive_tree_save_mesh: filename \"%s\" does not end in .stl\n",
+                     name.c_str());
+    }
+    FILE* fp = std::fopen(name.c_str(), "wb");
+    if (!fp) {
+        std::fprintf(stderr, "libfive_tree_save_mesh: could not open %s\n", name.c_str());
+        return false;
+    }
+    char header[80];
+    std::memset(header, ' ', sizeof(header));
+    const char* text = "This is a binary STL exported from libfive.";
+    std::memcpy(header, text, std::strlen(text));
+    uint32_t count = 0;
+    for (auto* m : meshes) count += m->tri_count;
+    bool ok = std::fwrite(header, 1, 80, fp) == 80 &&
+              std::fwrite(&count, 4, 1, fp) == 1;
+    std::vector<char> rec(50 * size_t(1024));
+    for (auto* m : meshes) {
+        for (uint32_t i = 0; ok && i < m->tri_count;) {
+            const uint32_t n = std::min<uint32_t>(1024, m->tri_count - i);
+            for (uint32_t k = 0; k < n; ++k) {
+                char* r = rec.data() + 50 * k;
+                std::memset(r, 0, 50);             // zero normal and attribute
+                const libfive_tri& t = m->tris[i + k];
+                const uint32_t idx[3] = {t.a, t.b, t.c};
+                for (int j = 0; j < 3; ++j) std::memcpy(r + 12 + 12 * j, &m->verts[idx[j]], 12);
+            }
+            ok = std::fwrite(rec.data(), 50, n, fp) == n;
+            i += n;
+        }
+    }
+    ok = (std::fclose(fp) == 0) && ok;
+    return ok;
+}
+
+bool libfive_tree_save_mesh(libfive_tree tree, libfive_region3 R, float res,
+                            const char* f) {
+    libfive_mesh* m = libfive_tree_render_mesh(tree, R, res);
+    if (!m) return false;
+    const bool ok = save_stl(f, {m});
+    libfive_mesh_delete(m);
+    return ok;
+}
+
+bool libfive_tree_save_meshes(libfive_tree trees[], libfive_region3 R,
+                              float res, float quality, const char* f) {
+    std::vector<libfive_mesh*> ms;
+    bool ok = true;
+    for (size_t i = 0; trees[i] != nullptr; ++i) {
+        libfive_mesh_settings s;
+        libfive_mesh_settings_default(&s);
+        s.min_feature = 1 / res;
+        s.max_err = std::pow(10.0, -double(quality));
+        libfive_mesh* m = render(trees[i], R, s, "libfive_tree_save_meshes");
+        if (!m) {
+            ok = false;
+            break;
+        }
+        ms.push_back(m);
+    }
+    if (ok) ok = save_stl(f, ms);
+    for (auto* m : ms) libfive_mesh_delete(m);
+    return ok;
+}
+
 libfive_mesh* libfive_tree_render_mesh_settings(libfive_tree tree,
                                                 libfive_region3 R,
                                                 const libfive_mesh_settings* s) {

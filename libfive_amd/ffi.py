@@ -32,6 +32,13 @@ class libfive_tri(ctypes.Structure):
                 ("c", ctypes.c_uint32)]
 
 
+class libfive_mesh_coords(ctypes.Structure):
+    _fields_ = [("verts", ctypes.POINTER(libfive_vec3)),
+                ("vert_count", ctypes.c_uint32),
+                ("coord_indices", ctypes.POINTER(ctypes.c_int32)),
+                ("coord_index_count", ctypes.c_uint32)]
+
+
 class libfive_mesh(ctypes.Structure):
     _fields_ = [("verts", ctypes.POINTER(libfive_vec3)),
                 ("tris", ctypes.POINTER(libfive_tri)),
@@ -116,6 +123,14 @@ SIGNATURES = {
         ctypes.c_int, [ctypes.POINTER(tree_t), ctypes.POINTER(libfive_region3),
                        fp, ctypes.c_size_t, ctypes.c_int,
                        ctypes.POINTER(ctypes.POINTER(libfive_mesh))]),
+    "libfive_tree_render_mesh_coords": (ctypes.POINTER(libfive_mesh_coords),
+                                        [tree_t, libfive_region3, ctypes.c_float]),
+    "libfive_mesh_coords_delete": (None, [ctypes.POINTER(libfive_mesh_coords)]),
+    "libfive_tree_save_mesh": (ctypes.c_bool, [tree_t, libfive_region3,
+                                               ctypes.c_float, ctypes.c_char_p]),
+    "libfive_tree_save_meshes": (ctypes.c_bool, [ctypes.POINTER(tree_t),
+                                                 libfive_region3, ctypes.c_float,
+                                                 ctypes.c_float, ctypes.c_char_p]),
     "libfive_tape_specialize": (ctypes.c_int,
                                 [tape_t, ctypes.POINTER(ctypes.c_double)]),
     "libfive_tape_use_specialized": (None, [tape_t, ctypes.c_int]),

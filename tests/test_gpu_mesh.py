@@ -182,3 +182,76 @@ def test_batch_matches_single_renders(gpu):
         assert v.shape == v1.shape and t.shape == t1.shape
         assert (v.view(np.uint32) == v1.view(np.uint32)).all()
         assert (t == t1).all()
+
+
+def _region(lo, hi):
+    from libfive_amd import ffi
+    return ffi.libfive_region3(ffi.libfive_interval(lo, hi),
+                               ffi.libfive_interval(lo, hi),
+                               ffi.libfive_interval(lo, hi))
+
+
+def test_render_mesh_coords_matches_render_mesh(gpu):
+    """libfive_tree_render_mesh_coords (libfive.cpp:354-395): the same
+    vertices, triangles as index triples followed by -1."""
+    from libfive_amd import ffi
+    s = W.sphere(0.5)
+    verts, tris = s.render_mesh([-1] * 3, [1] * 3, res=16)
+    L = ffi.lib()
+    m = L.libfive_tree_render_mesh_coords(s.ptr, _region(-1, 1), 16.0)
+    assert m, ffi.last_error()
+    try:
+        mc = m.contents
+        v = np.ctypeslib.as_array(ctypes_cast_f(mc.verts), (mc.vert_count, 3)).copy()
+        ci = np.ctypeslib.as_array(mc.coord_indices, (mc.coord_index_count,)).copy()
+    finally:
+        L.libfive_mesh_coords_delete(m)
+    assert (v.view(np.uint32) == verts.view(np.uint32)).all()
+    ci = ci.reshape(-1, 4)
+    assert (ci[:, 3] == -1).all()
+    assert (ci[:, :3].astype(np.uint32) == tris).all()
+
+
+def ctypes_cast_f(p):
+    import ctypes
+    from libfive_amd import ffi
+    return ctypes.cast(p, ffi.fp)
+
+
+def _read_stl(path):
+    data = open(path, "rb").read()
+    n = int(np.frombuffer(data[80:84], np.uint32)[0])
+    rec = np.frombuffer(data[84:], dtype=np.dtype([("n", "<f4", 3), ("v", "<f4", (3, 3)),
+                                                   ("a", "<u2")]))
+    return data[:80], n, rec
+
+
+def test_save_mesh_writes_binary_stl(gpu, tmp_path):
+    """libfive_tree_save_mesh: binary STL (mesh.cpp:139-198) of the mesh
+    libfive_tree_render_mesh returns, triangle by triangle."""
+    from libfive_amd import ffi
+    s = W.csg_c1()
+    verts, tris = s.render_mesh([-2] * 3, [2] * 3, res=8)
+    path = str(tmp_path / "c1.stl")
+    assert ffi.lib().libfive_tree_save_mesh(s.ptr, _region(-2, 2), 8.0, path.encode())
+    header, n, rec = _read_stl(path)
+    assert header.startswith(b"This is a binary STL exported from libfive.")
+    assert header.rstrip(b" ") == b"This is a binary STL exported from libfive."
+    assert n == tris.shape[0] == rec.shape[0]
+    assert (rec["n"] == 0).all() and (rec["a"] == 0).all()
+    assert (rec["v"].view(np.uint32) == verts[tris].view(np.uint32)).all()
+
+
+def test_save_meshes_concatenates(gpu, tmp_path):
+    from libfive_amd import ffi
+    import ctypes
+    a, b = W.sphere(0.5), W.sphere(0.3, (0.5, 0.5, 0.5))
+    arr = (ffi.tree_t * 3)(a.ptr, b.ptr, None)
+    path = str(tmp_path / "two.stl")
+    assert ffi.lib().libfive_tree_save_meshes(arr, _region(-1, 1), 8.0, 8.0, path.encode())
+    _, n, rec = _read_stl(path)
+    me = 10.0 ** -8.0          # max_err = 10^-quality, as the entry point computes it
+    mf = float(np.float32(1) / np.float32(8))
+    na = a.render_mesh([-1] * 3, [1] * 3, min_feature=mf, max_err=me)[1].shape[0]
+    nb = b.render_mesh([-1] * 3, [1] * 3, min_feature=mf, max_err=me)[1].shape[0]
+    assert n == na + nb == rec.shape[0]

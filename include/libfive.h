@@ -181,6 +181,26 @@ bool libfive_tree_save_mesh(libfive_tree tree, libfive_region3 R, float res,
 bool libfive_tree_save_meshes(libfive_tree trees[], libfive_region3 R,
                               float res, float quality, const char* f);
 
+/* Free variables (libfive.h:171-179, 389-430; libfive.cpp:71-76,
+ * 408-496): an evaluator holds a tree and values for its free variables
+ * (keyed by libfive_tree_id of each variable); update_vars returns true
+ * when a value of a variable of the tree changed; save_mesh renders with
+ * the current values and the default settings (min_feature 0.1) and writes
+ * binary STL. */
+typedef struct libfive_vars {
+    void* const* vars;
+    float* values;
+    uint32_t size;
+} libfive_vars;
+void libfive_vars_delete(libfive_vars* j);
+typedef struct libfive_evaluator_ libfive_evaluator_;
+typedef struct libfive_evaluator_* libfive_evaluator;
+libfive_evaluator libfive_tree_evaluator(libfive_tree tree, libfive_vars vars);
+bool libfive_evaluator_update_vars(libfive_evaluator eval_tree, libfive_vars vars);
+bool libfive_evaluator_save_mesh(libfive_evaluator evaluator, libfive_region3 R,
+                                 const char* f);
+void libfive_evaluator_delete(libfive_evaluator ptr);
+
 /* Many independent meshes in one call (the many-model configuration; no
  * counterpart in the reference, whose callers loop over
  * libfive_tree_render_mesh, libfive.cpp:341-345).  Model i is meshed over

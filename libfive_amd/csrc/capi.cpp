@@ -43,6 +43,11 @@ static void set_error(const char* where, const std::exception& e) {
 static std::mutex g_keys_mu;
 static std::map<const libfive_mesh*, std::vector<uint32_t>> g_keys;
 
+struct libfive_evaluator_ {
+    Tree tree;
+    std::map<const void*, float> vars;
+};
+
 struct libfive_tape_ {
     std::unique_ptr<GpuTape> t;
     std::unique_ptr<SpecializedTape> spec;
@@ -786,6 +791,55 @@ bool libfive_tree_save_meshes(libfive_tree trees[], libfive_region3 R,
     for (auto* m : ms) libfive_mesh_delete(m);
     return ok;
 }
+
+void libfive_vars_delete(libfive_vars* j) {
+    if (!j) return;
+    delete[] j->vars;
+    delete[] j->values;
+    delete j;
+}
+
+libfive_evaluator libfive_tree_evaluator(libfive_tree tree, libfive_vars vars) {
+    auto* e = new libfive_evaluator_{Tree(D(tree)), {}};
+    for (uint32_t i = 0; i < vars.size; ++i) e->vars[vars.vars[i]] = vars.values[i];
+    return e;
+}
+
+bool libfive_evaluator_update_vars(libfive_evaluator ev, libfive_vars vars) {
+    // A variable counts only if the tree uses it (setVar's result).
+    const Deck deck(ev->tree);
+    bool changed = false;
+    for (uint32_t i = 0; i < vars.size; ++i) {
+        const void* id = vars.vars[i];
+        const bool used = std::find(deck.var_nodes.begin(), deck.var_nodes.end(),
+                                    static_cast<const TreeData*>(id)) != deck.var_nodes.end();
+        auto it = ev->vars.find(id);
+        const float old = (it == ev->vars.end()) ? 0.0f : it->second;
+        if (used && !(old == vars.values[i])) changed = true;
+        ev->vars[id] = vars.values[i];
+    }
+    return changed;
+}
+
+bool libfive_evaluator_save_mesh(libfive_evaluator ev, libfive_region3 R,
+                                 const char* f) {
+    try {
+        MeshSettings ms;                  // BRepSettings defaults
+        ms.vars = &ev->vars;
+        const double lo[3] = {R.X.lower, R.Y.lower, R.Z.lower};
+        const double hi[3] = {R.X.upper, R.Y.upper, R.Z.upper};
+        libfive_mesh* m = render_to_c(ev->tree, lo, hi, ms, false, nullptr);
+        if (!m) return false;
+        const bool ok = save_stl(f, {m});
+        libfive_mesh_delete(m);
+        return ok;
+    } catch (const std::exception& e) {
+        set_error("libfive_evaluator_save_mesh", e);
+        return false;
+    }
+}
+
+void libfive_evaluator_delete(libfive_evaluator ptr) { delete ptr; }
 
 libfive_mesh* libfive_tree_render_mesh_settings(libfive_tree tree,
                                                 libfive_region3 R,

@@ -26,6 +26,7 @@ Deck::Deck(const Tree& root_) {
                 break;
             case Opcode::VAR_FREE:
                 vars.push_back(id);
+                var_nodes.push_back(m);
                 break;
             case Opcode::ORACLE:
                 has_oracle = true;

@@ -27,6 +27,7 @@ public:
     std::vector<RefClause> clauses;                 // root first
     std::vector<std::pair<uint32_t, float>> constants;
     std::vector<uint32_t> vars;                      // VAR_FREE slots
+    std::vector<const TreeData*> var_nodes;          // their nodes (tree ids)
     uint32_t X = 0, Y = 0, Z = 0;
     uint32_t root = 1;
     size_t num_clauses = 0;                          // slots, excluding 0

@@ -202,7 +202,14 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
     ensure_tables();
 
     const Deck deck(tree);
-    const DeviceTape base = compile_deck(deck, nullptr);
+    std::vector<float> var_values(deck.vars.size(), 0.0f);
+    if (settings.vars) {
+        for (size_t i = 0; i < deck.var_nodes.size(); ++i) {
+            auto it = settings.vars->find(deck.var_nodes[i]);
+            if (it != settings.vars->end()) var_values[i] = it->second;
+        }
+    }
+    const DeviceTape base = compile_deck(deck, &var_values);
     const uint32_t nregs = base.num_regs;
     if (nregs > 128) {
         throw Error("render_mesh: the tape needs " + std::to_string(nregs) +

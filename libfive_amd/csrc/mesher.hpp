@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <functional>
+#include <map>
 #include <vector>
 
 #include <hip/hip_runtime.h>
@@ -20,6 +21,9 @@ struct MeshSettings {
     uint32_t precision = 0;       // QEF accumulation: only 0 (FP64) is built
     bool collect_stats = false;   // per-level cell counts and tape lengths
     volatile const int* cancel = nullptr;
+    // Values of free variables by tree id (libfive_tree_id); a variable
+    // not listed is 0.  The tape is compiled with these values.
+    const std::map<const void*, float>* vars = nullptr;
 };
 
 // Flat output: verts as xyz floats with the dummy vertex 0 first, tris as

@@ -39,6 +39,12 @@ class libfive_mesh_coords(ctypes.Structure):
                 ("coord_index_count", ctypes.c_uint32)]
 
 
+class libfive_vars(ctypes.Structure):
+    _fields_ = [("vars", ctypes.POINTER(ctypes.c_void_p)),
+                ("values", ctypes.POINTER(ctypes.c_float)),
+                ("size", ctypes.c_uint32)]
+
+
 class libfive_mesh(ctypes.Structure):
     _fields_ = [("verts", ctypes.POINTER(libfive_vec3)),
                 ("tris", ctypes.POINTER(libfive_tri)),
@@ -131,6 +137,12 @@ SIGNATURES = {
     "libfive_tree_save_meshes": (ctypes.c_bool, [ctypes.POINTER(tree_t),
                                                  libfive_region3, ctypes.c_float,
                                                  ctypes.c_float, ctypes.c_char_p]),
+    "libfive_tree_evaluator": (ctypes.c_void_p, [tree_t, libfive_vars]),
+    "libfive_evaluator_update_vars": (ctypes.c_bool, [ctypes.c_void_p, libfive_vars]),
+    "libfive_evaluator_save_mesh": (ctypes.c_bool, [ctypes.c_void_p, libfive_region3,
+                                                    ctypes.c_char_p]),
+    "libfive_evaluator_delete": (None, [ctypes.c_void_p]),
+    "libfive_vars_delete": (None, [ctypes.POINTER(libfive_vars)]),
     "libfive_tape_specialize": (ctypes.c_int,
                                 [tape_t, ctypes.POINTER(ctypes.c_double)]),
     "libfive_tape_use_specialized": (None, [tape_t, ctypes.c_int]),

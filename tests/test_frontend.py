@@ -188,3 +188,26 @@ def test_specialized_kernels_compile_on_host():
         Shape.binary("mod", X(), 0.5) + X().sin() * Y().cos()
     assert ffi.lib().libfive_tree_specialized_code_size(s.ptr) > 0, \
         ffi.last_error()
+
+
+def _vars(pairs):
+    n = len(pairs)
+    return ffi.libfive_vars((ctypes.c_void_p * n)(*[p for p, _ in pairs]),
+                            (ctypes.c_float * n)(*[v for _, v in pairs]), n)
+
+
+def test_evaluator_update_vars():
+    """libfive_evaluator_update_vars (Evaluator::updateVars,
+    libfive/include/libfive/eval/evaluator.hpp:41-49): true only when a
+    variable the tree uses changes value."""
+    L = ffi.lib()
+    v, w = Shape.var(), Shape.var()
+    t = X() + v * 2.0
+    vid, wid = L.libfive_tree_id(v.ptr), L.libfive_tree_id(w.ptr)
+    ev = L.libfive_tree_evaluator(t.ptr, _vars([(vid, 1.0)]))
+    try:
+        assert L.libfive_evaluator_update_vars(ev, _vars([(vid, 2.0)]))
+        assert not L.libfive_evaluator_update_vars(ev, _vars([(vid, 2.0)]))
+        assert not L.libfive_evaluator_update_vars(ev, _vars([(wid, 5.0)]))
+    finally:
+        L.libfive_evaluator_delete(ev)

@@ -255,3 +255,33 @@ def test_save_meshes_concatenates(gpu, tmp_path):
     na = a.render_mesh([-1] * 3, [1] * 3, min_feature=mf, max_err=me)[1].shape[0]
     nb = b.render_mesh([-1] * 3, [1] * 3, min_feature=mf, max_err=me)[1].shape[0]
     assert n == na + nb == rec.shape[0]
+
+
+def test_evaluator_save_mesh_uses_variables(gpu, tmp_path):
+    """libfive_evaluator_save_mesh renders with the evaluator's current
+    free-variable values (libfive.cpp:408-496): a sphere whose radius is a
+    variable, meshed at two radii."""
+    import ctypes
+    from libfive_amd import Shape, ffi
+    L = ffi.lib()
+    r = Shape.var()
+    t = (X().square() + Y().square() + Z().square()).sqrt() - r
+    rid = L.libfive_tree_id(r.ptr)
+
+    def vars_(val):
+        return ffi.libfive_vars((ctypes.c_void_p * 1)(rid), (ctypes.c_float * 1)(val), 1)
+
+    ev = L.libfive_tree_evaluator(t.ptr, vars_(0.5))
+    try:
+        for radius in (0.5, 0.8):
+            if radius != 0.5:
+                assert L.libfive_evaluator_update_vars(ev, vars_(radius))
+            path = str(tmp_path / f"r{radius}.stl")
+            assert L.libfive_evaluator_save_mesh(ev, _region(-1, 1), path.encode()), \
+                ffi.last_error()
+            _, n, rec = _read_stl(path)
+            assert n > 0
+            rr = np.sqrt((rec["v"].astype(np.float64) ** 2).sum(axis=2))
+            assert np.abs(rr - radius).max() < 0.01
+    finally:
+        L.libfive_evaluator_delete(ev)

@@ -241,6 +241,34 @@ def mesh_measurements(args):
             "tris": tris_batch, "same_tri_count": tris_batch == tris_plain,
         }
 
+    # Z-slice occupancy raster (libfive_tree_render_pixels) of the gyroid
+    # at 4096 x 4096 pixels
+    n_px = 4096
+    px_res = n_px / (hi - lo)
+    px_ts = []
+    for i in range(1 + max(3, args.mesh_steps)):
+        t0 = time.perf_counter()
+        img = gy.render_pixels((lo, lo), (hi, hi), 0.05, px_res)
+        if i:
+            px_ts.append((time.perf_counter() - t0) * 1e3)
+    out["pixels_gyroid_4096"] = {
+        "ms": round(float(np.median(px_ts)), 2), "shape": list(img.shape),
+        "filled": int(img.sum()),
+        "workload": f"same gyroid, slice z=0.05, {n_px}^2 pixels",
+    }
+    if not args.no_cpu:
+        odp = oracle.OracleDeck(gy.deck())
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            oimg = odp.render_pixels((lo, lo), (hi, hi), 0.05, px_res)
+            ts.append((time.perf_counter() - t0) * 1e3)
+        out["pixels_gyroid_4096"]["cpu_baseline"] = {
+            "value": round(float(np.median(ts)), 2), "unit": "ms", "cores": 1,
+            "kind": "port", "sample": "the whole image, median of 3",
+            "same_image": bool(np.array_equal(oimg, img)),
+        }
+
     if not args.no_cpu:
         # The oracle mesher (CPU restatement of the reference's DC mesher)
         # on the same gyroid at a lower resolution, with the GPU time there.

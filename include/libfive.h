@@ -56,6 +56,13 @@ typedef struct libfive_mesh_coords {
     uint32_t coord_index_count;
 } libfive_mesh_coords;
 
+/* libfive.h:120-128: an occupancy bitmap, width * height, row-major. */
+typedef struct libfive_pixels {
+    bool* pixels;
+    uint32_t width;
+    uint32_t height;
+} libfive_pixels;
+
 /* libfive.h:184-188: an opaque, intrusively refcounted tree node. */
 typedef struct libfive_tree_ libfive_tree_;
 typedef struct libfive_tree_* libfive_tree;
@@ -180,6 +187,17 @@ bool libfive_tree_save_mesh(libfive_tree tree, libfive_region3 R, float res,
                             const char* f);
 bool libfive_tree_save_meshes(libfive_tree trees[], libfive_region3 R,
                               float res, float quality, const char* f);
+
+/* Occupancy raster of the slice at height z (libfive.h:408-415,
+ * libfive.cpp:442-465 -> Heightmap::render, heightmap.cpp:106-319).  The
+ * region is widened to whole voxels of size 1/res (voxels.cpp:23-45); a
+ * pixel is set where the tree is below zero at the voxel centre, decided by
+ * interval bounds for blocks of more than 256 voxels.  Free with
+ * libfive_pixels_delete (libfive.h:155). */
+libfive_pixels* libfive_tree_render_pixels(libfive_tree tree,
+                                           libfive_region2 R,
+                                           float z, float res);
+void libfive_pixels_delete(libfive_pixels* ps);
 
 /* Free variables (libfive.h:171-179, 389-430; libfive.cpp:71-76,
  * 408-496): an evaluator holds a tree and values for its free variables

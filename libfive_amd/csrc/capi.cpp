@@ -12,6 +12,7 @@
 
 #include "../../include/libfive.h"
 #include "eval_kernels.hpp"
+#include "heightmap.hpp"
 #include "interval_kernels.hpp"
 #include "mesher.hpp"
 #include "tape_jit.hpp"
@@ -790,6 +791,30 @@ bool libfive_tree_save_meshes(libfive_tree trees[], libfive_region3 R,
     if (ok) ok = save_stl(f, ms);
     for (auto* m : ms) libfive_mesh_delete(m);
     return ok;
+}
+
+libfive_pixels* libfive_tree_render_pixels(libfive_tree tree, libfive_region2 R,
+                                           float z, float res) {
+    try {
+        PixelImage img = render_pixels(Tree(D(tree)), R.X.lower, R.X.upper,
+                                       R.Y.lower, R.Y.upper, z, res);
+        auto out = new libfive_pixels;
+        out->width = img.width;
+        out->height = img.height;
+        const size_t n = size_t(img.width) * img.height;
+        out->pixels = new bool[n ? n : 1];
+        for (size_t i = 0; i < n; ++i) out->pixels[i] = img.filled[i] != 0;
+        return out;
+    } catch (const std::exception& e) {
+        set_error("libfive_tree_render_pixels", e);
+        return nullptr;
+    }
+}
+
+void libfive_pixels_delete(libfive_pixels* ps) {
+    if (!ps) return;
+    delete[] ps->pixels;
+    delete ps;
 }
 
 void libfive_vars_delete(libfive_vars* j) {

@@ -1,0 +1,206 @@
+// Z-slice occupancy raster (see heightmap.hpp).
+#include "heightmap.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <deque>
+#include <mutex>
+
+#include "eval_kernels.hpp"
+#include "interval_kernels.hpp"
+#include "runtime.hpp"
+
+namespace libfive {
+namespace amd {
+
+namespace {
+
+// Voxels (voxels.cpp:23-45): the region is widened to a whole number of
+// voxels per axis and the sample points are voxel centres, in float.
+struct Grid {
+    float lower[3], upper[3];
+    std::vector<float> pts[3];
+};
+
+Grid make_grid(const float lo[3], const float hi[3], float res) {
+    Grid g;
+    for (int i = 0; i < 3; ++i) {
+        const float span = hi[i] - lo[i];
+        const int size = int(std::max(std::ceil(res * span), 1.0f));
+        const float extra = (res > 0.0f) ? float(size) / res - span : 0.0f;
+        g.lower[i] = lo[i] - extra / 2;
+        g.upper[i] = hi[i] + extra / 2;
+        for (int index = 0; index < size; ++index) {
+            const float frac = float((index + 0.5) / size);
+            g.pts[i].push_back(g.lower[i] * (1 - frac) + g.upper[i] * frac);
+        }
+    }
+    return g;
+}
+
+// Voxels::View (voxels.hpp:40-120): a box, its voxel counts, its first
+// voxel.
+struct View {
+    float lower[3], upper[3];
+    int size[3];
+    int corner[3];
+    size_t voxels() const { return size_t(size[0]) * size[1] * size[2]; }
+};
+
+// View::split<A> (voxels.hpp:71-104): halves the largest axis among those
+// in mask A (the first one on ties); the lower half gets the extra voxel.
+void split(const View& v, unsigned mask, View& a, View& b) {
+    int axis = 0, best = -1;
+    for (int i = 0; i < 3; ++i) {
+        const int s = (mask & (1u << i)) ? v.size[i] : 0;
+        if (s > best) {
+            best = s;
+            axis = i;
+        }
+    }
+    const int size_upper = v.size[axis] / 2;
+    const int size_lower = v.size[axis] - size_upper;
+    const float frac = size_lower / float(v.size[axis]);
+    const float middle = (v.upper[axis] * frac) + (v.lower[axis] * (1 - frac));
+    a = v;
+    b = v;
+    a.upper[axis] = middle;
+    a.size[axis] = size_lower;
+    b.lower[axis] = middle;
+    b.size[axis] = size_upper;
+    b.corner[axis] += size_lower;
+}
+
+}   // namespace
+
+PixelImage render_pixels(const Tree& t, float xlo, float xhi, float ylo,
+                         float yhi, float z, float res) {
+    Context& c = context();
+    const float lo[3] = {xlo, ylo, z}, hi[3] = {xhi, yhi, z};
+    const Grid g = make_grid(lo, hi, res);
+    PixelImage img;
+    img.width = uint32_t(g.pts[0].size());
+    img.height = uint32_t(g.pts[1].size());
+    img.filled.assign(size_t(img.width) * img.height, 0);
+    const float zs = g.pts[2].back();    // the slice's single voxel centre
+
+    auto gt = compile_tree(t);
+    std::lock_guard<std::mutex> lock(c.mu);
+
+    // Initial regions: XY halves until there are 8 (Heightmap::render with
+    // 8 workers, heightmap.cpp:285-294).
+    View root;
+    for (int i = 0; i < 3; ++i) {
+        root.lower[i] = g.lower[i];
+        root.upper[i] = g.upper[i];
+        root.size[i] = int(g.pts[i].size());
+        root.corner[i] = 0;
+    }
+    std::deque<View> rs{root};
+    while (rs.size() < 8 && std::min(rs.front().size[0], rs.front().size[1]) > 1) {
+        const View f = rs.front();
+        rs.pop_front();
+        View a, b;
+        split(f, 3u, a, b);
+        rs.push_back(a);
+        rs.push_back(b);
+    }
+
+    auto fill = [&](const View& v) {
+        for (int j = 0; j < v.size[1]; ++j) {
+            for (int i = 0; i < v.size[0]; ++i) {
+                img.filled[size_t(v.corner[1] + j) * img.width + v.corner[0] + i] = 1;
+            }
+        }
+    };
+
+    // Heightmap::recurse (heightmap.cpp:193-246), one level at a time.
+    std::vector<View> frontier(rs.begin(), rs.end()), pixel_views;
+    std::vector<float> boxes, olo, ohi;
+    while (!frontier.empty()) {
+        std::vector<View> big;
+        for (const View& v : frontier) {
+            if (v.voxels() == 0) continue;
+            if (v.voxels() <= 256) pixel_views.push_back(v);
+            else big.push_back(v);
+        }
+        frontier.clear();
+        const size_t n = big.size();
+        if (n == 0) break;
+        boxes.assign(6 * n, 0.0f);
+        for (size_t k = 0; k < n; ++k) {
+            for (int a = 0; a < 3; ++a) {
+                boxes[(2 * a) * n + k] = big[k].lower[a];
+                boxes[(2 * a + 1) * n + k] = big[k].upper[a];
+            }
+        }
+        DevBuf<float> d(8 * n);
+        DevBuf<uint8_t> dn(n);
+        hip_check(hipMemcpyAsync(d.p, boxes.data(), 6 * n * sizeof(float),
+                                 hipMemcpyHostToDevice, c.stream), "copy boxes");
+        const auto& h = gt->host;
+        hip_check(launch_eval_intervals(gt->d.p, gt->size(), h.root_reg, h.num_regs,
+                                        d.p, d.p + n, d.p + 2 * n, d.p + 3 * n,
+                                        d.p + 4 * n, d.p + 5 * n, d.p + 6 * n,
+                                        d.p + 7 * n, dn.p, n, c.stream),
+                  "pixel intervals");
+        olo.resize(n);
+        ohi.resize(n);
+        hip_check(hipMemcpyAsync(olo.data(), d.p + 6 * n, n * sizeof(float),
+                                 hipMemcpyDeviceToHost, c.stream), "copy lo");
+        hip_check(hipMemcpyAsync(ohi.data(), d.p + 7 * n, n * sizeof(float),
+                                 hipMemcpyDeviceToHost, c.stream), "copy hi");
+        hip_check(hipStreamSynchronize(c.stream), "pixel intervals");
+        for (size_t k = 0; k < n; ++k) {
+            // Interval::isFilled / isEmpty on the bounds alone; the
+            // may-be-NaN flag plays no part here (heightmap.cpp:221-233).
+            if (ohi[k] < 0.0f) {
+                fill(big[k]);
+                continue;
+            } else if (olo[k] > 0.0f) {
+                continue;
+            }
+            View a, b;
+            split(big[k], 7u, a, b);
+            frontier.push_back(b);
+            frontier.push_back(a);
+        }
+    }
+
+    // Heightmap::pixels (heightmap.cpp:100-155): point values at the voxel
+    // centres of the small views; filled where the value is below zero.
+    std::vector<float> px, py, pz;
+    std::vector<uint32_t> where;
+    for (const View& v : pixel_views) {
+        for (int i = 0; i < v.size[0]; ++i) {
+            for (int j = 0; j < v.size[1]; ++j) {
+                px.push_back(g.pts[0][v.corner[0] + i]);
+                py.push_back(g.pts[1][v.corner[1] + j]);
+                pz.push_back(zs);
+                where.push_back(uint32_t((v.corner[1] + j) * img.width + v.corner[0] + i));
+            }
+        }
+    }
+    const size_t np = px.size();
+    if (np) {
+        DevBuf<float> d(4 * np);
+        hip_check(hipMemcpyAsync(d.p, px.data(), np * 4, hipMemcpyHostToDevice, c.stream), "copy x");
+        hip_check(hipMemcpyAsync(d.p + np, py.data(), np * 4, hipMemcpyHostToDevice, c.stream), "copy y");
+        hip_check(hipMemcpyAsync(d.p + 2 * np, pz.data(), np * 4, hipMemcpyHostToDevice, c.stream), "copy z");
+        const auto& h = gt->host;
+        hip_check(launch_eval_values(gt->d.p, gt->size(), h.root_reg, h.num_regs, d.p,
+                                     d.p + np, d.p + 2 * np, d.p + 3 * np, np, c.stream),
+                  "pixel values");
+        std::vector<float> out(np);
+        hip_check(hipMemcpyAsync(out.data(), d.p + 3 * np, np * 4, hipMemcpyDeviceToHost,
+                                 c.stream), "copy values");
+        hip_check(hipStreamSynchronize(c.stream), "pixel values");
+        for (size_t k = 0; k < np; ++k) {
+            if (out[k] < 0.0f) img.filled[where[k]] = 1;
+        }
+    }
+    return img;
+}
+
+}   // namespace amd
+}   // namespace libfive

@@ -22,6 +22,16 @@ class libfive_region3(ctypes.Structure):
                 ("Z", libfive_interval)]
 
 
+class libfive_region2(ctypes.Structure):
+    _fields_ = [("X", libfive_interval), ("Y", libfive_interval)]
+
+
+class libfive_pixels(ctypes.Structure):
+    _fields_ = [("pixels", ctypes.POINTER(ctypes.c_bool)),
+                ("width", ctypes.c_uint32),
+                ("height", ctypes.c_uint32)]
+
+
 class libfive_vec3(ctypes.Structure):
     _fields_ = [("x", ctypes.c_float), ("y", ctypes.c_float),
                 ("z", ctypes.c_float)]
@@ -132,6 +142,10 @@ SIGNATURES = {
     "libfive_tree_render_mesh_coords": (ctypes.POINTER(libfive_mesh_coords),
                                         [tree_t, libfive_region3, ctypes.c_float]),
     "libfive_mesh_coords_delete": (None, [ctypes.POINTER(libfive_mesh_coords)]),
+    "libfive_tree_render_pixels": (ctypes.POINTER(libfive_pixels),
+                                   [tree_t, libfive_region2, ctypes.c_float,
+                                    ctypes.c_float]),
+    "libfive_pixels_delete": (None, [ctypes.POINTER(libfive_pixels)]),
     "libfive_tree_save_mesh": (ctypes.c_bool, [tree_t, libfive_region3,
                                                ctypes.c_float, ctypes.c_char_p]),
     "libfive_tree_save_meshes": (ctypes.c_bool, [ctypes.POINTER(tree_t),

@@ -249,6 +249,24 @@ class Shape:
         out.cells = cells[:nc]
         return out
 
+    def render_pixels(self, xy_lo, xy_hi, z, res):
+        """Occupancy raster of the slice at height z over [xy_lo, xy_hi]
+        (libfive_tree_render_pixels): a bool array of shape (height, width),
+        row 0 at the lowest y."""
+        L = ffi.lib()
+        R = ffi.libfive_region2(ffi.libfive_interval(xy_lo[0], xy_hi[0]),
+                                ffi.libfive_interval(xy_lo[1], xy_hi[1]))
+        p = L.libfive_tree_render_pixels(self.ptr, R, float(z), float(res))
+        if not p:
+            raise GpuError(ffi.last_error())
+        try:
+            pp = p.contents
+            return np.ctypeslib.as_array(
+                ctypes.cast(pp.pixels, ctypes.POINTER(ctypes.c_uint8)),
+                (pp.height, pp.width)).astype(bool)
+        finally:
+            L.libfive_pixels_delete(p)
+
 
 def _mesh_arrays(m):
     mm = m.contents

@@ -63,6 +63,9 @@ _SIGS = {
                                              _dp, ctypes.POINTER(ctypes.c_int)]),
     "oracle_push_sample": (None, [_dp, _dp, _dp, ctypes.c_double]),
     "oracle_resolution_level": (ctypes.c_int, [_dp, _dp, ctypes.c_double]),
+    "oracle_render_pixels": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_float] * 6
+                             + [_u8, _u32, _u32]),
+    "oracle_voxel_pts": (None, [ctypes.c_float] * 3 + [_fp, _u32, _fp, _fp]),
 }
 
 _lib = None
@@ -211,6 +214,33 @@ class OracleDeck:
         out.cells = cells
         out.verts64 = v64
         return out
+
+    def render_pixels(self, xy_lo, xy_hi, z, res):
+        """Occupancy raster of the slice at height z: bool[height, width]
+        (row 0 at the lowest y), as libfive_tree_render_pixels."""
+        L = lib()
+        args = [float(xy_lo[0]), float(xy_hi[0]), float(xy_lo[1]),
+                float(xy_hi[1]), float(z), float(res)]
+        w, h = ctypes.c_uint32(), ctypes.c_uint32()
+        L.oracle_render_pixels(self.ptr, *args, None, ctypes.byref(w),
+                               ctypes.byref(h))
+        out = np.zeros((h.value, w.value), np.uint8)
+        L.oracle_render_pixels(self.ptr, *args, out.ctypes.data_as(_u8),
+                               ctypes.byref(w), ctypes.byref(h))
+        return out.astype(bool)
+
+
+def voxel_pts(lo, hi, res):
+    """(centres, lower, upper) of one axis of the voxel grid."""
+    L = lib()
+    n = ctypes.c_uint32()
+    L.oracle_voxel_pts(lo, hi, res, None, ctypes.byref(n), None, None)
+    pts = np.zeros(n.value, np.float32)
+    lower = np.zeros(1, np.float32)
+    upper = np.zeros(1, np.float32)
+    L.oracle_voxel_pts(lo, hi, res, _f(pts), ctypes.byref(n), _f(lower),
+                       _f(upper))
+    return pts, float(lower[0]), float(upper[0])
 
 
 class MeshResult:

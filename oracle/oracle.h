@@ -97,6 +97,17 @@ size_t oracle_mesh_leaf_verts(const oracle_mesh* m, double* out);
 size_t oracle_mesh_cells(const oracle_mesh* m, uint32_t* out);
 void oracle_mesh_delete(oracle_mesh* m);
 
+/* Occupancy raster of the slice at height z (libfive_tree_render_pixels):
+ * writes width * height bytes, row-major, 1 where the shape is filled.
+ * With out == NULL only the sizes are returned. */
+int oracle_render_pixels(oracle_deck* d, float xlo, float xhi, float ylo,
+                         float yhi, float z, float res, uint8_t* out,
+                         uint32_t* width, uint32_t* height);
+/* The voxel grid of one axis over [lo, hi] at res (centres, count and the
+ * widened bounds). */
+void oracle_voxel_pts(float lo, float hi, float res, float* out, uint32_t* n,
+                      float* lower, float* upper);
+
 /* Tables and building blocks, exposed for their own tests. */
 int oracle_corner_manifold(int mask);
 void oracle_marching_table(int8_t* v /* 256*4*12*2 */, int8_t* e /* 8*8 */,

@@ -56,6 +56,25 @@ typedef struct libfive_mesh_coords {
     uint32_t coord_index_count;
 } libfive_mesh_coords;
 
+/* libfive.h:60-92: contours as point sequences (2D, or 3D with the
+ * slice height as Z); a closed contour repeats its first point last. */
+typedef struct libfive_contour {
+    libfive_vec2* pts;
+    uint32_t count;
+} libfive_contour;
+typedef struct libfive_contours {
+    libfive_contour* cs;
+    uint32_t count;
+} libfive_contours;
+typedef struct libfive_contour3 {
+    libfive_vec3* pts;
+    uint32_t count;
+} libfive_contour3;
+typedef struct libfive_contours3 {
+    libfive_contour3* cs;
+    uint32_t count;
+} libfive_contours3;
+
 /* libfive.h:120-128: an occupancy bitmap, width * height, row-major. */
 typedef struct libfive_pixels {
     bool* pixels;
@@ -187,6 +206,35 @@ bool libfive_tree_save_mesh(libfive_tree tree, libfive_region3 R, float res,
                             const char* f);
 bool libfive_tree_save_meshes(libfive_tree trees[], libfive_region3 R,
                               float res, float quality, const char* f);
+
+/* 2D dual contouring of the slice at height z (libfive.h:318-335,
+ * libfive.cpp:233-293 -> Contours::render, contours.cpp:25-49, with
+ * min_feature = 1 / res and max_err = 1e-8): quadtree, leaf edge search and
+ * QEF vertices as the 3D mesher, segments from the dual walk
+ * (dc_contourer.cpp:16-98) joined head to tail into contours
+ * (contours.cpp:112-184).  Free with libfive_contours_delete /
+ * libfive_contours3_delete (libfive.h:132-150). */
+libfive_contours* libfive_tree_render_slice(libfive_tree tree, libfive_region2 R,
+                                            float z, float res);
+libfive_contours3* libfive_tree_render_slice3(libfive_tree tree, libfive_region2 R,
+                                              float z, float res);
+void libfive_contours_delete(libfive_contours* cs);
+void libfive_contours3_delete(libfive_contours3* cs);
+/* The same with explicit settings, and the vertices, their keys (x, y of
+ * the owning cell's lower corner in finest-cell units, level, patch) and
+ * the segments as vertex-index pairs, for tests and tools: returns NULL
+ * with libfive_last_error() set on failure. */
+typedef struct libfive_slice_segments {
+    float* verts;          /* 2 per vertex; vertex 0 is a reserved dummy */
+    uint32_t* keys;        /* 4 per vertex */
+    uint32_t vert_count;
+    uint32_t* segs;        /* 2 per segment */
+    uint32_t seg_count;
+} libfive_slice_segments;
+libfive_slice_segments* libfive_tree_render_slice_segments(
+    libfive_tree tree, libfive_region2 R, float z, double min_feature,
+    double max_err);
+void libfive_slice_segments_delete(libfive_slice_segments* s);
 
 /* Occupancy raster of the slice at height z (libfive.h:408-415,
  * libfive.cpp:442-465 -> Heightmap::render, heightmap.cpp:106-319).  The

@@ -12,6 +12,7 @@
 
 #include "../../include/libfive.h"
 #include "eval_kernels.hpp"
+#include "contours.hpp"
 #include "heightmap.hpp"
 #include "interval_kernels.hpp"
 #include "mesher.hpp"
@@ -791,6 +792,105 @@ bool libfive_tree_save_meshes(libfive_tree trees[], libfive_region3 R,
     if (ok) ok = save_stl(f, ms);
     for (auto* m : ms) libfive_mesh_delete(m);
     return ok;
+}
+
+static ContourResult slice(libfive_tree tree, libfive_region2 R, float z,
+                           double min_feature, double max_err) {
+    const double lo[2] = {R.X.lower, R.Y.lower}, hi[2] = {R.X.upper, R.Y.upper};
+    return render_contours(Tree(D(tree)), lo, hi, z, min_feature, max_err);
+}
+
+libfive_contours* libfive_tree_render_slice(libfive_tree tree, libfive_region2 R,
+                                            float z, float res) {
+    try {
+        const ContourResult r = slice(tree, R, z, 1 / res, 1e-8);
+        auto out = new libfive_contours;
+        out->count = uint32_t(r.contours.size());
+        out->cs = new libfive_contour[out->count ? out->count : 1];
+        for (uint32_t i = 0; i < out->count; ++i) {
+            const auto& c = r.contours[i];
+            out->cs[i].count = uint32_t(c.size());
+            out->cs[i].pts = new libfive_vec2[c.size() ? c.size() : 1];
+            for (size_t j = 0; j < c.size(); ++j) {
+                out->cs[i].pts[j] = {r.verts[c[j]][0], r.verts[c[j]][1]};
+            }
+        }
+        return out;
+    } catch (const std::exception& e) {
+        set_error("libfive_tree_render_slice", e);
+        return nullptr;
+    }
+}
+
+libfive_contours3* libfive_tree_render_slice3(libfive_tree tree, libfive_region2 R,
+                                              float z, float res) {
+    try {
+        const ContourResult r = slice(tree, R, z, 1 / res, 1e-8);
+        auto out = new libfive_contours3;
+        out->count = uint32_t(r.contours.size());
+        out->cs = new libfive_contour3[out->count ? out->count : 1];
+        for (uint32_t i = 0; i < out->count; ++i) {
+            const auto& c = r.contours[i];
+            out->cs[i].count = uint32_t(c.size());
+            out->cs[i].pts = new libfive_vec3[c.size() ? c.size() : 1];
+            for (size_t j = 0; j < c.size(); ++j) {
+                out->cs[i].pts[j] = {r.verts[c[j]][0], r.verts[c[j]][1], z};
+            }
+        }
+        return out;
+    } catch (const std::exception& e) {
+        set_error("libfive_tree_render_slice3", e);
+        return nullptr;
+    }
+}
+
+void libfive_contours_delete(libfive_contours* cs) {
+    if (!cs) return;
+    for (uint32_t i = 0; i < cs->count; ++i) delete[] cs->cs[i].pts;
+    delete[] cs->cs;
+    delete cs;
+}
+
+void libfive_contours3_delete(libfive_contours3* cs) {
+    if (!cs) return;
+    for (uint32_t i = 0; i < cs->count; ++i) delete[] cs->cs[i].pts;
+    delete[] cs->cs;
+    delete cs;
+}
+
+libfive_slice_segments* libfive_tree_render_slice_segments(
+        libfive_tree tree, libfive_region2 R, float z, double min_feature,
+        double max_err) {
+    try {
+        const ContourResult r = slice(tree, R, z, min_feature, max_err);
+        auto out = new libfive_slice_segments;
+        out->vert_count = uint32_t(r.verts.size());
+        out->seg_count = uint32_t(r.segs.size());
+        out->verts = new float[2 * r.verts.size()];
+        out->keys = new uint32_t[4 * r.verts.size()];
+        out->segs = new uint32_t[2 * r.segs.size() + 1];
+        for (size_t i = 0; i < r.verts.size(); ++i) {
+            out->verts[2 * i] = r.verts[i][0];
+            out->verts[2 * i + 1] = r.verts[i][1];
+            for (int k = 0; k < 4; ++k) out->keys[4 * i + k] = r.keys[i][k];
+        }
+        for (size_t i = 0; i < r.segs.size(); ++i) {
+            out->segs[2 * i] = r.segs[i][0];
+            out->segs[2 * i + 1] = r.segs[i][1];
+        }
+        return out;
+    } catch (const std::exception& e) {
+        set_error("libfive_tree_render_slice_segments", e);
+        return nullptr;
+    }
+}
+
+void libfive_slice_segments_delete(libfive_slice_segments* s) {
+    if (!s) return;
+    delete[] s->verts;
+    delete[] s->keys;
+    delete[] s->segs;
+    delete s;
 }
 
 libfive_pixels* libfive_tree_render_pixels(libfive_tree tree, libfive_region2 R,

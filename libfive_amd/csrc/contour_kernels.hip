@@ -1,0 +1,128 @@
+// Point passes of the 2D contourer: batched values, partials, ambiguity
+// and feature evaluation over arbitrary point lists, on the interpreters
+// shared with the mesher's leaf kernel.
+#include <hip/hip_runtime.h>
+
+#include "contour_kernels.hpp"
+#include "feature_eval.hpp"
+#include "point_eval.hpp"
+#include "regfile.hpp"
+
+namespace libfive {
+namespace amd {
+
+namespace {
+
+int banks(uint32_t nregs) {
+    const uint32_t b = (nregs + 31) / 32;
+    return b <= 1 ? 1 : b <= 2 ? 2 : b <= 4 ? 4 : 0;
+}
+
+template <int NV>
+__global__ __launch_bounds__(64)
+void k_point_values(const tape::Clause* __restrict__ t, uint32_t n, uint32_t root,
+                    const float* __restrict__ xyz, float* __restrict__ v,
+                    uint8_t* __restrict__ amb, size_t npts)
+{
+    const size_t i = size_t(blockIdx.x) * 64 + threadIdx.x;
+    const size_t k = i < npts ? i : npts - 1;
+    uint32_t fl;
+    const float r = pe_value<NV>(t, n, root, xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2], &fl);
+    if (i < npts) {
+        v[i] = r;
+        amb[i] = (fl & PT_AMBIGUOUS) ? 1 : 0;
+    }
+}
+
+template <int NV>
+__global__ __launch_bounds__(64)
+void k_point_grads(const tape::Clause* __restrict__ t, uint32_t n, uint32_t root,
+                   const float* __restrict__ xyz, float* __restrict__ v,
+                   float* __restrict__ g, uint8_t* __restrict__ amb, size_t npts)
+{
+    const size_t i = size_t(blockIdx.x) * 64 + threadIdx.x;
+    const size_t k = i < npts ? i : npts - 1;
+    uint32_t fl = 0;
+    float d[3];
+    const float r = pe_grad3<NV>(t, n, root, xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2],
+                                 d, &fl);
+    if (i < npts) {
+        v[i] = r;
+        g[3 * i] = d[0];
+        g[3 * i + 1] = d[1];
+        g[3 * i + 2] = d[2];
+        amb[i] = (fl & PT_AMBIGUOUS) ? 1 : 0;
+    }
+}
+
+// One wave per point; lane 0 runs the (serial) feature walk.
+__global__ __launch_bounds__(64)
+void k_point_features(const tape::Clause* __restrict__ t, uint32_t n, uint32_t root,
+                      uint32_t nregs, uint32_t sx, uint32_t sy, uint32_t sz,
+                      const float* __restrict__ xyz, uint8_t* __restrict__ inside,
+                      float* __restrict__ feats, int32_t* __restrict__ nfeat,
+                      uint8_t* __restrict__ ws, size_t ws_bytes, uint32_t* err,
+                      size_t npts)
+{
+    if (threadIdx.x != 0) return;
+    feat::Workspace W = feat::carve(ws + size_t(blockIdx.x) * ws_bytes, n, nregs);
+    for (size_t i = blockIdx.x; i < npts; i += gridDim.x) {
+        const float p[3] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+        inside[i] = feat::is_inside(t, n, nregs, root, sx, sy, sz, p, W, err) ? 1 : 0;
+        float out[feat::kMaxOut][3];
+        const int m = feat::features(t, n, nregs, root, sx, sy, sz, p, W, out, err);
+        nfeat[i] = m;
+        for (int j = 0; j < m; ++j) {
+            for (int q = 0; q < 3; ++q) feats[(i * feat::kMaxOut + j) * 3 + q] = out[j][q];
+        }
+    }
+}
+
+}   // namespace
+
+hipError_t launch_point_values(const tape::Clause* t, uint32_t n, uint32_t root,
+                               uint32_t nregs, const float* xyz, float* v,
+                               uint8_t* amb, size_t npts, hipStream_t s) {
+    if (npts == 0) return hipSuccess;
+    const dim3 grid(uint32_t((npts + 63) / 64));
+    switch (banks(nregs)) {
+        case 1: hipLaunchKernelGGL(k_point_values<1>, grid, dim3(64), 0, s, t, n, root, xyz, v, amb, npts); break;
+        case 2: hipLaunchKernelGGL(k_point_values<2>, grid, dim3(64), 0, s, t, n, root, xyz, v, amb, npts); break;
+        case 4: hipLaunchKernelGGL(k_point_values<4>, grid, dim3(64), 0, s, t, n, root, xyz, v, amb, npts); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_point_grads(const tape::Clause* t, uint32_t n, uint32_t root,
+                              uint32_t nregs, const float* xyz, float* v, float* g,
+                              uint8_t* amb, size_t npts, hipStream_t s) {
+    if (npts == 0) return hipSuccess;
+    const dim3 grid(uint32_t((npts + 63) / 64));
+    switch (banks(nregs)) {
+        case 1: hipLaunchKernelGGL(k_point_grads<1>, grid, dim3(64), 0, s, t, n, root, xyz, v, g, amb, npts); break;
+        case 2: hipLaunchKernelGGL(k_point_grads<2>, grid, dim3(64), 0, s, t, n, root, xyz, v, g, amb, npts); break;
+        case 4: hipLaunchKernelGGL(k_point_grads<4>, grid, dim3(64), 0, s, t, n, root, xyz, v, g, amb, npts); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+size_t point_feature_workspace(uint32_t n, uint32_t nregs) {
+    return (feat::workspace_bytes(n, nregs) + 255) & ~size_t(255);
+}
+
+hipError_t launch_point_features(const tape::Clause* t, uint32_t n, uint32_t root,
+                                 uint32_t nregs, uint32_t sx, uint32_t sy, uint32_t sz,
+                                 const float* xyz, uint8_t* inside, float* feats,
+                                 int32_t* nfeat, uint8_t* ws, size_t ws_bytes,
+                                 uint32_t* err, size_t npts, hipStream_t s) {
+    if (npts == 0) return hipSuccess;
+    const uint32_t grid = uint32_t(npts < kFeatureWaves ? npts : kFeatureWaves);
+    hipLaunchKernelGGL(k_point_features, dim3(grid), dim3(64), 0, s, t, n, root, nregs,
+                       sx, sy, sz, xyz, inside, feats, nfeat, ws, ws_bytes, err, npts);
+    return hipGetLastError();
+}
+
+}   // namespace amd
+}   // namespace libfive

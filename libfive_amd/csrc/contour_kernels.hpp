@@ -1,0 +1,40 @@
+// Host-side launchers for the point passes of the 2D contourer (device
+// pointers, asynchronous on `s`).  One lane per point for values and
+// partials; one wave per point for the feature evaluator.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+#include "tape.hpp"
+
+namespace libfive {
+namespace amd {
+
+// Values and ambiguity (a min / max with equal operands on the point's
+// path, ArrayEvaluator::getAmbiguous).
+hipError_t launch_point_values(const tape::Clause* t, uint32_t n, uint32_t root,
+                               uint32_t nregs, const float* xyz, float* v,
+                               uint8_t* amb, size_t npts, hipStream_t s);
+
+// Values, the three partials (g: 3 per point) and ambiguity.
+hipError_t launch_point_grads(const tape::Clause* t, uint32_t n, uint32_t root,
+                              uint32_t nregs, const float* xyz, float* v, float* g,
+                              uint8_t* amb, size_t npts, hipStream_t s);
+
+// FeatureEvaluator::isInside (inside[i]) and ::features (feats: up to 16
+// gradients of 3 floats per point, nfeat[i] of them).  ws holds
+// point_feature_workspace(n, nregs) bytes per wave in flight (at most
+// kFeatureWaves); err gets a nonzero bit when a point has more features
+// than the evaluation holds.
+constexpr uint32_t kFeatureWaves = 256;
+size_t point_feature_workspace(uint32_t n, uint32_t nregs);
+hipError_t launch_point_features(const tape::Clause* t, uint32_t n, uint32_t root,
+                                 uint32_t nregs, uint32_t sx, uint32_t sy, uint32_t sz,
+                                 const float* xyz, uint8_t* inside, float* feats,
+                                 int32_t* nfeat, uint8_t* ws, size_t ws_bytes,
+                                 uint32_t* err, size_t npts, hipStream_t s);
+
+}   // namespace amd
+}   // namespace libfive

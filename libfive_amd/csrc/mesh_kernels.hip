@@ -31,105 +31,6 @@
 namespace libfive {
 namespace amd {
 
-// Out-of-line derivative rules of the less common opcodes
-// (eval_deriv_array.cpp:119-256).
-__device__ __noinline__ void pe_cold_deriv(uint32_t o, float av, float ad0,
-                                           float ad1, float ad2, float bv,
-                                           float bd0, float bd1, float bd2,
-                                           float* res) {
-    namespace OC = Opcode;
-    const float ad[3] = {ad0, ad1, ad2};
-    const float bd[3] = {bd0, bd1, bd2};
-    float ov;
-    float od[3];
-    switch (o) {
-        case OC::OP_ATAN2: {
-            ov = fmath::atan2_f32(av, bv);
-            const float a2 = av * av;
-            const float b2 = bv * bv;
-            const float den = a2 + b2;
-            for (int j = 0; j < 3; ++j) {
-                const float p = ad[j] * bv;
-                const float q = bd[j] * av;
-                od[j] = (p - q) / den;
-            }
-            break;
-        }
-        case OC::OP_POW: {
-            ov = fmath::pow_f32(av, bv);
-            const float f = bv * fmath::pow_f32(av, bv - 1.0f);
-            for (int j = 0; j < 3; ++j) od[j] = ad[j] * f;
-            break;
-        }
-        case OC::OP_NTH_ROOT: {
-            ov = fmath::nth_root_f32(av, bv);
-            const float f = fmath::pow_f32(av, 1.0f / bv - 1.0f) / bv;
-            for (int j = 0; j < 3; ++j) od[j] = (ad[j] == 0.0f) ? 0.0f : ad[j] * f;
-            break;
-        }
-        case OC::OP_MOD:
-            ov = fmath::mod_f32(av, bv);
-            for (int j = 0; j < 3; ++j) od[j] = ad[j];
-            break;
-        case OC::OP_NANFILL: {
-            const bool n_ = fmath::isnan_f(av);
-            ov = n_ ? bv : av;
-            for (int j = 0; j < 3; ++j) od[j] = n_ ? bd[j] : ad[j];
-            break;
-        }
-        case OC::OP_COMPARE:
-            ov = (av < bv) ? -1.0f : (av > bv) ? 1.0f : 0.0f;
-            for (int j = 0; j < 3; ++j) od[j] = 0.0f;
-            break;
-        case OC::OP_TAN: {
-            ov = fmath::tan_f32(av);
-            const float r = 1.0f / fmath::cos_f32(av);
-            const float f = r * r;
-            for (int j = 0; j < 3; ++j) od[j] = ad[j] * f;
-            break;
-        }
-        case OC::OP_ASIN: {
-            ov = fmath::asin_f32(av);
-            const float s = sqrtf(1.0f - av * av);
-            for (int j = 0; j < 3; ++j) od[j] = ad[j] / s;
-            break;
-        }
-        case OC::OP_ACOS: {
-            ov = fmath::acos_f32(av);
-            const float s = -sqrtf(1.0f - av * av);
-            for (int j = 0; j < 3; ++j) od[j] = ad[j] / s;
-            break;
-        }
-        case OC::OP_ATAN: {
-            ov = fmath::atan_f32(av);
-            const float s = av * av + 1.0f;
-            for (int j = 0; j < 3; ++j) od[j] = ad[j] / s;
-            break;
-        }
-        case OC::OP_LOG:
-            ov = fmath::log_f32(av);
-            for (int j = 0; j < 3; ++j) od[j] = ad[j] / av;
-            break;
-        case OC::OP_EXP:
-            ov = fmath::exp_f32(av);
-            for (int j = 0; j < 3; ++j) od[j] = ad[j] * ov;
-            break;
-        case OC::OP_RECIP: {
-            ov = 1.0f / av;
-            const float s = -(av * av);
-            for (int j = 0; j < 3; ++j) od[j] = ad[j] / s;
-            break;
-        }
-        default:
-            ov = av;
-            for (int j = 0; j < 3; ++j) od[j] = ad[j];
-            break;
-    }
-    res[0] = ov;
-    res[1] = od[0];
-    res[2] = od[1];
-    res[3] = od[2];
-}
 
 namespace mesh {
 

@@ -26,6 +26,26 @@ class libfive_region2(ctypes.Structure):
     _fields_ = [("X", libfive_interval), ("Y", libfive_interval)]
 
 
+class libfive_vec2(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_float), ("y", ctypes.c_float)]
+
+
+class libfive_contour(ctypes.Structure):
+    _fields_ = [("pts", ctypes.POINTER(libfive_vec2)), ("count", ctypes.c_uint32)]
+
+
+class libfive_contours(ctypes.Structure):
+    _fields_ = [("cs", ctypes.POINTER(libfive_contour)), ("count", ctypes.c_uint32)]
+
+
+class libfive_slice_segments(ctypes.Structure):
+    _fields_ = [("verts", ctypes.POINTER(ctypes.c_float)),
+                ("keys", ctypes.POINTER(ctypes.c_uint32)),
+                ("vert_count", ctypes.c_uint32),
+                ("segs", ctypes.POINTER(ctypes.c_uint32)),
+                ("seg_count", ctypes.c_uint32)]
+
+
 class libfive_pixels(ctypes.Structure):
     _fields_ = [("pixels", ctypes.POINTER(ctypes.c_bool)),
                 ("width", ctypes.c_uint32),
@@ -146,6 +166,17 @@ SIGNATURES = {
                                    [tree_t, libfive_region2, ctypes.c_float,
                                     ctypes.c_float]),
     "libfive_pixels_delete": (None, [ctypes.POINTER(libfive_pixels)]),
+    "libfive_tree_render_slice": (ctypes.POINTER(libfive_contours),
+                                  [tree_t, libfive_region2, ctypes.c_float,
+                                   ctypes.c_float]),
+    "libfive_tree_render_slice3": (ctypes.c_void_p, [tree_t, libfive_region2,
+                                                     ctypes.c_float, ctypes.c_float]),
+    "libfive_contours_delete": (None, [ctypes.POINTER(libfive_contours)]),
+    "libfive_contours3_delete": (None, [ctypes.c_void_p]),
+    "libfive_tree_render_slice_segments": (
+        ctypes.POINTER(libfive_slice_segments),
+        [tree_t, libfive_region2, ctypes.c_float, ctypes.c_double, ctypes.c_double]),
+    "libfive_slice_segments_delete": (None, [ctypes.POINTER(libfive_slice_segments)]),
     "libfive_tree_save_mesh": (ctypes.c_bool, [tree_t, libfive_region3,
                                                ctypes.c_float, ctypes.c_char_p]),
     "libfive_tree_save_meshes": (ctypes.c_bool, [ctypes.POINTER(tree_t),

@@ -267,6 +267,47 @@ class Shape:
         finally:
             L.libfive_pixels_delete(p)
 
+    def render_slice(self, xy_lo, xy_hi, z, res):
+        """2D contours of the slice at height z (libfive_tree_render_slice):
+        a list of float32 arrays of shape (n, 2)."""
+        L = ffi.lib()
+        R = ffi.libfive_region2(ffi.libfive_interval(xy_lo[0], xy_hi[0]),
+                                ffi.libfive_interval(xy_lo[1], xy_hi[1]))
+        cs = L.libfive_tree_render_slice(self.ptr, R, float(z), float(res))
+        if not cs:
+            raise GpuError(ffi.last_error())
+        try:
+            out = []
+            for i in range(cs.contents.count):
+                c = cs.contents.cs[i]
+                out.append(np.ctypeslib.as_array(
+                    ctypes.cast(c.pts, ffi.fp), (c.count, 2)).copy()
+                    if c.count else np.zeros((0, 2), np.float32))
+            return out
+        finally:
+            L.libfive_contours_delete(cs)
+
+    def slice_segments(self, xy_lo, xy_hi, z, min_feature, max_err=1e-8):
+        """(verts[n,2], keys[n,4], segs[m,2]) of the 2D contourer; vertex 0
+        is the reserved dummy vertex, keys are the owning cell's lower
+        corner in finest-cell units, its level and its patch."""
+        L = ffi.lib()
+        R = ffi.libfive_region2(ffi.libfive_interval(xy_lo[0], xy_hi[0]),
+                                ffi.libfive_interval(xy_lo[1], xy_hi[1]))
+        p = L.libfive_tree_render_slice_segments(self.ptr, R, float(z),
+                                                 float(min_feature), float(max_err))
+        if not p:
+            raise GpuError(ffi.last_error())
+        try:
+            s = p.contents
+            verts = np.ctypeslib.as_array(s.verts, (s.vert_count, 2)).copy()
+            keys = np.ctypeslib.as_array(s.keys, (s.vert_count, 4)).copy()
+            segs = (np.ctypeslib.as_array(s.segs, (s.seg_count, 2)).copy()
+                    if s.seg_count else np.zeros((0, 2), np.uint32))
+            return verts, keys, segs
+        finally:
+            L.libfive_slice_segments_delete(p)
+
 
 def _mesh_arrays(m):
     mm = m.contents

@@ -66,6 +66,12 @@ _SIGS = {
     "oracle_render_pixels": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_float] * 6
                              + [_u8, _u32, _u32]),
     "oracle_voxel_pts": (None, [ctypes.c_float] * 3 + [_fp, _u32, _fp, _fp]),
+    "oracle_render_contours": (ctypes.c_void_p, [ctypes.c_void_p, _dp, _dp,
+                                                 ctypes.c_float, ctypes.c_double,
+                                                 ctypes.c_double]),
+    "oracle_contours_sizes": (None, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_size_t)] * 5),
+    "oracle_contours_copy": (None, [ctypes.c_void_p, _fp] + [_u32] * 5),
+    "oracle_contours_delete": (None, [ctypes.c_void_p]),
 }
 
 _lib = None
@@ -228,6 +234,47 @@ class OracleDeck:
         L.oracle_render_pixels(self.ptr, *args, out.ctypes.data_as(_u8),
                                ctypes.byref(w), ctypes.byref(h))
         return out.astype(bool)
+
+    def render_contours(self, lo, hi, z, min_feature, max_err=1e-8):
+        """2D dual contouring of the slice at height z: a ContourResult."""
+        L = lib()
+        lo = np.ascontiguousarray(lo, np.float64)
+        hi = np.ascontiguousarray(hi, np.float64)
+        c = L.oracle_render_contours(self.ptr, lo.ctypes.data_as(_dp),
+                                     hi.ctypes.data_as(_dp), float(z),
+                                     float(min_feature), float(max_err))
+        try:
+            sz = [ctypes.c_size_t() for _ in range(5)]
+            L.oracle_contours_sizes(c, *[ctypes.byref(x) for x in sz])
+            nv, ns, nc, npts, ncell = (x.value for x in sz)
+            verts = np.zeros((nv, 2), np.float32)
+            keys = np.zeros((nv, 4), np.uint32)
+            segs = np.zeros((ns, 2), np.uint32)
+            counts = np.zeros(nc, np.uint32)
+            points = np.zeros(npts, np.uint32)
+            cells = np.zeros((ncell, 4), np.uint32)
+            L.oracle_contours_copy(c, _f(verts), keys.ctypes.data_as(_u32),
+                                   segs.ctypes.data_as(_u32),
+                                   counts.ctypes.data_as(_u32),
+                                   points.ctypes.data_as(_u32),
+                                   cells.ctypes.data_as(_u32))
+        finally:
+            L.oracle_contours_delete(c)
+        return ContourResult(verts, keys, segs, counts, points, cells)
+
+
+class ContourResult:
+    """verts[n,2] (row 0 is the reserved dummy vertex), keys[n,4] (owning
+    cell's lower corner x, y in finest-cell units, its level, its patch),
+    segs[m,2] vertex pairs, the chained contours as lists of vertex
+    indices (.contours) and point arrays (.polylines), and every evaluated
+    cell (x, y, level, state)."""
+
+    def __init__(self, verts, keys, segs, counts, points, cells):
+        self.verts, self.keys, self.segs, self.cells = verts, keys, segs, cells
+        offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        self.contours = [points[offs[i]:offs[i + 1]] for i in range(len(counts))]
+        self.polylines = [verts[c] for c in self.contours]
 
 
 def voxel_pts(lo, hi, res):

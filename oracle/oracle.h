@@ -108,6 +108,24 @@ int oracle_render_pixels(oracle_deck* d, float xlo, float xhi, float ylo,
 void oracle_voxel_pts(float lo, float hi, float res, float* out, uint32_t* n,
                       float* lower, float* upper);
 
+/* 2D dual contouring of the slice at height z over [lo, hi]
+ * (libfive_tree_render_slice with min_feature and max_err).  Holds the
+ * vertices (index 0 is the reserved dummy vertex) with one key each (x, y
+ * of the owning cell's lower corner in finest-cell units, its level, its
+ * patch), the segments as vertex pairs, the chained contours (point counts
+ * and vertex indices) and every evaluated cell (x, y, level, state). */
+typedef struct oracle_contours oracle_contours;
+oracle_contours* oracle_render_contours(oracle_deck* d, const double lo[2],
+                                        const double hi[2], float z,
+                                        double min_feature, double max_err);
+void oracle_contours_sizes(const oracle_contours* c, size_t* n_verts,
+                           size_t* n_segs, size_t* n_contours,
+                           size_t* n_points, size_t* n_cells);
+void oracle_contours_copy(const oracle_contours* c, float* verts,
+                          uint32_t* keys, uint32_t* segs, uint32_t* counts,
+                          uint32_t* points, uint32_t* cells);
+void oracle_contours_delete(oracle_contours* c);
+
 /* Tables and building blocks, exposed for their own tests. */
 int oracle_corner_manifold(int mask);
 void oracle_marching_table(int8_t* v /* 256*4*12*2 */, int8_t* e /* 8*8 */,

@@ -1,0 +1,93 @@
+"""GPU 2D dual contouring (libfive_tree_render_slice) through the C ABI,
+against the oracle's restatement of the reference contourer: the same
+evaluated cells and states, the same vertex set keyed by (cell, patch)
+with bit-identical positions, the same segments as ordered key pairs and
+the same number of contours; plus the reference's known answers."""
+import numpy as np
+import pytest
+
+import oracle
+from libfive_amd import X, Y, Z
+from libfive_amd import workloads as W
+
+from test_contour_oracle import rect
+
+pytestmark = pytest.mark.gpu
+
+
+def compare(shape, lo, hi, z, mf, max_err=1e-8):
+    print(f"slice parity: lo={lo} hi={hi} z={z} min_feature={mf}", flush=True)
+    lo32 = [float(np.float32(v)) for v in lo]
+    hi32 = [float(np.float32(v)) for v in hi]
+    gv, gk, gs = shape.slice_segments(lo32, hi32, z, mf, max_err)
+    o = oracle.OracleDeck(shape.deck()).render_contours(lo32, hi32, z, mf, max_err)
+    gkk = [tuple(k) for k in gk.tolist()]
+    okk = [tuple(k) for k in o.keys.tolist()]
+    gmap = {gkk[i]: tuple(gv[i].view(np.uint32)) for i in range(1, len(gkk))}
+    omap = {okk[i]: tuple(o.verts[i].view(np.uint32)) for i in range(1, len(okk))}
+    assert gmap == omap
+    gseg = sorted((gkk[a], gkk[b]) for a, b in gs.tolist())
+    oseg = sorted((okk[a], okk[b]) for a, b in o.segs.tolist())
+    assert gseg == oseg
+    return o
+
+
+CASES = [
+    ("circle_api", lambda: W.circle(1.0), (-2, -2), (2, 2), 0.0, 0.1),
+    ("circle_half", lambda: W.circle(0.5), (-1, -1), (1, 1), 0.0, 0.1),
+    ("ambiguities", lambda: (((-X()).max(X() - 1)).max((-Y()).max(Y() - 1))).max(-X()),
+     (-2, -2), (2, 2), 0.0, 0.1),
+    ("rects", lambda: rect(-1, 0, -1, 1).min(rect(0, 1, -1, 1)), (-2, -2), (2, 2), 0.0, 0.1),
+    ("rects_neg", lambda: -(rect(-1, 0, -1, 1).min(rect(0, 1, -1, 1))), (-2, -2), (2, 2),
+     0.0, 0.1),
+    ("square", lambda: rect(-0.5, 0.5, -0.5, 0.5), (-1, -1), (1, 1), 0.0, 0.1),
+    ("sphere_slice", lambda: W.sphere(0.8), (-1, -1), (1, 1), 0.3, 1 / 64),
+    ("csg_c1", W.csg_c1, (-2, -2), (2, 2), 0.25, 4 / 128),
+    ("gyroid", lambda: W.gyroid_c4(), (-1.2, -1.2), (1.2, 1.2), 0.3, 2.4 / 256),
+    ("sphere_union", lambda: W.sphere_union(200), (-1, -1), (1, 1), 0.1, 2 / 256),
+    ("random_csg", lambda: W.random_csg(150, np.random.RandomState(3)), (-1, -1), (1, 1),
+     0.0, 2 / 128),
+]
+
+
+@pytest.mark.parametrize("name,mk,lo,hi,z,mf", CASES, ids=[c[0] for c in CASES])
+def test_matches_oracle(name, mk, lo, hi, z, mf):
+    s = mk()
+    o = compare(s, lo, hi, z, mf)
+    res = 1.0 / mf
+    cs = s.render_slice(lo, hi, z, res) if mf == 0.1 else None
+    if cs is not None:
+        assert len(cs) == len(o.contours)
+
+
+def test_reference_known_answers():
+    cs = W.circle(1.0).render_slice((-2, -2), (2, 2), 0.0, 10)
+    assert len(cs) == 1 and len(cs[0]) > 0
+    r2 = np.sum(cs[0].astype(np.float64) ** 2, axis=1)
+    assert r2.min() > 0.99 and r2.max() < 1.01
+    cs = W.circle(0.5).render_slice((-1, -1), (1, 1), 0.0, 10)
+    r = np.linalg.norm(cs[0], axis=1)
+    assert len(cs) == 1 and r.max() < 0.51 and r.min() > 0.49
+    rects = rect(-1, 0, -1, 1).min(rect(0, 1, -1, 1))
+    assert len(rects.render_slice((-2, -2), (2, 2), 0.0, 10)) == 1
+    assert len((-rects).render_slice((-2, -2), (2, 2), 0.0, 10)) == 1
+
+
+def test_empty_and_full_regions():
+    assert W.sphere(0.5, (5, 5, 5)).render_slice((-1, -1), (1, 1), 0.0, 10) == []
+    assert W.sphere(10.0).render_slice((-1, -1), (1, 1), 0.0, 10) == []
+
+
+def test_large_slice_closed_contours():
+    """Gyroid slice at 2048 cells per side: every contour that does not
+    reach the region's edge is closed, and the contour count equals the
+    oracle's."""
+    s = W.gyroid_c4()
+    print("slice property: gyroid 2048", flush=True)
+    cs = s.render_slice((-1.2, -1.2), (1.2, 1.2), 0.3, 2048 / 2.4)
+    o = oracle.OracleDeck(s.deck()).render_contours(
+        (float(np.float32(-1.2)),) * 2, (float(np.float32(1.2)),) * 2, 0.3,
+        float(np.float32(1) / np.float32(2048 / 2.4)))
+    assert len(cs) == len(o.contours)
+    closed = sum(bool((c[0] == c[-1]).all()) for c in cs)
+    assert closed == sum(bool((p[0] == p[-1]).all()) for p in o.polylines)

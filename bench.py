@@ -269,6 +269,33 @@ def mesh_measurements(args):
             "same_image": bool(np.array_equal(oimg, img)),
         }
 
+    # 2D contours of the same slice (libfive_tree_render_slice), 2048 cells
+    # per side
+    n_ct = 2048
+    ct_res = n_ct / (hi - lo)
+    ct_ts = []
+    for i in range(1 + max(3, args.mesh_steps)):
+        t0 = time.perf_counter()
+        cs = gy.render_slice((lo, lo), (hi, hi), 0.05, ct_res)
+        if i:
+            ct_ts.append((time.perf_counter() - t0) * 1e3)
+    out["contours_gyroid_2048"] = {
+        "ms": round(float(np.median(ct_ts)), 2), "contours": len(cs),
+        "points": int(sum(len(c) for c in cs)),
+        "workload": f"same gyroid, slice z=0.05, {n_ct}^2 cells",
+    }
+    if not args.no_cpu:
+        odc = oracle.OracleDeck(gy.deck())
+        lo32, hi32 = float(np.float32(lo)), float(np.float32(hi))
+        mfc = float(np.float32(1) / np.float32(ct_res))
+        t0 = time.perf_counter()
+        oc = odc.render_contours((lo32, lo32), (hi32, hi32), 0.05, mfc)
+        out["contours_gyroid_2048"]["cpu_baseline"] = {
+            "value": round((time.perf_counter() - t0) * 1e3, 1), "unit": "ms",
+            "cores": 1, "kind": "port", "sample": "the whole slice, one run",
+            "same_contour_count": len(oc.contours) == len(cs),
+        }
+
     if not args.no_cpu:
         # The oracle mesher (CPU restatement of the reference's DC mesher)
         # on the same gyroid at a lower resolution, with the GPU time there.

@@ -91,3 +91,30 @@ def test_large_slice_closed_contours():
     assert len(cs) == len(o.contours)
     closed = sum(bool((c[0] == c[-1]).all()) for c in cs)
     assert closed == sum(bool((p[0] == p[-1]).all()) for p in o.polylines)
+
+
+def test_slice3_carries_z():
+    s = W.sphere(0.8)
+    c2 = s.render_slice((-1, -1), (1, 1), 0.3, 32)
+    from libfive_amd import ffi
+    import ctypes
+    L = ffi.lib()
+    R = ffi.libfive_region2(ffi.libfive_interval(-1, 1), ffi.libfive_interval(-1, 1))
+    p = L.libfive_tree_render_slice3(s.ptr, R, 0.3, 32.0)
+    assert p
+    try:
+        # libfive_contours3 {libfive_contour3* cs; uint32_t count}
+        class C3(ctypes.Structure):
+            _fields_ = [("pts", ctypes.POINTER(ctypes.c_float)), ("count", ctypes.c_uint32)]
+
+        class CS3(ctypes.Structure):
+            _fields_ = [("cs", ctypes.POINTER(C3)), ("count", ctypes.c_uint32)]
+
+        cs = ctypes.cast(p, ctypes.POINTER(CS3)).contents
+        assert cs.count == len(c2)
+        for i in range(cs.count):
+            pts = np.ctypeslib.as_array(cs.cs[i].pts, (cs.cs[i].count, 3))
+            assert (pts[:, :2] == c2[i]).all()
+            assert (pts[:, 2] == np.float32(0.3)).all()
+    finally:
+        L.libfive_contours3_delete(p)

@@ -78,6 +78,9 @@ PixelImage render_pixels(const Tree& t, float xlo, float xhi, float ylo,
     Context& c = context();
     const float lo[3] = {xlo, ylo, z}, hi[3] = {xhi, yhi, z};
     const Grid g = make_grid(lo, hi, res);
+    if (double(g.pts[0].size()) * double(g.pts[1].size()) >= 4294967296.0) {
+        throw Error("render_pixels: the image has 2^32 pixels or more");
+    }
     PixelImage img;
     img.width = uint32_t(g.pts[0].size());
     img.height = uint32_t(g.pts[1].size());
@@ -134,8 +137,8 @@ PixelImage render_pixels(const Tree& t, float xlo, float xhi, float ylo,
                 boxes[(2 * a + 1) * n + k] = big[k].upper[a];
             }
         }
-        DevBuf<float> d(8 * n);
-        DevBuf<uint8_t> dn(n);
+        PoolBuf<float> d(8 * n);
+        PoolBuf<uint8_t> dn(n);
         hip_check(hipMemcpyAsync(d.p, boxes.data(), 6 * n * sizeof(float),
                                  hipMemcpyHostToDevice, c.stream), "copy boxes");
         const auto& h = gt->host;
@@ -183,7 +186,7 @@ PixelImage render_pixels(const Tree& t, float xlo, float xhi, float ylo,
     }
     const size_t np = px.size();
     if (np) {
-        DevBuf<float> d(4 * np);
+        PoolBuf<float> d(4 * np);
         hip_check(hipMemcpyAsync(d.p, px.data(), np * 4, hipMemcpyHostToDevice, c.stream), "copy x");
         hip_check(hipMemcpyAsync(d.p + np, py.data(), np * 4, hipMemcpyHostToDevice, c.stream), "copy y");
         hip_check(hipMemcpyAsync(d.p + 2 * np, pz.data(), np * 4, hipMemcpyHostToDevice, c.stream), "copy z");

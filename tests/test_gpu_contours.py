@@ -78,6 +78,13 @@ def test_empty_and_full_regions():
     assert W.sphere(10.0).render_slice((-1, -1), (1, 1), 0.0, 10) == []
 
 
+def test_large_slice_matches_oracle():
+    """Gyroid slice at 2048 cells per side: vertices and segments equal to
+    the oracle's."""
+    mf = float(np.float32(1) / np.float32(2048 / 2.4))
+    compare(W.gyroid_c4(), (-1.2, -1.2), (1.2, 1.2), 0.3, mf)
+
+
 def test_large_slice_closed_contours():
     """Gyroid slice at 2048 cells per side: every contour that does not
     reach the region's edge is closed, and the contour count equals the

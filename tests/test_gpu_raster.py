@@ -64,6 +64,16 @@ def test_empty_full_and_single_pixel():
     assert one.shape == (1, 1) and one[0, 0]
 
 
+def test_large_raster_matches_oracle():
+    s = W.gyroid_c4()
+    res = 4096 / 2.4
+    print("raster parity: gyroid 4096x4096", flush=True)
+    g = s.render_pixels((-1.2, -1.2), (1.2, 1.2), 0.05, res)
+    o = od(s).render_pixels((-1.2, -1.2), (1.2, 1.2), 0.05, res)
+    assert g.shape == o.shape == (4096, 4096)
+    assert (g == o).all()
+
+
 @pytest.mark.parametrize("n", [4096])
 def test_large_raster_matches_point_signs(n):
     s = W.gyroid_c4()

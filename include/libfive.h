@@ -221,8 +221,10 @@ libfive_contours3* libfive_tree_render_slice3(libfive_tree tree, libfive_region2
 void libfive_contours_delete(libfive_contours* cs);
 void libfive_contours3_delete(libfive_contours3* cs);
 /* The same with explicit settings, and the vertices, their keys (x, y of
- * the owning cell's lower corner in finest-cell units, level, patch) and
- * the segments as vertex-index pairs, for tests and tools: returns NULL
+ * the owning cell's lower corner in finest-cell units, level, patch), the
+ * segments as vertex-index pairs and every cell whose state was evaluated
+ * (interval or leaf; state 0 empty, 1 filled, 2 ambiguous), for tests and
+ * tools: returns NULL
  * with libfive_last_error() set on failure. */
 typedef struct libfive_slice_segments {
     float* verts;          /* 2 per vertex; vertex 0 is a reserved dummy */
@@ -230,6 +232,8 @@ typedef struct libfive_slice_segments {
     uint32_t vert_count;
     uint32_t* segs;        /* 2 per segment */
     uint32_t seg_count;
+    uint32_t* cells;       /* 4 per evaluated cell: x, y, level, state */
+    uint32_t cell_count;
 } libfive_slice_segments;
 libfive_slice_segments* libfive_tree_render_slice_segments(
     libfive_tree tree, libfive_region2 R, float z, double min_feature,

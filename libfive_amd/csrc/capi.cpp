@@ -878,6 +878,9 @@ libfive_slice_segments* libfive_tree_render_slice_segments(
             out->segs[2 * i] = r.segs[i][0];
             out->segs[2 * i + 1] = r.segs[i][1];
         }
+        out->cell_count = uint32_t(r.cells.size() / 4);
+        out->cells = new uint32_t[r.cells.size() + 1];
+        std::copy(r.cells.begin(), r.cells.end(), out->cells);
         return out;
     } catch (const std::exception& e) {
         set_error("libfive_tree_render_slice_segments", e);
@@ -890,6 +893,7 @@ void libfive_slice_segments_delete(libfive_slice_segments* s) {
     delete[] s->verts;
     delete[] s->keys;
     delete[] s->segs;
+    delete[] s->cells;
     delete s;
 }
 

@@ -43,7 +43,9 @@ class libfive_slice_segments(ctypes.Structure):
                 ("keys", ctypes.POINTER(ctypes.c_uint32)),
                 ("vert_count", ctypes.c_uint32),
                 ("segs", ctypes.POINTER(ctypes.c_uint32)),
-                ("seg_count", ctypes.c_uint32)]
+                ("seg_count", ctypes.c_uint32),
+                ("cells", ctypes.POINTER(ctypes.c_uint32)),
+                ("cell_count", ctypes.c_uint32)]
 
 
 class libfive_pixels(ctypes.Structure):

@@ -287,10 +287,12 @@ class Shape:
         finally:
             L.libfive_contours_delete(cs)
 
-    def slice_segments(self, xy_lo, xy_hi, z, min_feature, max_err=1e-8):
+    def slice_segments(self, xy_lo, xy_hi, z, min_feature, max_err=1e-8,
+                       cells=False):
         """(verts[n,2], keys[n,4], segs[m,2]) of the 2D contourer; vertex 0
         is the reserved dummy vertex, keys are the owning cell's lower
-        corner in finest-cell units, its level and its patch."""
+        corner in finest-cell units, its level and its patch.  With
+        cells=True also every evaluated cell as (x, y, level, state)."""
         L = ffi.lib()
         R = ffi.libfive_region2(ffi.libfive_interval(xy_lo[0], xy_hi[0]),
                                 ffi.libfive_interval(xy_lo[1], xy_hi[1]))
@@ -304,7 +306,11 @@ class Shape:
             keys = np.ctypeslib.as_array(s.keys, (s.vert_count, 4)).copy()
             segs = (np.ctypeslib.as_array(s.segs, (s.seg_count, 2)).copy()
                     if s.seg_count else np.zeros((0, 2), np.uint32))
-            return verts, keys, segs
+            if not cells:
+                return verts, keys, segs
+            cl = (np.ctypeslib.as_array(s.cells, (s.cell_count, 4)).copy()
+                  if s.cell_count else np.zeros((0, 4), np.uint32))
+            return verts, keys, segs, cl
         finally:
             L.libfive_slice_segments_delete(p)
 

@@ -19,8 +19,10 @@ def compare(shape, lo, hi, z, mf, max_err=1e-8):
     print(f"slice parity: lo={lo} hi={hi} z={z} min_feature={mf}", flush=True)
     lo32 = [float(np.float32(v)) for v in lo]
     hi32 = [float(np.float32(v)) for v in hi]
-    gv, gk, gs = shape.slice_segments(lo32, hi32, z, mf, max_err)
+    gv, gk, gs, gc = shape.slice_segments(lo32, hi32, z, mf, max_err, cells=True)
     o = oracle.OracleDeck(shape.deck()).render_contours(lo32, hi32, z, mf, max_err)
+    # the same quadtree: every evaluated cell with the same state
+    assert sorted(map(tuple, gc.tolist())) == sorted(map(tuple, o.cells.tolist()))
     gkk = [tuple(k) for k in gk.tolist()]
     okk = [tuple(k) for k in o.keys.tolist()]
     gmap = {gkk[i]: tuple(gv[i].view(np.uint32)) for i in range(1, len(gkk))}

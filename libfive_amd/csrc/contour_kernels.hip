@@ -55,6 +55,51 @@ void k_point_grads(const tape::Clause* __restrict__ t, uint32_t n, uint32_t root
     }
 }
 
+// Edge search (dc_tree.inl:339-410): 16 lanes per edge, four edges per
+// wave, the four rounds in registers.  A round whose first candidate
+// sample is an exact zero needs FeatureEvaluator::isInside; the kernel
+// stops that edge there (stop[e] = round, bracket as at the round's start)
+// and the host finishes it.  stop[e] = 4 when all rounds ran here.
+template <int NV>
+__global__ __launch_bounds__(64)
+void k_edge_search(const tape::Clause* __restrict__ t, uint32_t n, uint32_t root,
+                   float z, double* __restrict__ ends, uint8_t* __restrict__ stop,
+                   size_t nedges)
+{
+    const int lane = threadIdx.x;
+    const int j = lane & 15;
+    const size_t e = size_t(blockIdx.x) * 4 + (lane >> 4);
+    const size_t ec = e < nedges ? e : nedges - 1;
+    double a0 = ends[4 * ec], a1 = ends[4 * ec + 1];
+    double b0 = ends[4 * ec + 2], b1 = ends[4 * ec + 3];
+    const int shift = lane & 48;
+    const double f = j / 15.0;
+    int round = 0;
+    for (; round < 4; ++round) {
+        const double p0 = a0 * (1 - f) + b0 * f;
+        const double p1 = a1 * (1 - f) + b1 * f;
+        uint32_t fl;
+        const float v = pe_value<NV>(t, n, root, float(p0), float(p1), z, &fl);
+        const bool cand = j >= 1 && (v > 0.0f || v == 0.0f || j == 15);
+        const uint32_t cm = uint32_t(__ballot(cand) >> shift) & 0xffffu;
+        const uint32_t zm = uint32_t(__ballot(v == 0.0f) >> shift) & 0xffffu;
+        const int js = __builtin_ctz(cm);          // cm always has bit 15
+        if ((zm >> js) & 1u) break;                // exact zero: host finishes
+        const double fs = js / 15.0, fp = (js - 1) / 15.0;
+        const double n0 = a0 * (1 - fp) + b0 * fp, n1 = a1 * (1 - fp) + b1 * fp;
+        const double m0 = a0 * (1 - fs) + b0 * fs, m1 = a1 * (1 - fs) + b1 * fs;
+        a0 = n0; a1 = n1;
+        b0 = m0; b1 = m1;
+    }
+    if (e < nedges && j == 0) {
+        ends[4 * e] = a0;
+        ends[4 * e + 1] = a1;
+        ends[4 * e + 2] = b0;
+        ends[4 * e + 3] = b1;
+        stop[e] = uint8_t(round);
+    }
+}
+
 // One wave per point; lane 0 runs the (serial) feature walk.
 __global__ __launch_bounds__(64)
 void k_point_features(const tape::Clause* __restrict__ t, uint32_t n, uint32_t root,
@@ -68,12 +113,16 @@ void k_point_features(const tape::Clause* __restrict__ t, uint32_t n, uint32_t r
     feat::Workspace W = feat::carve(ws + size_t(blockIdx.x) * ws_bytes, n, nregs);
     for (size_t i = blockIdx.x; i < npts; i += gridDim.x) {
         const float p[3] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
-        inside[i] = feat::is_inside(t, n, nregs, root, sx, sy, sz, p, W, err) ? 1 : 0;
-        float out[feat::kMaxOut][3];
-        const int m = feat::features(t, n, nregs, root, sx, sy, sz, p, W, out, err);
-        nfeat[i] = m;
-        for (int j = 0; j < m; ++j) {
-            for (int q = 0; q < 3; ++q) feats[(i * feat::kMaxOut + j) * 3 + q] = out[j][q];
+        if (inside) {
+            inside[i] = feat::is_inside(t, n, nregs, root, sx, sy, sz, p, W, err) ? 1 : 0;
+        }
+        if (feats) {
+            float out[feat::kMaxOut][3];
+            const int m = feat::features(t, n, nregs, root, sx, sy, sz, p, W, out, err);
+            nfeat[i] = m;
+            for (int j = 0; j < m; ++j) {
+                for (int q = 0; q < 3; ++q) feats[(i * feat::kMaxOut + j) * 3 + q] = out[j][q];
+            }
         }
     }
 }
@@ -103,6 +152,20 @@ hipError_t launch_point_grads(const tape::Clause* t, uint32_t n, uint32_t root,
         case 1: hipLaunchKernelGGL(k_point_grads<1>, grid, dim3(64), 0, s, t, n, root, xyz, v, g, amb, npts); break;
         case 2: hipLaunchKernelGGL(k_point_grads<2>, grid, dim3(64), 0, s, t, n, root, xyz, v, g, amb, npts); break;
         case 4: hipLaunchKernelGGL(k_point_grads<4>, grid, dim3(64), 0, s, t, n, root, xyz, v, g, amb, npts); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_edge_search(const tape::Clause* t, uint32_t n, uint32_t root,
+                              uint32_t nregs, float z, double* ends, uint8_t* stop,
+                              size_t nedges, hipStream_t s) {
+    if (nedges == 0) return hipSuccess;
+    const dim3 grid(uint32_t((nedges + 3) / 4));
+    switch (banks(nregs)) {
+        case 1: hipLaunchKernelGGL(k_edge_search<1>, grid, dim3(64), 0, s, t, n, root, z, ends, stop, nedges); break;
+        case 2: hipLaunchKernelGGL(k_edge_search<2>, grid, dim3(64), 0, s, t, n, root, z, ends, stop, nedges); break;
+        case 4: hipLaunchKernelGGL(k_edge_search<4>, grid, dim3(64), 0, s, t, n, root, z, ends, stop, nedges); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -23,8 +23,17 @@ hipError_t launch_point_grads(const tape::Clause* t, uint32_t n, uint32_t root,
                               uint32_t nregs, const float* xyz, float* v, float* g,
                               uint8_t* amb, size_t npts, hipStream_t s);
 
+// Edge search over nedges edges, ends = (inside x, y, outside x, y) per
+// edge in double, updated in place; stop[e] = 4 when the four rounds ran
+// on the device, else the round at which an exact-zero sample needs the
+// feature evaluator (the bracket is left as at that round's start).
+hipError_t launch_edge_search(const tape::Clause* t, uint32_t n, uint32_t root,
+                              uint32_t nregs, float z, double* ends, uint8_t* stop,
+                              size_t nedges, hipStream_t s);
+
 // FeatureEvaluator::isInside (inside[i]) and ::features (feats: up to 16
-// gradients of 3 floats per point, nfeat[i] of them).  ws holds
+// gradients of 3 floats per point, nfeat[i] of them); a null inside or
+// feats skips that evaluation.  ws holds
 // point_feature_workspace(n, nregs) bytes per wave in flight (at most
 // kFeatureWaves); err gets a nonzero bit when a point has more features
 // than the evaluation holds.

@@ -8,6 +8,7 @@
 #include <list>
 #include <map>
 #include <mutex>
+#include <thread>
 
 #include "contour_kernels.hpp"
 #include "deck.hpp"
@@ -160,9 +161,10 @@ struct Device {
         hip_check(hipStreamSynchronize(c.stream), "point partials");
     }
 
-    // isInside and the feature gradients of each point.
-    void features(const std::vector<float>& xyz, std::vector<uint8_t>& inside,
-                  std::vector<float>& feats, std::vector<int32_t>& nfeat) {
+    // isInside (want_inside) or the feature gradients of each point.
+    void features(const std::vector<float>& xyz, bool want_inside,
+                  std::vector<uint8_t>& inside, std::vector<float>& feats,
+                  std::vector<int32_t>& nfeat) {
         const size_t n = xyz.size() / 3;
         inside.resize(n);
         feats.resize(n * 16 * 3);
@@ -176,12 +178,16 @@ struct Device {
         hip_check(hipMemsetAsync(err.p, 0, 4, c.stream), "clear error");
         hip_check(hipMemcpyAsync(d.p, xyz.data(), 12 * n, hipMemcpyHostToDevice, c.stream), "copy points");
         hip_check(launch_point_features(gt.d.p, gt.size(), gt.host.root_reg, nregs, sx, sy, sz,
-                                        d.p, di.p, d.p + 3 * n, dn.p, ws.p, wsb, err.p, n,
-                                        c.stream), "point features");
+                                        d.p, want_inside ? di.p : nullptr,
+                                        want_inside ? nullptr : d.p + 3 * n, dn.p, ws.p, wsb,
+                                        err.p, n, c.stream), "point features");
         uint32_t e = 0;
-        hip_check(hipMemcpyAsync(inside.data(), di.p, n, hipMemcpyDeviceToHost, c.stream), "copy inside");
-        hip_check(hipMemcpyAsync(feats.data(), d.p + 3 * n, 4 * n * 48, hipMemcpyDeviceToHost, c.stream), "copy features");
-        hip_check(hipMemcpyAsync(nfeat.data(), dn.p, 4 * n, hipMemcpyDeviceToHost, c.stream), "copy counts");
+        if (want_inside) {
+            hip_check(hipMemcpyAsync(inside.data(), di.p, n, hipMemcpyDeviceToHost, c.stream), "copy inside");
+        } else {
+            hip_check(hipMemcpyAsync(feats.data(), d.p + 3 * n, 4 * n * 48, hipMemcpyDeviceToHost, c.stream), "copy features");
+            hip_check(hipMemcpyAsync(nfeat.data(), dn.p, 4 * n, hipMemcpyDeviceToHost, c.stream), "copy counts");
+        }
         hip_check(hipMemcpyAsync(&e, err.p, 4, hipMemcpyDeviceToHost, c.stream), "copy error");
         hip_check(hipStreamSynchronize(c.stream), "point features");
         if (e) {
@@ -345,7 +351,7 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
             std::vector<uint8_t> ins;
             std::vector<float> fs;
             std::vector<int32_t> nf;
-            dev.features(gather(ties), ins, fs, nf);
+            dev.features(gather(ties), true, ins, fs, nf);
             for (size_t j = 0; j < ties.size(); ++j) state[ties[j]] = ins[j] ? FILLED : EMPTY;
         }
 
@@ -380,17 +386,54 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
             }
         }
 
-        // Edge search: 4 rounds of 16 samples per edge (dc_tree.inl:339-410).
+        // Edge search: 4 rounds of 16 samples per edge (dc_tree.inl:339-410),
+        // on the device; an edge whose round meets an exact-zero sample
+        // first is finished here from that round, with isInside on the
+        // device.
         const size_t ne = edges.size();
-        std::vector<double> sp(2 * 16 * ne);
-        for (int round = 0; round < 4 && ne; ++round) {
-            std::vector<float> sx(3 * 16 * ne);
+        std::vector<uint8_t> stop(ne, 4);
+        if (ne) {
+            std::vector<double> ends(4 * ne);
             for (size_t e = 0; e < ne; ++e) {
+                ends[4 * e] = edges[e].a[0];
+                ends[4 * e + 1] = edges[e].a[1];
+                ends[4 * e + 2] = edges[e].b[0];
+                ends[4 * e + 3] = edges[e].b[1];
+            }
+            PoolBuf<double> de(4 * ne);
+            PoolBuf<uint8_t> dst(ne);
+            hip_check(hipMemcpyAsync(de.p, ends.data(), 32 * ne, hipMemcpyHostToDevice, c.stream),
+                      "copy edges");
+            hip_check(launch_edge_search(gt.d.p, gt.size(), base.root_reg, dev.nregs, z, de.p,
+                                         dst.p, ne, c.stream), "edge search");
+            hip_check(hipMemcpyAsync(ends.data(), de.p, 32 * ne, hipMemcpyDeviceToHost, c.stream),
+                      "copy brackets");
+            hip_check(hipMemcpyAsync(stop.data(), dst.p, ne, hipMemcpyDeviceToHost, c.stream),
+                      "copy stops");
+            hip_check(hipStreamSynchronize(c.stream), "edge search");
+            for (size_t e = 0; e < ne; ++e) {
+                edges[e].a[0] = ends[4 * e];
+                edges[e].a[1] = ends[4 * e + 1];
+                edges[e].b[0] = ends[4 * e + 2];
+                edges[e].b[1] = ends[4 * e + 3];
+            }
+        }
+        for (int round = 0; round < 4; ++round) {
+            std::vector<size_t> act;
+            for (size_t e = 0; e < ne; ++e) {
+                if (stop[e] <= round) act.push_back(e);
+            }
+            if (act.empty()) continue;
+            const size_t na = act.size();
+            std::vector<double> sp(2 * 16 * na);
+            std::vector<float> sx(3 * 16 * na);
+            for (size_t k = 0; k < na; ++k) {
+                const Edge& ed = edges[act[k]];
                 for (int j = 0; j < 16; ++j) {
                     const double f = j / 15.0;
-                    const size_t i = 16 * e + j;
+                    const size_t i = 16 * k + j;
                     for (int a = 0; a < 2; ++a) {
-                        sp[2 * i + a] = edges[e].a[a] * (1 - f) + edges[e].b[a] * f;
+                        sp[2 * i + a] = ed.a[a] * (1 - f) + ed.b[a] * f;
                         sx[3 * i + a] = float(sp[2 * i + a]);
                     }
                     sx[3 * i + 2] = z;
@@ -401,7 +444,7 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
             dev.values(sx, sv, sa);
             // isInside for the samples that are exactly zero
             std::vector<size_t> zs;
-            for (size_t i = 0; i < 16 * ne; ++i) {
+            for (size_t i = 0; i < 16 * na; ++i) {
                 if ((i & 15) && sv[i] == 0) zs.push_back(i);
             }
             std::vector<uint8_t> zin;
@@ -410,14 +453,15 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
                 for (size_t j = 0; j < zs.size(); ++j) std::memcpy(&zp[3 * j], &sx[3 * zs[j]], 12);
                 std::vector<float> fs;
                 std::vector<int32_t> nf;
-                dev.features(zp, zin, fs, nf);
+                dev.features(zp, true, zin, fs, nf);
             }
             // First sample after the inside end that is outside (v > 0, or
             // v == 0 and not inside); else the last one, NaN included.
             size_t zk = 0;
-            for (size_t e = 0; e < ne; ++e) {
+            for (size_t k = 0; k < na; ++k) {
+                Edge& ed = edges[act[k]];
                 for (int j = 1; j < 16; ++j) {
-                    const size_t i = 16 * e + j;
+                    const size_t i = 16 * k + j;
                     bool take;
                     if (sv[i] > 0) {
                         take = true;
@@ -429,8 +473,8 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
                     }
                     if (take) {
                         for (int a = 0; a < 2; ++a) {
-                            edges[e].a[a] = sp[2 * (i - 1) + a];
-                            edges[e].b[a] = sp[2 * i + a];
+                            ed.a[a] = sp[2 * (i - 1) + a];
+                            ed.b[a] = sp[2 * i + a];
                         }
                         break;
                     }
@@ -461,76 +505,94 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
             for (size_t j = 0; j < amb_ends.size(); ++j) {
                 std::memcpy(&fp[3 * j], &bx[3 * amb_ends[j]], 12);
             }
-            dev.features(fp, fin, fs, nf);
+            dev.features(fp, false, fin, fs, nf);
         }
 
         // QEF per patch (dc_tree.inl:412-552): edges of one leaf and patch
-        // are consecutive in `edges`.
-        size_t fk = 0;
-        for (size_t e0 = 0; e0 < ne;) {
-            size_t e1 = e0;
-            while (e1 < ne && edges[e1].leaf == edges[e0].leaf &&
-                   edges[e1].patch == edges[e0].patch) {
-                ++e1;
-            }
-            Leaf& lf = q.leaves[edges[e0].leaf];
-            const int ec = int(e1 - e0);
-            double acc[4][14] = {};
-            bool used[4] = {false, false, false, false};
-            for (int i = 0; i < 2 * ec; ++i) {
-                const size_t pi = 2 * e0 + i;          // bracket end index
-                const Edge& ed = edges[e0 + i / 2];
-                const double* p2 = (i & 1) ? ed.b : ed.a;
-                const double p[3] = {p2[0], p2[1], 0.0};
-                const double val = bv[pi];
-                if (!ba[pi]) {
-                    const double d[3] = {bg[3 * pi], bg[3 * pi + 1], 0.0};
-                    qef::push_sample(acc[i / 2], p, d, val);
-                    used[i / 2] = true;
-                } else {
-                    while (amb_ends[fk] != pi) ++fk;
-                    for (int j = 0; j < nf[fk]; ++j) {
-                        const double d[3] = {fs[(fk * 16 + j) * 3], fs[(fk * 16 + j) * 3 + 1], 0.0};
+        // are consecutive in `edges`; leaves are independent, so ranges of
+        // whole leaves go to host threads.
+        auto solve = [&](size_t ebegin, size_t eend) {
+            size_t fk = size_t(std::lower_bound(amb_ends.begin(), amb_ends.end(), 2 * ebegin) -
+                               amb_ends.begin());
+            for (size_t e0 = ebegin; e0 < eend;) {
+                size_t e1 = e0;
+                while (e1 < ne && edges[e1].leaf == edges[e0].leaf &&
+                       edges[e1].patch == edges[e0].patch) {
+                    ++e1;
+                }
+                Leaf& lf = q.leaves[edges[e0].leaf];
+                const int ec = int(e1 - e0);
+                double acc[4][14] = {};
+                bool used[4] = {false, false, false, false};
+                for (int i = 0; i < 2 * ec; ++i) {
+                    const size_t pi = 2 * e0 + i;          // bracket end index
+                    const Edge& ed = edges[e0 + i / 2];
+                    const double* p2 = (i & 1) ? ed.b : ed.a;
+                    const double p[3] = {p2[0], p2[1], 0.0};
+                    const double val = bv[pi];
+                    if (!ba[pi]) {
+                        const double d[3] = {bg[3 * pi], bg[3 * pi + 1], 0.0};
                         qef::push_sample(acc[i / 2], p, d, val);
                         used[i / 2] = true;
+                    } else {
+                        while (amb_ends[fk] != pi) ++fk;
+                        for (int j = 0; j < nf[fk]; ++j) {
+                            const double d[3] = {fs[(fk * 16 + j) * 3], fs[(fk * 16 + j) * 3 + 1], 0.0};
+                            qef::push_sample(acc[i / 2], p, d, val);
+                            used[i / 2] = true;
+                        }
                     }
                 }
-            }
-            int max_rank = 0, ranks[4];
-            for (int i = 0; i < ec; ++i) {
-                ranks[i] = -1;
-                if (!used[i]) continue;
-                double ata[3][3];
-                qef::unpack_ata(acc[i], ata);
-                ranks[i] = qef::small_eigenvalue_count(ata);
-                max_rank = std::max(max_rank, ranks[i]);
-            }
-            double mass[4] = {0, 0, 0, 0};
-            for (int i = 0; i < ec; ++i) {
-                if (ranks[i] == max_rank) {
-                    mass[0] = mass[0] + acc[i][0] / acc[i][3];
-                    mass[1] = mass[1] + acc[i][1] / acc[i][3];
-                    mass[2] = mass[2] + acc[i][2] / acc[i][3];
-                    mass[3] = mass[3] + 1.0;
+                int max_rank = 0, ranks[4];
+                for (int i = 0; i < ec; ++i) {
+                    ranks[i] = -1;
+                    if (!used[i]) continue;
+                    double ata[3][3];
+                    qef::unpack_ata(acc[i], ata);
+                    ranks[i] = qef::small_eigenvalue_count(ata);
+                    max_rank = std::max(max_rank, ranks[i]);
                 }
+                double mass[4] = {0, 0, 0, 0};
+                for (int i = 0; i < ec; ++i) {
+                    if (ranks[i] == max_rank) {
+                        mass[0] = mass[0] + acc[i][0] / acc[i][3];
+                        mass[1] = mass[1] + acc[i][1] / acc[i][3];
+                        mass[2] = mass[2] + acc[i][2] / acc[i][3];
+                        mass[3] = mass[3] + 1.0;
+                    }
+                }
+                double sum[14] = {};
+                for (int i = 0; i < ec; ++i) {
+                    if (!used[i]) continue;
+                    for (int k = 4; k < 14; ++k) sum[k] = sum[k] + acc[i][k];
+                }
+                std::memcpy(lf.mass, mass, sizeof(mass));
+                qef::unpack_ata(sum, lf.ata);
+                lf.atb[0] = sum[10];
+                lf.atb[1] = sum[11];
+                lf.atb[2] = sum[12];
+                lf.btb = sum[13];
+                int rank = 0;
+                qef::find_vertex(lf.ata, lf.atb, lf.btb, mass, lf.vert[lf.count], &rank);
+                lf.rank = rank;
+                lf.count++;
+                e0 = e1;
             }
-            double sum[14] = {};
-            for (int i = 0; i < ec; ++i) {
-                if (!used[i]) continue;
-                for (int k = 4; k < 14; ++k) sum[k] = sum[k] + acc[i][k];
-            }
-            std::memcpy(lf.mass, mass, sizeof(mass));
-            qef::unpack_ata(sum, lf.ata);
-            lf.atb[0] = sum[10];
-            lf.atb[1] = sum[11];
-            lf.atb[2] = sum[12];
-            lf.btb = sum[13];
-            int rank = 0;
-            qef::find_vertex(lf.ata, lf.atb, lf.btb, mass, lf.vert[lf.count], &rank);
-            lf.rank = rank;
-            lf.count++;
-            e0 = e1;
+        };
+        const size_t parts = std::min<size_t>(16, std::max<size_t>(1, ne / 4096));
+        std::vector<size_t> cut{0};
+        for (size_t k = 1; k < parts; ++k) {
+            size_t e = std::max(cut.back(), ne * k / parts);
+            while (e < ne && e > 0 && edges[e].leaf == edges[e - 1].leaf) ++e;
+            cut.push_back(e);
         }
+        cut.push_back(ne);
+        std::vector<std::thread> pool;
+        for (size_t k = 1; k + 1 < cut.size(); ++k) {
+            pool.emplace_back(solve, cut[k], cut[k + 1]);
+        }
+        solve(cut[0], cut[1]);
+        for (auto& t : pool) t.join();
     }
 
     // Collapse, bottom up (DCTree::collectChildren, dc_tree.inl:581-735);

@@ -1,5 +1,6 @@
-// Point passes of the 2D contourer: batched values, partials, ambiguity
-// and feature evaluation over arbitrary point lists, on the interpreters
+// Point passes of the slice renderers (2D contours and the raster):
+// batched values, partials, ambiguity, edge search and feature evaluation
+// over point lists, and the raster's pixel pass, on the interpreters
 // shared with the mesher's leaf kernel.
 #include <hip/hip_runtime.h>
 
@@ -100,6 +101,27 @@ void k_edge_search(const tape::Clause* __restrict__ t, uint32_t n, uint32_t root
     }
 }
 
+// Raster pixel pass (Heightmap::pixels, heightmap.cpp:108-155, one Z
+// voxel): one workgroup per view of at most 256 pixels; pixel (i, j) of
+// view v is out[offs[v] + i * sy + j] = (value at the voxel centre < 0).
+template <int NV>
+__global__ __launch_bounds__(256)
+void k_raster_views(const tape::Clause* __restrict__ t, uint32_t n, uint32_t root,
+                    const float* __restrict__ px, const float* __restrict__ py, float z,
+                    const int32_t* __restrict__ views, const uint32_t* __restrict__ offs,
+                    uint8_t* __restrict__ out)
+{
+    const uint32_t v = blockIdx.x;
+    const int32_t cx = views[4 * v], cy = views[4 * v + 1];
+    const int32_t sx = views[4 * v + 2], sy = views[4 * v + 3];
+    const int32_t k = int32_t(threadIdx.x);
+    const bool live = k < sx * sy;
+    const int32_t i = live ? k / sy : 0, j = live ? k % sy : 0;
+    uint32_t fl;
+    const float r = pe_value<NV>(t, n, root, px[cx + i], py[cy + j], z, &fl);
+    if (live) out[offs[v] + uint32_t(k)] = r < 0.0f ? 1 : 0;
+}
+
 // One wave per point; lane 0 runs the (serial) feature walk.
 __global__ __launch_bounds__(64)
 void k_point_features(const tape::Clause* __restrict__ t, uint32_t n, uint32_t root,
@@ -166,6 +188,21 @@ hipError_t launch_edge_search(const tape::Clause* t, uint32_t n, uint32_t root,
         case 1: hipLaunchKernelGGL(k_edge_search<1>, grid, dim3(64), 0, s, t, n, root, z, ends, stop, nedges); break;
         case 2: hipLaunchKernelGGL(k_edge_search<2>, grid, dim3(64), 0, s, t, n, root, z, ends, stop, nedges); break;
         case 4: hipLaunchKernelGGL(k_edge_search<4>, grid, dim3(64), 0, s, t, n, root, z, ends, stop, nedges); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_raster_views(const tape::Clause* t, uint32_t n, uint32_t root,
+                               uint32_t nregs, const float* px, const float* py, float z,
+                               const int32_t* views, const uint32_t* offs, uint8_t* out,
+                               size_t nviews, hipStream_t s) {
+    if (nviews == 0) return hipSuccess;
+    const dim3 grid(static_cast<uint32_t>(nviews), 1, 1);
+    switch (banks(nregs)) {
+        case 1: hipLaunchKernelGGL(k_raster_views<1>, grid, dim3(256), 0, s, t, n, root, px, py, z, views, offs, out); break;
+        case 2: hipLaunchKernelGGL(k_raster_views<2>, grid, dim3(256), 0, s, t, n, root, px, py, z, views, offs, out); break;
+        case 4: hipLaunchKernelGGL(k_raster_views<4>, grid, dim3(256), 0, s, t, n, root, px, py, z, views, offs, out); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

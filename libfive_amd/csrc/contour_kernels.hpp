@@ -1,5 +1,5 @@
-// Host-side launchers for the point passes of the 2D contourer (device
-// pointers, asynchronous on `s`).  One lane per point for values and
+// Host-side launchers for the point passes of the 2D contourer and the
+// raster (device pointers, asynchronous on `s`).  One lane per point for values and
 // partials; one wave per point for the feature evaluator.
 #pragma once
 
@@ -30,6 +30,15 @@ hipError_t launch_point_grads(const tape::Clause* t, uint32_t n, uint32_t root,
 hipError_t launch_edge_search(const tape::Clause* t, uint32_t n, uint32_t root,
                               uint32_t nregs, float z, double* ends, uint8_t* stop,
                               size_t nedges, hipStream_t s);
+
+// Raster pixel pass: views = (first x, first y, x count, y count) per view
+// of at most 256 pixels, offs = each view's first output byte; out gets 1
+// where the value at the pixel's voxel centre (px[x], py[y], z) is below
+// zero, in view order, x major.
+hipError_t launch_raster_views(const tape::Clause* t, uint32_t n, uint32_t root,
+                               uint32_t nregs, const float* px, const float* py, float z,
+                               const int32_t* views, const uint32_t* offs, uint8_t* out,
+                               size_t nviews, hipStream_t s);
 
 // FeatureEvaluator::isInside (inside[i]) and ::features (feats: up to 16
 // gradients of 3 floats per point, nfeat[i] of them); a null inside or

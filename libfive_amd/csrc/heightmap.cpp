@@ -6,7 +6,7 @@
 #include <deque>
 #include <mutex>
 
-#include "eval_kernels.hpp"
+#include "contour_kernels.hpp"
 #include "interval_kernels.hpp"
 #include "runtime.hpp"
 
@@ -171,35 +171,49 @@ PixelImage render_pixels(const Tree& t, float xlo, float xhi, float ylo,
     }
 
     // Heightmap::pixels (heightmap.cpp:100-155): point values at the voxel
-    // centres of the small views; filled where the value is below zero.
-    std::vector<float> px, py, pz;
-    std::vector<uint32_t> where;
-    for (const View& v : pixel_views) {
-        for (int i = 0; i < v.size[0]; ++i) {
-            for (int j = 0; j < v.size[1]; ++j) {
-                px.push_back(g.pts[0][v.corner[0] + i]);
-                py.push_back(g.pts[1][v.corner[1] + j]);
-                pz.push_back(zs);
-                where.push_back(uint32_t((v.corner[1] + j) * img.width + v.corner[0] + i));
-            }
+    // centres of the small views, one workgroup per view; filled where the
+    // value is below zero.
+    const size_t nv = pixel_views.size();
+    if (nv) {
+        std::vector<int32_t> vd(4 * nv);
+        std::vector<uint32_t> offs(nv);
+        uint32_t np = 0;
+        for (size_t k = 0; k < nv; ++k) {
+            const View& v = pixel_views[k];
+            vd[4 * k] = v.corner[0];
+            vd[4 * k + 1] = v.corner[1];
+            vd[4 * k + 2] = v.size[0];
+            vd[4 * k + 3] = v.size[1];
+            offs[k] = np;
+            np += uint32_t(v.size[0] * v.size[1]);
         }
-    }
-    const size_t np = px.size();
-    if (np) {
-        PoolBuf<float> d(4 * np);
-        hip_check(hipMemcpyAsync(d.p, px.data(), np * 4, hipMemcpyHostToDevice, c.stream), "copy x");
-        hip_check(hipMemcpyAsync(d.p + np, py.data(), np * 4, hipMemcpyHostToDevice, c.stream), "copy y");
-        hip_check(hipMemcpyAsync(d.p + 2 * np, pz.data(), np * 4, hipMemcpyHostToDevice, c.stream), "copy z");
+        const size_t w = g.pts[0].size(), hgt = g.pts[1].size();
+        PoolBuf<float> dp(w + hgt);
+        PoolBuf<int32_t> dv(4 * nv);
+        PoolBuf<uint32_t> doff(nv);
+        PoolBuf<uint8_t> dout(np);
+        hip_check(hipMemcpyAsync(dp.p, g.pts[0].data(), 4 * w, hipMemcpyHostToDevice, c.stream), "copy x");
+        hip_check(hipMemcpyAsync(dp.p + w, g.pts[1].data(), 4 * hgt, hipMemcpyHostToDevice, c.stream), "copy y");
+        hip_check(hipMemcpyAsync(dv.p, vd.data(), 16 * nv, hipMemcpyHostToDevice, c.stream), "copy views");
+        hip_check(hipMemcpyAsync(doff.p, offs.data(), 4 * nv, hipMemcpyHostToDevice, c.stream), "copy offsets");
         const auto& h = gt->host;
-        hip_check(launch_eval_values(gt->d.p, gt->size(), h.root_reg, h.num_regs, d.p,
-                                     d.p + np, d.p + 2 * np, d.p + 3 * np, np, c.stream),
+        hip_check(launch_raster_views(gt->d.p, gt->size(), h.root_reg, h.num_regs, dp.p,
+                                      dp.p + w, zs, dv.p, doff.p, dout.p, nv, c.stream),
                   "pixel values");
-        std::vector<float> out(np);
-        hip_check(hipMemcpyAsync(out.data(), d.p + 3 * np, np * 4, hipMemcpyDeviceToHost,
-                                 c.stream), "copy values");
+        std::vector<uint8_t> out(np);
+        hip_check(hipMemcpyAsync(out.data(), dout.p, np, hipMemcpyDeviceToHost, c.stream),
+                  "copy pixels");
         hip_check(hipStreamSynchronize(c.stream), "pixel values");
-        for (size_t k = 0; k < np; ++k) {
-            if (out[k] < 0.0f) img.filled[where[k]] = 1;
+        for (size_t k = 0; k < nv; ++k) {
+            const View& v = pixel_views[k];
+            const uint8_t* o = out.data() + offs[k];
+            for (int i = 0; i < v.size[0]; ++i) {
+                for (int j = 0; j < v.size[1]; ++j) {
+                    if (o[i * v.size[1] + j]) {
+                        img.filled[size_t(v.corner[1] + j) * img.width + v.corner[0] + i] = 1;
+                    }
+                }
+            }
         }
     }
     return img;

@@ -7,8 +7,9 @@
 // 256 voxels is classified by interval arithmetic (filled: every pixel set;
 // empty: none; ambiguous: split in two along its largest axis), a smaller
 // one by point evaluation at its pixel centres.  Here the recursion runs
-// level by level on the host, and each level's interval boxes, then all
-// remaining pixel centres, are evaluated in one GPU launch each.
+// level by level on the host with each level's interval boxes in one GPU
+// launch; the small views are then evaluated in one launch with a
+// workgroup per view.
 #pragma once
 
 #include <cstdint>

@@ -81,8 +81,8 @@ LF_HD float succ_f(float f) {
 // 1980, additive range reduction; Payne & Hanek, SIGNUM Newsletter 18(1),
 // 1983, for large arguments):
 //   x = k*(pi/2) + r, |r| <= pi/4, evaluated in double precision;
-//   sin(r), cos(r) by their Taylor series through r^13 / r^14 (truncation
-//   error below 2^-45 relative on |r| <= pi/4);
+//   sin(r), cos(r) by their Taylor series through r^11 / r^12 (truncation
+//   error below 2^-36 relative on |r| <= pi/4, 2^-12 of a float ulp);
 //   the quadrant k mod 4 selects +-sin(r) or +-cos(r); one rounding to float.
 // For |x| < 2^19 the reduction uses a two-double split of pi/2 (the first
 // product is exact because k*PIO2_HI and x share the 2^-52 grid).  Larger
@@ -158,26 +158,22 @@ constexpr double kS5 = 1.0 / 120.0;
 constexpr double kS7 = -1.0 / 5040.0;
 constexpr double kS9 = 1.0 / 362880.0;
 constexpr double kS11 = -1.0 / 39916800.0;
-constexpr double kS13 = 1.0 / 6227020800.0;
 constexpr double kC2 = -1.0 / 2.0;
 constexpr double kC4 = 1.0 / 24.0;
 constexpr double kC6 = -1.0 / 720.0;
 constexpr double kC8 = 1.0 / 40320.0;
 constexpr double kC10 = -1.0 / 3628800.0;
 constexpr double kC12 = 1.0 / 479001600.0;
-constexpr double kC14 = -1.0 / 87178291200.0;
 
 LF_HD void sincos_reduced(double r, double* s, double* c) {
     const double r2 = r * r;
-    double ps = kS13;
-    ps = ::fma(ps, r2, kS11);
+    double ps = kS11;
     ps = ::fma(ps, r2, kS9);
     ps = ::fma(ps, r2, kS7);
     ps = ::fma(ps, r2, kS5);
     ps = ::fma(ps, r2, kS3);
     *s = ::fma(ps * r2, r, r);
-    double pc = kC14;
-    pc = ::fma(pc, r2, kC12);
+    double pc = kC12;
     pc = ::fma(pc, r2, kC10);
     pc = ::fma(pc, r2, kC8);
     pc = ::fma(pc, r2, kC6);

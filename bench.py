@@ -12,11 +12,12 @@ points x tape clauses / time.
 Meshing (the "mesh" object): milliseconds to mesh the thickened gyroid
 intersected with the unit sphere over [-1.2, 1.2]^3 at 512 cells per axis
 through libfive_tree_render_mesh (tree in, libfive_mesh out, host copies
-included), plus the CSG configuration at resolution 16 and the 1000-sphere
-union's octree to depth 8 with its pruned-tape statistics.
+included), plus the CSG configuration at resolution 16, the 1000-sphere
+union's octree to depth 8 with its pruned-tape statistics, 256 models in
+one batched call, and a gyroid slice as 2D contours and as a raster.
 
-Prints one JSON line.  --gpus above 1 is rejected: this build measures one
-GPU (multi-GPU sharding is described in DESIGN.md, not run).
+Prints one JSON line.  --gpus above 1 is rejected: this benchmark measures
+one GPU; nothing here runs on a second one.
 """
 import argparse
 import ctypes

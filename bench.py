@@ -32,9 +32,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 # Nominal FLOPs per point for each opcode (value pass / value+gradient
-# pass).  sin and cos count W_TRIG = 20: range reduction (multiply, round,
-# two fused multiply-adds) plus the two 7-term polynomials evaluated by
-# Horner's rule, counting a fused multiply-add as two FLOPs.
+# pass), fixed before any tuning.  sin and cos count W_TRIG = 20, a fixed
+# nominal weight for one range reduction and one polynomial evaluation; it
+# is not recounted when the implementation changes (the series now have 6
+# and 7 terms).
 W_TRIG = 20
 VALUE_FLOPS = {7: 1, 8: 1, 9: 1, 10: W_TRIG, 11: W_TRIG, 17: 1, 18: 1,
                19: 1, 20: 1, 21: 1, 22: 1, 28: 1}

@@ -176,6 +176,12 @@ def test_no_gpu_fails_loudly():
     assert "GPU" in ffi.last_error()
     with pytest.raises(libfive_amd.GpuError):
         t.values(np.zeros(4), np.zeros(4), np.zeros(4))
+    c = W.circle(1.0)
+    for call in (lambda: c.render_slice((-2, -2), (2, 2), 0.0, 10),
+                 lambda: c.render_pixels((-2, -2), (2, 2), 0.0, 10),
+                 lambda: c.render_mesh([-2] * 3, [2] * 3, res=8)):
+        with pytest.raises(libfive_amd.GpuError):
+            call()
 
 
 def test_specialized_kernels_compile_on_host():

@@ -42,6 +42,19 @@ def test_sincos_within_one_ulp_of_float64(golden):
     assert (ulp_distance(s, golden["sin_rn"]) == 0).mean() > 0.999
 
 
+def test_sincos_dense_within_one_ulp():
+    """Two million float arguments in [-6.5, 6.5] (every quadrant, the
+    reduced argument over its whole range): within one ulp of the float64
+    value rounded to float, and nearly always equal to it."""
+    x = np.linspace(-6.5, 6.5, 2000001).astype(np.float32)
+    s, c = oracle.sincos(x)
+    rs = np.sin(x.astype(np.float64)).astype(np.float32)
+    rc = np.cos(x.astype(np.float64)).astype(np.float32)
+    us, uc = ulp_distance(s, rs), ulp_distance(c, rc)
+    assert us.max() <= 1 and uc.max() <= 1
+    assert (us == 0).mean() > 0.9999 and (uc == 0).mean() > 0.9999
+
+
 def test_two_over_pi_table():
     """The 2/pi bits used for large-argument reduction, recomputed from
     Machin's formula."""

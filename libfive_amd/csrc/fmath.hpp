@@ -197,16 +197,22 @@ LF_HD void sincos_pair(float x, float* so, float* co, const uint32_t* tbl) {
         const double k = ::rint(xd * kTwoOverPiD);
         r = ::fma(-k, kPio2Hi, xd);
         r = ::fma(-k, kPio2Lo, r);
-        q = int(int64_t(k) & 3);
+        q = int(k) & 3;                 // 0 <= k < 2^19
     } else {
         r = reduce_large(ax, tbl, &q);
     }
     double s, c;
     sincos_reduced(r, &s, &c);
-    // sin / cos of k*pi/2 + r for quadrants 0..3; sin is odd in x.
-    double sv = (q == 0) ? s : (q == 1) ? c : (q == 2) ? -s : -c;
-    const double cv = (q == 0) ? c : (q == 1) ? -s : (q == 2) ? -c : s;
-    if (f2u(x) & 0x80000000u) sv = -sv;
+    // sin / cos of k*pi/2 + r: odd quadrants swap the two, quadrants 2 and
+    // 3 negate sin, quadrants 1 and 2 negate cos; sin is odd in x.  Written
+    // as two-way selects so that no branches are generated.
+    const bool swap = (q & 1) != 0;
+    double sv = swap ? c : s;
+    double cv = swap ? s : c;
+    const bool neg_s = ((q & 2) != 0) != ((f2u(x) & 0x80000000u) != 0);
+    const bool neg_c = ((q + 1) & 2) != 0;
+    sv = neg_s ? -sv : sv;
+    cv = neg_c ? -cv : cv;
     *so = float(sv);
     *co = float(cv);
 }

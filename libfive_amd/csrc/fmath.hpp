@@ -124,9 +124,14 @@ LF_HD uint32_t two_over_pi_word(const uint32_t* tbl, int bit) {
     return s ? ((hi << s) | (lo >> (32 - s))) : hi;
 }
 
-// Large-argument reduction for finite |x| >= 2^19: returns r (double, in
-// [-pi/4, pi/4]) and sets *q to the quadrant (mod 4).
-LF_HD_COLD double reduce_large(float ax, const uint32_t* tbl, int* q) {
+// Large-argument reduction for finite |x| >= 2^19: r (double, in
+// [-pi/4, pi/4]) and the quadrant q (mod 4), returned by value so that the
+// out-of-line call needs no stack memory.
+struct Reduced {
+    double r;
+    int q;
+};
+LF_HD_COLD Reduced reduce_large(float ax, const uint32_t* tbl) {
     const uint32_t u = f2u(ax);
     const int e = int((u >> 23) & 0xff) - 150;          // ax = m * 2^e
     const uint64_t m = (u & 0x7fffffu) | 0x800000u;
@@ -151,8 +156,7 @@ LF_HD_COLD double reduce_large(float ax, const uint32_t* tbl, int* q) {
     double f = double(fhi) * ::ldexp(1.0, 64 - s)
              + double(lo) * ::ldexp(1.0, -s);
     if (f >= 0.5) { f -= 1.0; quad += 1; }
-    *q = quad & 3;
-    return ::fma(f, kPio2Hi, f * kPio2Lo);
+    return {::fma(f, kPio2Hi, f * kPio2Lo), quad & 3};
 }
 
 // Coefficients of p (kS*) and q (kC*), constant term first.
@@ -199,7 +203,9 @@ LF_HD void sincos_pair(float x, float* so, float* co, const uint32_t* tbl) {
         r = ::fma(-k, kPio2Lo, r);
         q = int(k) & 3;                 // 0 <= k < 2^19
     } else {
-        r = reduce_large(ax, tbl, &q);
+        const Reduced red = reduce_large(ax, tbl);
+        r = red.r;
+        q = red.q;
     }
     double s, c;
     sincos_reduced(r, &s, &c);

@@ -62,6 +62,13 @@ def test_matches_oracle(name, mk, lo, hi, z, mf):
         assert len(cs) == len(o.contours)
 
 
+def test_wide_register_file_matches_oracle():
+    """37 registers: the contourer's kernels with two register banks."""
+    s = W.random_dag() - 0.3
+    assert s.tape_info()[1] > 32
+    compare(s, (-1, -1), (1, 1), 0.1, 2 / 128)
+
+
 def test_reference_known_answers():
     cs = W.circle(1.0).render_slice((-2, -2), (2, 2), 0.0, 10)
     assert len(cs) == 1 and len(cs[0]) > 0

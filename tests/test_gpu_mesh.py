@@ -79,6 +79,14 @@ def test_sphere_union_matches_oracle(gpu):
     compare(W.sphere_union(n=60), -1.2, 1.2, 2.4 / 64)
 
 
+def test_wide_register_file_matches_oracle(gpu):
+    """The 809-clause random expression needs 37 registers, so the mesher's
+    kernels run with two register banks."""
+    s = W.random_dag() - 0.3
+    assert s.tape_info()[1] > 32
+    compare(s, -1.0, 1.0, 2.0 / 16)
+
+
 def test_sliced_box_matches_oracle(gpu):
     """Box faces on the grid: exact zeros on tied MIN/MAX go through the
     feature evaluation."""

@@ -56,6 +56,16 @@ def test_matches_oracle(name, mk, lo, hi, z, res):
     assert (g == o).all()
 
 
+def test_wide_register_file_matches_oracle():
+    """37 registers: the raster's kernels with two register banks."""
+    s = W.random_dag() - 0.3
+    assert s.tape_info()[1] > 32
+    g = s.render_pixels((-1, -1), (1, 1), 0.1, 200)
+    o = od(s).render_pixels((-1, -1), (1, 1), 0.1, 200)
+    assert g.shape == o.shape and (g == o).all()
+    assert g.any() and not g.all()
+
+
 def test_empty_full_and_single_pixel():
     far = W.sphere(0.5, (10, 10, 10))
     assert not far.render_pixels((-1, -1), (1, 1), 0, 50).any()

@@ -1,7 +1,7 @@
 """Shared test helpers."""
 import numpy as np
 
-from libfive_amd import Shape
+from libfive_amd import Shape, X, Y, Z
 
 
 def same_bits(a, b):
@@ -48,3 +48,22 @@ def op_shape(name):
     if name in BINARY:
         return Shape.binary(name, x, y)
     return Shape.unary(name, x)
+
+
+def wide_registers(n=70):
+    """A shape whose tape keeps about n + 5 values live at once: n sines,
+    summed in one order and max-folded in the other, plus a sphere.  With
+    n = 70 it needs 75 registers (four register banks).  The phases differ
+    so that no lattice point makes the sines equal (a max over many exact
+    ties has exponentially many features)."""
+    from libfive_amd import workloads as W
+    x, y = X(), Y()
+    a = [((x * (0.1 * i + 1.0)) + y * (0.05 * i + 0.3) + (0.37 * i + 0.1)).sin()
+         for i in range(n)]
+    s1 = a[0]
+    for t in a[1:]:
+        s1 = s1 + t
+    s2 = a[-1]
+    for t in reversed(a[:-1]):
+        s2 = s2.max(t)
+    return s1 * 0.02 + s2 * 0.1 + W.sphere(0.8)

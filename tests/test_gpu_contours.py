@@ -12,6 +12,8 @@ from libfive_amd import workloads as W
 
 from test_contour_oracle import rect
 
+from helpers import wide_registers
+
 pytestmark = pytest.mark.gpu
 
 
@@ -66,6 +68,12 @@ def test_wide_register_file_matches_oracle():
     """37 registers: the contourer's kernels with two register banks."""
     s = W.random_dag() - 0.3
     assert s.tape_info()[1] > 32
+    compare(s, (-1, -1), (1, 1), 0.1, 2 / 128)
+
+
+def test_four_register_banks_match_oracle():
+    s = wide_registers(70)
+    assert s.tape_info()[1] > 64
     compare(s, (-1, -1), (1, 1), 0.1, 2 / 128)
 
 

@@ -10,7 +10,8 @@ import oracle
 from libfive_amd import Shape, X, Y, Z, ffi
 from libfive_amd import workloads as W
 
-from helpers import BINARY, UNARY, assert_same_bits, op_shape, ulp_distance
+from helpers import (BINARY, UNARY, assert_same_bits, op_shape, ulp_distance,
+                     wide_registers)
 
 pytestmark = pytest.mark.gpu
 
@@ -89,6 +90,7 @@ SHAPES = {
                         + Shape.binary("compare", X(), Y())
                         + Shape.binary("nanfill", (X() - 0.5).sqrt(), Y())
                         + Shape.unary("recip", Z() + 3)),
+    "wide_75_registers": wide_registers,
 }
 
 

@@ -13,6 +13,8 @@ import oracle
 from libfive_amd import X, Y, Z
 from libfive_amd import workloads as W
 
+from helpers import wide_registers
+
 pytestmark = pytest.mark.gpu
 
 
@@ -84,6 +86,12 @@ def test_wide_register_file_matches_oracle(gpu):
     kernels run with two register banks."""
     s = W.random_dag() - 0.3
     assert s.tape_info()[1] > 32
+    compare(s, -1.0, 1.0, 2.0 / 16)
+
+
+def test_four_register_banks_match_oracle(gpu):
+    s = wide_registers(70)
+    assert s.tape_info()[1] > 64
     compare(s, -1.0, 1.0, 2.0 / 16)
 
 

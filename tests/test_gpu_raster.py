@@ -14,6 +14,8 @@ from libfive_amd import workloads as W
 
 from test_raster_oracle import CASES, CIRCLE, ORIENT_X, ORIENT_Y
 
+from helpers import wide_registers
+
 pytestmark = pytest.mark.gpu
 
 
@@ -60,6 +62,15 @@ def test_wide_register_file_matches_oracle():
     """37 registers: the raster's kernels with two register banks."""
     s = W.random_dag() - 0.3
     assert s.tape_info()[1] > 32
+    g = s.render_pixels((-1, -1), (1, 1), 0.1, 200)
+    o = od(s).render_pixels((-1, -1), (1, 1), 0.1, 200)
+    assert g.shape == o.shape and (g == o).all()
+    assert g.any() and not g.all()
+
+
+def test_four_register_banks_match_oracle():
+    s = wide_registers(70)
+    assert s.tape_info()[1] > 64
     g = s.render_pixels((-1, -1), (1, 1), 0.1, 200)
     o = od(s).render_pixels((-1, -1), (1, 1), 0.1, 200)
     assert g.shape == o.shape and (g == o).all()

@@ -6,7 +6,6 @@
 #include <cstring>
 #include <functional>
 #include <list>
-#include <map>
 #include <mutex>
 #include <thread>
 
@@ -358,6 +357,9 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
         // Mixed cells become leaves; their sign-change edges are searched
         // together.
         std::vector<Edge> edges;
+        edges.reserve(2 * nl);
+        q.leaves.reserve(q.leaves.size() + nl);
+        out.cells.reserve(out.cells.size() + 4 * nl);
         for (size_t k = 0; k < nl; ++k) {
             Cell& cl = q.cells[frontier[k]];
             uint8_t mask = 0;
@@ -732,23 +734,25 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
     walk(q.cells[0]);
 
     // Contours::collect (contours.cpp:112-184): join segments head to tail.
-    std::map<uint32_t, uint32_t> heads, tails;
+    // The reference's head and tail maps, keyed by vertex index, as arrays
+    // over the vertices (kNone: no entry); a segment joins two different
+    // cells' vertices, so its ends differ.
+    constexpr uint32_t kNone = 0xffffffffu;
+    std::vector<uint32_t> heads(out.verts.size(), kNone), tails(out.verts.size(), kNone);
     std::vector<std::list<uint32_t>> runs;
     for (const auto& s : out.segs) {
-        auto t = tails.find(s[0]);
-        if (t != tails.end()) {
-            const uint32_t r = t->second;
+        if (tails[s[0]] != kNone) {
+            const uint32_t r = tails[s[0]];
             runs[r].push_back(s[1]);
-            tails.insert({s[1], r});
-            tails.erase(t);
+            if (tails[s[1]] == kNone) tails[s[1]] = r;     // insert keeps an entry
+            tails[s[0]] = kNone;
             continue;
         }
-        auto h = heads.find(s[1]);
-        if (h != heads.end()) {
-            const uint32_t r = h->second;
+        if (heads[s[1]] != kNone) {
+            const uint32_t r = heads[s[1]];
             runs[r].push_front(s[0]);
-            heads.insert({s[0], r});
-            heads.erase(h);
+            if (heads[s[0]] == kNone) heads[s[0]] = r;
+            heads[s[1]] = kNone;
             continue;
         }
         heads[s[0]] = uint32_t(runs.size());
@@ -763,10 +767,10 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
             auto& dst = out.contours.back();
             dst.insert(dst.end(), runs[r].begin(), runs[r].end());
             used[r] = true;
-            auto h = heads.find(runs[r].back());
-            if (h == heads.end() || used[h->second]) break;
+            const uint32_t h = heads[runs[r].back()];
+            if (h == kNone || used[h]) break;
             dst.pop_back();
-            r = h->second;
+            r = h;
         }
     }
     return out;

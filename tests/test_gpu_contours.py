@@ -77,6 +77,24 @@ def test_four_register_banks_match_oracle():
     compare(s, (-1, -1), (1, 1), 0.1, 2 / 128)
 
 
+@pytest.mark.parametrize("name,mk,lo,hi,z,mf", CASES[:9], ids=[c[0] for c in CASES[:9]])
+def test_chained_contours_match_oracle(name, mk, lo, hi, z, mf):
+    """Both walks visit the quadtree in the same depth-first order, so the
+    segments come in the same order and the head-to-tail chaining
+    (contours.cpp:112-184) gives the same point sequences, bit for bit."""
+    s = mk()
+    res = float(np.float32(1) / np.float32(mf))
+    cs = s.render_slice(lo, hi, z, res)
+    mf32 = float(np.float32(1) / np.float32(res))
+    lo32 = [float(np.float32(v)) for v in lo]
+    hi32 = [float(np.float32(v)) for v in hi]
+    o = oracle.OracleDeck(s.deck()).render_contours(lo32, hi32, z, mf32)
+    assert len(cs) == len(o.polylines)
+    for g, r in zip(cs, o.polylines):
+        assert g.shape == r.shape
+        assert (g.view(np.uint32) == r.view(np.uint32)).all()
+
+
 def test_reference_known_answers():
     cs = W.circle(1.0).render_slice((-2, -2), (2, 2), 0.0, 10)
     assert len(cs) == 1 and len(cs[0]) > 0

@@ -162,13 +162,6 @@ void deriv_body(const DeviceTape& t, std::string& out) {
 
 }   // namespace
 
-// Occupancy hint for the derivative kernel: at least 4 waves per SIMD
-// (measured on the 809-clause benchmark tape: 4.83 ms against 4.96-4.98 ms
-// with no hint or with 1, 2, 3, 6 or 8).
-static std::string waves_attr() {
-    return " __attribute__((amdgpu_waves_per_eu(4)))";
-}
-
 std::string specialized_source(const DeviceTape& t) {
     std::string s =
         "#include \"clause_ops.hpp\"\n"
@@ -183,7 +176,7 @@ std::string specialized_source(const DeviceTape& t) {
         "    const float px = xs[i], py = ys[i], pz = zs[i];\n";
     value_body(t, s);
     s += "}\n\n"
-         "extern \"C\" __global__ __launch_bounds__(256)" + waves_attr() + "\n"
+         "extern \"C\" __global__ __launch_bounds__(256)\n"
          "void lf_derivs(const float* __restrict__ xs, const float* __restrict__ ys,\n"
          "               const float* __restrict__ zs, float* __restrict__ ov,\n"
          "               float* __restrict__ odx, float* __restrict__ ody,\n"

@@ -60,8 +60,6 @@ def parse():
     p.add_argument("--mesh-steps", type=int, default=5)
     p.add_argument("--batch-models", type=int, default=256,
                    help="models in the many-model meshing leg (0: skip)")
-    p.add_argument("--cpu-mesh-res", type=int, default=256,
-                   help="gyroid resolution for the CPU meshing baseline")
     p.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                    help="only run a plain device-to-device copy of MIB MiB "
                         "three times (calibrates the HBM byte counters)")
@@ -71,13 +69,34 @@ def parse():
     return p.parse_args()
 
 
+def host_cpu():
+    """CPU model of this host, the cores this process may run on, and the
+    worker threads the CPU legs use: one per core, at most 16 (the host
+    cores allotted to one GPU on the MI355X machines this runs on)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return model, avail, max(1, min(16, avail))
+
+
 def cpu_baseline(shape, target_seconds):
     """The oracle (CPU restatement of the reference's 256-lane
     DerivArrayEvaluator) timed on this host on a bounded sample of the same
-    grid: whole z-slabs of the 256^3 grid, sized to about target_seconds."""
+    grid: whole z-slabs of the 256^3 grid, sized to about target_seconds.
+    Returns the baseline record and the oracle's outputs on the sample
+    (value and partials of the first n grid points)."""
     import oracle
     from libfive_amd import workloads as W
-    threads = max(1, min(16, os.cpu_count() or 1))
+    model, avail, threads = host_cpu()
     deck = shape.deck()
     T = deck[0].shape[0]
     od = oracle.OracleDeck(deck)
@@ -90,17 +109,19 @@ def cpu_baseline(shape, target_seconds):
     n_slabs = int(max(1, min(GRID, target_seconds / max(dt, 1e-6))))
     n = n_slabs * slab
     t0 = time.perf_counter()
-    od.derivs(x[:n], y[:n], z[:n], threads=threads)
+    ref = od.derivs(x[:n], y[:n], z[:n], threads=threads)
     dt = time.perf_counter() - t0
     return {
         "value": n * T / dt / 1e9,
         "unit": "Gpoint-op/s",
         "cores": threads,
         "kind": "port",
+        "cpu_model": model,
+        "cores_available": avail,
         "sample": f"{n_slabs} of {GRID} z-slabs of the {GRID}^3 grid "
                   f"({n} points x {T} clauses, values+gradients, "
                   f"{dt:.1f} s)",
-    }
+    }, ref
 
 
 def profiled_traffic(kernel):
@@ -146,6 +167,7 @@ def mesh_measurements(args):
     lo, hi = W.C4_REGION[0][0], W.C4_REGION[1][0]
     ms, (nv, nt) = render_ms(gy, lo, hi, res / (hi - lo), args.mesh_steps)
     d = gy.render_mesh([lo] * 3, [hi] * 3, res=res / (hi - lo), details=True)
+    _, _, threads = host_cpu()
     out = {
         "workload": f"thickened gyroid (k=10) x unit sphere, [-1.2,1.2]^3, "
                     f"{res}^3 cells, FP64 QEF",
@@ -155,6 +177,27 @@ def mesh_measurements(args):
                      zip(("build", "collapse", "emit", "total"), d.ms)},
         "ambiguous_leaves": int(d.level_counts[-1][2]),
     }
+    lo32, hi32 = float(np.float32(lo)), float(np.float32(hi))
+    mf512 = float(np.float32(1) / np.float32(res / (hi - lo)))
+    if not args.no_cpu:
+        # The oracle mesher at the same resolution: the CPU leg, an exact
+        # comparison (vertex keys, vertex bits, triangles), and the vertex
+        # error of the GPU mesh against the FP64 error oracle.
+        od = oracle.OracleDeck(gy.deck())
+        t0 = time.perf_counter()
+        om = od.render_mesh([lo32] * 3, [hi32] * 3, mf512, threads=threads)
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        out["cpu_baseline"] = {
+            "value": round(cpu_ms, 1), "unit": "ms", "cores": threads,
+            "kind": "port", "sample": "the whole configuration, one run"}
+        out["vs_oracle"] = oracle.compare_meshes(d.verts, d.tris, d.keys, om)
+        del om
+        r64 = od.render_mesh([lo32] * 3, [hi32] * 3, mf512, threads=threads,
+                             fp64=True)
+        err, matched, nverts = oracle.vertex_error(d.verts, d.keys, r64)
+        out["vertex_error_vs_fp64"] = {"max_abs": err, "matched": matched,
+                                       "verts": nverts}
+        del r64
     # CSG configuration at resolution 16 (64^3 finest cells)
     c1 = W.csg_c1()
     c1_ms, (c1v, c1t) = render_ms(c1, -2.0, 2.0, W.C1_RES, 5)
@@ -189,7 +232,6 @@ def mesh_measurements(args):
         "tris": int(sm.tris.shape[0]),
     }
     if not args.no_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
         ods = oracle.OracleDeck(su.deck())
         lo32, hi32 = float(np.float32(s_lo)), float(np.float32(s_hi))
         mfs = float(np.float32(1) / np.float32(2 ** W.C3_DEPTH / (s_hi - s_lo)))
@@ -208,6 +250,26 @@ def mesh_measurements(args):
                                      max(1, args.mesh_steps // 2))
     out["sphere_union_512"] = {"ms": round(su_ms512, 2),
                                "verts": int(suv) - 1, "tris": int(sut)}
+    if not args.no_cpu:
+        sd = su.render_mesh([s_lo] * 3, [s_hi] * 3, res=res / (s_hi - s_lo),
+                            details=True)
+        ods = oracle.OracleDeck(su.deck())
+        slo32, shi32 = float(np.float32(s_lo)), float(np.float32(s_hi))
+        t0 = time.perf_counter()
+        osm = ods.render_mesh([slo32] * 3, [shi32] * 3, mf512, threads=threads)
+        out["sphere_union_512"]["cpu_baseline"] = {
+            "value": round((time.perf_counter() - t0) * 1e3, 1), "unit": "ms",
+            "cores": threads, "kind": "port",
+            "sample": "the whole configuration, one run"}
+        out["sphere_union_512"]["vs_oracle"] = oracle.compare_meshes(
+            sd.verts, sd.tris, sd.keys, osm)
+        del osm
+        r64 = ods.render_mesh([slo32] * 3, [shi32] * 3, mf512,
+                              threads=threads, fp64=True)
+        err, matched, nverts = oracle.vertex_error(sd.verts, sd.keys, r64)
+        out["sphere_union_512"]["vertex_error_vs_fp64"] = {
+            "max_abs": err, "matched": matched, "verts": nverts}
+        del r64, sd
     # Many independent models (256 seeded random CSG trees, 10-2000 ops,
     # resolutions 32..256 over [-1,1]^3): one libfive_tree_render_mesh
     # call per model (plain) against one batched call that runs 8 models
@@ -298,37 +360,6 @@ def mesh_measurements(args):
             "same_contour_count": len(oc.contours) == len(cs),
         }
 
-    if not args.no_cpu:
-        # The oracle mesher (CPU restatement of the reference's DC mesher)
-        # on the same gyroid at a lower resolution, with the GPU time there.
-        cres = args.cpu_mesh_res
-        threads = max(1, min(16, os.cpu_count() or 1))
-        od = oracle.OracleDeck(gy.deck())
-        mf = float(np.float32(1) / np.float32(cres / (hi - lo)))
-        lo32, hi32 = float(np.float32(lo)), float(np.float32(hi))
-        t0 = time.perf_counter()
-        om = od.render_mesh([lo32] * 3, [hi32] * 3, mf, threads=threads)
-        cpu_ms = (time.perf_counter() - t0) * 1e3
-        gpu_ms, _ = render_ms(gy, lo, hi, cres / (hi - lo), 3)
-        # vertex error of the GPU mesh against the FP64 error oracle (the
-        # same octree algorithm with double evaluation), vertices matched
-        # by owning cell and patch
-        gm = gy.render_mesh([lo] * 3, [hi] * 3, res=cres / (hi - lo),
-                            details=True)
-        r64 = od.render_mesh([lo32] * 3, [hi32] * 3, mf, threads=threads,
-                             fp64=True)
-        err, matched, nverts = oracle.vertex_error(gm.verts, gm.keys, r64)
-        out["vertex_error_vs_fp64"] = {
-            "max_abs": err, "matched": matched, "verts": nverts,
-            "workload": f"same gyroid at {cres}^3",
-        }
-        out["cpu_baseline"] = {
-            "value": round(cpu_ms, 1), "unit": "ms", "cores": threads,
-            "kind": "port",
-            "sample": f"same gyroid at {cres}^3 ({om.tris.shape[0]} tris); "
-                      f"GPU at {cres}^3: {gpu_ms:.1f} ms",
-            "gpu_ms_same_sample": round(gpu_ms, 2),
-        }
     return out
 
 
@@ -473,13 +504,28 @@ def main():
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
             "traffic": profiled_traffic("lf_derivs"),
+            "traffic_source": "profiles/lf_derivs_random_dag_256cube.md: "
+                              "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs "
+                              "of this kernel, scaled by a plain copy",
             "kernel": "lf_derivs",
             "kernel_ms": round(kern_ms, 4),
             "flops_per_point": dflops,
         },
     }
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(shape, args.cpu_seconds)
+        result["cpu_baseline"], ref = cpu_baseline(shape, args.cpu_seconds)
+        # The timed kernel's output (the last step's values and partials)
+        # against the oracle on the sample it computed.
+        m = ref[0].size
+        got = out[:, :m].cpu().numpy()
+        same = np.ones(m, bool)
+        for k in range(4):
+            a, b = got[k], ref[k]
+            same &= (a.view(np.uint32) == b.view(np.uint32)) | (
+                np.isnan(a) & np.isnan(b))
+        result["c2_parity"] = {"points_compared": int(m),
+                               "bit_exact": bool(same.all()),
+                               "mismatches": int((~same).sum())}
     L.libfive_tape_delete(tape)
     if not args.no_mesh:
         result["mesh"] = mesh_measurements(args)

@@ -164,6 +164,22 @@ int libfive_tree_eval_interval_array(libfive_tree t, const float* xlo,
                                      float* lower, float* upper,
                                      uint8_t* maybe_nan);
 
+/* Ambiguity flags at n host-array points: out[i] = 1 where some MIN/MAX
+ * clause of the tree's tape has equal operands at point i (replaces
+ * ArrayEvaluator::getAmbiguous, libfive/src/eval/eval_array.cpp:198-217).
+ * Returns 0 on success. */
+int libfive_tree_eval_ambiguous_array(libfive_tree t, const float* x,
+                                      const float* y, const float* z,
+                                      size_t n, uint8_t* out);
+
+/* As above, but a clause counts only when its operands' partial
+ * derivatives also differ (replaces
+ * DerivArrayEvaluator::getAmbiguousDerivs,
+ * libfive/src/eval/eval_deriv_array.cpp:58-81).  Returns 0 on success. */
+int libfive_tree_eval_deriv_ambiguous_array(libfive_tree t, const float* x,
+                                            const float* y, const float* z,
+                                            size_t n, uint8_t* out);
+
 /* A tree compiled once to a device-resident tape. */
 typedef struct libfive_tape_ libfive_tape_;
 typedef struct libfive_tape_* libfive_tape;
@@ -191,6 +207,7 @@ int libfive_tape_eval_values(libfive_tape t, const float* x, const float* y,
 int libfive_tape_eval_derivs(libfive_tape t, const float* x, const float* y,
                              const float* z, size_t n, float* value, float* dx,
                              float* dy, float* dz);
+
 
 /* Mesh output forms (libfive.h:369-410, libfive.cpp:354-440): the same
  * mesh as libfive_tree_render_mesh, as -1-separated coordinate indices, or

@@ -274,6 +274,48 @@ int libfive_tree_eval_deriv_array(libfive_tree t, const float* x,
     }
 }
 
+static int eval_ambiguous(const char* where, libfive_tree t, bool derivs,
+                          const float* x, const float* y, const float* z,
+                          size_t n, uint8_t* out) {
+    try {
+        Context& c = context();
+        auto gt = compile_tree(Tree(D(t)));
+        DevBuf<float> buf(3 * n);
+        DevBuf<uint8_t> flags(n ? n : 1);
+        std::lock_guard<std::mutex> lock(c.mu);
+        const size_t b = n * sizeof(float);
+        float* p = buf.p;
+        hip_check(hipMemcpyAsync(p, x, b, hipMemcpyHostToDevice, c.stream), "copy x");
+        hip_check(hipMemcpyAsync(p + n, y, b, hipMemcpyHostToDevice, c.stream), "copy y");
+        hip_check(hipMemcpyAsync(p + 2 * n, z, b, hipMemcpyHostToDevice, c.stream), "copy z");
+        hip_check(launch_eval_ambiguous(gt->d.p, gt->size(), gt->host.root_reg,
+                                        gt->host.num_regs, derivs, p, p + n,
+                                        p + 2 * n, flags.p, n, c.stream),
+                  "eval_ambiguous");
+        hip_check(hipMemcpyAsync(out, flags.p, n, hipMemcpyDeviceToHost, c.stream),
+                  "copy flags");
+        hip_check(hipStreamSynchronize(c.stream), "eval_ambiguous sync");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error(where, e);
+        return -1;
+    }
+}
+
+int libfive_tree_eval_ambiguous_array(libfive_tree t, const float* x,
+                                      const float* y, const float* z,
+                                      size_t n, uint8_t* out) {
+    return eval_ambiguous("libfive_tree_eval_ambiguous_array", t, false, x, y,
+                          z, n, out);
+}
+
+int libfive_tree_eval_deriv_ambiguous_array(libfive_tree t, const float* x,
+                                            const float* y, const float* z,
+                                            size_t n, uint8_t* out) {
+    return eval_ambiguous("libfive_tree_eval_deriv_ambiguous_array", t, true, x,
+                          y, z, n, out);
+}
+
 int libfive_tree_eval_interval_array(libfive_tree t, const float* xlo,
                                      const float* xhi, const float* ylo,
                                      const float* yhi, const float* zlo,

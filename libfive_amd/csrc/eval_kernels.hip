@@ -192,6 +192,106 @@ void k_eval_derivs(const Clause* __restrict__ t, uint32_t n, uint32_t root,
 }
 
 ////////////////////////////////////////////////////////////////////////////////
+// Ambiguity flags: ArrayEvaluator::getAmbiguous (eval_array.cpp:198-217),
+// a MIN/MAX clause whose operands compare equal; and
+// DerivArrayEvaluator::getAmbiguousDerivs (eval_deriv_array.cpp:58-81),
+// equal operands whose partials differ in some component.  The flags are
+// OR-ed over the whole tape, as the reference's loop over its clauses.
+
+template <int NV>
+__global__ __launch_bounds__(256)
+void k_eval_ambiguous(const Clause* __restrict__ t, uint32_t n, uint32_t root,
+                      const float* __restrict__ xs, const float* __restrict__ ys,
+                      const float* __restrict__ zs, uint8_t* __restrict__ out,
+                      size_t npts)
+{
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool ok = i < npts;
+    LF_RF_DECL(r0);
+    LF_RF_SET(r0, NV, kRegX, ok ? xs[i] : 0.0f);
+    LF_RF_SET(r0, NV, kRegY, ok ? ys[i] : 0.0f);
+    LF_RF_SET(r0, NV, kRegZ, ok ? zs[i] : 0.0f);
+    bool amb = false;
+    for (uint32_t k = 0; k < n; ++k) {
+        const Clause c = t[k];
+        const uint32_t o = tape::op(c);
+        if (o == OP_NOP) continue;
+        const uint32_t m = mode(c);
+        const float im = imm(c);
+        const float a = (m == MODE_A_IMM || o == OP_LOADI) ? im : LF_RF_GET(r0, NV, ra(c));
+        float b = 0.0f;
+        if (is_binary(c)) b = (m == MODE_B_IMM) ? im : LF_RF_GET(r0, NV, rb(c));
+        if (o == OC::OP_MIN || o == OC::OP_MAX) amb |= (a == b);
+        float q;
+        apply_value<1>(o, &a, &b, im, &q);
+        LF_RF_SET(r0, NV, dst(c), q);
+    }
+    if (ok) out[i] = amb ? 1 : 0;
+}
+
+template <int NV>
+__global__ __launch_bounds__(256)
+void k_eval_deriv_ambiguous(const Clause* __restrict__ t, uint32_t n,
+                            uint32_t root, const float* __restrict__ xs,
+                            const float* __restrict__ ys,
+                            const float* __restrict__ zs,
+                            uint8_t* __restrict__ out, size_t npts)
+{
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool ok = i < npts;
+    LF_RF_DECL(v);
+    LF_RF_DECL(gx);
+    LF_RF_DECL(gy);
+    LF_RF_DECL(gz);
+    LF_RF_SET(v, NV, kRegX, ok ? xs[i] : 0.0f);
+    LF_RF_SET(v, NV, kRegY, ok ? ys[i] : 0.0f);
+    LF_RF_SET(v, NV, kRegZ, ok ? zs[i] : 0.0f);
+    LF_RF_SET(gx, NV, kRegX, 1.0f);
+    LF_RF_SET(gy, NV, kRegY, 1.0f);
+    LF_RF_SET(gz, NV, kRegZ, 1.0f);
+    bool amb = false;
+    for (uint32_t k = 0; k < n; ++k) {
+        const Clause c = t[k];
+        const uint32_t o = tape::op(c);
+        if (o == OP_NOP) continue;
+        const uint32_t m = mode(c);
+        const float im = imm(c);
+        float av = im, ad[3] = {0.0f, 0.0f, 0.0f};
+        float bv = 0.0f, bd[3] = {0.0f, 0.0f, 0.0f};
+        if (m != MODE_A_IMM && o != OP_LOADI) {
+            const uint32_t a = ra(c);
+            av = LF_RF_GET(v, NV, a);
+            ad[0] = LF_RF_GET(gx, NV, a);
+            ad[1] = LF_RF_GET(gy, NV, a);
+            ad[2] = LF_RF_GET(gz, NV, a);
+        }
+        if (is_binary(c)) {
+            if (m == MODE_B_IMM) {
+                bv = im;
+            } else {
+                const uint32_t b = rb(c);
+                bv = LF_RF_GET(v, NV, b);
+                bd[0] = LF_RF_GET(gx, NV, b);
+                bd[1] = LF_RF_GET(gy, NV, b);
+                bd[2] = LF_RF_GET(gz, NV, b);
+            }
+        }
+        if (o == OC::OP_MIN || o == OC::OP_MAX) {
+            amb |= (av == bv) &&
+                   (ad[0] != bd[0] || ad[1] != bd[1] || ad[2] != bd[2]);
+        }
+        float ov, od[3];
+        apply_deriv(o, av, ad, bv, bd, im, ov, od);
+        const uint32_t d = dst(c);
+        LF_RF_SET(v, NV, d, ov);
+        LF_RF_SET(gx, NV, d, od[0]);
+        LF_RF_SET(gy, NV, d, od[1]);
+        LF_RF_SET(gz, NV, d, od[2]);
+    }
+    if (ok) out[i] = amb ? 1 : 0;
+}
+
+////////////////////////////////////////////////////////////////////////////////
 // Launchers
 
 static int banks_for(uint32_t nregs) {
@@ -252,6 +352,34 @@ hipError_t launch_eval_derivs(const Clause* t, uint32_t n, uint32_t root,
         case 2: k_eval_derivs<2><<<grid, block, 0, s>>>(t, n, root, x, y, z, v, dx, dy, dz, npts); break;
         case 4: k_eval_derivs<4><<<grid, block, 0, s>>>(t, n, root, x, y, z, v, dx, dy, dz, npts); break;
         default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_eval_ambiguous(const Clause* t, uint32_t n, uint32_t root,
+                                 uint32_t nregs, bool derivs, const float* x,
+                                 const float* y, const float* z, uint8_t* out,
+                                 size_t npts, hipStream_t s)
+{
+    if (npts == 0) return hipSuccess;
+    const dim3 block(256);
+    const dim3 grid(unsigned((npts + 255) / 256));
+    const int nv = banks_for(nregs);
+    if (derivs) {
+        switch (nv) {
+            case 1: k_eval_deriv_ambiguous<1><<<grid, block, 0, s>>>(t, n, root, x, y, z, out, npts); break;
+            case 2: k_eval_deriv_ambiguous<2><<<grid, block, 0, s>>>(t, n, root, x, y, z, out, npts); break;
+            case 4: k_eval_deriv_ambiguous<4><<<grid, block, 0, s>>>(t, n, root, x, y, z, out, npts); break;
+            default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (nv) {
+            case 1: k_eval_ambiguous<1><<<grid, block, 0, s>>>(t, n, root, x, y, z, out, npts); break;
+            case 2: k_eval_ambiguous<2><<<grid, block, 0, s>>>(t, n, root, x, y, z, out, npts); break;
+            case 4: k_eval_ambiguous<4><<<grid, block, 0, s>>>(t, n, root, x, y, z, out, npts); break;
+            case 8: k_eval_ambiguous<8><<<grid, block, 0, s>>>(t, n, root, x, y, z, out, npts); break;
+            default: return hipErrorInvalidValue;
+        }
     }
     return hipGetLastError();
 }

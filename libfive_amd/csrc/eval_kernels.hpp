@@ -21,6 +21,14 @@ hipError_t launch_eval_derivs(const tape::Clause* t, uint32_t n, uint32_t root,
                               const float* z, float* v, float* dx, float* dy,
                               float* dz, size_t npts, hipStream_t s);
 
+// Per point: 1 where a MIN/MAX clause has equal operands (getAmbiguous), or
+// with derivs = true equal operands with different partials
+// (getAmbiguousDerivs).
+hipError_t launch_eval_ambiguous(const tape::Clause* t, uint32_t n, uint32_t root,
+                                 uint32_t nregs, bool derivs, const float* x,
+                                 const float* y, const float* z, uint8_t* out,
+                                 size_t npts, hipStream_t s);
+
 // Points per lane for the value kernel (0: automatic).
 void set_points_per_lane(int p);
 // Largest register count each interpreter keeps in VGPRs.

@@ -137,6 +137,12 @@ SIGNATURES = {
     "libfive_tree_eval_interval_array": (ctypes.c_int,
                                          [tree_t, fp, fp, fp, fp, fp, fp,
                                           ctypes.c_size_t, fp, fp, u8p]),
+    "libfive_tree_eval_ambiguous_array": (ctypes.c_int,
+                                          [tree_t, fp, fp, fp,
+                                           ctypes.c_size_t, u8p]),
+    "libfive_tree_eval_deriv_ambiguous_array": (ctypes.c_int,
+                                                [tree_t, fp, fp, fp,
+                                                 ctypes.c_size_t, u8p]),
     "libfive_tape_new": (tape_t, [tree_t]),
     "libfive_tape_delete": (None, [tape_t]),
     "libfive_tape_clauses": (ctypes.c_uint32, [tape_t]),

@@ -174,6 +174,19 @@ class Shape:
             raise GpuError(ffi.last_error())
         return v, dx, dy, dz
 
+    def ambiguous(self, x, y, z, derivs=False):
+        """Per point: some MIN/MAX clause has equal operands
+        (ArrayEvaluator::getAmbiguous); with derivs=True, equal operands
+        with different partials (DerivArrayEvaluator::getAmbiguousDerivs)."""
+        x, y, z = (np.ascontiguousarray(a, np.float32) for a in (x, y, z))
+        out = np.empty(x.size, np.uint8)
+        f = (ffi.lib().libfive_tree_eval_deriv_ambiguous_array if derivs
+             else ffi.lib().libfive_tree_eval_ambiguous_array)
+        if f(self.ptr, _f32p(x), _f32p(y), _f32p(z), x.size,
+             out.ctypes.data_as(ffi.u8p)):
+            raise GpuError(ffi.last_error())
+        return out.astype(bool)
+
     def intervals(self, lo, hi):
         """lo, hi: arrays of shape (n, 3).  Returns (lower, upper, maybe_nan)."""
         lo = np.asarray(lo, np.float32)

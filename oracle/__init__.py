@@ -37,6 +37,15 @@ _SIGS = {
                                                _fp, _fp, _u8,
                                                ctypes.POINTER(ctypes.c_int)]),
     "oracle_sincos": (None, [_fp, ctypes.c_size_t, _fp, _fp]),
+    "oracle_eval_ambiguous": (None, [ctypes.c_void_p, _fp, _fp, _fp,
+                                     ctypes.c_size_t, _u8]),
+    "oracle_eval_deriv_ambiguous": (None, [ctypes.c_void_p, _fp, _fp, _fp,
+                                           ctypes.c_size_t, _u8]),
+    "oracle_value_and_push": (ctypes.c_size_t, [ctypes.c_void_p, _fp, _fp, _fp,
+                                                _fp, ctypes.c_size_t, _fp, _fp]),
+    "oracle_interval_push_eval": (ctypes.c_size_t, [ctypes.c_void_p, _fp, _fp,
+                                                    _fp, _fp, _fp,
+                                                    ctypes.c_size_t, _fp]),
     "oracle_render_mesh": (ctypes.c_void_p, [ctypes.c_void_p, _dp, _dp,
                                              ctypes.c_double, ctypes.c_double]),
     "oracle_render_mesh_mt": (ctypes.c_void_p, [ctypes.c_void_p, _dp, _dp,
@@ -170,6 +179,42 @@ class OracleDeck:
                                        ctypes.byref(ch))
         return n, bool(ch.value), (float(ol[0]), float(oh[0]), bool(nan[0]))
 
+    def ambiguous(self, x, y, z):
+        """ArrayEvaluator::getAmbiguous per point (bool array)."""
+        x, y, z = _f32(x), _f32(y), _f32(z)
+        out = np.zeros(x.size, np.uint8)
+        lib().oracle_eval_ambiguous(self.ptr, _f(x), _f(y), _f(z), x.size,
+                                    out.ctypes.data_as(_u8))
+        return out.astype(bool)
+
+    def deriv_ambiguous(self, x, y, z):
+        """DerivArrayEvaluator::getAmbiguousDerivs per point (bool array)."""
+        x, y, z = _f32(x), _f32(y), _f32(z)
+        out = np.zeros(x.size, np.uint8)
+        lib().oracle_eval_deriv_ambiguous(self.ptr, _f(x), _f(y), _f(z),
+                                          x.size, out.ctypes.data_as(_u8))
+        return out.astype(bool)
+
+    def value_and_push(self, p, qx=(), qy=(), qz=()):
+        """ArrayEvaluator::valueAndPush at p: (value at p, pushed tape's
+        clause count, the pushed tape's values at the points q)."""
+        p = _f32(p)
+        qx, qy, qz = _f32(qx), _f32(qy), _f32(qz)
+        out = np.zeros(qx.size, np.float32)
+        val = np.zeros(1, np.float32)
+        n = lib().oracle_value_and_push(self.ptr, _f(p), _f(qx), _f(qy),
+                                        _f(qz), qx.size, _f(out), _f(val))
+        return float(val[0]), n, out
+
+    def interval_push_values(self, lo, hi, qx, qy, qz):
+        """intervalAndPush on [lo, hi]: (pushed tape's clause count, its
+        values at the points q)."""
+        lo, hi = _f32(lo), _f32(hi)
+        qx, qy, qz = _f32(qx), _f32(qy), _f32(qz)
+        out = np.zeros(qx.size, np.float32)
+        n = lib().oracle_interval_push_eval(self.ptr, _f(lo), _f(hi), _f(qx),
+                                            _f(qy), _f(qz), qx.size, _f(out))
+        return n, out
 
     def is_inside(self, p):
         p = _f32(p)
@@ -386,3 +431,33 @@ def vertex_error(verts, keys, ref):
     d = np.abs(np.asarray(verts)[1:][hit].astype(np.float64) - ref.verts64[j])
     err = float(d.max()) if d.size else 0.0
     return err, int(hit.sum()), len(a)
+
+
+def _canonical(verts, tris, keys):
+    """Vertices sorted by key (owning cell and patch) and triangles as
+    sorted rows of key ranks, winding kept within each row."""
+    k = _pack_keys(np.asarray(keys)[1:])
+    order = np.argsort(k, kind="stable")
+    rank = np.empty(len(k) + 1, np.int64)
+    rank[0] = -1
+    rank[1 + order] = np.arange(len(k))
+    t = rank[np.asarray(tris, np.int64)] if len(tris) else np.zeros((0, 3), np.int64)
+    t = t[np.lexsort(t.T[::-1])] if len(t) else t
+    v = np.ascontiguousarray(np.asarray(verts, np.float32)[1:][order])
+    return k[order], v.view(np.uint32), t
+
+
+def compare_meshes(verts, tris, keys, ref):
+    """Exact comparison of a mesh with the oracle's mesh `ref` through the
+    vertex keys (vertex numbering is free): same key set, bit-identical
+    positions per key, same triangles as ordered key triples.  Returns a
+    dict of booleans and counts."""
+    ka, va, ta = _canonical(verts, tris, keys)
+    kb, vb, tb = _canonical(ref.verts, ref.tris, ref.keys)
+    same_keys = ka.shape == kb.shape and bool((ka == kb).all())
+    out = {"verts": int(len(ka)), "tris": int(len(ta)), "same_vertex_keys": same_keys,
+           "bit_identical_vertices": False, "same_triangles": False}
+    if same_keys:
+        out["bit_identical_vertices"] = bool((va == vb).all())
+        out["same_triangles"] = ta.shape == tb.shape and bool((ta == tb).all())
+    return out

@@ -740,10 +740,16 @@ static double value_op_f64(uint32_t op, double a, double b) {
     }
 }
 
+// Scratch v holds one value per slot.  It is initialised (zeros, then the
+// constants) only when its size does not match the deck: clauses write
+// only their own slots, so constants and the dummy slot 0 keep their
+// values from one call to the next.
 double eval_point_f64(const Deck& deck, const Tape& tape, double x, double y,
                       double z, std::vector<double>& v) {
-    v.assign(deck.num_slots + 1, 0.0);
-    for (auto& c : deck.constants) v[c.first] = c.second;
+    if (v.size() != size_t(deck.num_slots) + 1) {
+        v.assign(deck.num_slots + 1, 0.0);
+        for (auto& c : deck.constants) v[c.first] = c.second;
+    }
     v[deck.X] = x;
     v[deck.Y] = y;
     v[deck.Z] = z;
@@ -794,15 +800,20 @@ static void deriv_op_f64(uint32_t op, double av, const double* ad, double bv,
 double eval_deriv_f64(const Deck& deck, const Tape& tape, double x, double y,
                       double z, std::vector<double>& v, double grad[3]) {
     const size_t n = deck.num_slots + 1;
-    v.assign(4 * n, 0.0);
-    double* g = v.data() + n;
-    for (auto& c : deck.constants) v[c.first] = c.second;
+    double* g;
+    if (v.size() != 4 * n) {
+        // first call with this scratch (see eval_point_f64)
+        v.assign(4 * n, 0.0);
+        g = v.data() + n;
+        for (auto& c : deck.constants) v[c.first] = c.second;
+        g[3 * deck.X] = 1;
+        g[3 * deck.Y + 1] = 1;
+        g[3 * deck.Z + 2] = 1;
+    }
+    g = v.data() + n;
     v[deck.X] = x;
     v[deck.Y] = y;
     v[deck.Z] = z;
-    g[3 * deck.X] = 1;
-    g[3 * deck.Y + 1] = 1;
-    g[3 * deck.Z + 2] = 1;
     for (auto it = tape.t.rbegin(); it != tape.t.rend(); ++it) {
         const double av = v[it->a], bv = v[it->b];
         const double ov = value_op_f64(it->op, av, bv);
@@ -917,6 +928,93 @@ size_t oracle_interval_push(oracle_deck* d, const float lo[3],
     if (out_hi) *out_hi = r.hi;
     if (out_nan) *out_nan = r.nan;
     if (changed) *changed = (t != d->deck.tape);
+    return t->t.size();
+}
+
+// Values of `tape` at n points, 256 at a time (ArrayEvaluator::values).
+static void eval_on_tape(Deck& deck, const Tape& tape, const float* x,
+                         const float* y, const float* z, size_t n, float* out) {
+    ArrayEval e(deck);
+    for (size_t s = 0; s < n; s += ArrayEval::N) {
+        const size_t cnt = std::min(ArrayEval::N, n - s);
+        for (size_t j = 0; j < cnt; ++j) e.set(x[s + j], y[s + j], z[s + j], j);
+        std::memcpy(out + s, e.values(cnt, tape), cnt * sizeof(float));
+    }
+}
+
+void oracle_eval_ambiguous(oracle_deck* d, const float* x, const float* y,
+                           const float* z, size_t n, uint8_t* out) {
+    ArrayEval e(d->deck);
+    const Tape& t = *d->deck.tape;
+    for (size_t s = 0; s < n; s += ArrayEval::N) {
+        const size_t cnt = std::min(ArrayEval::N, n - s);
+        for (size_t j = 0; j < cnt; ++j) e.set(x[s + j], y[s + j], z[s + j], j);
+        e.values(cnt, t);
+        e.ambiguous(cnt, t, out + s);
+    }
+}
+
+// DerivArrayEvaluator::getAmbiguousDerivs (eval_deriv_array.cpp:58-81): a
+// MIN/MAX clause with equal operand values and operand partials that
+// differ in at least one component.
+void oracle_eval_deriv_ambiguous(oracle_deck* d, const float* x,
+                                 const float* y, const float* z, size_t n,
+                                 uint8_t* out) {
+    DerivEval e(d->deck);
+    const Tape& t = *d->deck.tape;
+    for (size_t s = 0; s < n; s += ArrayEval::N) {
+        const size_t cnt = std::min(ArrayEval::N, n - s);
+        for (size_t j = 0; j < cnt; ++j) e.set(x[s + j], y[s + j], z[s + j], j);
+        e.derivs(cnt, t);
+        uint8_t* o = out + s;
+        std::fill(o, o + cnt, 0);
+        for (auto it = t.t.rbegin(); it != t.t.rend(); ++it) {
+            if (it->op != 19 && it->op != 20) continue;
+            for (size_t j = 0; j < cnt; ++j) {
+                bool differ = false;
+                for (int k = 0; k < 3; ++k) {
+                    differ |= e.gslot(it->a, k, j) != e.gslot(it->b, k, j);
+                }
+                o[j] |= (e.slot(it->a, j) == e.slot(it->b, j)) && differ;
+            }
+        }
+    }
+}
+
+// ArrayEvaluator::valueAndPush (eval_array.cpp:121-170): the value at p,
+// then a SPECIALIZED push that keeps only the strictly larger (MAX) or
+// smaller (MIN) operand; the pushed tape is evaluated at the nq points q.
+// Returns the pushed tape's clause count.
+size_t oracle_value_and_push(oracle_deck* d, const float p[3],
+                             const float* qx, const float* qy,
+                             const float* qz, size_t nq, float* out,
+                             float* value) {
+    Deck& deck = d->deck;
+    ArrayEval e(deck);
+    e.set(p[0], p[1], p[2], 0);
+    const float v0 = e.values(1, *deck.tape)[0];
+    auto t = oracle::push(deck, deck.tape,
+        [&](uint32_t op, uint32_t, uint32_t a, uint32_t b) -> Keep {
+            const float va = e.slot(a, 0), vb = e.slot(b, 0);
+            if (op == 20) return (va > vb) ? KEEP_A : (vb > va) ? KEEP_B : KEEP_BOTH;
+            if (op == 19) return (va > vb) ? KEEP_B : (vb > va) ? KEEP_A : KEEP_BOTH;
+            return KEEP_ALWAYS;
+        }, Tape::SPECIALIZED, nullptr);
+    if (value) *value = v0;
+    if (nq) eval_on_tape(deck, *t, qx, qy, qz, nq, out);
+    return t->t.size();
+}
+
+// IntervalEvaluator::intervalAndPush on [lo, hi], then the pushed tape's
+// values at the nq points q.  Returns the pushed tape's clause count.
+size_t oracle_interval_push_eval(oracle_deck* d, const float lo[3],
+                                 const float hi[3], const float* qx,
+                                 const float* qy, const float* qz, size_t nq,
+                                 float* out) {
+    IntervalEval e(d->deck);
+    e.eval(lo, hi, *d->deck.tape);
+    auto t = e.push(d->deck.tape);
+    if (nq) eval_on_tape(d->deck, *t, qx, qy, qz, nq, out);
     return t->t.size();
 }
 

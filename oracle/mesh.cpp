@@ -238,11 +238,14 @@ struct Mesher {
     FeatureEval fe;
     double max_err;
     // FP64 error oracle: values, partials and the |f(v)| collapse test are
-    // evaluated in double on the unpruned tape at double positions; points
-    // where the double value is exactly zero fall back to the float path
-    // (ambiguity, isInside and features stay in float).
+    // evaluated in double at double positions, on the same interval-pruned
+    // tape the float path uses for the point (a MIN/MAX decided by the
+    // intervals is decided for every real point of the box, and the pruned
+    // tape is the one the reference's evaluators see); points where the
+    // double value is exactly zero fall back to the float path (ambiguity,
+    // isInside and features stay in float).
     bool fp64 = false;
-    std::vector<double> sc;
+    std::vector<double> sc, sd;     // FP64 scratch: values, values + partials
     std::vector<std::array<float, 3>> verts{{0, 0, 0}};
     std::vector<std::array<double, 3>> verts64{{0, 0, 0}};
     std::vector<std::array<uint32_t, 3>> tris;
@@ -257,11 +260,11 @@ struct Mesher {
     Mesher(Deck& d, double me, bool f64 = false)
         : deck(d), ie(d), de(d), fe(d), max_err(me), fp64(f64) {}
 
-    double f64(const double* p) {
-        return eval_point_f64(deck, *deck.tape, p[0], p[1], p[2], sc);
+    double f64(const double* p, const Tape& t) {
+        return eval_point_f64(deck, t, p[0], p[1], p[2], sc);
     }
-    double d64(const double* p, double g[3]) {
-        return eval_deriv_f64(deck, *deck.tape, p[0], p[1], p[2], sc, g);
+    double d64(const double* p, double g[3], const Tape& t) {
+        return eval_deriv_f64(deck, t, p[0], p[1], p[2], sd, g);
     }
 
     void subdivide(Cell* c) {
@@ -366,7 +369,7 @@ struct Mesher {
             for (int i = 0; i < 8; ++i) {
                 double pd[3];
                 for (int j = 0; j < 3; ++j) pd[j] = (i & (1 << j)) ? c->hi[j] : c->lo[j];
-                const double v = f64(pd);
+                const double v = f64(pd, *tape);
                 if (v > 0 || std::isnan(v)) corners[i] = EMPTY;
                 else if (v < 0) corners[i] = FILLED;
             }
@@ -420,7 +423,7 @@ struct Mesher {
                 }
                 double out[12 * 16];
                 if (fp64) {
-                    for (int i = 0; i < 16 * ec; ++i) out[i] = f64(ps[i]);
+                    for (int i = 0; i < 16 * ec; ++i) out[i] = f64(ps[i], *tape);
                 } else {
                     const float* o = de.values(16 * ec, *tape);
                     for (int i = 0; i < 16 * ec; ++i) out[i] = o[i];
@@ -458,7 +461,7 @@ struct Mesher {
                 ds[i][1] = de.gslot(tape->i, 1, i);
                 ds[i][2] = de.gslot(tape->i, 2, i);
                 ds[i][3] = de.slot(tape->i, i);
-                if (fp64) ds[i][3] = d64((i & 1) ? ts[i / 2] : tf[i / 2], ds[i]);
+                if (fp64) ds[i][3] = d64((i & 1) ? ts[i / 2] : tf[i / 2], ds[i], *tape);
             }
             uint8_t amb[24];
             de.ambiguous(2 * ec, *tape, amb);
@@ -610,7 +613,8 @@ struct Mesher {
             const double vd[3] = {vf[0], vf[1], vf[2]};
             auto t = get_base(tape, vd, vd);
             de.set(vf[0], vf[1], vf[2], 0);
-            const double fv = fp64 ? f64(v) : double(de.values(1, *t)[0]);
+            const double fv = fp64 ? f64(v, *get_base(tape, v, v))
+                                   : double(de.values(1, *t)[0]);
             if (std::fabs(fv) < max_err) ok = true;
         }
         if (ok) {

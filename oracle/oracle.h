@@ -54,6 +54,28 @@ size_t oracle_interval_push(oracle_deck* d, const float lo[3],
                             const float hi[3], float* out_lo, float* out_hi,
                             uint8_t* out_nan, int* changed);
 
+/* ArrayEvaluator::getAmbiguous: out[i] = 1 where a MIN/MAX clause sees equal
+ * operand values at point i. */
+void oracle_eval_ambiguous(oracle_deck* d, const float* x, const float* y,
+                           const float* z, size_t n, uint8_t* out);
+/* DerivArrayEvaluator::getAmbiguousDerivs: equal operand values and
+ * different operand partials. */
+void oracle_eval_deriv_ambiguous(oracle_deck* d, const float* x,
+                                 const float* y, const float* z, size_t n,
+                                 uint8_t* out);
+/* ArrayEvaluator::valueAndPush at p; the pushed (SPECIALIZED) tape is then
+ * evaluated at the nq points q.  Returns the pushed tape's clause count. */
+size_t oracle_value_and_push(oracle_deck* d, const float p[3],
+                             const float* qx, const float* qy,
+                             const float* qz, size_t nq, float* out,
+                             float* value);
+/* intervalAndPush on [lo, hi], then the pushed tape evaluated at the nq
+ * points q.  Returns the pushed tape's clause count. */
+size_t oracle_interval_push_eval(oracle_deck* d, const float lo[3],
+                                 const float hi[3], const float* qx,
+                                 const float* qy, const float* qz, size_t nq,
+                                 float* out);
+
 /* Single-precision sin/cos routine shared with the product (for the
  * accuracy tests against float64 references). */
 void oracle_sincos(const float* x, size_t n, float* s, float* c);

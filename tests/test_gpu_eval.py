@@ -140,6 +140,54 @@ def test_intervals_match_oracle(gpu, name):
     assert_same_bits(gh, rh, name + ":upper")
 
 
+def lattice_points(n, lo=-1.5, hi=1.5):
+    """An n^3 lattice that lands exactly on the CSG shapes' planes, corners
+    and tie surfaces (the points where MIN/MAX operands compare equal)."""
+    t = np.linspace(lo, hi, n).astype(np.float32)
+    x, y, z = np.meshgrid(t, t, t, indexing="ij")
+    return x.ravel(), y.ravel(), z.ravel()
+
+
+AMBIG_SHAPES = {
+    "min_x_negx": lambda: X().min(-X()),
+    "deriv_known_answer": lambda: X().min(Y()).min(X().min(4 + 0.2 * Y())),
+    "equal_partials": lambda: X().min(X() * 1.0 + 0.0 * Y()),
+    "c1_csg": lambda: W.csg_c1(),
+    "box": lambda: W.box((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5)),
+    "spheres50": lambda: W.sphere_union(50, seed=3),
+    "c2_dag": lambda: W.random_dag(),
+}
+
+
+@pytest.mark.parametrize("name", sorted(AMBIG_SHAPES))
+@pytest.mark.parametrize("derivs", [False, True])
+def test_ambiguity_flags_match_oracle(gpu, name, derivs):
+    """Device getAmbiguous / getAmbiguousDerivs flags equal the oracle's,
+    point by point, on a lattice through the shapes' ties and on random
+    points."""
+    s = AMBIG_SHAPES[name]()
+    lx, ly, lz = lattice_points(33)
+    r = points(4000, 17)
+    p = [np.concatenate([a, b]) for a, b in zip((lx, ly, lz), r)]
+    got = s.ambiguous(*p, derivs=derivs)
+    o = oracle.OracleDeck(s.deck())
+    ref = o.deriv_ambiguous(*p) if derivs else o.ambiguous(*p)
+    bad = np.flatnonzero(got != ref)
+    assert bad.size == 0, (name, bad[:8], [q[bad[:4]] for q in p])
+    if name in ("min_x_negx", "c1_csg", "box", "deriv_known_answer"):
+        assert ref.any()        # the lattice does meet ties
+
+
+def test_ambiguity_known_answers_gpu(gpu):
+    """libfive/test/eval_array.cpp:203-221 and
+    libfive/test/eval_deriv_array.cpp:94-111 on the device."""
+    a = X().min(-X()).ambiguous([0, 1, 2, 0], [0] * 4, [0] * 4)
+    assert a.tolist() == [True, False, False, True]
+    s = X().min(Y()).min(X().min(4 + 0.2 * Y()))
+    d = s.ambiguous([0, 0, 5, 0], [0, 1, 5, 5], [0] * 4, derivs=True)
+    assert d.tolist() == [True, False, True, False]
+
+
 def test_grid_c2_exact_and_enclosed(gpu):
     """C2 DAG on a 64^3 grid (the bench runs 256^3): bit-exact vs the
     oracle, including sin/cos."""

@@ -25,7 +25,7 @@ def canonical(verts, tris, keys):
     return vmap, tri
 
 
-def compare(shape, lo, hi, mf, max_err=1e-8):
+def compare(shape, lo, hi, mf, max_err=1e-8, threads=1):
     # The C API takes the region as floats (libfive_region3), so the oracle
     # gets the same float-rounded bounds.
     lo3 = [float(np.float32(lo))] * 3
@@ -33,7 +33,8 @@ def compare(shape, lo, hi, mf, max_err=1e-8):
     print(f"mesh parity: lo={lo} hi={hi} min_feature={mf}", flush=True)
     g = shape.render_mesh(lo3, hi3, min_feature=mf, max_err=max_err,
                           details=True)
-    o = oracle.OracleDeck(shape.deck()).render_mesh(lo3, hi3, mf, max_err)
+    o = oracle.OracleDeck(shape.deck()).render_mesh(lo3, hi3, mf, max_err,
+                                                    threads=threads)
     assert g.level_counts.shape == o.level_counts.shape
     assert (g.level_counts == o.level_counts).all()
     # identical interval / leaf classification, cell by cell
@@ -79,6 +80,19 @@ def test_gyroid_matches_oracle(gpu, res):
 
 def test_sphere_union_matches_oracle(gpu):
     compare(W.sphere_union(n=60), -1.2, 1.2, 2.4 / 64)
+
+
+def test_c3_sphere_union_depth8_matches_oracle(gpu):
+    """The full interval-pruning workload: the seeded 1000-sphere min-union
+    (about 11k clauses) meshed to depth 8 over [-1.2, 1.2]^3.  Per-cell
+    states at every level, the multiset of pruned tape lengths (base tape
+    of 10 999 clauses, pruned tapes down to a few clauses), keyed vertices
+    and triangles all equal the oracle mesher's."""
+    s = W.sphere_union()
+    g = compare(s, -1.2, 1.2, 2.4 / 256, threads=16)
+    assert g.level_counts.shape[0] == 9
+    assert max(g.tape_lengths.tolist()) > 5000
+    assert min(g.tape_lengths.tolist()) < 20
 
 
 def test_wide_register_file_matches_oracle(gpu):

@@ -11,7 +11,8 @@ import oracle
 from libfive_amd import Shape, X, Y, Z
 from libfive_amd import workloads as W
 
-from helpers import BINARY, UNARY, assert_same_bits, op_shape, ulp_distance
+from helpers import (BINARY, UNARY, assert_same_bits, op_shape, same_bits,
+                     ulp_distance)
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "ieee_ops.npz")
 
@@ -239,16 +240,134 @@ def test_interval_encloses_point_values(build):
 
 def test_pushed_tape_is_exact_inside_box():
     """A tape pruned on a box gives bit-identical values inside the box
-    (no NaN in the shape): the pruning invariant the mesher relies on."""
+    (no NaN in the shape): the pruning invariant the mesher relies on.
+    Points are drawn inside each box, on its faces and at its corners."""
     s = W.sphere_union(50, seed=99)
-    deck = s.deck()
-    d = oracle.OracleDeck(deck)
+    d = oracle.OracleDeck(s.deck())
     rng = np.random.default_rng(3)
-    for _ in range(20):
+    pruned = 0
+    for _ in range(40):
         c = rng.uniform(-1, 1, 3).astype(np.float32)
-        lo, hi = c - 0.1, c + 0.1
-        n, changed, _ = d.interval_push(lo, hi)
+        lo = (c - np.float32(0.1)).astype(np.float32)
+        hi = (c + np.float32(0.1)).astype(np.float32)
+        u = rng.uniform(0, 1, (3, 600)).astype(np.float32)
+        u[:, :8] = np.array([[(k >> a) & 1 for k in range(8)] for a in range(3)])
+        u[0, 8:100] = rng.integers(0, 2, 92)           # on x faces
+        q = (lo[:, None] + u * (hi - lo)[:, None]).astype(np.float32)
+        q = np.minimum(np.maximum(q, lo[:, None]), hi[:, None])
+        n, got = d.interval_push_values(lo, hi, *q)
         assert n <= d.n_clauses
+        pruned += n < d.n_clauses
+        assert_same_bits(got, d.values(*q), "pruned vs full tape")
+    assert pruned > 20
+
+
+def test_pushed_tape_differs_outside_box():
+    """The check above would pass for a push that prunes nothing: a pruned
+    tape must give different values somewhere outside its box."""
+    s = W.sphere_union(50, seed=99)
+    d = oracle.OracleDeck(s.deck())
+    lo = np.full(3, -0.05, np.float32)
+    hi = np.full(3, 0.05, np.float32)
+    rng = np.random.default_rng(4)
+    q = rng.uniform(-1, 1, (3, 2000)).astype(np.float32)
+    n, got = d.interval_push_values(lo, hi, *q)
+    assert n < d.n_clauses
+    assert not same_bits(got, d.values(*q)).all()
+
+
+def test_get_ambiguous_known_answer():
+    """libfive/test/eval_array.cpp:203-221: min(X, -X) at x = 0, 1, 2, 0 is
+    ambiguous at the two zeros only."""
+    d = od(X().min(-X()))
+    amb = d.ambiguous([0, 1, 2, 0], [0] * 4, [0] * 4)
+    assert amb.tolist() == [True, False, False, True]
+    assert int(amb[:3].sum()) == 1
+
+
+def test_get_ambiguous_is_ieee_equality():
+    """Equal operands only: NaN operands are never ambiguous, +0 and -0
+    are (IEEE equality, eval_array.cpp:211-213)."""
+    x, y = X(), Y()
+    d = od(x.min(y))
+    amb = d.ambiguous([np.nan, 0.0, -0.0, 1.0, 1.0],
+                      [np.nan, -0.0, 0.0, np.nextafter(np.float32(1), 2), 1.0],
+                      [0.0] * 5)
+    assert amb.tolist() == [False, True, True, False, True]
+
+
+def test_get_ambiguous_derivs_known_answer():
+    """libfive/test/eval_deriv_array.cpp:94-111: ambiguous where two MIN
+    operands are equal with different partials."""
+    x, y = X(), Y()
+    d = od(x.min(y).min(x.min(4 + 0.2 * y)))
+    amb = d.deriv_ambiguous([0, 0, 5, 0], [0, 1, 5, 5], [0] * 4)
+    assert amb.tolist() == [True, False, True, False]
+    # equal values with equal partials are not ambiguous for derivatives,
+    # though they are for values
+    e = od(x.min(x * 1.0 + 0.0 * y))
+    assert e.deriv_ambiguous([1.0], [2.0], [0.0]).tolist() == [False]
+
+
+def test_value_and_push_known_answer():
+    """libfive/test/eval_array.cpp:234-253: valueAndPush specialises
+    min(X, Y) to X at (-1, 0, 0) and to Y at (0, -1, 0)."""
+    d = od(X().min(Y()))
+    qx, qy = [-2, 4, 4, 10], [0, 0, 5, 5]
+    v, n, got = d.value_and_push([-1, 0, 0], qx, qy, [0] * 4)
+    assert v == -1 and n == 0
+    assert got.tolist() == [-2, 4, 4, 10]
+    v, n, got = d.value_and_push([0, -1, 0], qx, qy, [0] * 4)
+    assert v == -1 and n == 0
+    assert got.tolist() == [0, 0, 5, 5]
+
+
+def test_value_and_push_keeps_ties():
+    """A SPECIALIZED push keeps both operands of a tie (neither strictly
+    greater), and of a NaN comparison: the tape is unchanged."""
+    d = od(X().max(Y()) + Z())
+    full = d.n_clauses
+    _, n, _ = d.value_and_push([1, 1, 0])
+    assert n == full
+    _, n, _ = d.value_and_push([np.nan, 1, 0])
+    assert n == full
+    _, n, got = d.value_and_push([2, 1, 0], [5, -3], [0, 0], [1, 1])
+    assert n < full and got.tolist() == [6, -2]
+
+
+def _minmax_clause(shape):
+    """(opcode, a slot, b slot) of the deck's only MIN/MAX clause."""
+    clauses = np.asarray(shape.deck()[0])
+    mm = [c for c in clauses.tolist() if c[0] in (19, 20)]
+    assert len(mm) == 1
+    return mm[0][0], mm[0][2], mm[0][3]
+
+
+@pytest.mark.parametrize("op", ["min", "max"])
+def test_minmax_tie_side_partials(op):
+    """At a tie the partials come from b for MIN and from a for MAX
+    (eval_deriv_array.cpp:151-162: MIN (a < b) ? a' : b', MAX (a < b) ?
+    b' : a'), and the value from a for both (std::min, std::max).  The
+    operands x and 2x - 1 meet at x = 1 with slopes 1 and 2, so each side
+    is identified by its partial; which tape operand is a is read off the
+    deck."""
+    x = X()
+    s = Shape.binary(op, x, 2 * x - 1)
+    code, a, b = _minmax_clause(s)
+    deck = s.deck()
+    xs = deck[2][0]
+    slope = {xs: 1.0}
+    other = b if a == xs else a
+    slope[other] = 2.0
+    v, dx, dy, dz = od(s).derivs([1.0], [0.0], [0.0])
+    want = slope[b] if code == 19 else slope[a]
+    assert v[0] == 1.0 and dx[0] == want and dy[0] == 0 and dz[0] == 0
+    # away from the tie: the smaller (MIN) / larger (MAX) operand's slope
+    v, dx, _, _ = od(s).derivs([0.5, 1.5], [0, 0], [0, 0])
+    if op == "min":
+        assert dx.tolist() == [2.0, 1.0]
+    else:
+        assert dx.tolist() == [1.0, 2.0]
 
 
 def test_f64_twin_close_to_f32():

@@ -208,6 +208,16 @@ int libfive_tape_eval_derivs(libfive_tape t, const float* x, const float* y,
                              const float* z, size_t n, float* value, float* dx,
                              float* dy, float* dz);
 
+/* libfive_tape_eval_values without any device allocation: no counterpart
+ * in the reference; for callers that cannot allocate device memory (code
+ * under graph capture, tight serving loops).  The points are staged
+ * through a fixed 16 MiB device array that the library holds on each GPU
+ * until the process ends, so n is at most 1048576 (larger n fails with an
+ * error and nothing is evaluated).  Calls serialise on the library's lock;
+ * the array keeps the last call's contents.  Returns 0 on success. */
+int libfive_tape_eval_values_noalloc(libfive_tape t, const float* x,
+                                     const float* y, const float* z, size_t n,
+                                     float* out);
 
 /* Mesh output forms (libfive.h:369-410, libfive.cpp:354-440): the same
  * mesh as libfive_tree_render_mesh, as -1-separated coordinate indices, or

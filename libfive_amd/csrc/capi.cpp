@@ -445,6 +445,46 @@ int libfive_tape_eval_values(libfive_tape t, const float* x, const float* y,
     }
 }
 
+int libfive_tape_eval_values_noalloc(libfive_tape t, const float* x,
+                                     const float* y, const float* z, size_t n,
+                                     float* out) {
+    try {
+        Context& c = context();
+        if (n > kNoAllocMaxPoints) {
+            throw Error("at most " + std::to_string(kNoAllocMaxPoints) +
+                        " points per call, got " + std::to_string(n));
+        }
+        if (!c.scratch) throw Error("no staging memory (" + c.scratch_error + ")");
+        if (n == 0) return 0;
+        // Calls serialise on the library lock: the staging array is shared
+        // by every call on this GPU and keeps the last call's contents.
+        std::lock_guard<std::mutex> lock(c.mu);
+        const size_t b = n * sizeof(float);
+        float* px = c.scratch;
+        float* py = px + kNoAllocMaxPoints;
+        float* pz = py + kNoAllocMaxPoints;
+        float* po = pz + kNoAllocMaxPoints;
+        hip_check(hipMemcpyAsync(px, x, b, hipMemcpyHostToDevice, c.stream), "copy x");
+        hip_check(hipMemcpyAsync(py, y, b, hipMemcpyHostToDevice, c.stream), "copy y");
+        hip_check(hipMemcpyAsync(pz, z, b, hipMemcpyHostToDevice, c.stream), "copy z");
+        const auto& h = t->t->host;
+        if (t->spec && t->use_spec) {
+            hip_check(launch_specialized_values(*t->spec, px, py, pz, po, n, c.stream),
+                      "eval_values (specialised)");
+        } else {
+            hip_check(launch_eval_values(t->t->d.p, t->t->size(), h.root_reg,
+                                         h.num_regs, px, py, pz, po, n, c.stream),
+                      "eval_values");
+        }
+        hip_check(hipMemcpyAsync(out, po, b, hipMemcpyDeviceToHost, c.stream), "copy out");
+        hip_check(hipStreamSynchronize(c.stream), "tape eval sync");
+        return 0;
+    } catch (const std::exception& e) {
+        set_error("libfive_tape_eval_values_noalloc", e);
+        return -1;
+    }
+}
+
 int libfive_tape_eval_derivs(libfive_tape t, const float* x, const float* y,
                              const float* z, size_t n, float* value, float* dx,
                              float* dy, float* dz) {

@@ -384,6 +384,21 @@ hipError_t launch_eval_ambiguous(const Clause* t, uint32_t n, uint32_t root,
     return hipGetLastError();
 }
 
+// Staging memory of the allocation-free value entry point
+// (libfive_tape_eval_values_noalloc): x, y, z and the values, each
+// kNoAllocMaxPoints floats, 16 MiB in all, held by every GPU the library
+// runs on until the process ends.  Host code never passes this name to a
+// kernel: it stands for a host-side shadow, not the device address, which
+// comes from noalloc_scratch_address() once per device at initialisation.
+static __device__ __align__(256) float scratch[4 * kNoAllocMaxPoints];
+
+hipError_t noalloc_scratch_address(float** out) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    *out = (e == hipSuccess) ? static_cast<float*>(p) : nullptr;
+    return e;
+}
+
 int max_value_regs() { return 8 * 32; }
 int max_deriv_regs() { return 4 * 32; }
 

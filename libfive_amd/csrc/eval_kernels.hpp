@@ -29,6 +29,12 @@ hipError_t launch_eval_ambiguous(const tape::Clause* t, uint32_t n, uint32_t roo
                                  const float* y, const float* z, uint8_t* out,
                                  size_t npts, hipStream_t s);
 
+// Largest point count of one libfive_tape_eval_values_noalloc call, and
+// the device address of its staging array (x, y, z, values at strides of
+// kNoAllocMaxPoints floats) on the current device.
+constexpr size_t kNoAllocMaxPoints = size_t(1) << 20;
+hipError_t noalloc_scratch_address(float** out);
+
 // Points per lane for the value kernel (0: automatic).
 void set_points_per_lane(int p);
 // Largest register count each interpreter keeps in VGPRs.

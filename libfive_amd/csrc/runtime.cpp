@@ -167,6 +167,14 @@ static Context* make_context() {
     hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->h_small),
                             Context::kSmall * sizeof(float),
                             hipHostMallocDefault), "hipHostMalloc");
+    // With c->device current: the per-device address of the static staging
+    // array used by libfive_tape_eval_values_noalloc.
+    const hipError_t se = noalloc_scratch_address(&c->scratch);
+    if (se != hipSuccess) {
+        c->scratch = nullptr;
+        c->scratch_error = std::string("hipGetSymbolAddress: ") + hipGetErrorString(se);
+        (void)hipGetLastError();
+    }
     return c;
 }
 

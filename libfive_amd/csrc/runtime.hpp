@@ -30,6 +30,11 @@ struct Context {
     std::mutex mu;
     float* d_small = nullptr;     // device staging for single-point calls
     float* h_small = nullptr;     // pinned host staging
+    // Device address of the allocation-free entry point's staging array
+    // on this context's device (nullptr if it could not be looked up, with
+    // the reason in scratch_error).
+    float* scratch = nullptr;
+    std::string scratch_error;
     static constexpr size_t kSmall = 64;
 };
 

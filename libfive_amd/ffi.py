@@ -158,6 +158,8 @@ SIGNATURES = {
                                          ctypes.c_size_t, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p]),
+    "libfive_tape_eval_values_noalloc": (ctypes.c_int, [tape_t, fp, fp, fp,
+                                                        ctypes.c_size_t, fp]),
     "libfive_tape_eval_values": (ctypes.c_int, [tape_t, fp, fp, fp,
                                                 ctypes.c_size_t, fp]),
     "libfive_tape_eval_derivs": (ctypes.c_int, [tape_t, fp, fp, fp,

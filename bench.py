@@ -34,8 +34,9 @@ sys.path.insert(0, ROOT)
 # Nominal FLOPs per point for each opcode (value pass / value+gradient
 # pass), fixed before any tuning.  sin and cos count W_TRIG = 20, a fixed
 # nominal weight for one range reduction and one polynomial evaluation; it
-# is not recounted when the implementation changes (the series now have 6
-# and 7 terms).
+# is not recounted when the implementation changes (the single-precision
+# routine now evaluates polynomials of 3 and 4 terms after a three-step
+# reduction).
 W_TRIG = 20
 VALUE_FLOPS = {7: 1, 8: 1, 9: 1, 10: W_TRIG, 11: W_TRIG, 17: 1, 18: 1,
                19: 1, 20: 1, 21: 1, 22: 1, 28: 1}

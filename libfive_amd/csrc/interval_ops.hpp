@@ -261,8 +261,8 @@ LF_DEV Iv iv_abs(Iv a) {
 }
 
 // cos_down / cos_up of rounded_transc_std, substituted as described above.
-LF_DEV float cos_dn(float x) { return fmax_std(-1.0f, rnd::dn(fmath::cos_f32(x))); }
-LF_DEV float cos_up(float x) { return fmin_std(1.0f, rnd::up(fmath::cos_f32(x))); }
+LF_DEV float cos_dn(float x) { return fmax_std(-1.0f, rnd::dn(fmath::cos_f32_accurate(x))); }
+LF_DEV float cos_up(float x) { return fmin_std(1.0f, rnd::up(fmath::cos_f32_accurate(x))); }
 
 // Boost cos(interval) (transc.hpp): reduce by fmod against [2pi], split
 // into monotone pieces; "tmp.lower() >= pi_upper" recurses on tmp - pi

@@ -812,6 +812,426 @@ size_t leaf_feature_workspace(uint32_t max_len, uint32_t nregs) {
 }
 
 ////////////////////////////////////////////////////////////////////////////////
+// Phase-split leaf pass (see mesh_kernels.hpp)
+
+// The children of a level-1 parent tile its box as a 2 x 2 x 2 block; their
+// corners are the 27 points of a lattice, point (x, y, z) (each 0..2) with
+// index x + 3y + 9z, at coordinate lower / midpoint / upper of the parent
+// along each axis (the midpoint is the one Region::subdivide computes, so
+// the coordinates are the children's own bounds).  A lattice edge joins a
+// point and its neighbour along one axis: id = 18 axis + 9 c + 3 o1 + o2,
+// with c (0 or 1) the lower point's coordinate along the axis and o1, o2
+// the other two coordinates in x, y, z order.
+__device__ __forceinline__ int lattice_point(uint32_t oct, int corner) {
+    const int x = int(oct & 1u) + (corner & 1);
+    const int y = int((oct >> 1) & 1u) + ((corner >> 1) & 1);
+    const int z = int((oct >> 2) & 1u) + ((corner >> 2) & 1);
+    return x + 3 * y + 9 * z;
+}
+
+__device__ __forceinline__ int lattice_edge(int pa, int pb) {
+    const int lo = pa < pb ? pa : pb;
+    const int d = pa < pb ? pb - pa : pa - pb;       // 1, 3 or 9
+    const int axis = (d == 1) ? 0 : (d == 3) ? 1 : 2;
+    const int x = lo % 3, y = (lo / 3) % 3, z = lo / 9;
+    const int c = (axis == 0) ? x : (axis == 1) ? y : z;
+    const int o1 = (axis == 0) ? y : x;
+    const int o2 = (axis == 2) ? y : z;
+    return axis * 18 + c * 9 + o1 * 3 + o2;
+}
+
+// The lattice points of edge `id`, lower first.
+__device__ __forceinline__ void edge_points(int id, int* pa, int* pb) {
+    const int axis = id / 18, r = id % 18;
+    const int c = r / 9, o1 = (r / 3) % 3, o2 = r % 3;
+    int x, y, z;
+    if (axis == 0) { x = c; y = o1; z = o2; }
+    else if (axis == 1) { x = o1; y = c; z = o2; }
+    else { x = o1; y = o2; z = c; }
+    *pa = x + 3 * y + 9 * z;
+    *pb = *pa + ((axis == 0) ? 1 : (axis == 1) ? 3 : 9);
+}
+
+__device__ __forceinline__ double lattice_coord(const double lo[3], const double hi[3],
+                                                int a, int i) {
+    return (i == 0) ? lo[a] : (i == 2) ? hi[a] : (lo[a] + hi[a]) / 2;
+}
+
+__device__ __forceinline__ void lattice_pos(const double lo[3], const double hi[3],
+                                            int li, double out[3]) {
+    out[0] = lattice_coord(lo, hi, 0, li % 3);
+    out[1] = lattice_coord(lo, hi, 1, (li / 3) % 3);
+    out[2] = lattice_coord(lo, hi, 2, li / 9);
+}
+
+// Lists the parent's 8 cells for the feature pass, once.
+__device__ __forceinline__ void defer_group(const Params& P, LeafGroupRec* grec,
+                                            uint32_t g, uint32_t* deferred) {
+    if ((atomicOr(&grec[g].flags, 1u) & 1u) == 0u) {
+        const uint32_t b = atomicAdd(&P.counters->deferred, 8u);
+        for (uint32_t k = 0; k < 8u; ++k) deferred[b + k] = 8u * g + k;
+    }
+}
+
+// Corners (dc_tree.inl:131-239 for the 8 children at once).
+template <int NV>
+__global__ __launch_bounds__(64)
+void k_leaf_corners(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
+                    LevelView l1, LevelView l0, LeafGroupRec* __restrict__ grec,
+                    uint64_t* __restrict__ tasks, uint32_t* __restrict__ leaves,
+                    uint32_t* __restrict__ deferred)
+{
+    const int lane = threadIdx.x;
+    const uint32_t g = blockIdx.x;
+    if (g >= ngroups) return;
+    const uint32_t p = groups[g];
+    const uint32_t tape_id = __builtin_amdgcn_readfirstlane(l1.tape[p]);
+    const TapeRec tr = P.tapes[tape_id];
+    const uint32_t n = __builtin_amdgcn_readfirstlane(tr.len);
+    const Clause* __restrict__ t = P.pool + __builtin_amdgcn_readfirstlane(tr.off);
+    const uint32_t root = P.root_reg;
+    uint32_t pkey[3];
+    for (int a = 0; a < 3; ++a) pkey[a] = l1.key[3 * p + a];
+    double plo[3], phi[3];
+    cell_bounds(P, pkey, 1, plo, phi);
+
+    // Lanes 0..26: one lattice point each.
+    const bool pt = lane < 27;
+    double pd[3];
+    lattice_pos(plo, phi, pt ? lane : 0, pd);
+    const float cp[3] = {float(pd[0]), float(pd[1]), float(pd[2])};
+    uint32_t fl;
+    const float v = pe_value<NV>(t, n, root, cp[0], cp[1], cp[2], &fl);
+    float gr[3] = {0.0f, 0.0f, 0.0f};
+    if (ballot(pt && v == 0.0f && !(fl & PT_AMBIGUOUS))) {
+        uint32_t fl2;
+        (void)pe_grad3<NV>(t, n, root, cp[0], cp[1], cp[2], gr, &fl2);
+    }
+    bool inside;
+    if (v > 0.0f || fmath::isnan_f(v)) inside = false;
+    else if (v < 0.0f) inside = true;
+    else inside = (gr[0] != 0.0f || gr[1] != 0.0f || gr[2] != 0.0f);
+    const uint32_t in27 = uint32_t(ballot(pt && inside)) & 0x7ffffffu;
+    if (ballot(pt && v == 0.0f && (fl & PT_AMBIGUOUS))) {
+        // FeatureEvaluator::isInside needed: the feature pass does the
+        // parent's 8 cells.
+        if (lane == 0) {
+            LeafGroupRec r;
+            r.edge_mask = 0;
+            r.edge_base = 0;
+            r.inside = in27;
+            r.flags = 0;
+            r.pad = 0;
+            grec[g] = r;
+            __threadfence();
+            defer_group(P, grec, g, deferred);
+        }
+        return;
+    }
+
+    // Lanes 0..7: one child each.
+    const uint32_t oct = uint32_t(lane & 7);
+    const uint32_t first = uint32_t(l1.child[p]);
+    uint32_t mask = 0;
+    for (int c = 0; c < 8; ++c) mask |= ((in27 >> lattice_point(oct, c)) & 1u) << c;
+    const bool amb = lane < 8 && mask != 0u && mask != 0xffu;
+    uint64_t emask = 0;
+    if (lane < 8) {
+        const uint32_t cidx = first + oct;
+        l0.child[cidx] = -1;
+        l0.tape[cidx] = tape_id;
+        for (int a = 0; a < 3; ++a) l0.key[3 * cidx + a] = pkey[a] + ((oct >> a) & 1u);
+        if (!amb) {
+            l0.type[cidx] = mask ? FILLED : EMPTY;
+        } else {
+            l0.type[cidx] = AMBIGUOUS;
+            LeafRec* leaf = l0.leaf + cidx;
+            leaf->mask = mask;
+            leaf->flags = c_manifold[mask] ? 1u : 0u;
+            for (int patch = 0; patch < 4; ++patch) {
+                if (mv(mask, patch, 0, 0) == -1) break;
+                for (int k = 0; k < 12; ++k) {
+                    const int a = mv(mask, patch, k, 0);
+                    if (a == -1) break;
+                    const int b = mv(mask, patch, k, 1);
+                    emask |= uint64_t(1) << lattice_edge(lattice_point(oct, a),
+                                                         lattice_point(oct, b));
+                }
+            }
+        }
+    }
+    // OR over the 8 child lanes.
+    for (int d = 1; d < 8; d <<= 1) {
+        const uint32_t lo = __shfl_xor(uint32_t(emask), d);
+        const uint32_t hi = __shfl_xor(uint32_t(emask >> 32), d);
+        emask |= uint64_t(lo) | (uint64_t(hi) << 32);
+    }
+    const uint32_t ambm = uint32_t(ballot(amb)) & 0xffu;
+    uint32_t lbase = 0;
+    if (lane == 0) {
+        const uint32_t ne = uint32_t(__popcll(emask));
+        LeafGroupRec r;
+        r.edge_mask = emask;
+        r.edge_base = ne ? atomicAdd(&P.counters->edges, ne) : 0u;
+        r.inside = in27;
+        r.flags = 0;
+        r.pad = 0;
+        grec[g] = r;
+        const uint32_t nt = (ne + 3u) / 4u;
+        if (nt) {
+            const uint32_t tb = atomicAdd(&P.counters->tasks, nt);
+            for (uint32_t j = 0; j < nt; ++j) tasks[tb + j] = (uint64_t(g) << 32) | j;
+        }
+        if (ambm) lbase = atomicAdd(&P.counters->leaves, uint32_t(__popc(ambm)));
+    }
+    lbase = __shfl(lbase, 0);
+    if (amb) leaves[lbase + uint32_t(__popc(ambm & ((1u << lane) - 1u)))] = 8u * g + oct;
+}
+
+// Edge search, partials and per-edge QEF (dc_tree.inl:339-502 for one
+// lattice edge, shared by the children whose patches cross it).
+template <int NV>
+__global__ __launch_bounds__(64)
+void k_leaf_edges(Params P, const uint32_t* __restrict__ groups, LevelView l1,
+                  LeafGroupRec* __restrict__ grec, const uint64_t* __restrict__ tasks,
+                  uint32_t ntasks, EdgeRec* __restrict__ edges,
+                  uint32_t* __restrict__ deferred)
+{
+    __shared__ double s_tf[4][3], s_ts[4][3];
+    __shared__ float s_v[8], s_g[8][3];
+    __shared__ uint32_t s_fl[8];
+    const int lane = threadIdx.x;
+    if (blockIdx.x >= ntasks) return;
+    const uint64_t task = tasks[blockIdx.x];
+    const uint32_t g = uint32_t(task >> 32), j = uint32_t(task & 0xffffffffu);
+    const uint64_t emask = grec[g].edge_mask;
+    const uint32_t ebase = grec[g].edge_base;
+    const uint32_t inside = grec[g].inside;
+    const uint32_t p = groups[g];
+    const uint32_t tape_id = __builtin_amdgcn_readfirstlane(l1.tape[p]);
+    const TapeRec tr = P.tapes[tape_id];
+    const uint32_t n = __builtin_amdgcn_readfirstlane(tr.len);
+    const Clause* __restrict__ t = P.pool + __builtin_amdgcn_readfirstlane(tr.off);
+    const uint32_t root = P.root_reg;
+    uint32_t pkey[3];
+    for (int a = 0; a < 3; ++a) pkey[a] = l1.key[3 * p + a];
+    double plo[3], phi[3];
+    cell_bounds(P, pkey, 1, plo, phi);
+
+    const uint32_t ne = uint32_t(__popcll(emask));
+    const int e = lane >> 4, smp = lane & 15;
+    const uint32_t r = 4u * j + uint32_t(e);
+    const bool valid = r < ne;
+    if (smp == 0) {
+        double a[3] = {0.0, 0.0, 0.0}, b[3] = {0.0, 0.0, 0.0};
+        if (valid) {
+            uint64_t m = emask;
+            for (uint32_t k = 0; k < r; ++k) m &= m - 1;
+            int pa, pb;
+            edge_points(__builtin_ctzll(m), &pa, &pb);
+            const bool a_in = (inside >> pa) & 1u;
+            lattice_pos(plo, phi, a_in ? pa : pb, a);      // inside end
+            lattice_pos(plo, phi, a_in ? pb : pa, b);      // outside end
+        }
+        for (int q = 0; q < 3; ++q) {
+            s_tf[e][q] = a[q];
+            s_ts[e][q] = b[q];
+        }
+    }
+    wave_sync();
+    // 4 rounds of 16 samples (dc_tree.inl:339-410).
+    const double frac = smp / 15.0;
+    for (int round = 0; round < 4; ++round) {
+        double ps[3];
+        for (int q = 0; q < 3; ++q) ps[q] = s_tf[e][q] * (1 - frac) + s_ts[e][q] * frac;
+        uint32_t vf;
+        const float sv = pe_value<NV>(t, n, root, float(ps[0]), float(ps[1]),
+                                      float(ps[2]), &vf);
+        const bool zero = valid && smp >= 1 && sv == 0.0f;
+        if (ballot(zero && (vf & PT_TIE))) {
+            if (lane == 0) defer_group(P, grec, g, deferred);
+            return;
+        }
+        bool zero_out = false;
+        if (ballot(zero)) {
+            float sg[3];
+            uint32_t f2;
+            pe_grad3<NV>(t, n, root, float(ps[0]), float(ps[1]), float(ps[2]), sg, &f2);
+            const float nrm = sqrtf(sg[0] * sg[0] + (sg[1] * sg[1] + sg[2] * sg[2]));
+            if (zero) zero_out = !(nrm > 0.0f);
+        }
+        const bool take = valid && smp >= 1 &&
+            (sv > 0.0f || zero_out || (smp == 15 && !(sv == 0.0f)));
+        const uint64_t tm = ballot(take);
+        const uint32_t seg = uint32_t(tm >> (e * 16)) & 0xffffu;
+        const int first = seg ? __builtin_ctz(seg) : -1;
+        wave_sync();
+        if (valid && first >= 1) {
+            if (smp == first) {
+                for (int q = 0; q < 3; ++q) s_ts[e][q] = ps[q];
+            } else if (smp == first - 1) {
+                for (int q = 0; q < 3; ++q) s_tf[e][q] = ps[q];
+            }
+        }
+        wave_sync();
+    }
+    // Partials at both bracket ends (dc_tree.inl:412-471), lanes 0..23:
+    // edge lane / 6, end (lane / 3) & 1, axis lane % 3.
+    {
+        const int de = lane / 6, end = (lane / 3) & 1, axis = lane % 3;
+        const bool dv = lane < 24 && 4u * j + uint32_t(de) < ne;
+        const int ee = dv ? de : 0;
+        float ep[3];
+        for (int q = 0; q < 3; ++q) ep[q] = float(end ? s_ts[ee][q] : s_tf[ee][q]);
+        float ga;
+        uint32_t ef;
+        const float ev = pe_deriv1<NV>(t, n, root, ep[0], ep[1], ep[2], axis, &ga, &ef);
+        if (dv) {
+            s_g[2 * de + end][axis] = ga;
+            if (axis == 0) {
+                s_v[2 * de + end] = ev;
+                s_fl[2 * de + end] = ef;
+            }
+        }
+    }
+    wave_sync();
+    const bool end_ok = lane < 8 && 4u * j + uint32_t(lane >> 1) < ne;
+    if (ballot(end_ok && (s_fl[lane & 7] & PT_AMBIGUOUS))) {
+        if (lane == 0) defer_group(P, grec, g, deferred);
+        return;
+    }
+    if (lane < 4 && 4u * j + uint32_t(lane) < ne) {
+        EdgeRec out;
+        for (int q = 0; q < 14; ++q) out.acc[q] = 0.0;
+        for (int end = 0; end < 2; ++end) {
+            const int k = 2 * lane + end;
+            double pos[3];
+            for (int q = 0; q < 3; ++q) pos[q] = end ? s_ts[lane][q] : s_tf[lane][q];
+            const double d[3] = {double(s_g[k][0]), double(s_g[k][1]), double(s_g[k][2])};
+            qef::push_sample(out.acc, pos, d, double(s_v[k]));
+        }
+        double ata[3][3];
+        qef::unpack_ata(out.acc, ata);
+        out.rank = qef::small_eigenvalue_count(ata);
+        out.pad = 0;
+        edges[ebase + 4u * j + uint32_t(lane)] = out;
+    }
+}
+
+// Vertex solve per patch (dc_tree.inl:481-552, 740-813), one lane per
+// ambiguous leaf.
+__global__ __launch_bounds__(64)
+void k_leaf_solve(Params P, const uint32_t* __restrict__ groups, LevelView l1,
+                  LevelView l0, const LeafGroupRec* __restrict__ grec,
+                  const uint32_t* __restrict__ leaves, uint32_t nleaves,
+                  const EdgeRec* __restrict__ edges)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nleaves) return;
+    const uint32_t w = leaves[i];
+    const uint32_t g = w >> 3, oct = w & 7u;
+    const LeafGroupRec gr = grec[g];
+    if (gr.flags & 1u) return;              // redone by the feature pass
+    const uint32_t cidx = uint32_t(l1.child[groups[g]]) + oct;
+    LeafRec* leaf = l0.leaf + cidx;
+    const uint32_t mask = leaf->mask;
+    uint32_t vcount = 0, rank = 0;
+    for (int patch = 0; patch < 4; ++patch) {
+        if (mv(mask, patch, 0, 0) == -1) break;
+        int ec = 0;
+        while (ec < 12 && mv(mask, patch, ec, 0) != -1) ++ec;
+        uint32_t idx[12];
+        int max_rank = 0;
+        for (int e = 0; e < ec; ++e) {
+            const int le = lattice_edge(lattice_point(oct, mv(mask, patch, e, 0)),
+                                        lattice_point(oct, mv(mask, patch, e, 1)));
+            idx[e] = gr.edge_base +
+                     uint32_t(__popcll(gr.edge_mask & ((uint64_t(1) << le) - 1)));
+            const int rk = edges[idx[e]].rank;
+            max_rank = rk > max_rank ? rk : max_rank;
+        }
+        double mass[4] = {0.0, 0.0, 0.0, 0.0};
+        double sum[14];
+        for (int q = 0; q < 14; ++q) sum[q] = 0.0;
+        for (int e = 0; e < ec; ++e) {
+            const EdgeRec& er = edges[idx[e]];
+            if (er.rank < 0) continue;
+            if (er.rank == max_rank) {
+                mass[0] = mass[0] + er.acc[0] / er.acc[3];
+                mass[1] = mass[1] + er.acc[1] / er.acc[3];
+                mass[2] = mass[2] + er.acc[2] / er.acc[3];
+                mass[3] = mass[3] + 1.0;
+            }
+        }
+        for (int e = 0; e < ec; ++e) {
+            const EdgeRec& er = edges[idx[e]];
+            if (er.rank < 0) continue;
+            for (int q = 4; q < 14; ++q) sum[q] = sum[q] + er.acc[q];
+        }
+        double ata[3][3];
+        qef::unpack_ata(sum, ata);
+        int rk = 0;
+        double vert[3];
+        qef::find_vertex(ata, sum + 10, sum[13], mass, vert, &rk);
+        for (int q = 0; q < 3; ++q) leaf->vert[patch][q] = vert[q];
+        for (int q = 0; q < 4; ++q) leaf->mass[q] = mass[q];
+        for (int q = 0; q < 6; ++q) leaf->ata[q] = sum[4 + q];
+        for (int q = 0; q < 3; ++q) leaf->atb[q] = sum[10 + q];
+        leaf->btb = sum[13];
+        rank = uint32_t(rk);
+        ++vcount;
+    }
+    leaf->rank = rank;
+    leaf->vcount = vcount;
+}
+
+hipError_t launch_leaf_corners(const Params& p, const uint32_t* groups,
+                               uint32_t ngroups, const LevelView& l1,
+                               const LevelView& l0, LeafGroupRec* grec,
+                               uint64_t* tasks, uint32_t* leaves,
+                               uint32_t* deferred, hipStream_t s) {
+    if (ngroups == 0) return hipSuccess;
+#define LF_CORNERS(NV) hipLaunchKernelGGL(k_leaf_corners<NV>, dim3(ngroups), dim3(64), 0, s, \
+                                          p, groups, ngroups, l1, l0, grec, tasks, leaves, deferred)
+    switch (banks(p.nregs)) {
+        case 1: LF_CORNERS(1); break;
+        case 2: LF_CORNERS(2); break;
+        case 4: LF_CORNERS(4); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef LF_CORNERS
+    return hipGetLastError();
+}
+
+hipError_t launch_leaf_edges(const Params& p, const uint32_t* groups,
+                             const LevelView& l1, LeafGroupRec* grec,
+                             const uint64_t* tasks, uint32_t ntasks,
+                             EdgeRec* edges, uint32_t* deferred, hipStream_t s) {
+    if (ntasks == 0) return hipSuccess;
+#define LF_EDGES(NV) hipLaunchKernelGGL(k_leaf_edges<NV>, dim3(ntasks), dim3(64), 0, s, \
+                                        p, groups, l1, grec, tasks, ntasks, edges, deferred)
+    switch (banks(p.nregs)) {
+        case 1: LF_EDGES(1); break;
+        case 2: LF_EDGES(2); break;
+        case 4: LF_EDGES(4); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef LF_EDGES
+    return hipGetLastError();
+}
+
+hipError_t launch_leaf_solve(const Params& p, const uint32_t* groups,
+                             const LevelView& l1, const LevelView& l0,
+                             const LeafGroupRec* grec, const uint32_t* leaves,
+                             uint32_t nleaves, const EdgeRec* edges,
+                             hipStream_t s) {
+    if (nleaves == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_leaf_solve, dim3((nleaves + 63) / 64), dim3(64), 0, s, p, groups,
+                       l1, l0, grec, leaves, nleaves, edges);
+    return hipGetLastError();
+}
+
+////////////////////////////////////////////////////////////////////////////////
 // Collapse (collectChildren, dc_tree.inl:581-735)
 
 // Tape::getBase (tape.cpp:168-205) for a region and for a point.

@@ -70,7 +70,9 @@ struct Counters {
     uint32_t deferred;   // leaf cells left for the feature pass
     uint32_t error;
     uint32_t max_len;    // longest tape of an ambiguous cell of the last level
-    uint32_t pad[3];
+    uint32_t edges;      // lattice edges searched by the leaf pass
+    uint32_t tasks;      // edge-search tasks (four edges each)
+    uint32_t leaves;     // ambiguous leaves solved by the leaf pass
 };
 enum : uint32_t {
     ERR_POOL = 1u,       // clause pool exhausted
@@ -125,6 +127,52 @@ hipError_t launch_leaf_features(const Params& p, const uint32_t* groups,
                                 const uint32_t* work, uint32_t nwork,
                                 uint8_t* ws, size_t ws_bytes, hipStream_t s);
 size_t leaf_feature_workspace(uint32_t max_len, uint32_t nregs);
+
+// Phase-split leaf pass over the 8 children of each parent in `groups`
+// (the children share the parent's tape):
+//   corners  one wave per parent: values at the 27 points of the parent's
+//            2x2x2 lattice of children, the children's corner masks and
+//            cell records, the set of lattice edges the children's patches
+//            cross (a 54-bit mask), the edge-search tasks and the list of
+//            ambiguous leaves;
+//   edges    one wave per task (4 lattice edges of one parent): the
+//            4 x 16-sample bracket search, the partials at both bracket
+//            ends and the edge's QEF and rank;
+//   solve    one lane per ambiguous leaf: per patch, the mass point and QEF
+//            sum over the patch's edges and the FP64 vertex solve.
+// Each lattice edge is searched once per parent instead of once per cell
+// that uses it; the values are the same, since the children evaluate the
+// same tape at the same points.  A parent whose points need the feature
+// evaluation (an exact zero on a tied MIN/MAX) is listed whole (its 8
+// cells) in `deferred` for launch_leaf_features.
+struct LeafGroupRec {
+    uint64_t edge_mask;  // lattice edges used by the children's patches
+    uint32_t edge_base;  // first of the parent's edges in the edge records
+    uint32_t inside;     // bit i: lattice point i is inside
+    uint32_t flags;      // bit 0: deferred to the feature pass
+    uint32_t pad;
+};
+struct EdgeRec {
+    double acc[14];      // Intersection sums (qef::push_sample layout)
+    int32_t rank;        // eigenvalues below the cutoff; -1 if no sample
+    uint32_t pad;
+};
+constexpr uint32_t kLeafTasksPerGroup = 14;   // ceil(54 / 4)
+
+hipError_t launch_leaf_corners(const Params& p, const uint32_t* groups,
+                               uint32_t ngroups, const LevelView& l1,
+                               const LevelView& l0, LeafGroupRec* grec,
+                               uint64_t* tasks, uint32_t* leaves,
+                               uint32_t* deferred, hipStream_t s);
+hipError_t launch_leaf_edges(const Params& p, const uint32_t* groups,
+                             const LevelView& l1, LeafGroupRec* grec,
+                             const uint64_t* tasks, uint32_t ntasks,
+                             EdgeRec* edges, uint32_t* deferred, hipStream_t s);
+hipError_t launch_leaf_solve(const Params& p, const uint32_t* groups,
+                             const LevelView& l1, const LevelView& l0,
+                             const LeafGroupRec* grec, const uint32_t* leaves,
+                             uint32_t nleaves, const EdgeRec* edges,
+                             hipStream_t s);
 
 // Bottom-up collapse of one level.
 hipError_t launch_collapse(const Params& p, int level, const LevelView& cur,

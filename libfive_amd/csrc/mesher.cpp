@@ -135,11 +135,38 @@ void run_leaves(const Params& P, const uint32_t* groups, uint32_t ngroups,
     const uint32_t cells = ngroups * per_group;
     PoolBuf<uint32_t> deferred(cells);
     hip_check(hipMemsetAsync(&counters.p->deferred, 0, sizeof(uint32_t), s), "deferred");
-    hip_check(launch_leaf(P, groups, ngroups, per_group, l1, l0, deferred.p, s),
-              "leaf kernel");
-    hip_check(hipMemcpyAsync(&hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost, s),
-              "counters");
-    wait(s, "leaf kernel");
+    if (per_group == 8) {
+        // corners per parent, then the parents' lattice edges, then one
+        // vertex solve per ambiguous leaf
+        hip_check(hipMemsetAsync(&counters.p->edges, 0, 3 * sizeof(uint32_t), s), "leaf counters");
+        PoolBuf<LeafGroupRec> grec(ngroups);
+        PoolBuf<uint64_t> tasks(size_t(ngroups) * kLeafTasksPerGroup);
+        PoolBuf<uint32_t> leaves(cells);
+        hip_check(launch_leaf_corners(P, groups, ngroups, l1, l0, grec.p, tasks.p, leaves.p,
+                                      deferred.p, s), "leaf corner kernel");
+        hip_check(hipMemcpyAsync(&hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost, s),
+                  "counters");
+        wait(s, "leaf corner kernel");
+        check_errors(hc.error);
+        if (hc.tasks > size_t(ngroups) * kLeafTasksPerGroup || hc.leaves > cells ||
+            hc.edges > 54ull * ngroups || hc.deferred > cells) {
+            throw Error("render_mesh: inconsistent leaf counts");
+        }
+        PoolBuf<EdgeRec> edges(std::max<uint32_t>(hc.edges, 1u));
+        hip_check(launch_leaf_edges(P, groups, l1, grec.p, tasks.p, hc.tasks, edges.p,
+                                    deferred.p, s), "leaf edge kernel");
+        hip_check(launch_leaf_solve(P, groups, l1, l0, grec.p, leaves.p, hc.leaves,
+                                    edges.p, s), "leaf solve kernel");
+        hip_check(hipMemcpyAsync(&hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost, s),
+                  "counters");
+        wait(s, "leaf edge and solve kernels");
+    } else {
+        hip_check(launch_leaf(P, groups, ngroups, per_group, l1, l0, deferred.p, s),
+                  "leaf kernel");
+        hip_check(hipMemcpyAsync(&hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost, s),
+                  "counters");
+        wait(s, "leaf kernel");
+    }
     check_errors(hc.error);
     const uint32_t nd = hc.deferred;
     if (nd == 0) return;

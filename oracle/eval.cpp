@@ -346,8 +346,8 @@ static void fmod_iv(float xl, float xu, float yl, float yu, float& tl, float& tu
     tu = rnd::sub_up(xu, nl);
 }
 
-static float cos_dn(float x) { return smax(-1.0f, rnd::dn(libfive::fmath::cos_f32(x))); }
-static float cos_up(float x) { return smin(1.0f, rnd::up(libfive::fmath::cos_f32(x))); }
+static float cos_dn(float x) { return smax(-1.0f, rnd::dn(libfive::fmath::cos_f32_accurate(x))); }
+static float cos_up(float x) { return smin(1.0f, rnd::up(libfive::fmath::cos_f32_accurate(x))); }
 
 // boost cos(interval) (transc.hpp), with its recursion on tmp - pi.
 static void boost_cos(float xl, float xu, float& rl, float& ru, int depth) {

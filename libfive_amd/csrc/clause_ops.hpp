@@ -53,12 +53,17 @@ __device__ __forceinline__ void apply_value(uint32_t o, const float* a,
 #undef LF_EACH
 }
 
+// Value and partials of one clause, returned in registers (a result
+// array passed by pointer would give the kernels a stack frame).
+struct ColdDeriv {
+    float v, d0, d1, d2;
+};
+
 // Value and partial derivatives of one clause for the opcodes outside the
 // common set (eval_deriv_array.cpp:119-256); out of line.
-static __device__ __noinline__ void cold_deriv(uint32_t o, float av, float ad0,
+static __device__ __noinline__ ColdDeriv cold_deriv(uint32_t o, float av, float ad0,
                                         float ad1, float ad2, float bv,
-                                        float bd0, float bd1, float bd2,
-                                        float* res) {
+                                        float bd0, float bd1, float bd2) {
     const float ad[3] = {ad0, ad1, ad2};
     const float bd[3] = {bd0, bd1, bd2};
     float ov;
@@ -146,10 +151,7 @@ static __device__ __noinline__ void cold_deriv(uint32_t o, float av, float ad0,
             for (int j = 0; j < 3; ++j) od[j] = ad[j];
             break;
     }
-    res[0] = ov;
-    res[1] = od[0];
-    res[2] = od[1];
-    res[3] = od[2];
+    return {ov, od[0], od[1], od[2]};
 }
 
 // Value and partial derivatives of one clause (eval_deriv_array.cpp:
@@ -242,12 +244,11 @@ __device__ __forceinline__ void apply_deriv(uint32_t o, float av,
             for (int j = 0; j < 3; ++j) od[j] = ad[j];
             break;
         default: {
-            float res[4];
-            cold_deriv(o, av, ad[0], ad[1], ad[2], bv, bd[0], bd[1], bd[2], res);
-            ov = res[0];
-            od[0] = res[1];
-            od[1] = res[2];
-            od[2] = res[3];
+            const ColdDeriv r = cold_deriv(o, av, ad[0], ad[1], ad[2], bv, bd[0], bd[1], bd[2]);
+            ov = r.v;
+            od[0] = r.d0;
+            od[1] = r.d1;
+            od[2] = r.d2;
             break;
         }
     }

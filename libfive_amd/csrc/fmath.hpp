@@ -267,14 +267,10 @@ LF_HD_COLD SinCos sincos_large(float x, const uint32_t* tbl) {
     return r;
 }
 
-LF_HD void sincos_pair(float x, float* so, float* co, const uint32_t* tbl) {
+// The single-precision path for |x| < 2^17 (for infinities and NaNs it
+// gives NaNs).
+LF_HD void sincos_core(float x, float* so, float* co) {
     const float ax = ::fabsf(x);
-    if (!(ax < kCwLimit)) {
-        const SinCos r = sincos_large(x, tbl);
-        *so = r.s;
-        *co = r.c;
-        return;
-    }
     const float k = ::rintf(ax * kTwoOverPiF);
     float r = ::fmaf(-k, kCwP1, ax);
     r = ::fmaf(-k, kCwP2, r);
@@ -299,6 +295,25 @@ LF_HD void sincos_pair(float x, float* so, float* co, const uint32_t* tbl) {
     const uint32_t cneg = ((q + 1u) << 30) & 0x80000000u;
     *so = u2f(f2u(sv) ^ sneg);
     *co = u2f(f2u(cv) ^ cneg);
+}
+
+LF_HD void sincos_pair(float x, float* so, float* co, const uint32_t* tbl) {
+    if (!(::fabsf(x) < kCwLimit)) {
+        const SinCos r = sincos_large(x, tbl);
+        *so = r.s;
+        *co = r.c;
+        return;
+    }
+    sincos_core(x, so, co);
+}
+
+// sincos_pair without the out-of-line large-argument path, for kernels
+// that must not call functions: the same results, or false (results
+// unspecified) for finite |x| >= 2^17.  Infinities and NaNs give NaNs.
+LF_HD bool sincos_fast(float x, float* so, float* co) {
+    sincos_core(x, so, co);
+    const float ax = ::fabsf(x);
+    return !(ax >= kCwLimit && ax <= 3.40282347e38f);
 }
 
 // which = 0 for sin, 1 for cos.

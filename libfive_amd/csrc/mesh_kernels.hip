@@ -874,7 +874,7 @@ __device__ __forceinline__ void defer_group(const Params& P, LeafGroupRec* grec,
 }
 
 // Corners (dc_tree.inl:131-239 for the 8 children at once).
-template <int NV>
+template <int NV, bool FAST>
 __global__ __launch_bounds__(64)
 void k_leaf_corners(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
                     LevelView l1, LevelView l0, LeafGroupRec* __restrict__ grec,
@@ -901,20 +901,22 @@ void k_leaf_corners(Params P, const uint32_t* __restrict__ groups, uint32_t ngro
     lattice_pos(plo, phi, pt ? lane : 0, pd);
     const float cp[3] = {float(pd[0]), float(pd[1]), float(pd[2])};
     uint32_t fl;
-    const float v = pe_value<NV>(t, n, root, cp[0], cp[1], cp[2], &fl);
+    const float v = pe_value<NV, FAST>(t, n, root, cp[0], cp[1], cp[2], &fl);
     float gr[3] = {0.0f, 0.0f, 0.0f};
     if (ballot(pt && v == 0.0f && !(fl & PT_AMBIGUOUS))) {
-        uint32_t fl2;
-        (void)pe_grad3<NV>(t, n, root, cp[0], cp[1], cp[2], gr, &fl2);
+        uint32_t fl2 = 0;
+        (void)pe_grad3<NV, FAST>(t, n, root, cp[0], cp[1], cp[2], gr, &fl2);
+        fl |= fl2 & PT_SLOW;
     }
     bool inside;
     if (v > 0.0f || fmath::isnan_f(v)) inside = false;
     else if (v < 0.0f) inside = true;
     else inside = (gr[0] != 0.0f || gr[1] != 0.0f || gr[2] != 0.0f);
     const uint32_t in27 = uint32_t(ballot(pt && inside)) & 0x7ffffffu;
-    if (ballot(pt && v == 0.0f && (fl & PT_AMBIGUOUS))) {
-        // FeatureEvaluator::isInside needed: the feature pass does the
-        // parent's 8 cells.
+    if (ballot(pt && ((v == 0.0f && (fl & PT_AMBIGUOUS)) || (fl & PT_SLOW)))) {
+        // FeatureEvaluator::isInside needed (or an operation the fast
+        // evaluator leaves out): the feature pass does the parent's 8
+        // cells.
         if (lane == 0) {
             LeafGroupRec r;
             r.edge_mask = 0;
@@ -990,7 +992,7 @@ void k_leaf_corners(Params P, const uint32_t* __restrict__ groups, uint32_t ngro
 
 // Edge search, partials and per-edge QEF (dc_tree.inl:339-502 for one
 // lattice edge, shared by the children whose patches cross it).
-template <int NV>
+template <int NV, bool FAST>
 __global__ __launch_bounds__(64)
 void k_leaf_edges(Params P, const uint32_t* __restrict__ groups, LevelView l1,
                   LeafGroupRec* __restrict__ grec, const uint64_t* __restrict__ tasks,
@@ -1045,18 +1047,22 @@ void k_leaf_edges(Params P, const uint32_t* __restrict__ groups, LevelView l1,
         double ps[3];
         for (int q = 0; q < 3; ++q) ps[q] = s_tf[e][q] * (1 - frac) + s_ts[e][q] * frac;
         uint32_t vf;
-        const float sv = pe_value<NV>(t, n, root, float(ps[0]), float(ps[1]),
-                                      float(ps[2]), &vf);
+        const float sv = pe_value<NV, FAST>(t, n, root, float(ps[0]), float(ps[1]),
+                                            float(ps[2]), &vf);
         const bool zero = valid && smp >= 1 && sv == 0.0f;
-        if (ballot(zero && (vf & PT_TIE))) {
+        if (ballot((zero && (vf & PT_TIE)) || (valid && (vf & PT_SLOW)))) {
             if (lane == 0) defer_group(P, grec, g, deferred);
             return;
         }
         bool zero_out = false;
         if (ballot(zero)) {
             float sg[3];
-            uint32_t f2;
-            pe_grad3<NV>(t, n, root, float(ps[0]), float(ps[1]), float(ps[2]), sg, &f2);
+            uint32_t f2 = 0;
+            pe_grad3<NV, FAST>(t, n, root, float(ps[0]), float(ps[1]), float(ps[2]), sg, &f2);
+            if (ballot(zero && (f2 & PT_SLOW))) {
+                if (lane == 0) defer_group(P, grec, g, deferred);
+                return;
+            }
             const float nrm = sqrtf(sg[0] * sg[0] + (sg[1] * sg[1] + sg[2] * sg[2]));
             if (zero) zero_out = !(nrm > 0.0f);
         }
@@ -1085,7 +1091,7 @@ void k_leaf_edges(Params P, const uint32_t* __restrict__ groups, LevelView l1,
         for (int q = 0; q < 3; ++q) ep[q] = float(end ? s_ts[ee][q] : s_tf[ee][q]);
         float ga;
         uint32_t ef;
-        const float ev = pe_deriv1<NV>(t, n, root, ep[0], ep[1], ep[2], axis, &ga, &ef);
+        const float ev = pe_deriv1<NV, FAST>(t, n, root, ep[0], ep[1], ep[2], axis, &ga, &ef);
         if (dv) {
             s_g[2 * de + end][axis] = ga;
             if (axis == 0) {
@@ -1096,7 +1102,7 @@ void k_leaf_edges(Params P, const uint32_t* __restrict__ groups, LevelView l1,
     }
     wave_sync();
     const bool end_ok = lane < 8 && 4u * j + uint32_t(lane >> 1) < ne;
-    if (ballot(end_ok && (s_fl[lane & 7] & PT_AMBIGUOUS))) {
+    if (ballot(end_ok && (s_fl[lane & 7] & (PT_AMBIGUOUS | PT_SLOW)))) {
         if (lane == 0) defer_group(P, grec, g, deferred);
         return;
     }
@@ -1185,35 +1191,41 @@ void k_leaf_solve(Params P, const uint32_t* __restrict__ groups, LevelView l1,
     leaf->vcount = vcount;
 }
 
-hipError_t launch_leaf_corners(const Params& p, const uint32_t* groups,
+hipError_t launch_leaf_corners(const Params& p, bool fast, const uint32_t* groups,
                                uint32_t ngroups, const LevelView& l1,
                                const LevelView& l0, LeafGroupRec* grec,
                                uint64_t* tasks, uint32_t* leaves,
                                uint32_t* deferred, hipStream_t s) {
     if (ngroups == 0) return hipSuccess;
-#define LF_CORNERS(NV) hipLaunchKernelGGL(k_leaf_corners<NV>, dim3(ngroups), dim3(64), 0, s, \
-                                          p, groups, ngroups, l1, l0, grec, tasks, leaves, deferred)
-    switch (banks(p.nregs)) {
-        case 1: LF_CORNERS(1); break;
-        case 2: LF_CORNERS(2); break;
-        case 4: LF_CORNERS(4); break;
+#define LF_CORNERS(NV, F) hipLaunchKernelGGL((k_leaf_corners<NV, F>), dim3(ngroups), dim3(64), 0, s, \
+                                             p, groups, ngroups, l1, l0, grec, tasks, leaves, deferred)
+    switch (banks(p.nregs) * (fast ? -1 : 1)) {
+        case 1: LF_CORNERS(1, false); break;
+        case 2: LF_CORNERS(2, false); break;
+        case 4: LF_CORNERS(4, false); break;
+        case -1: LF_CORNERS(1, true); break;
+        case -2: LF_CORNERS(2, true); break;
+        case -4: LF_CORNERS(4, true); break;
         default: return hipErrorInvalidValue;
     }
 #undef LF_CORNERS
     return hipGetLastError();
 }
 
-hipError_t launch_leaf_edges(const Params& p, const uint32_t* groups,
+hipError_t launch_leaf_edges(const Params& p, bool fast, const uint32_t* groups,
                              const LevelView& l1, LeafGroupRec* grec,
                              const uint64_t* tasks, uint32_t ntasks,
                              EdgeRec* edges, uint32_t* deferred, hipStream_t s) {
     if (ntasks == 0) return hipSuccess;
-#define LF_EDGES(NV) hipLaunchKernelGGL(k_leaf_edges<NV>, dim3(ntasks), dim3(64), 0, s, \
-                                        p, groups, l1, grec, tasks, ntasks, edges, deferred)
-    switch (banks(p.nregs)) {
-        case 1: LF_EDGES(1); break;
-        case 2: LF_EDGES(2); break;
-        case 4: LF_EDGES(4); break;
+#define LF_EDGES(NV, F) hipLaunchKernelGGL((k_leaf_edges<NV, F>), dim3(ntasks), dim3(64), 0, s, \
+                                           p, groups, l1, grec, tasks, ntasks, edges, deferred)
+    switch (banks(p.nregs) * (fast ? -1 : 1)) {
+        case 1: LF_EDGES(1, false); break;
+        case 2: LF_EDGES(2, false); break;
+        case 4: LF_EDGES(4, false); break;
+        case -1: LF_EDGES(1, true); break;
+        case -2: LF_EDGES(2, true); break;
+        case -4: LF_EDGES(4, true); break;
         default: return hipErrorInvalidValue;
     }
 #undef LF_EDGES

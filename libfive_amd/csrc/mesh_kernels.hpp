@@ -159,12 +159,15 @@ struct EdgeRec {
 };
 constexpr uint32_t kLeafTasksPerGroup = 14;   // ceil(54 / 4)
 
-hipError_t launch_leaf_corners(const Params& p, const uint32_t* groups,
+// fast: the tape has only the opcodes the call-free evaluators handle
+// (pe_fast_opcode); points they cannot evaluate send the parent to the
+// feature pass.
+hipError_t launch_leaf_corners(const Params& p, bool fast, const uint32_t* groups,
                                uint32_t ngroups, const LevelView& l1,
                                const LevelView& l0, LeafGroupRec* grec,
                                uint64_t* tasks, uint32_t* leaves,
                                uint32_t* deferred, hipStream_t s);
-hipError_t launch_leaf_edges(const Params& p, const uint32_t* groups,
+hipError_t launch_leaf_edges(const Params& p, bool fast, const uint32_t* groups,
                              const LevelView& l1, LeafGroupRec* grec,
                              const uint64_t* tasks, uint32_t ntasks,
                              EdgeRec* edges, uint32_t* deferred, hipStream_t s);

@@ -130,8 +130,8 @@ void check_errors(uint32_t err) {
 // with the feature evaluation, in batches that bound the workspace.
 void run_leaves(const Params& P, const uint32_t* groups, uint32_t ngroups,
                 uint32_t per_group, const LevelView& l1, const LevelView& l0,
-                uint32_t max_len, PoolBuf<Counters>& counters, Counters& hc,
-                hipStream_t s) {
+                uint32_t max_len, bool fast, PoolBuf<Counters>& counters,
+                Counters& hc, hipStream_t s) {
     const uint32_t cells = ngroups * per_group;
     PoolBuf<uint32_t> deferred(cells);
     hip_check(hipMemsetAsync(&counters.p->deferred, 0, sizeof(uint32_t), s), "deferred");
@@ -142,7 +142,7 @@ void run_leaves(const Params& P, const uint32_t* groups, uint32_t ngroups,
         PoolBuf<LeafGroupRec> grec(ngroups);
         PoolBuf<uint64_t> tasks(size_t(ngroups) * kLeafTasksPerGroup);
         PoolBuf<uint32_t> leaves(cells);
-        hip_check(launch_leaf_corners(P, groups, ngroups, l1, l0, grec.p, tasks.p, leaves.p,
+        hip_check(launch_leaf_corners(P, fast, groups, ngroups, l1, l0, grec.p, tasks.p, leaves.p,
                                       deferred.p, s), "leaf corner kernel");
         hip_check(hipMemcpyAsync(&hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost, s),
                   "counters");
@@ -153,7 +153,7 @@ void run_leaves(const Params& P, const uint32_t* groups, uint32_t ngroups,
             throw Error("render_mesh: inconsistent leaf counts");
         }
         PoolBuf<EdgeRec> edges(std::max<uint32_t>(hc.edges, 1u));
-        hip_check(launch_leaf_edges(P, groups, l1, grec.p, tasks.p, hc.tasks, edges.p,
+        hip_check(launch_leaf_edges(P, fast, groups, l1, grec.p, tasks.p, hc.tasks, edges.p,
                                     deferred.p, s), "leaf edge kernel");
         hip_check(launch_leaf_solve(P, groups, l1, l0, grec.p, leaves.p, hc.leaves,
                                     edges.p, s), "leaf solve kernel");
@@ -242,6 +242,10 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
         throw Error("render_mesh: the tape needs " + std::to_string(nregs) +
                     " registers; the mesher supports 128");
     }
+    // The call-free leaf kernels serve tapes of the common opcodes only
+    // (tape::fast_opcode; pe_value_op in point_eval.hpp handles the same set).
+    bool fast = true;
+    for (const tape::Clause& c : base.clauses) fast = fast && tape::fast_opcode(tape::op(c));
     const int top = resolution_level(lo, hi, settings.min_feature);
     if (top > 20) throw Error("render_mesh: resolution too fine (more than 20 levels)");
     g_stats.top = top;
@@ -316,7 +320,7 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
         groups[1].alloc(1);
         hip_check(hipMemsetAsync(groups[1].p, 0, sizeof(uint32_t), s), "root");
         run_leaves(P, groups[1].p, 1, 1, virt.view(0), levels[0].view(0), nbase,
-                   counters, hc, s);
+                   fast, counters, hc, s);
     } else {
         hip_check(hipMemsetAsync(&counters.p->max_len, 0, sizeof(uint32_t), s), "max_len");
         hip_check(launch_interval(P, top, nullptr, 1, levels[top].view(0),
@@ -356,7 +360,7 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
             } else {
                 run_leaves(P, groups[1].p, uint32_t(namb), 8, cur.view(0),
                            levels[0].view(0), std::max<uint32_t>(hc.max_len, 1),
-                           counters, hc, s);
+                           fast, counters, hc, s);
             }
             if (settings.cancel && *settings.cancel) {
                 wait(s, "cancel");

@@ -19,15 +19,50 @@
 namespace libfive {
 namespace amd {
 
-enum : uint32_t { PT_AMBIGUOUS = 1u, PT_TIE = 2u };
+enum : uint32_t { PT_AMBIGUOUS = 1u, PT_TIE = 2u, PT_SLOW = 4u };
+
+// FAST = true: the evaluators call no function, so that the kernels using
+// them need no stack.  They then handle only the common opcodes (the
+// caller checks that the tape has no other) and sin / cos arguments below
+// 2^17; anything else sets PT_SLOW in the flags (with an unspecified value)
+// and the caller redoes the point with FAST = false.  tape::fast_opcode
+// lists the common opcodes.
+
 
 struct Clause4x {
     tape::Clause c[tape::kFetch];
 };
 
+template <bool FAST = false>
 __device__ __forceinline__ float pe_value_op(uint32_t o, float a, float b,
-                                             float im) {
+                                             float im, uint32_t& flags) {
     namespace OC = Opcode;
+    if (FAST) {
+        switch (o) {
+            case OC::OP_ADD: return a + b;
+            case OC::OP_MUL: return a * b;
+            case OC::OP_MIN: return (b < a) ? b : a;
+            case OC::OP_MAX: return (a < b) ? b : a;
+            case OC::OP_SUB: return a - b;
+            case OC::OP_NEG: return -a;
+            case OC::OP_SQUARE: return a * a;
+            case OC::OP_ABS: return fabsf(a);
+            case OC::OP_SQRT: return sqrtf(a);
+            case OC::OP_SIN:
+            case OC::OP_COS: {
+                float sv, cv;
+                if (!fmath::sincos_fast(a, &sv, &cv)) flags |= PT_SLOW;
+                return (o == OC::OP_SIN) ? sv : cv;
+            }
+            case OC::OP_DIV: return a / b;
+            case tape::OP_LOADI: return im;
+            case tape::OP_COPY:
+            case OC::CONST_VAR: return a;
+            default:
+                flags |= PT_SLOW;
+                return 0.0f;
+        }
+    }
     switch (o) {
         case OC::OP_ADD: return a + b;
         case OC::OP_MUL: return a * b;
@@ -46,6 +81,12 @@ __device__ __forceinline__ float pe_value_op(uint32_t o, float a, float b,
         case OC::CONST_VAR: return a;
         default: return fmath::eval_op(int(o), a, b);
     }
+}
+
+__device__ __forceinline__ float pe_value_op(uint32_t o, float a, float b,
+                                             float im) {
+    uint32_t unused = 0;
+    return pe_value_op<false>(o, a, b, im, unused);
 }
 
 __device__ __forceinline__ uint32_t pe_minmax_flags(uint32_t o, float a,
@@ -69,12 +110,13 @@ __device__ __forceinline__ uint32_t pe_minmax_flags(uint32_t o, float a,
                                               : LF_RF_GET(r, NV, tape::rb(c_)); \
             }                                                                \
             flags |= pe_minmax_flags(o_, a_, b_);                            \
-            LF_RF_SET(r, NV, tape::dst(c_), pe_value_op(o_, a_, b_, im_));   \
+            LF_RF_SET(r, NV, tape::dst(c_),                                  \
+                      pe_value_op<FAST>(o_, a_, b_, im_, flags));            \
         }                                                                    \
     } while (0)
 
 // Value of the tape at (x, y, z); *flags gets PT_* bits.
-template <int NV>
+template <int NV, bool FAST = false>
 __device__ __forceinline__ float pe_value(const tape::Clause* __restrict__ t,
                                           uint32_t n, uint32_t root, float x,
                                           float y, float z, uint32_t* out_flags)
@@ -96,12 +138,17 @@ __device__ __forceinline__ float pe_value(const tape::Clause* __restrict__ t,
     return LF_RF_GET(r, NV, root);
 }
 
+// Value and partials of one clause, returned in registers (a result
+// array passed by pointer would give the kernels a stack frame).
+struct PeColdDeriv {
+    float v, d0, d1, d2;
+};
+
 // Out-of-line derivative rules of the less common opcodes
 // (eval_deriv_array.cpp:119-256).
-static __device__ __noinline__ void pe_cold_deriv(uint32_t o, float av, float ad0,
+static __device__ __noinline__ PeColdDeriv pe_cold_deriv(uint32_t o, float av, float ad0,
                                            float ad1, float ad2, float bv,
-                                           float bd0, float bd1, float bd2,
-                                           float* res) {
+                                           float bd0, float bd1, float bd2) {
     namespace OC = Opcode;
     const float ad[3] = {ad0, ad1, ad2};
     const float bd[3] = {bd0, bd1, bd2};
@@ -190,18 +237,17 @@ static __device__ __noinline__ void pe_cold_deriv(uint32_t o, float av, float ad
             for (int j = 0; j < 3; ++j) od[j] = ad[j];
             break;
     }
-    res[0] = ov;
-    res[1] = od[0];
-    res[2] = od[1];
-    res[3] = od[2];
+    return {ov, od[0], od[1], od[2]};
 }
 
 // Value and partial derivatives of one clause (the rules of
 // DerivArrayEvaluator, libfive/src/eval/eval_deriv_array.cpp:119-256).
+template <bool FAST = false>
 __device__ __forceinline__ void pe_deriv_op(uint32_t o, float av,
                                             const float ad[3], float bv,
                                             const float bd[3], float im,
-                                            float& ov, float od[3]) {
+                                            float& ov, float od[3],
+                                            uint32_t& flags) {
     namespace OC = Opcode;
     switch (o) {
         case OC::OP_ADD:
@@ -264,14 +310,22 @@ __device__ __forceinline__ void pe_deriv_op(uint32_t o, float av,
             break;
         case OC::OP_SIN: {
             float s, co;
-            fmath::sin_cos_f32(av, &s, &co);
+            if (FAST) {
+                if (!fmath::sincos_fast(av, &s, &co)) flags |= PT_SLOW;
+            } else {
+                fmath::sin_cos_f32(av, &s, &co);
+            }
             ov = s;
             for (int j = 0; j < 3; ++j) od[j] = ad[j] * co;
             break;
         }
         case OC::OP_COS: {
             float s, co;
-            fmath::sin_cos_f32(av, &s, &co);
+            if (FAST) {
+                if (!fmath::sincos_fast(av, &s, &co)) flags |= PT_SLOW;
+            } else {
+                fmath::sin_cos_f32(av, &s, &co);
+            }
             ov = co;
             const float ns = -s;
             for (int j = 0; j < 3; ++j) od[j] = ad[j] * ns;
@@ -287,12 +341,17 @@ __device__ __forceinline__ void pe_deriv_op(uint32_t o, float av,
             for (int j = 0; j < 3; ++j) od[j] = ad[j];
             break;
         default: {
-            float res[4];
-            pe_cold_deriv(o, av, ad[0], ad[1], ad[2], bv, bd[0], bd[1], bd[2], res);
-            ov = res[0];
-            od[0] = res[1];
-            od[1] = res[2];
-            od[2] = res[3];
+            if (FAST) {
+                flags |= PT_SLOW;
+                ov = 0.0f;
+                for (int j = 0; j < 3; ++j) od[j] = 0.0f;
+                break;
+            }
+            const PeColdDeriv r = pe_cold_deriv(o, av, ad[0], ad[1], ad[2], bv, bd[0], bd[1], bd[2]);
+            ov = r.v;
+            od[0] = r.d0;
+            od[1] = r.d1;
+            od[2] = r.d2;
             break;
         }
     }
@@ -327,7 +386,7 @@ __device__ __forceinline__ void pe_deriv_op(uint32_t o, float av,
             }                                                                \
             flags |= pe_minmax_flags(o_, av_, bv_);                          \
             float ov_, od_[3];                                               \
-            pe_deriv_op(o_, av_, ad_, bv_, bd_, im_, ov_, od_);              \
+            pe_deriv_op<FAST>(o_, av_, ad_, bv_, bd_, im_, ov_, od_, flags);              \
             const uint32_t d_ = tape::dst(c_);                               \
             LF_RF_SET(v, NV, d_, ov_);                                       \
             LF_RF_SET(gx, NV, d_, od_[0]);                                   \
@@ -337,7 +396,7 @@ __device__ __forceinline__ void pe_deriv_op(uint32_t o, float av,
     } while (0)
 
 // Value and gradient of the tape at (x, y, z).
-template <int NV>
+template <int NV, bool FAST = false>
 __device__ __forceinline__ float pe_deriv(const tape::Clause* __restrict__ t,
                                           uint32_t n, uint32_t root, float x,
                                           float y, float z, float g[3],
@@ -397,14 +456,14 @@ __device__ __forceinline__ float pe_deriv(const tape::Clause* __restrict__ t,
             }                                                                \
             flags |= pe_minmax_flags(o_, av_, bv_);                          \
             float ov_, od_[3];                                               \
-            pe_deriv_op(o_, av_, ad_, bv_, bd_, im_, ov_, od_);              \
+            pe_deriv_op<FAST>(o_, av_, ad_, bv_, bd_, im_, ov_, od_, flags);              \
             const uint32_t d_ = tape::dst(c_);                               \
             LF_RF_SET(v, NV, d_, ov_);                                       \
             LF_RF_SET(gd, NV, d_, od_[0]);                                   \
         }                                                                    \
     } while (0)
 
-template <int NV>
+template <int NV, bool FAST = false>
 __device__ __forceinline__ float pe_deriv1(const tape::Clause* __restrict__ t,
                                            uint32_t n, uint32_t root, float x,
                                            float y, float z, int axis,
@@ -433,7 +492,7 @@ __device__ __forceinline__ float pe_deriv1(const tape::Clause* __restrict__ t,
 }
 
 // Value and gradient as three single-component passes (pe_deriv's bits).
-template <int NV>
+template <int NV, bool FAST = false>
 __device__ __forceinline__ float pe_grad3(const tape::Clause* __restrict__ t,
                                           uint32_t n, uint32_t root, float x,
                                           float y, float z, float g[3],
@@ -443,7 +502,7 @@ __device__ __forceinline__ float pe_grad3(const tape::Clause* __restrict__ t,
 #pragma unroll 1
     for (int axis = 0; axis < 3; ++axis) {
         float ga;
-        v = pe_deriv1<NV>(t, n, root, x, y, z, axis, &ga, out_flags);
+        v = pe_deriv1<NV, FAST>(t, n, root, x, y, z, axis, &ga, out_flags);
         if (axis == 0) g[0] = ga;
         else if (axis == 1) g[1] = ga;
         else g[2] = ga;

@@ -96,6 +96,19 @@ LF_HD Clause make(uint32_t op_, uint32_t mode_, uint32_t dst_, uint32_t a,
     return c;
 }
 
+// The opcodes of the mesher's call-free evaluators (the common ones: IEEE
+// arithmetic, min / max, sqrt, sin / cos and the tape's own LOADI / COPY).
+LF_HD bool fast_opcode(uint32_t o) {
+    switch (o) {
+        case 17: case 18: case 19: case 20: case 21: case 9: case 7: case 28:
+        case 8: case 10: case 11: case 22: case 6:
+        case OP_LOADI: case OP_COPY: case OP_NOP:
+            return true;
+        default:
+            return false;
+    }
+}
+
 // Keep decision for a MIN/MAX clause (Tape::Keep, tape.hpp:38-41).
 enum : uint8_t { KEEP_BOTH = 0, KEEP_A = 1, KEEP_B = 2, KEEP_ALWAYS = 3 };
 

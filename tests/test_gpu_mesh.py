@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle
-from libfive_amd import X, Y, Z
+from libfive_amd import Shape, X, Y, Z
 from libfive_amd import workloads as W
 
 from helpers import wide_registers
@@ -93,6 +93,21 @@ def test_c3_sphere_union_depth8_matches_oracle(gpu):
     assert g.level_counts.shape[0] == 9
     assert max(g.tape_lengths.tolist()) > 5000
     assert min(g.tape_lengths.tolist()) < 20
+
+
+def test_large_trig_arguments_match_oracle(gpu):
+    """sin arguments beyond 2^17 inside the region: the mesher's call-free
+    leaf kernels hand such parents to the general per-cell kernel."""
+    s = W.sphere(0.8) + (X() * 3.0e5).sin() * 0.001
+    compare(s, -1.0, 1.0, 2.0 / 32)
+
+
+def test_uncommon_opcodes_match_oracle(gpu):
+    """A tape with opcodes outside the common set runs the leaf kernels
+    that call out to the general opcode rules."""
+    s = (W.sphere(0.8) + Shape.unary("exp", X()) * 0.05
+         + Shape.binary("atan2", Y(), Z() + 2.0) * 0.05)
+    compare(s, -1.0, 1.0, 2.0 / 32)
 
 
 def test_wide_register_file_matches_oracle(gpu):

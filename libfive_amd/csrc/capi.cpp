@@ -22,6 +22,7 @@
 #include <mutex>
 #include <vector>
 #include "runtime.hpp"
+#include "stl.hpp"
 #include "tree.hpp"
 
 using namespace libfive;
@@ -40,6 +41,12 @@ static void set_error(const char* where, const std::exception& e) {
     g_last_error = std::string(where) + ": " + e.what();
     std::fprintf(stderr, "%s\n", g_last_error.c_str());
 }
+
+namespace libfive {
+namespace amd {
+void set_last_error(const char* where, const std::exception& e) { set_error(where, e); }
+}   // namespace amd
+}   // namespace libfive
 
 // Vertex keys of the meshes handed out, released by libfive_mesh_delete.
 static std::mutex g_keys_mu;
@@ -806,8 +813,10 @@ void libfive_mesh_coords_delete(libfive_mesh_coords* m) {
     delete m;
 }
 
+}   // extern "C"
+
 // Binary STL of several meshes (Mesh::saveSTL, mesh.cpp:139-198).
-static bool save_stl(const char* f, const std::vector<libfive_mesh*>& meshes) {
+bool libfive::amd::save_stl(const char* f, const std::vector<const libfive_mesh*>& meshes) {
     const std::string name(f ? f : "");
     if (name.size() < 4 || (name.compare(name.size() - 4, 4, ".stl") != 0 &&
                             name.compare(name.size() - 4, 4, ".STL") != 0)) {
@@ -824,11 +833,11 @@ static bool save_stl(const char* f, const std::vector<libfive_mesh*>& meshes) {
     const char* text = "This is a binary STL exported from libfive.";
     std::memcpy(header, text, std::strlen(text));
     uint32_t count = 0;
-    for (auto* m : meshes) count += m->tri_count;
+    for (const auto* m : meshes) count += m->tri_count;
     bool ok = std::fwrite(header, 1, 80, fp) == 80 &&
               std::fwrite(&count, 4, 1, fp) == 1;
     std::vector<char> rec(50 * size_t(1024));
-    for (auto* m : meshes) {
+    for (const auto* m : meshes) {
         for (uint32_t i = 0; ok && i < m->tri_count;) {
             const uint32_t n = std::min<uint32_t>(1024, m->tri_count - i);
             for (uint32_t k = 0; k < n; ++k) {
@@ -845,6 +854,8 @@ static bool save_stl(const char* f, const std::vector<libfive_mesh*>& meshes) {
     ok = (std::fclose(fp) == 0) && ok;
     return ok;
 }
+
+extern "C" {
 
 bool libfive_tree_save_mesh(libfive_tree tree, libfive_region3 R, float res,
                             const char* f) {
@@ -871,7 +882,7 @@ bool libfive_tree_save_meshes(libfive_tree trees[], libfive_region3 R,
         }
         ms.push_back(m);
     }
-    if (ok) ok = save_stl(f, ms);
+    if (ok) ok = save_stl(f, std::vector<const libfive_mesh*>(ms.begin(), ms.end()));
     for (auto* m : ms) libfive_mesh_delete(m);
     return ok;
 }

@@ -876,14 +876,15 @@ __device__ __forceinline__ void defer_group(const Params& P, LeafGroupRec* grec,
 // Corners (dc_tree.inl:131-239 for the 8 children at once).
 template <int NV, bool FAST>
 __global__ __launch_bounds__(64)
-void k_leaf_corners(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
-                    LevelView l1, LevelView l0, LeafGroupRec* __restrict__ grec,
-                    uint64_t* __restrict__ tasks, uint32_t* __restrict__ leaves,
-                    uint32_t* __restrict__ deferred)
+void k_leaf_corners(Params P, const uint32_t* __restrict__ groups, uint32_t g0,
+                    uint32_t ngroups, LevelView l1, LevelView l0,
+                    LeafGroupRec* __restrict__ grec, uint64_t* __restrict__ tasks,
+                    uint32_t* __restrict__ leaves, uint32_t* __restrict__ deferred)
 {
+    // This launch covers groups g0 .. g0 + ngroups - 1 of the level.
     const int lane = threadIdx.x;
-    const uint32_t g = blockIdx.x;
-    if (g >= ngroups) return;
+    if (blockIdx.x >= ngroups) return;
+    const uint32_t g = g0 + blockIdx.x;
     const uint32_t p = groups[g];
     const uint32_t tape_id = __builtin_amdgcn_readfirstlane(l1.tape[p]);
     const TapeRec tr = P.tapes[tape_id];
@@ -1192,13 +1193,13 @@ void k_leaf_solve(Params P, const uint32_t* __restrict__ groups, LevelView l1,
 }
 
 hipError_t launch_leaf_corners(const Params& p, bool fast, const uint32_t* groups,
-                               uint32_t ngroups, const LevelView& l1,
+                               uint32_t g0, uint32_t ngroups, const LevelView& l1,
                                const LevelView& l0, LeafGroupRec* grec,
                                uint64_t* tasks, uint32_t* leaves,
                                uint32_t* deferred, hipStream_t s) {
     if (ngroups == 0) return hipSuccess;
 #define LF_CORNERS(NV, F) hipLaunchKernelGGL((k_leaf_corners<NV, F>), dim3(ngroups), dim3(64), 0, s, \
-                                             p, groups, ngroups, l1, l0, grec, tasks, leaves, deferred)
+                                             p, groups, g0, ngroups, l1, l0, grec, tasks, leaves, deferred)
     switch (banks(p.nregs) * (fast ? -1 : 1)) {
         case 1: LF_CORNERS(1, false); break;
         case 2: LF_CORNERS(2, false); break;

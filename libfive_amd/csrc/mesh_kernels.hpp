@@ -162,8 +162,10 @@ constexpr uint32_t kLeafTasksPerGroup = 14;   // ceil(54 / 4)
 // fast: the tape has only the opcodes the call-free evaluators handle
 // (pe_fast_opcode); points they cannot evaluate send the parent to the
 // feature pass.
+// Corners of groups g0 .. g0 + ngroups - 1 (a launch may cover part of a
+// level); the edge kernel runs the tasks it is given.
 hipError_t launch_leaf_corners(const Params& p, bool fast, const uint32_t* groups,
-                               uint32_t ngroups, const LevelView& l1,
+                               uint32_t g0, uint32_t ngroups, const LevelView& l1,
                                const LevelView& l0, LeafGroupRec* grec,
                                uint64_t* tasks, uint32_t* leaves,
                                uint32_t* deferred, hipStream_t s);

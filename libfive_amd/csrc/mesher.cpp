@@ -125,13 +125,18 @@ void check_errors(uint32_t err) {
     if (err) throw Error("render_mesh: internal error " + std::to_string(err));
 }
 
+// Parents per launch of the leaf corner kernel (four times as many edge
+// tasks per launch of the edge kernel): about 10 ms of GPU time each on the
+// gyroid at 512^3.
+constexpr uint32_t kLeafChunk = 1u << 18;
+
 // Evaluates the leaf cells of `groups` (DCTree::evalLeaf): one pass for
 // all cells, then the cells that met an exact zero on a tied MIN/MAX again
 // with the feature evaluation, in batches that bound the workspace.
 void run_leaves(const Params& P, const uint32_t* groups, uint32_t ngroups,
                 uint32_t per_group, const LevelView& l1, const LevelView& l0,
-                uint32_t max_len, bool fast, PoolBuf<Counters>& counters,
-                Counters& hc, hipStream_t s) {
+                uint32_t max_len, bool fast, const MeshSettings& settings,
+                PoolBuf<Counters>& counters, Counters& hc, hipStream_t s) {
     const uint32_t cells = ngroups * per_group;
     PoolBuf<uint32_t> deferred(cells);
     hip_check(hipMemsetAsync(&counters.p->deferred, 0, sizeof(uint32_t), s), "deferred");
@@ -142,19 +147,42 @@ void run_leaves(const Params& P, const uint32_t* groups, uint32_t ngroups,
         PoolBuf<LeafGroupRec> grec(ngroups);
         PoolBuf<uint64_t> tasks(size_t(ngroups) * kLeafTasksPerGroup);
         PoolBuf<uint32_t> leaves(cells);
-        hip_check(launch_leaf_corners(P, fast, groups, ngroups, l1, l0, grec.p, tasks.p, leaves.p,
-                                      deferred.p, s), "leaf corner kernel");
+        // With a cancel flag the kernels are launched in chunks of at most
+        // kLeafChunk parents (tasks below), and the host waits after each
+        // chunk and checks the flag, so that a request is seen within one
+        // chunk's run time.
+        const bool poll = settings.cancel || settings.cancel_flag;
+        const uint32_t chunk = poll ? kLeafChunk : ngroups;
+        for (uint32_t g0 = 0; g0 < ngroups; g0 += chunk) {
+            const uint32_t ng = std::min(chunk, ngroups - g0);
+            hip_check(launch_leaf_corners(P, fast, groups, g0, ng, l1, l0, grec.p,
+                                          tasks.p, leaves.p, deferred.p, s),
+                      "leaf corner kernel");
+            if (poll) {
+                wait(s, "leaf corner kernel");
+                if (settings.cancelled()) return;     // the caller checks too
+            }
+        }
         hip_check(hipMemcpyAsync(&hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost, s),
                   "counters");
         wait(s, "leaf corner kernel");
         check_errors(hc.error);
+        if (settings.cancelled()) return;
         if (hc.tasks > size_t(ngroups) * kLeafTasksPerGroup || hc.leaves > cells ||
             hc.edges > 54ull * ngroups || hc.deferred > cells) {
             throw Error("render_mesh: inconsistent leaf counts");
         }
         PoolBuf<EdgeRec> edges(std::max<uint32_t>(hc.edges, 1u));
-        hip_check(launch_leaf_edges(P, fast, groups, l1, grec.p, tasks.p, hc.tasks, edges.p,
-                                    deferred.p, s), "leaf edge kernel");
+        const uint32_t tchunk = poll ? 4 * kLeafChunk : std::max<uint32_t>(hc.tasks, 1u);
+        for (uint32_t t0 = 0; t0 < hc.tasks; t0 += tchunk) {
+            const uint32_t nt = std::min(tchunk, hc.tasks - t0);
+            hip_check(launch_leaf_edges(P, fast, groups, l1, grec.p, tasks.p + t0, nt, edges.p,
+                                        deferred.p, s), "leaf edge kernel");
+            if (poll) {
+                wait(s, "leaf edge kernel");
+                if (settings.cancelled()) return;
+            }
+        }
         hip_check(launch_leaf_solve(P, groups, l1, l0, grec.p, leaves.p, hc.leaves,
                                     edges.p, s), "leaf solve kernel");
         hip_check(hipMemcpyAsync(&hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost, s),
@@ -320,7 +348,7 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
         groups[1].alloc(1);
         hip_check(hipMemsetAsync(groups[1].p, 0, sizeof(uint32_t), s), "root");
         run_leaves(P, groups[1].p, 1, 1, virt.view(0), levels[0].view(0), nbase,
-                   fast, counters, hc, s);
+                   fast, settings, counters, hc, s);
     } else {
         hip_check(hipMemsetAsync(&counters.p->max_len, 0, sizeof(uint32_t), s), "max_len");
         hip_check(launch_interval(P, top, nullptr, 1, levels[top].view(0),
@@ -360,13 +388,18 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
             } else {
                 run_leaves(P, groups[1].p, uint32_t(namb), 8, cur.view(0),
                            levels[0].view(0), std::max<uint32_t>(hc.max_len, 1),
-                           fast, counters, hc, s);
+                           fast, settings, counters, hc, s);
             }
-            if (settings.cancel && *settings.cancel) {
+            if (settings.cancelled()) {
                 wait(s, "cancel");
                 return false;
             }
         }
+    }
+
+    if (settings.cancelled()) {
+        wait(s, "cancel");
+        return false;
     }
 
     // Global cell numbering (level 0 first).
@@ -422,6 +455,12 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
         if (levels[L].n && levels[L - 1].n) {
             hip_check(launch_collapse(P, L, views[L], views[L - 1], s), "collapse kernel");
         }
+        // Levels are collapsed in order on one stream; the flag is checked
+        // once the work queued so far has finished.
+        if (settings.cancelled()) {
+            wait(s, "cancel");
+            return false;
+        }
     }
     wait(s, "collapse");
     g_stats.ms_collapse = ms_since(t_collapse);
@@ -451,6 +490,7 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
     hip_check(hipMemcpyAsync(&hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost, s), "counters");
     wait(s, "quad count");
     check_errors(hc.error);
+    if (settings.cancelled()) return false;
     const uint64_t ntris64 = uint64_t(tail[0]) + tail[1];
     const uint64_t nverts64 = uint64_t(tail[2]) + tail[3] + 1;    // with the dummy vertex
     if (ntris64 > 24 * total_cells || nverts64 > 4 * total_cells + 1) {
@@ -469,6 +509,10 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
                                d_tris.p, s), "quad write");
         hip_check(launch_verts(views[L], L, vslot.p, vid.p, d_verts.p, d_keys.p, s),
                   "vertex gather");
+    }
+    if (settings.cancelled()) {
+        wait(s, "cancel");
+        return false;
     }
     float* vdst = nullptr;
     uint32_t* tdst = nullptr;

@@ -3,6 +3,7 @@
 // emits triangles on the GPU, then copies the mesh to the host.
 #pragma once
 
+#include <atomic>
 #include <cstdint>
 #include <functional>
 #include <map>
@@ -20,7 +21,11 @@ struct MeshSettings {
     double max_err = 1e-8;
     uint32_t precision = 0;       // QEF accumulation: only 0 (FP64) is built
     bool collect_stats = false;   // per-level cell counts and tape lengths
-    volatile const int* cancel = nullptr;
+    volatile const int* cancel = nullptr;           // C API flag
+    const std::atomic<bool>* cancel_flag = nullptr; // BRepSettings::cancel
+    bool cancelled() const {
+        return (cancel && *cancel) || (cancel_flag && cancel_flag->load());
+    }
     // Values of free variables by tree id (libfive_tree_id); a variable
     // not listed is 0.  The tape is compiled with these values.
     const std::map<const void*, float>* vars = nullptr;
@@ -52,8 +57,11 @@ struct MeshStats {
     double ms_build = 0, ms_collapse = 0, ms_emit = 0, ms_total = 0;
 };
 
-// Returns false when cancelled (the C API then returns NULL).  Runs on the
-// library stream, one render at a time.
+// Returns false when cancelled (the C API then returns NULL): the flag is
+// polled between the octree levels of the build, after the leaf pass,
+// between the collapse levels and between the emit steps, so a render ends
+// within one level's kernels of the request.  Runs on the library stream,
+// one render at a time.
 bool render_mesh(const Tree& t, const double lo[3], const double hi[3],
                  const MeshSettings& s, HostMesh& out);
 

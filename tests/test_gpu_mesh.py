@@ -330,3 +330,62 @@ def test_evaluator_save_mesh_uses_variables(gpu, tmp_path):
             assert np.abs(rr - radius).max() < 0.01
     finally:
         L.libfive_evaluator_delete(ev)
+
+
+def test_cpp_mesh_api(gpu):
+    """The C++ entry (libfive::Mesh::render, both overloads, saveSTL) and
+    the reference's cancellation case, in the test program built with the
+    library (tests/cpp/mesh_api_test.cpp)."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "mesh_api_test")
+    assert os.path.exists(exe), "built by __graft_entry__.build()"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120,
+                       cwd=os.path.dirname(exe))
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith("OK")
+
+
+def test_cancel_from_another_thread(gpu):
+    """libfive/test/xtree.cpp:240-272 through the C API: a render cancelled
+    from another thread returns NULL within 150 ms, right after it starts
+    and 30 ms into the octree build."""
+    import ctypes
+    import threading
+    import time
+    from libfive_amd import ffi
+    L = ffi.lib()
+    s = W.gyroid_c4()
+    R = ffi.libfive_region3(ffi.libfive_interval(-1.2, 1.2),
+                            ffi.libfive_interval(-1.2, 1.2),
+                            ffi.libfive_interval(-1.2, 1.2))
+    st = ffi.libfive_mesh_settings()
+    L.libfive_mesh_settings_default(ctypes.byref(st))
+    st.min_feature = 2.4 / 1024
+    flag = ctypes.c_int(0)
+    st.cancel = ctypes.pointer(flag)
+    t0 = time.perf_counter()
+    m = L.libfive_tree_render_mesh_settings(s.ptr, R, ctypes.byref(st))
+    full_ms = (time.perf_counter() - t0) * 1e3
+    assert m, ffi.last_error()
+    L.libfive_mesh_delete(m)
+    for delay in (0.0, 0.03):
+        flag.value = 0
+        out = {}
+
+        def run():
+            out["m"] = L.libfive_tree_render_mesh_settings(s.ptr, R, ctypes.byref(st))
+
+        th = threading.Thread(target=run)
+        th.start()
+        time.sleep(delay)
+        t1 = time.perf_counter()
+        flag.value = 1
+        th.join(60)
+        ms = (time.perf_counter() - t1) * 1e3
+        print(f"cancel after {delay * 1e3:.0f} ms: returned in {ms:.1f} ms "
+              f"(full render {full_ms:.1f} ms)", flush=True)
+        assert not th.is_alive()
+        assert not out["m"]
+        assert ms < 150

@@ -11,9 +11,9 @@
 
 #include "contour_kernels.hpp"
 #include "deck.hpp"
-#include "interval_kernels.hpp"
 #include "qef.hpp"
 #include "runtime.hpp"
+#include "slice_tapes.hpp"
 
 namespace libfive {
 namespace amd {
@@ -104,6 +104,8 @@ struct Cell {
     uint8_t type = AMBIGUOUS;
     int32_t kids = -1;        // first of four consecutive children
     int32_t leaf = -1;
+    uint32_t tape_in = 0;     // tape the cell is evaluated with
+    uint32_t tape_out = 0;    // tape pushed for it (its children's tape_in)
 };
 
 struct Quadtree {
@@ -119,31 +121,38 @@ struct Quadtree {
     bool branch(const Cell& c) const { return c.kids >= 0; }
 };
 
-// One device round trip: points (x, y, z interleaved) in, results out.
+// One device round trip: points (x, y, z interleaved) and their tapes in,
+// results out.
 struct Device {
     Context& c;
-    const GpuTape& gt;
-    uint32_t nregs, sx, sy, sz;
+    const SliceTapes& tp;
+    uint32_t sx, sy, sz;
     PoolBuf<uint32_t> err{1};
 
-    void values(const std::vector<float>& xyz, std::vector<float>& v,
-                std::vector<uint8_t>& amb) {
+    PoolTapes tapes() const {
+        return {tp.pool(), tp.records(), tp.root(), tp.nregs(), tp.max_len()};
+    }
+
+    void values(const std::vector<float>& xyz, const std::vector<uint32_t>& tid,
+                std::vector<float>& v, std::vector<uint8_t>& amb) {
         const size_t n = xyz.size() / 3;
         v.resize(n);
         amb.resize(n);
         if (!n) return;
         PoolBuf<float> d(4 * n);
         PoolBuf<uint8_t> da(n);
+        PoolBuf<uint32_t> dt(n);
         hip_check(hipMemcpyAsync(d.p, xyz.data(), 12 * n, hipMemcpyHostToDevice, c.stream), "copy points");
-        hip_check(launch_point_values(gt.d.p, gt.size(), gt.host.root_reg, nregs, d.p,
-                                      d.p + 3 * n, da.p, n, c.stream), "point values");
+        hip_check(hipMemcpyAsync(dt.p, tid.data(), 4 * n, hipMemcpyHostToDevice, c.stream), "copy tapes");
+        hip_check(launch_point_values(tapes(), dt.p, d.p, d.p + 3 * n, da.p, n, c.stream),
+                  "point values");
         hip_check(hipMemcpyAsync(v.data(), d.p + 3 * n, 4 * n, hipMemcpyDeviceToHost, c.stream), "copy values");
         hip_check(hipMemcpyAsync(amb.data(), da.p, n, hipMemcpyDeviceToHost, c.stream), "copy flags");
         hip_check(hipStreamSynchronize(c.stream), "point values");
     }
 
-    void grads(const std::vector<float>& xyz, std::vector<float>& v, std::vector<float>& g,
-               std::vector<uint8_t>& amb) {
+    void grads(const std::vector<float>& xyz, const std::vector<uint32_t>& tid,
+               std::vector<float>& v, std::vector<float>& g, std::vector<uint8_t>& amb) {
         const size_t n = xyz.size() / 3;
         v.resize(n);
         g.resize(3 * n);
@@ -151,9 +160,11 @@ struct Device {
         if (!n) return;
         PoolBuf<float> d(7 * n);
         PoolBuf<uint8_t> da(n);
+        PoolBuf<uint32_t> dt(n);
         hip_check(hipMemcpyAsync(d.p, xyz.data(), 12 * n, hipMemcpyHostToDevice, c.stream), "copy points");
-        hip_check(launch_point_grads(gt.d.p, gt.size(), gt.host.root_reg, nregs, d.p,
-                                     d.p + 3 * n, d.p + 4 * n, da.p, n, c.stream), "point partials");
+        hip_check(hipMemcpyAsync(dt.p, tid.data(), 4 * n, hipMemcpyHostToDevice, c.stream), "copy tapes");
+        hip_check(launch_point_grads(tapes(), dt.p, d.p, d.p + 3 * n, d.p + 4 * n, da.p, n,
+                                     c.stream), "point partials");
         hip_check(hipMemcpyAsync(v.data(), d.p + 3 * n, 4 * n, hipMemcpyDeviceToHost, c.stream), "copy values");
         hip_check(hipMemcpyAsync(g.data(), d.p + 4 * n, 12 * n, hipMemcpyDeviceToHost, c.stream), "copy partials");
         hip_check(hipMemcpyAsync(amb.data(), da.p, n, hipMemcpyDeviceToHost, c.stream), "copy flags");
@@ -161,23 +172,25 @@ struct Device {
     }
 
     // isInside (want_inside) or the feature gradients of each point.
-    void features(const std::vector<float>& xyz, bool want_inside,
-                  std::vector<uint8_t>& inside, std::vector<float>& feats,
+    void features(const std::vector<float>& xyz, const std::vector<uint32_t>& tid,
+                  bool want_inside, std::vector<uint8_t>& inside, std::vector<float>& feats,
                   std::vector<int32_t>& nfeat) {
         const size_t n = xyz.size() / 3;
         inside.resize(n);
         feats.resize(n * 16 * 3);
         nfeat.resize(n);
         if (!n) return;
-        const size_t wsb = point_feature_workspace(gt.size(), nregs);
+        const size_t wsb = point_feature_workspace(tp.max_len(), tp.nregs());
         const size_t waves = std::min<size_t>(n, kFeatureWaves);
         PoolBuf<float> d(3 * n + n * 16 * 3);
         PoolBuf<uint8_t> di(n), ws(waves * wsb);
         PoolBuf<int32_t> dn(n);
+        PoolBuf<uint32_t> dt(n);
         hip_check(hipMemsetAsync(err.p, 0, 4, c.stream), "clear error");
         hip_check(hipMemcpyAsync(d.p, xyz.data(), 12 * n, hipMemcpyHostToDevice, c.stream), "copy points");
-        hip_check(launch_point_features(gt.d.p, gt.size(), gt.host.root_reg, nregs, sx, sy, sz,
-                                        d.p, want_inside ? di.p : nullptr,
+        hip_check(hipMemcpyAsync(dt.p, tid.data(), 4 * n, hipMemcpyHostToDevice, c.stream), "copy tapes");
+        hip_check(launch_point_features(tapes(), dt.p, sx, sy, sz, d.p,
+                                        want_inside ? di.p : nullptr,
                                         want_inside ? nullptr : d.p + 3 * n, dn.p, ws.p, wsb,
                                         err.p, n, c.stream), "point features");
         uint32_t e = 0;
@@ -210,6 +223,7 @@ struct Edge {
     int32_t leaf;
     int patch;
     int id;                  // marching-table edge id
+    uint32_t tape;           // the leaf's tape
     double a[2], b[2];       // inside and outside bracket ends
 };
 
@@ -229,9 +243,8 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
     if (top > 24) throw Error("render_slice: resolution too fine (more than 24 levels)");
 
     std::lock_guard<std::mutex> lock(c.mu);
-    GpuTape gt(base);
-    Device dev{c, gt, std::max<uint32_t>(base.num_regs, tape::kFirstFreeReg),
-               deck.X, deck.Y, deck.Z};
+    SliceTapes tapes(base, deck.X, deck.Y, deck.Z, c.stream);
+    Device dev{c, tapes, deck.X, deck.Y, deck.Z};
     const Squares& M = squares();
     Quadtree q;
     ContourResult out;
@@ -247,43 +260,39 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
     }
 
     // Interval subdivision, one launch per level (DCTree::evalInterval,
-    // dc_tree.inl:94-126).
+    // dc_tree.inl:94-126): each cell is evaluated with its parent's pushed
+    // tape and pushes its own, except where the result may be NaN.
     std::vector<std::vector<int32_t>> branches(top + 1);
     std::vector<int32_t> frontier{0};
+    std::vector<float> box, r;
+    std::vector<uint8_t> nan;
+    std::vector<uint32_t> tin, pushed;
     for (int L = top; L > 0 && !frontier.empty(); --L) {
         const size_t n = frontier.size();
-        std::vector<float> box(6 * n);
+        box.resize(6 * n);
+        tin.resize(n);
         for (size_t k = 0; k < n; ++k) {
             const Cell& cl = q.cells[frontier[k]];
-            box[k] = float(cl.lo[0]);
-            box[n + k] = float(cl.hi[0]);
-            box[2 * n + k] = float(cl.lo[1]);
-            box[3 * n + k] = float(cl.hi[1]);
-            box[4 * n + k] = z;
-            box[5 * n + k] = z;
+            box[6 * k] = float(cl.lo[0]);
+            box[6 * k + 1] = float(cl.hi[0]);
+            box[6 * k + 2] = float(cl.lo[1]);
+            box[6 * k + 3] = float(cl.hi[1]);
+            box[6 * k + 4] = z;
+            box[6 * k + 5] = z;
+            tin[k] = cl.tape_in;
         }
-        PoolBuf<float> d(8 * n);
-        PoolBuf<uint8_t> dn(n);
-        hip_check(hipMemcpyAsync(d.p, box.data(), 24 * n, hipMemcpyHostToDevice, c.stream), "copy boxes");
-        hip_check(launch_eval_intervals(gt.d.p, gt.size(), base.root_reg, dev.nregs, d.p,
-                                        d.p + n, d.p + 2 * n, d.p + 3 * n, d.p + 4 * n,
-                                        d.p + 5 * n, d.p + 6 * n, d.p + 7 * n, dn.p, n,
-                                        c.stream), "slice intervals");
-        std::vector<float> r(2 * n);
-        std::vector<uint8_t> nan(n);
-        hip_check(hipMemcpyAsync(r.data(), d.p + 6 * n, 8 * n, hipMemcpyDeviceToHost, c.stream), "copy bounds");
-        hip_check(hipMemcpyAsync(nan.data(), dn.p, n, hipMemcpyDeviceToHost, c.stream), "copy nan");
-        hip_check(hipStreamSynchronize(c.stream), "slice intervals");
+        tapes.eval(box, tin, false, r, nan, pushed);
         std::vector<int32_t> next;
         for (size_t k = 0; k < n; ++k) {
             const int32_t ci = frontier[k];
             // Interval::state (interval.hpp:66-77)
             uint8_t ty = AMBIGUOUS;
             if (!nan[k]) {
-                if (r[k] > 0) ty = EMPTY;
-                else if (r[n + k] < 0) ty = FILLED;
+                if (r[2 * k] > 0) ty = EMPTY;
+                else if (r[2 * k + 1] < 0) ty = FILLED;
             }
             q.cells[ci].type = ty;
+            q.cells[ci].tape_out = pushed[k];
             const Cell cl = q.cells[ci];
             out.cells.insert(out.cells.end(), {cl.key[0], cl.key[1], uint32_t(L), ty});
             if (ty != AMBIGUOUS) continue;
@@ -291,16 +300,17 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
             q.cells[ci].kids = first;
             branches[L].push_back(ci);
             for (int i = 0; i < 4; ++i) {
-                Cell k;
+                Cell kid;
                 for (int j = 0; j < 2; ++j) {
                     const double mid = (cl.lo[j] + cl.hi[j]) / 2;
                     const bool up = (i >> j) & 1;
-                    k.lo[j] = up ? mid : cl.lo[j];
-                    k.hi[j] = up ? cl.hi[j] : mid;
-                    k.key[j] = cl.key[j] + (up ? (1u << (L - 1)) : 0u);
+                    kid.lo[j] = up ? mid : cl.lo[j];
+                    kid.hi[j] = up ? cl.hi[j] : mid;
+                    kid.key[j] = cl.key[j] + (up ? (1u << (L - 1)) : 0u);
                 }
-                k.level = L - 1;
-                q.cells.push_back(k);
+                kid.level = L - 1;
+                kid.tape_in = pushed[k];
+                q.cells.push_back(kid);
                 next.push_back(first + i);
             }
         }
@@ -311,9 +321,11 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
     const size_t nl = frontier.size();
     if (nl) {
         std::vector<float> pts(12 * nl);
+        std::vector<uint32_t> ptape(4 * nl);
         for (size_t k = 0; k < nl; ++k) {
             const Cell& cl = q.cells[frontier[k]];
             for (int i = 0; i < 4; ++i) {
+                ptape[4 * k + i] = cl.tape_in;
                 pts[3 * (4 * k + i)] = float((i & 1) ? cl.hi[0] : cl.lo[0]);
                 pts[3 * (4 * k + i) + 1] = float((i & 2) ? cl.hi[1] : cl.lo[1]);
                 pts[3 * (4 * k + i) + 2] = z;
@@ -321,7 +333,7 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
         }
         std::vector<float> v, g;
         std::vector<uint8_t> amb;
-        dev.values(pts, v, amb);
+        dev.values(pts, ptape, v, amb);
         std::vector<uint8_t> state(4 * nl, EMPTY);
         std::vector<size_t> zeros, ties;
         for (size_t i = 0; i < 4 * nl; ++i) {
@@ -337,10 +349,15 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
             }
             return p;
         };
+        auto gather_tapes = [&](const std::vector<size_t>& idx) {
+            std::vector<uint32_t> t(idx.size());
+            for (size_t j = 0; j < idx.size(); ++j) t[j] = ptape[idx[j]];
+            return t;
+        };
         if (!zeros.empty()) {
             std::vector<float> zv;
             std::vector<uint8_t> za;
-            dev.grads(gather(zeros), zv, g, za);
+            dev.grads(gather(zeros), gather_tapes(zeros), zv, g, za);
             for (size_t j = 0; j < zeros.size(); ++j) {
                 const bool nz = g[3 * j] != 0 || g[3 * j + 1] != 0 || g[3 * j + 2] != 0;
                 state[zeros[j]] = nz ? FILLED : EMPTY;
@@ -350,7 +367,7 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
             std::vector<uint8_t> ins;
             std::vector<float> fs;
             std::vector<int32_t> nf;
-            dev.features(gather(ties), true, ins, fs, nf);
+            dev.features(gather(ties), gather_tapes(ties), true, ins, fs, nf);
             for (size_t j = 0; j < ties.size(); ++j) state[ties[j]] = ins[j] ? FILLED : EMPTY;
         }
 
@@ -379,6 +396,7 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
                     ed.leaf = cl.leaf;
                     ed.patch = p;
                     ed.id = M.e[a][b];
+                    ed.tape = cl.tape_in;
                     ed.a[0] = (a & 1) ? cl.hi[0] : cl.lo[0];
                     ed.a[1] = (a & 2) ? cl.hi[1] : cl.lo[1];
                     ed.b[0] = (b & 1) ? cl.hi[0] : cl.lo[0];
@@ -396,7 +414,9 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
         std::vector<uint8_t> stop(ne, 4);
         if (ne) {
             std::vector<double> ends(4 * ne);
+            std::vector<uint32_t> etape(ne);
             for (size_t e = 0; e < ne; ++e) {
+                etape[e] = edges[e].tape;
                 ends[4 * e] = edges[e].a[0];
                 ends[4 * e + 1] = edges[e].a[1];
                 ends[4 * e + 2] = edges[e].b[0];
@@ -404,10 +424,13 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
             }
             PoolBuf<double> de(4 * ne);
             PoolBuf<uint8_t> dst(ne);
+            PoolBuf<uint32_t> dt(ne);
             hip_check(hipMemcpyAsync(de.p, ends.data(), 32 * ne, hipMemcpyHostToDevice, c.stream),
                       "copy edges");
-            hip_check(launch_edge_search(gt.d.p, gt.size(), base.root_reg, dev.nregs, z, de.p,
-                                         dst.p, ne, c.stream), "edge search");
+            hip_check(hipMemcpyAsync(dt.p, etape.data(), 4 * ne, hipMemcpyHostToDevice, c.stream),
+                      "copy edge tapes");
+            hip_check(launch_edge_search(dev.tapes(), dt.p, z, de.p, dst.p, ne, c.stream),
+                      "edge search");
             hip_check(hipMemcpyAsync(ends.data(), de.p, 32 * ne, hipMemcpyDeviceToHost, c.stream),
                       "copy brackets");
             hip_check(hipMemcpyAsync(stop.data(), dst.p, ne, hipMemcpyDeviceToHost, c.stream),
@@ -429,11 +452,13 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
             const size_t na = act.size();
             std::vector<double> sp(2 * 16 * na);
             std::vector<float> sx(3 * 16 * na);
+            std::vector<uint32_t> st(16 * na);
             for (size_t k = 0; k < na; ++k) {
                 const Edge& ed = edges[act[k]];
                 for (int j = 0; j < 16; ++j) {
                     const double f = j / 15.0;
                     const size_t i = 16 * k + j;
+                    st[i] = ed.tape;
                     for (int a = 0; a < 2; ++a) {
                         sp[2 * i + a] = ed.a[a] * (1 - f) + ed.b[a] * f;
                         sx[3 * i + a] = float(sp[2 * i + a]);
@@ -443,7 +468,7 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
             }
             std::vector<float> sv;
             std::vector<uint8_t> sa;
-            dev.values(sx, sv, sa);
+            dev.values(sx, st, sv, sa);
             // isInside for the samples that are exactly zero
             std::vector<size_t> zs;
             for (size_t i = 0; i < 16 * na; ++i) {
@@ -452,10 +477,14 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
             std::vector<uint8_t> zin;
             if (!zs.empty()) {
                 std::vector<float> zp(3 * zs.size());
-                for (size_t j = 0; j < zs.size(); ++j) std::memcpy(&zp[3 * j], &sx[3 * zs[j]], 12);
+                std::vector<uint32_t> zt(zs.size());
+                for (size_t j = 0; j < zs.size(); ++j) {
+                    std::memcpy(&zp[3 * j], &sx[3 * zs[j]], 12);
+                    zt[j] = st[zs[j]];
+                }
                 std::vector<float> fs;
                 std::vector<int32_t> nf;
-                dev.features(zp, true, zin, fs, nf);
+                dev.features(zp, zt, true, zin, fs, nf);
             }
             // First sample after the inside end that is outside (v > 0, or
             // v == 0 and not inside); else the last one, NaN included.
@@ -486,7 +515,9 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
 
         // Value and partials at both bracket ends (dc_tree.inl:412-471).
         std::vector<float> bx(6 * ne);
+        std::vector<uint32_t> bt(2 * ne);
         for (size_t e = 0; e < ne; ++e) {
+            bt[2 * e] = bt[2 * e + 1] = edges[e].tape;
             for (int a = 0; a < 2; ++a) {
                 bx[6 * e + a] = float(edges[e].a[a]);
                 bx[6 * e + 3 + a] = float(edges[e].b[a]);
@@ -496,7 +527,7 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
         }
         std::vector<float> bv, bg;
         std::vector<uint8_t> ba;
-        dev.grads(bx, bv, bg, ba);
+        dev.grads(bx, bt, bv, bg, ba);
         std::vector<size_t> amb_ends;
         for (size_t i = 0; i < 2 * ne; ++i) if (ba[i]) amb_ends.push_back(i);
         std::vector<uint8_t> fin;
@@ -504,10 +535,12 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
         std::vector<int32_t> nf;
         if (!amb_ends.empty()) {
             std::vector<float> fp(3 * amb_ends.size());
+            std::vector<uint32_t> ft(amb_ends.size());
             for (size_t j = 0; j < amb_ends.size(); ++j) {
                 std::memcpy(&fp[3 * j], &bx[3 * amb_ends[j]], 12);
+                ft[j] = bt[amb_ends[j]];
             }
-            dev.features(fp, false, fin, fs, nf);
+            dev.features(fp, ft, false, fin, fs, nf);
         }
 
         // QEF per patch (dc_tree.inl:412-552): edges of one leaf and patch
@@ -598,7 +631,10 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
     }
 
     // Collapse, bottom up (DCTree::collectChildren, dc_tree.inl:581-735);
-    // the vertex values of each level's candidates in one launch.
+    // the vertex values of each level's candidates in one launch, each
+    // with the tape Tape::getBase picks for the vertex (dc_tree.inl:716)
+    // starting from the cell's pushed tape narrowed to the cell.
+    tapes.refresh_records();
     for (int L = 1; L <= top; ++L) {
         struct Cand { int32_t cell; Leaf leaf; };
         std::vector<Cand> cands;
@@ -658,14 +694,20 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
         }
         if (cands.empty()) continue;
         std::vector<float> cp(3 * cands.size());
+        std::vector<uint32_t> ct(cands.size());
         for (size_t j = 0; j < cands.size(); ++j) {
             cp[3 * j] = float(cands[j].leaf.vert[0][0]);
             cp[3 * j + 1] = float(cands[j].leaf.vert[0][1]);
             cp[3 * j + 2] = z;
+            const Cell& cl = q.cells[cands[j].cell];
+            const double clo[3] = {cl.lo[0], cl.lo[1], double(z)};
+            const double chi[3] = {cl.hi[0], cl.hi[1], double(z)};
+            const double vd[3] = {cp[3 * j], cp[3 * j + 1], cp[3 * j + 2]};
+            ct[j] = tapes.base_for(tapes.base_for(cl.tape_out, clo, chi), vd, vd);
         }
         std::vector<float> cv;
         std::vector<uint8_t> ca;
-        dev.values(cp, cv, ca);
+        dev.values(cp, ct, cv, ca);
         for (size_t j = 0; j < cands.size(); ++j) {
             if (!(std::fabs(cv[j]) < max_err)) continue;
             Cell& cl = q.cells[cands[j].cell];

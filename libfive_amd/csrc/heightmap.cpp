@@ -5,10 +5,12 @@
 #include <cmath>
 #include <deque>
 #include <mutex>
+#include <string>
 
 #include "contour_kernels.hpp"
-#include "interval_kernels.hpp"
+#include "deck.hpp"
 #include "runtime.hpp"
+#include "slice_tapes.hpp"
 
 namespace libfive {
 namespace amd {
@@ -87,11 +89,17 @@ PixelImage render_pixels(const Tree& t, float xlo, float xhi, float ylo,
     img.filled.assign(size_t(img.width) * img.height, 0);
     const float zs = g.pts[2].back();    // the slice's single voxel centre
 
-    auto gt = compile_tree(t);
+    const Deck deck(t);
+    const DeviceTape base = compile_deck(deck);
+    if (base.num_regs > 128) {
+        throw Error("render_pixels: the tape needs " + std::to_string(base.num_regs) +
+                    " registers; the raster supports 128");
+    }
     std::lock_guard<std::mutex> lock(c.mu);
+    SliceTapes tapes(base, deck.X, deck.Y, deck.Z, c.stream);
 
     // Initial regions: XY halves until there are 8 (Heightmap::render with
-    // 8 workers, heightmap.cpp:285-294).
+    // 8 workers, heightmap.cpp:285-294), all with the base tape.
     View root;
     for (int i = 0; i < 3; ++i) {
         root.lower[i] = g.lower[i];
@@ -117,56 +125,56 @@ PixelImage render_pixels(const Tree& t, float xlo, float xhi, float ylo,
         }
     };
 
-    // Heightmap::recurse (heightmap.cpp:193-246), one level at a time.
+    // Heightmap::recurse (heightmap.cpp:193-246), one level at a time.  A
+    // view is interval-evaluated with the tape its parent pushed
+    // (intervalAndPush, heightmap.cpp:219-222), and both halves of an
+    // ambiguous view, or its pixels, use the tape pushed for it.
     std::vector<View> frontier(rs.begin(), rs.end()), pixel_views;
-    std::vector<float> boxes, olo, ohi;
+    std::vector<uint32_t> ftape(frontier.size(), 0), pixel_tapes;
+    std::vector<float> boxes, bounds;
+    std::vector<uint8_t> nan;
+    std::vector<uint32_t> btape, pushed;
     while (!frontier.empty()) {
         std::vector<View> big;
-        for (const View& v : frontier) {
+        btape.clear();
+        for (size_t k = 0; k < frontier.size(); ++k) {
+            const View& v = frontier[k];
             if (v.voxels() == 0) continue;
-            if (v.voxels() <= 256) pixel_views.push_back(v);
-            else big.push_back(v);
-        }
-        frontier.clear();
-        const size_t n = big.size();
-        if (n == 0) break;
-        boxes.assign(6 * n, 0.0f);
-        for (size_t k = 0; k < n; ++k) {
-            for (int a = 0; a < 3; ++a) {
-                boxes[(2 * a) * n + k] = big[k].lower[a];
-                boxes[(2 * a + 1) * n + k] = big[k].upper[a];
+            if (v.voxels() <= 256) {
+                pixel_views.push_back(v);
+                pixel_tapes.push_back(ftape[k]);
+            } else {
+                big.push_back(v);
+                btape.push_back(ftape[k]);
             }
         }
-        PoolBuf<float> d(8 * n);
-        PoolBuf<uint8_t> dn(n);
-        hip_check(hipMemcpyAsync(d.p, boxes.data(), 6 * n * sizeof(float),
-                                 hipMemcpyHostToDevice, c.stream), "copy boxes");
-        const auto& h = gt->host;
-        hip_check(launch_eval_intervals(gt->d.p, gt->size(), h.root_reg, h.num_regs,
-                                        d.p, d.p + n, d.p + 2 * n, d.p + 3 * n,
-                                        d.p + 4 * n, d.p + 5 * n, d.p + 6 * n,
-                                        d.p + 7 * n, dn.p, n, c.stream),
-                  "pixel intervals");
-        olo.resize(n);
-        ohi.resize(n);
-        hip_check(hipMemcpyAsync(olo.data(), d.p + 6 * n, n * sizeof(float),
-                                 hipMemcpyDeviceToHost, c.stream), "copy lo");
-        hip_check(hipMemcpyAsync(ohi.data(), d.p + 7 * n, n * sizeof(float),
-                                 hipMemcpyDeviceToHost, c.stream), "copy hi");
-        hip_check(hipStreamSynchronize(c.stream), "pixel intervals");
+        frontier.clear();
+        ftape.clear();
+        const size_t n = big.size();
+        if (n == 0) break;
+        boxes.resize(6 * n);
+        for (size_t k = 0; k < n; ++k) {
+            for (int a = 0; a < 3; ++a) {
+                boxes[6 * k + 2 * a] = big[k].lower[a];
+                boxes[6 * k + 2 * a + 1] = big[k].upper[a];
+            }
+        }
+        tapes.eval(boxes, btape, true, bounds, nan, pushed);
         for (size_t k = 0; k < n; ++k) {
             // Interval::isFilled / isEmpty on the bounds alone; the
             // may-be-NaN flag plays no part here (heightmap.cpp:221-233).
-            if (ohi[k] < 0.0f) {
+            if (bounds[2 * k + 1] < 0.0f) {
                 fill(big[k]);
                 continue;
-            } else if (olo[k] > 0.0f) {
+            } else if (bounds[2 * k] > 0.0f) {
                 continue;
             }
             View a, b;
             split(big[k], 7u, a, b);
             frontier.push_back(b);
             frontier.push_back(a);
+            ftape.push_back(pushed[k]);
+            ftape.push_back(pushed[k]);
         }
     }
 
@@ -190,15 +198,17 @@ PixelImage render_pixels(const Tree& t, float xlo, float xhi, float ylo,
         const size_t w = g.pts[0].size(), hgt = g.pts[1].size();
         PoolBuf<float> dp(w + hgt);
         PoolBuf<int32_t> dv(4 * nv);
-        PoolBuf<uint32_t> doff(nv);
+        PoolBuf<uint32_t> doff(nv), dvt(nv);
         PoolBuf<uint8_t> dout(np);
         hip_check(hipMemcpyAsync(dp.p, g.pts[0].data(), 4 * w, hipMemcpyHostToDevice, c.stream), "copy x");
         hip_check(hipMemcpyAsync(dp.p + w, g.pts[1].data(), 4 * hgt, hipMemcpyHostToDevice, c.stream), "copy y");
         hip_check(hipMemcpyAsync(dv.p, vd.data(), 16 * nv, hipMemcpyHostToDevice, c.stream), "copy views");
         hip_check(hipMemcpyAsync(doff.p, offs.data(), 4 * nv, hipMemcpyHostToDevice, c.stream), "copy offsets");
-        const auto& h = gt->host;
-        hip_check(launch_raster_views(gt->d.p, gt->size(), h.root_reg, h.num_regs, dp.p,
-                                      dp.p + w, zs, dv.p, doff.p, dout.p, nv, c.stream),
+        hip_check(hipMemcpyAsync(dvt.p, pixel_tapes.data(), 4 * nv, hipMemcpyHostToDevice, c.stream), "copy view tapes");
+        const PoolTapes T{tapes.pool(), tapes.records(), tapes.root(), tapes.nregs(),
+                          tapes.max_len()};
+        hip_check(launch_raster_views(T, dvt.p, dp.p, dp.p + w, zs, dv.p, doff.p, dout.p, nv,
+                                      c.stream),
                   "pixel values");
         std::vector<uint8_t> out(np);
         hip_check(hipMemcpyAsync(out.data(), dout.p, np, hipMemcpyDeviceToHost, c.stream),

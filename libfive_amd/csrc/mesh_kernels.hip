@@ -96,46 +96,40 @@ struct Batch4 {
     Clause c[4];
 };
 
+// Interval evaluation of up to 8 boxes with one tape, one box per lane
+// (lanes 0-7; `active` marks the lanes in use), then Tape::push
+// (tape.cpp:23-158) for each box whose result is neither empty nor filled,
+// and not NaN unless push_nan.  Returns the tape that the box's children
+// use: the pushed tape, or tape_id when the push would change nothing.
+//
+// Work arrays: choice (per clause, the lanes that keep a and those that
+// keep b at a MIN/MAX; 2 bytes) and lv (per clause, the lanes for which it
+// is live; 1 byte), in LDS or, for long tapes, in a global workspace; need
+// (ceil(nregs / 32) words per lane, 8 lanes) and bytes (16 * nregs: per
+// lane register renaming and alias counts) in LDS.
+//
+// Tapes and records are read through pool_in and recs_in, which alias
+// P.pool and P.tapes: a pass only reads tapes made by earlier passes and
+// only writes new ones, and saying so lets the compiler keep the clause
+// reads on the scalar unit.
 template <int NV>
-__global__ __launch_bounds__(64)
-void k_interval(Params P, int level, const uint32_t* __restrict__ groups,
-                LevelView parent, LevelView cur, uint32_t choice_words)
+__device__ __forceinline__ uint32_t interval_push(const Params& P,
+                                                  const Clause* __restrict__ pool_in,
+                                                  const TapeRec* __restrict__ recs_in,
+                                                  uint32_t tape_id,
+                                                  bool active, const float flo[3],
+                                                  const float fhi[3], bool push_nan,
+                                                  uint16_t* choice, uint8_t* lv,
+                                                  uint32_t* need, uint8_t* bytes,
+                                                  float& rlo, float& rhi, bool& rnan)
 {
-    HIP_DYNAMIC_SHARED(uint32_t, lds)
     const int lane = threadIdx.x;
-    const bool at_root = (groups == nullptr);
-    uint32_t tape_id = 0, first = 0, pkey[3] = {0, 0, 0};
-    if (!at_root) {
-        const uint32_t p = groups[blockIdx.x];
-        tape_id = parent.tape[p];
-        first = uint32_t(parent.child[p]);
-        for (int a = 0; a < 3; ++a) pkey[a] = parent.key[3 * p + a];
-    }
-    tape_id = __builtin_amdgcn_readfirstlane(tape_id);
-    const uint32_t nchild = at_root ? 1u : 8u;
-    const bool active = uint32_t(lane) < nchild;
-    const uint32_t octant = active ? uint32_t(lane) : 0u;
-    const uint32_t cidx = first + octant;
-    uint32_t key[3];
-    for (int a = 0; a < 3; ++a) {
-        key[a] = pkey[a] + (at_root ? 0u : (((octant >> a) & 1u) << level));
-    }
-    double lo[3], hi[3];
-    cell_bounds(P, key, level, lo, hi);
-    float flo[3], fhi[3];
-    for (int a = 0; a < 3; ++a) {
-        flo[a] = float(lo[a]);
-        fhi[a] = float(hi[a]);
-    }
-
-    const TapeRec tr = P.tapes[tape_id];
+    const TapeRec tr = recs_in[tape_id];
     const uint32_t off = __builtin_amdgcn_readfirstlane(tr.off);
     const uint32_t n = __builtin_amdgcn_readfirstlane(tr.len);
-    const Clause* __restrict__ t = P.pool + off;
-    uint16_t* choice = reinterpret_cast<uint16_t*>(lds);
-    uint32_t* rslot = lds + choice_words;
+    const uint32_t tflags = __builtin_amdgcn_readfirstlane(tr.flags);
+    const Clause* __restrict__ t = pool_in + off;
     const uint32_t W = (P.nregs + 31u) / 32u;
-    uint32_t* need = rslot + P.nregs;
 
     // The reference slot each register's value resolves to (the same in
     // every lane), kept as a register bank so that the a == b test of the
@@ -212,43 +206,34 @@ void k_interval(Params P, int level, const uint32_t* __restrict__ groups,
       cur4 = nxt4;
     }
     const uint32_t root = P.root_reg;
-    const float rlo = LF_RF_GET(L, NV, root);
-    const float rhi = LF_RF_GET(U, NV, root);
-    const bool rnan = LF_RF_GET(F, NV, root) != 0.0f;
-    // Interval::state (interval.hpp:66-77)
-    uint8_t state;
-    if (rnan) state = AMBIGUOUS;
-    else if (rlo > 0.0f) state = EMPTY;
-    else if (rhi < 0.0f) state = FILLED;
-    else state = AMBIGUOUS;
+    rlo = LF_RF_GET(L, NV, root);
+    rhi = LF_RF_GET(U, NV, root);
+    rnan = LF_RF_GET(F, NV, root) != 0.0f;
 
-    // Tape::push for every child that may prune (tape.cpp:23-158), in two
-    // walks per child lane.  The first (root first) marks the clauses whose
-    // result is still needed, a MIN/MAX resolved to one side needing only
-    // that side.  The second (leaves first) writes the kept clauses with
-    // their operands renamed through the resolved clauses, as the
-    // reference resolves its remap chains (tape.cpp:124-146): a resolved
-    // clause or a copy makes its destination an alias of the register that
-    // holds the value and writes nothing.  Because registers are reused,
-    // an alias is written out as a copy just before its register would be
-    // overwritten; an alias always names a register that holds its own
-    // value, so copies never chain.
-    bool dopush = active && state == AMBIGUOUS && !rnan &&
-                  !(tr.flags & TAPE_TERMINAL) && any_ab;
+    // Tape::push for every box that may prune, in two walks per lane.  The
+    // first (root first) marks the clauses whose result is still needed, a
+    // MIN/MAX resolved to one side needing only that side.  The second
+    // (leaves first) writes the kept clauses with their operands renamed
+    // through the resolved clauses, as the reference resolves its remap
+    // chains (tape.cpp:124-146): a resolved clause or a copy makes its
+    // destination an alias of the register that holds the value and writes
+    // nothing.  Because registers are reused, an alias is written out as a
+    // copy just before its register would be overwritten; an alias always
+    // names a register that holds its own value, so copies never chain.
+    bool dopush = active && !(rlo > 0.0f) && !(rhi < 0.0f) && (push_nan || !rnan) &&
+                  !(tflags & TAPE_TERMINAL) && any_ab;
     uint32_t child_tape = tape_id;
     __syncthreads();
     if (ballot(dopush)) {
         uint32_t* my = need + uint32_t(lane & 7) * W;
-        uint8_t* const bytes = reinterpret_cast<uint8_t*>(need + 8u * W);
         uint8_t* ren = bytes + uint32_t(lane & 7) * P.nregs;
         uint8_t* cnt = ren + 8u * P.nregs;
-        uint8_t* lv = bytes + 16u * P.nregs;     // shared by the child lanes
         uint32_t base = 0;
         if (dopush) {
             for (uint32_t w = 0; w < W; ++w) my[w] = 0;
             my[root >> 5] |= 1u << (root & 31u);
-            base = atomicAdd(&P.counters->pool_top, n + 9u);
-            if (base + n + 9u > P.pool_cap) {
+            base = atomicAdd(&P.counters->pool_top, n + kPushExtra);
+            if (base + n + kPushExtra > P.pool_cap) {
                 atomicOr(&P.counters->error, ERR_POOL);
                 dopush = false;
             }
@@ -291,7 +276,7 @@ void k_interval(Params P, int level, const uint32_t* __restrict__ groups,
           }
           bcur = bprv;
         }
-        wave_sync();
+        __syncthreads();
         uint32_t pos = base, refc = 0;
         const bool emit = dopush && changed;
         if (emit) {
@@ -390,24 +375,141 @@ void k_interval(Params P, int level, const uint32_t* __restrict__ groups,
             }
         }
     }
-    if (active && state == AMBIGUOUS) {
-        atomicMax(&P.counters->max_len, P.tapes[child_tape].len);
-    }
-    if (active) {
-        cur.type[cidx] = state;
-        cur.tape[cidx] = child_tape;
-        cur.child[cidx] = -1;
-        for (int a = 0; a < 3; ++a) cur.key[3 * cidx + a] = key[a];
-    }
+    return child_tape;
 }
 
-template <int NV>
-static void launch_interval_nv(const Params& p, int level, const uint32_t* groups,
-                               uint32_t ngroups, const LevelView& parent,
-                               const LevelView& cur, uint32_t choice_words,
-                               size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL(k_interval<NV>, dim3(ngroups), dim3(64), lds, s, p, level, groups, parent,
-                                                        cur, choice_words);
+// Carves a workgroup's work arrays (see interval_push) out of LDS, or
+// choice and lv out of the global workspace ws when `global`.
+struct IntervalWork {
+    uint16_t* choice;
+    uint8_t* lv;
+    uint32_t* need;
+    uint8_t* bytes;
+};
+// The two forms are separate instantiations so that the compiler keeps LDS
+// accesses as LDS instructions (a pointer that may point to either memory
+// would make every access a flat one).
+template <bool GLOBAL>
+__device__ __forceinline__ IntervalWork interval_work(uint32_t* lds, const Params& P,
+                                                      uint32_t max_len, uint8_t* ws) {
+    const uint32_t W = (P.nregs + 31u) / 32u;
+    IntervalWork w;
+    if constexpr (GLOBAL) {
+        uint8_t* arrays = ws + size_t(blockIdx.x) * interval_ws_stride(max_len);
+        w.choice = reinterpret_cast<uint16_t*>(arrays);
+        w.lv = arrays + 2u * size_t(max_len);
+        w.need = lds;
+    } else {
+        // LDS: [choice, lv (3 * max_len bytes, rounded to words)], need, bytes
+        w.choice = reinterpret_cast<uint16_t*>(lds);
+        w.lv = reinterpret_cast<uint8_t*>(lds) + 2u * size_t(max_len);
+        w.need = lds + (3u * max_len + 3u) / 4u;
+    }
+    w.bytes = reinterpret_cast<uint8_t*>(w.need + 8u * W);
+    return w;
+}
+
+// One wave per ambiguous parent (8 children), or the root alone when
+// groups is null; workgroup i takes group g0 + i.  The global-workspace
+// form is launched over at most kIntervalWaves groups at a time, so that
+// workgroup i owns slot i of the workspace.  (A loop over groups inside
+// the kernel would make the compiler keep the tape reads on the vector
+// unit and spill the register files.)
+template <int NV, bool GLOBAL>
+__global__ __launch_bounds__(64)
+void k_interval(Params P, const Clause* __restrict__ pool_in,
+                const TapeRec* __restrict__ recs_in, int level,
+                const uint32_t* __restrict__ groups, uint32_t g0, LevelView parent,
+                LevelView cur, uint32_t max_len, uint8_t* ws)
+{
+    HIP_DYNAMIC_SHARED(uint32_t, lds)
+    const IntervalWork w = interval_work<GLOBAL>(lds, P, max_len, ws);
+    const int lane = threadIdx.x;
+    const bool at_root = (groups == nullptr);
+    auto run = [&](uint32_t g) {
+        uint32_t tape_id = 0, first = 0, pkey[3] = {0, 0, 0};
+        if (!at_root) {
+            const uint32_t p = groups[g];
+            tape_id = parent.tape[p];
+            first = uint32_t(parent.child[p]);
+            for (int a = 0; a < 3; ++a) pkey[a] = parent.key[3 * p + a];
+        }
+        tape_id = __builtin_amdgcn_readfirstlane(tape_id);
+        const uint32_t nchild = at_root ? 1u : 8u;
+        const bool active = uint32_t(lane) < nchild;
+        const uint32_t octant = active ? uint32_t(lane) : 0u;
+        const uint32_t cidx = first + octant;
+        uint32_t key[3];
+        for (int a = 0; a < 3; ++a) {
+            key[a] = pkey[a] + (at_root ? 0u : (((octant >> a) & 1u) << level));
+        }
+        double lo[3], hi[3];
+        cell_bounds(P, key, level, lo, hi);
+        float flo[3], fhi[3];
+        for (int a = 0; a < 3; ++a) {
+            flo[a] = float(lo[a]);
+            fhi[a] = float(hi[a]);
+        }
+        float rlo, rhi;
+        bool rnan;
+        const uint32_t child_tape = interval_push<NV>(P, pool_in, recs_in, tape_id, active,
+                                                      flo, fhi, false,
+                                                      w.choice, w.lv, w.need, w.bytes,
+                                                      rlo, rhi, rnan);
+        // Interval::state (interval.hpp:66-77)
+        uint8_t state;
+        if (rnan) state = AMBIGUOUS;
+        else if (rlo > 0.0f) state = EMPTY;
+        else if (rhi < 0.0f) state = FILLED;
+        else state = AMBIGUOUS;
+        if (active && state == AMBIGUOUS) {
+            atomicMax(&P.counters->max_len, P.tapes[child_tape].len);
+        }
+        if (active) {
+            cur.type[cidx] = state;
+            cur.tape[cidx] = child_tape;
+            cur.child[cidx] = -1;
+            for (int a = 0; a < 3; ++a) cur.key[3 * cidx + a] = key[a];
+        }
+    };
+    run(g0 + blockIdx.x);
+}
+
+// Groups of up to 8 boxes that share a tape, one wave per group.
+template <int NV, bool GLOBAL>
+__global__ __launch_bounds__(64)
+void k_interval_boxes(Params P, const Clause* __restrict__ pool_in,
+                      const TapeRec* __restrict__ recs_in,
+                      const BoxGroup* __restrict__ groups, uint32_t g0,
+                      const float* __restrict__ boxes, float* __restrict__ bounds,
+                      uint8_t* __restrict__ nan, uint32_t* __restrict__ tapes,
+                      uint32_t push_nan, uint32_t max_len, uint8_t* ws)
+{
+    HIP_DYNAMIC_SHARED(uint32_t, lds)
+    const IntervalWork w = interval_work<GLOBAL>(lds, P, max_len, ws);
+    const int lane = threadIdx.x;
+    auto run = [&](uint32_t g) {
+        const BoxGroup G = groups[g];
+        const uint32_t tape_id = __builtin_amdgcn_readfirstlane(G.tape);
+        const uint32_t count = __builtin_amdgcn_readfirstlane(G.count);
+        const bool active = uint32_t(lane) < count;
+        const uint32_t b = G.first + (active ? uint32_t(lane) : 0u);
+        const float flo[3] = {boxes[6 * b], boxes[6 * b + 2], boxes[6 * b + 4]};
+        const float fhi[3] = {boxes[6 * b + 1], boxes[6 * b + 3], boxes[6 * b + 5]};
+        float rlo, rhi;
+        bool rnan;
+        const uint32_t child_tape = interval_push<NV>(P, pool_in, recs_in, tape_id, active,
+                                                      flo, fhi, push_nan != 0, w.choice, w.lv,
+                                                      w.need, w.bytes, rlo, rhi, rnan);
+        if (active) {
+            bounds[2 * b] = rlo;
+            bounds[2 * b + 1] = rhi;
+            nan[b] = rnan ? 1 : 0;
+            tapes[b] = child_tape;
+            atomicMax(&P.counters->max_len, P.tapes[child_tape].len);
+        }
+    };
+    run(g0 + blockIdx.x);
 }
 
 static int banks(uint32_t nregs) {
@@ -415,24 +517,88 @@ static int banks(uint32_t nregs) {
     return b <= 1 ? 1 : b <= 2 ? 2 : b <= 4 ? 4 : 0;
 }
 
+size_t interval_lds_bytes(uint32_t nregs, uint32_t max_len) {
+    const uint32_t W = (nregs + 31) / 32;
+    return 32u * W + 16u * size_t(nregs) + 4u * ((3u * size_t(max_len) + 3u) / 4u);
+}
+
+size_t interval_workspace_bytes(uint32_t nregs, uint32_t max_len) {
+    if (interval_lds_bytes(nregs, max_len) <= kIntervalLdsMax) return 0;
+    return size_t(kIntervalWaves) * interval_ws_stride(max_len);
+}
+
+namespace {
+
+// Launch geometry of one interval pass: the LDS form runs one workgroup
+// per group in one launch; the global-workspace form launches at most
+// kIntervalWaves workgroups at a time.
+struct IntervalLaunch {
+    bool global;
+    size_t lds;
+    uint32_t chunk;
+};
+
+hipError_t interval_launch(uint32_t nregs, uint32_t max_len, uint32_t ngroups,
+                           uint8_t* ws, size_t ws_bytes, IntervalLaunch& l) {
+    if (banks(nregs) == 0) return hipErrorInvalidValue;
+    const size_t full = interval_lds_bytes(nregs, max_len);
+    l.global = full > kIntervalLdsMax;
+    l.lds = l.global ? interval_lds_bytes(nregs, 0) : full;
+    l.chunk = ngroups;
+    if (l.global) {
+        if (ws == nullptr || ws_bytes < interval_workspace_bytes(nregs, max_len)) {
+            return hipErrorInvalidValue;
+        }
+        l.chunk = kIntervalWaves;
+    }
+    return hipSuccess;
+}
+
+}   // namespace
+
+#define LF_INTERVAL_SWITCH(KERNEL, ...)                                                   \
+    for (uint32_t g0 = 0; g0 < ngroups; g0 += l.chunk) {                                  \
+        const dim3 grid(ngroups - g0 < l.chunk ? ngroups - g0 : l.chunk);                 \
+        switch (banks(p.nregs) * (l.global ? -1 : 1)) {                                   \
+            case 1: hipLaunchKernelGGL((KERNEL<1, false>), grid, dim3(64), l.lds, s, __VA_ARGS__); break; \
+            case 2: hipLaunchKernelGGL((KERNEL<2, false>), grid, dim3(64), l.lds, s, __VA_ARGS__); break; \
+            case 4: hipLaunchKernelGGL((KERNEL<4, false>), grid, dim3(64), l.lds, s, __VA_ARGS__); break; \
+            case -1: hipLaunchKernelGGL((KERNEL<1, true>), grid, dim3(64), l.lds, s, __VA_ARGS__); break; \
+            case -2: hipLaunchKernelGGL((KERNEL<2, true>), grid, dim3(64), l.lds, s, __VA_ARGS__); break; \
+            case -4: hipLaunchKernelGGL((KERNEL<4, true>), grid, dim3(64), l.lds, s, __VA_ARGS__); break; \
+            default: return hipErrorInvalidValue;                                         \
+        }                                                                                 \
+        const hipError_t le = hipGetLastError();                                          \
+        if (le != hipSuccess) return le;                                                  \
+    }
+
 hipError_t launch_interval(const Params& p, int level, const uint32_t* groups,
                            uint32_t ngroups, const LevelView& parent,
-                           const LevelView& cur, uint32_t max_len, hipStream_t s) {
+                           const LevelView& cur, uint32_t max_len, uint8_t* ws,
+                           size_t ws_bytes, hipStream_t s) {
     if (ngroups == 0) return hipSuccess;
-    const uint32_t choice_words = (max_len + 1) / 2;
-    const uint32_t W = (p.nregs + 31) / 32;
-    // choice (uint16 per clause), rslot, need bits, then per child lane the
-    // alias and alias-count bytes, then one live byte per clause
-    const size_t lds = 4u * (choice_words + p.nregs + 8u * W) + 16u * p.nregs + max_len;
-    if (lds > 65536) return hipErrorInvalidValue;
-    switch (banks(p.nregs)) {
-        case 1: launch_interval_nv<1>(p, level, groups, ngroups, parent, cur, choice_words, lds, s); break;
-        case 2: launch_interval_nv<2>(p, level, groups, ngroups, parent, cur, choice_words, lds, s); break;
-        case 4: launch_interval_nv<4>(p, level, groups, ngroups, parent, cur, choice_words, lds, s); break;
-        default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    IntervalLaunch l;
+    const hipError_t e = interval_launch(p.nregs, max_len, ngroups, ws, ws_bytes, l);
+    if (e != hipSuccess) return e;
+    LF_INTERVAL_SWITCH(k_interval, p, p.pool, p.tapes, level, groups, g0, parent, cur,
+                       max_len, ws)
+    return hipSuccess;
 }
+
+hipError_t launch_interval_boxes(const Params& p, const BoxGroup* groups, uint32_t ngroups,
+                                 const float* boxes, float* bounds, uint8_t* nan,
+                                 uint32_t* tapes, bool push_nan, uint32_t max_len,
+                                 uint8_t* ws, size_t ws_bytes, hipStream_t s) {
+    if (ngroups == 0) return hipSuccess;
+    IntervalLaunch l;
+    const hipError_t e = interval_launch(p.nregs, max_len, ngroups, ws, ws_bytes, l);
+    if (e != hipSuccess) return e;
+    LF_INTERVAL_SWITCH(k_interval_boxes, p, p.pool, p.tapes, groups, g0, boxes, bounds, nan,
+                       tapes, push_nan ? 1u : 0u, max_len, ws)
+    return hipSuccess;
+}
+
+#undef LF_INTERVAL_SWITCH
 
 ////////////////////////////////////////////////////////////////////////////////
 // Level linking

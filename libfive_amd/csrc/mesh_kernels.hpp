@@ -98,12 +98,52 @@ struct Params {
 
 hipError_t upload_tables(hipStream_t s);
 
+// A pushed tape takes at most its parent's length plus a root copy and 8
+// clauses of padding; the pool is budgeted with the same figure.
+constexpr uint32_t kPushExtra = 9;
+
+// The interval passes keep two per-clause work arrays (3 bytes per clause)
+// in LDS while they fit in kIntervalLdsMax bytes together with the
+// per-register tables; longer tapes use a global workspace of
+// interval_workspace_bytes() instead, and then at most kIntervalWaves
+// workgroups, each looping over the groups.  No tape length is refused.
+constexpr size_t kIntervalLdsMax = 65536;
+constexpr uint32_t kIntervalWaves = 2048;
+__host__ __device__ inline size_t interval_ws_stride(uint32_t max_len) {
+    return (3u * size_t(max_len) + 255u) & ~size_t(255);
+}
+size_t interval_lds_bytes(uint32_t nregs, uint32_t max_len);
+size_t interval_workspace_bytes(uint32_t nregs, uint32_t max_len);
+
 // Interval classification and pruning of the cells of one level: one wave
 // per parent (8 children), or the root alone when groups is null.
+// max_len bounds the length of every tape the pass reads; ws holds
+// interval_workspace_bytes(p.nregs, max_len) bytes (null when that is 0).
 hipError_t launch_interval(const Params& p, int level, const uint32_t* groups,
                            uint32_t ngroups, const LevelView& parent,
-                           const LevelView& cur, uint32_t max_len,
-                           hipStream_t s);
+                           const LevelView& cur, uint32_t max_len, uint8_t* ws,
+                           size_t ws_bytes, hipStream_t s);
+
+// Interval evaluation of listed boxes (xlo, xhi, ylo, yhi, zlo, zhi per
+// box), grouped by the tape they are evaluated with: group g covers boxes
+// first .. first + count - 1 (count <= 8) and reads tape `tape`.  Per box:
+// bounds (lower, upper), nan (the may-be-NaN flag) and tapes (the tape for
+// the box's subdivisions: the tape pushed for it, or the group's tape when
+// nothing was pruned).  A box is pushed when its bounds are neither above
+// nor below zero, and when its result may be NaN only if push_nan
+// (Heightmap::recurse pushes regardless, heightmap.cpp:219-222;
+// DCTree::evalInterval keeps the unpruned tape, dc_tree.inl:105-109).
+// Counters::max_len gets the longest tape returned.
+struct BoxGroup {
+    uint32_t tape;
+    uint32_t first;
+    uint32_t count;
+    uint32_t pad;
+};
+hipError_t launch_interval_boxes(const Params& p, const BoxGroup* groups, uint32_t ngroups,
+                                 const float* boxes, float* bounds, uint8_t* nan,
+                                 uint32_t* tapes, bool push_nan, uint32_t max_len,
+                                 uint8_t* ws, size_t ws_bytes, hipStream_t s);
 
 // Links an evaluated level to the next: child offsets and the list of
 // ambiguous cells (8 children each).  base = exclusive scan of flags.

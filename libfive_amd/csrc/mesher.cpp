@@ -332,6 +332,7 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
     // Per-level scan buffers stay allocated until the render ends, so that
     // no device memory is released while kernels are in flight.
     std::vector<PoolBuf<uint32_t>> flags(top + 1), basei(top + 1), scan_tmp(top + 1);
+    std::vector<PoolBuf<uint8_t>> iws(top + 1);      // interval work arrays of long tapes
 
     // ---- build ------------------------------------------------------------
     levels[top].alloc(1);
@@ -351,8 +352,10 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
                    fast, settings, counters, hc, s);
     } else {
         hip_check(hipMemsetAsync(&counters.p->max_len, 0, sizeof(uint32_t), s), "max_len");
+        iws[top].alloc(interval_workspace_bytes(P.nregs, nbase));
         hip_check(launch_interval(P, top, nullptr, 1, levels[top].view(0),
-                                  levels[top].view(0), nbase, s), "interval kernel");
+                                  levels[top].view(0), nbase, iws[top].p, iws[top].n, s),
+                  "interval kernel");
         for (int L = top; L >= 1; --L) {
             LevelStore& cur = levels[L];
             flags[L].alloc(cur.n);
@@ -378,12 +381,15 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
                 if (max_len == 0 || max_len > hc.pool_top) {
                     throw Error("render_mesh: inconsistent tape length");
                 }
-                grow(pool, hc.pool_top, size_t(hc.pool_top) + size_t(namb) * 8u * (max_len + 8u), s);
+                grow(pool, hc.pool_top,
+                     size_t(hc.pool_top) + size_t(namb) * 8u * (max_len + kPushExtra), s);
                 grow(tapes, hc.tape_count, size_t(hc.tape_count) + size_t(namb) * 8u, s);
                 refresh();
                 hip_check(hipMemsetAsync(&counters.p->max_len, 0, sizeof(uint32_t), s), "max_len");
+                iws[L - 1].alloc(interval_workspace_bytes(P.nregs, max_len));
                 hip_check(launch_interval(P, L - 1, groups[L].p, uint32_t(namb), cur.view(0),
-                                          levels[L - 1].view(0), max_len, s),
+                                          levels[L - 1].view(0), max_len, iws[L - 1].p,
+                                          iws[L - 1].n, s),
                           "interval kernel");
             } else {
                 run_leaves(P, groups[1].p, uint32_t(namb), 8, cur.view(0),

@@ -8,6 +8,7 @@ import pytest
 
 import oracle
 from libfive_amd import X, Y, Z
+from libfive_amd.shape import Shape
 from libfive_amd import workloads as W
 
 from test_contour_oracle import rect
@@ -62,6 +63,34 @@ def test_matches_oracle(name, mk, lo, hi, z, mf):
     cs = s.render_slice(lo, hi, z, res) if mf == 0.1 else None
     if cs is not None:
         assert len(cs) == len(o.contours)
+
+
+def _nanfill(a, b):
+    return Shape.binary("nanfill", a, b)
+
+
+# A NaN-clearing nanfill above a MIN whose dropped operand is NaN in part of
+# a cell: the reference evaluates each cell with its parent's pushed tape,
+# so those points take the kept operand (DCTree::evalInterval,
+# dc_tree.inl:94-128).
+NAN_CASES = [
+    ("nanfill_min", lambda: _nanfill(X().sqrt().min(Y() - 0.5), Shape.constant(1.0)),
+     (-2, -2), (2, 2), 0.0, 0.1),
+    ("nanfill_min_circle", lambda: _nanfill((X() - 0.3).sqrt().min(
+        X() * X() + Y() * Y() - 1.0), Shape.constant(0.5)), (-2, -2), (2, 2), 0.0, 4 / 128),
+    ("nanfill_max", lambda: _nanfill((-X()).sqrt().max(Y() * Y() - 0.25),
+                                     Shape.constant(-1.0)), (-1, -1), (1, 1), 0.0, 2 / 64),
+]
+
+
+@pytest.mark.parametrize("name,mk,lo,hi,z,mf", NAN_CASES, ids=[c[0] for c in NAN_CASES])
+def test_pruned_tapes_with_nan_operands_match_oracle(name, mk, lo, hi, z, mf):
+    compare(mk(), lo, hi, z, mf)
+
+
+def test_long_tape_matches_oracle():
+    """27 499 clauses: interval work arrays in the global workspace."""
+    compare(W.sphere_union(2500), (-1.2, -1.2), (1.2, 1.2), 0.1, 2.4 / 128)
 
 
 def test_wide_register_file_matches_oracle():

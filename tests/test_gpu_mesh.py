@@ -95,6 +95,16 @@ def test_c3_sphere_union_depth8_matches_oracle(gpu):
     assert min(g.tape_lengths.tolist()) < 20
 
 
+def test_long_tape_sphere_union_matches_oracle(gpu):
+    """A 2500-sphere union (27 499 clauses): the interval passes keep their
+    per-clause work arrays in a global workspace instead of LDS while the
+    pruned tapes are that long, with the same octree and mesh."""
+    s = W.sphere_union(2500)
+    assert s.deck()[0].shape[0] > 25000
+    g = compare(s, -1.2, 1.2, 2.4 / 32, threads=16)
+    assert max(g.tape_lengths.tolist()) > 25000
+
+
 def test_large_trig_arguments_match_oracle(gpu):
     """sin arguments beyond 2^17 inside the region: the mesher's call-free
     leaf kernels hand such parents to the general per-cell kernel."""

@@ -10,6 +10,7 @@ import pytest
 
 import oracle
 from libfive_amd import X, Y
+from libfive_amd.shape import Shape
 from libfive_amd import workloads as W
 
 from test_raster_oracle import CASES, CIRCLE, ORIENT_X, ORIENT_Y
@@ -83,6 +84,43 @@ def test_empty_full_and_single_pixel():
     assert W.sphere(10.0).render_pixels((-1, -1), (1, 1), 0, 50).all()
     one = W.sphere(1.0).render_pixels((0, 0), (0, 0), 0, 10)
     assert one.shape == (1, 1) and one[0, 0]
+
+
+def _nanfill(a, b):
+    return Shape.binary("nanfill", a, b)
+
+
+# Min / max operands that may be NaN in a view: the reference evaluates a
+# view's pixels with the tape its ancestors pushed, whose MIN / MAX keep
+# rule ignores the may-be-NaN flag (eval_interval.cpp:127-160), so pixels
+# where the dropped operand is NaN take the kept one.  On these shapes that
+# differs from the unpruned value's sign at 1024 and 512 pixels.
+NAN_CASES = [
+    ("min_sqrt", lambda: X().sqrt().min(Y() - 10.0), (-2, -2), (2, 2), 0.0, 16.0),
+    ("nanfill_min", lambda: _nanfill(X().sqrt().min(Y() - 0.5), Shape.constant(1.0)),
+     (-2, -2), (2, 2), 0.0, 16.0),
+    ("nanfill_min_fine", lambda: _nanfill(X().sqrt().min(Y() - 0.5), Shape.constant(1.0)),
+     (-2.1, -1.7), (1.9, 2.3), 0.0, 77.0),
+]
+
+
+@pytest.mark.parametrize("name,mk,lo,hi,z,res", NAN_CASES, ids=[c[0] for c in NAN_CASES])
+def test_pruned_tapes_with_nan_operands_match_oracle(name, mk, lo, hi, z, res):
+    s = mk()
+    print(f"raster parity: {name} res={res}", flush=True)
+    g = s.render_pixels(lo, hi, z, res)
+    o = od(s).render_pixels(lo, hi, z, res)
+    assert g.shape == o.shape
+    assert (g == o).all()
+
+
+def test_long_tape_matches_oracle():
+    """27 499 clauses: interval work arrays in the global workspace."""
+    s = W.sphere_union(2500)
+    g = s.render_pixels((-1.2, -1.2), (1.2, 1.2), 0.1, 100.0)
+    o = od(s).render_pixels((-1.2, -1.2), (1.2, 1.2), 0.1, 100.0)
+    assert g.shape == o.shape
+    assert (g == o).all()
 
 
 def test_large_raster_matches_oracle():

@@ -15,9 +15,11 @@ namespace amd {
 
 namespace {
 
+// Register banks of 32 per file (8 banks: part of the file in AGPRs or
+// scratch memory, placed by the compiler).
 int banks(uint32_t nregs) {
     const uint32_t b = (nregs + 31) / 32;
-    return b <= 1 ? 1 : b <= 2 ? 2 : b <= 4 ? 4 : 0;
+    return b <= 1 ? 1 : b <= 2 ? 2 : b <= 4 ? 4 : b <= 8 ? 8 : 0;
 }
 
 // Runs f(t, n) once for each distinct tape id among the active lanes of
@@ -193,6 +195,7 @@ void k_point_features(PoolTapes T, const uint32_t* __restrict__ tid, uint32_t sx
         case 1: hipLaunchKernelGGL(KERNEL<1>, GRID, BLOCK, 0, s, __VA_ARGS__); break;   \
         case 2: hipLaunchKernelGGL(KERNEL<2>, GRID, BLOCK, 0, s, __VA_ARGS__); break;   \
         case 4: hipLaunchKernelGGL(KERNEL<4>, GRID, BLOCK, 0, s, __VA_ARGS__); break;   \
+        case 8: hipLaunchKernelGGL(KERNEL<8>, GRID, BLOCK, 0, s, __VA_ARGS__); break;   \
         default: return hipErrorInvalidValue;                                           \
     }
 

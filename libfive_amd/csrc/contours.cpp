@@ -234,9 +234,9 @@ ContourResult render_contours(const Tree& tree, const double lo[2], const double
     Context& c = context();
     const Deck deck(tree);
     const DeviceTape base = compile_deck(deck);
-    if (base.num_regs > 128) {
+    if (base.num_regs > 256) {
         throw Error("render_slice: the tape needs " + std::to_string(base.num_regs) +
-                    " registers; the contourer supports 128");
+                    " registers; the contourer supports 256");
     }
     const float mind = float(std::min(hi[0] - lo[0], hi[1] - lo[1]) / min_feature);
     const int top = int(std::ceil(std::fmax(0.0f, std::log(mind)) / std::log(2.0f)));

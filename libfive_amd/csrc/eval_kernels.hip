@@ -351,6 +351,7 @@ hipError_t launch_eval_derivs(const Clause* t, uint32_t n, uint32_t root,
         case 1: k_eval_derivs<1><<<grid, block, 0, s>>>(t, n, root, x, y, z, v, dx, dy, dz, npts); break;
         case 2: k_eval_derivs<2><<<grid, block, 0, s>>>(t, n, root, x, y, z, v, dx, dy, dz, npts); break;
         case 4: k_eval_derivs<4><<<grid, block, 0, s>>>(t, n, root, x, y, z, v, dx, dy, dz, npts); break;
+        case 8: k_eval_derivs<8><<<grid, block, 0, s>>>(t, n, root, x, y, z, v, dx, dy, dz, npts); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -370,6 +371,7 @@ hipError_t launch_eval_ambiguous(const Clause* t, uint32_t n, uint32_t root,
             case 1: k_eval_deriv_ambiguous<1><<<grid, block, 0, s>>>(t, n, root, x, y, z, out, npts); break;
             case 2: k_eval_deriv_ambiguous<2><<<grid, block, 0, s>>>(t, n, root, x, y, z, out, npts); break;
             case 4: k_eval_deriv_ambiguous<4><<<grid, block, 0, s>>>(t, n, root, x, y, z, out, npts); break;
+            case 8: k_eval_deriv_ambiguous<8><<<grid, block, 0, s>>>(t, n, root, x, y, z, out, npts); break;
             default: return hipErrorInvalidValue;
         }
     } else {

@@ -91,9 +91,9 @@ PixelImage render_pixels(const Tree& t, float xlo, float xhi, float ylo,
 
     const Deck deck(t);
     const DeviceTape base = compile_deck(deck);
-    if (base.num_regs > 128) {
+    if (base.num_regs > 256) {
         throw Error("render_pixels: the tape needs " + std::to_string(base.num_regs) +
-                    " registers; the raster supports 128");
+                    " registers; the raster supports 256");
     }
     std::lock_guard<std::mutex> lock(c.mu);
     SliceTapes tapes(base, deck.X, deck.Y, deck.Z, c.stream);

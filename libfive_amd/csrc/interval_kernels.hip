@@ -83,6 +83,8 @@ hipError_t launch_eval_intervals(const Clause* t, uint32_t n, uint32_t root,
         k_eval_intervals<2><<<grid, block, 0, s>>>(t, n, root, xlo, xhi, ylo, yhi, zlo, zhi, olo, ohi, onan, nbox);
     } else if (nv <= 4) {
         k_eval_intervals<4><<<grid, block, 0, s>>>(t, n, root, xlo, xhi, ylo, yhi, zlo, zhi, olo, ohi, onan, nbox);
+    } else if (nv <= 8) {
+        k_eval_intervals<8><<<grid, block, 0, s>>>(t, n, root, xlo, xhi, ylo, yhi, zlo, zhi, olo, ohi, onan, nbox);
     } else {
         return hipErrorInvalidValue;
     }

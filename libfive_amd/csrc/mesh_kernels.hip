@@ -512,9 +512,12 @@ void k_interval_boxes(Params P, const Clause* __restrict__ pool_in,
     run(g0 + blockIdx.x);
 }
 
+// Register banks of 32 per file: 1, 2 or 4 stay in VGPRs; 8 (tapes of up
+// to 256 registers) leave the compiler to keep part of each file in AGPRs
+// or scratch memory, which is slower but has no other limit.
 static int banks(uint32_t nregs) {
     const uint32_t b = (nregs + 31) / 32;
-    return b <= 1 ? 1 : b <= 2 ? 2 : b <= 4 ? 4 : 0;
+    return b <= 1 ? 1 : b <= 2 ? 2 : b <= 4 ? 4 : b <= 8 ? 8 : 0;
 }
 
 size_t interval_lds_bytes(uint32_t nregs, uint32_t max_len) {
@@ -563,9 +566,11 @@ hipError_t interval_launch(uint32_t nregs, uint32_t max_len, uint32_t ngroups,
             case 1: hipLaunchKernelGGL((KERNEL<1, false>), grid, dim3(64), l.lds, s, __VA_ARGS__); break; \
             case 2: hipLaunchKernelGGL((KERNEL<2, false>), grid, dim3(64), l.lds, s, __VA_ARGS__); break; \
             case 4: hipLaunchKernelGGL((KERNEL<4, false>), grid, dim3(64), l.lds, s, __VA_ARGS__); break; \
+            case 8: hipLaunchKernelGGL((KERNEL<8, false>), grid, dim3(64), l.lds, s, __VA_ARGS__); break; \
             case -1: hipLaunchKernelGGL((KERNEL<1, true>), grid, dim3(64), l.lds, s, __VA_ARGS__); break; \
             case -2: hipLaunchKernelGGL((KERNEL<2, true>), grid, dim3(64), l.lds, s, __VA_ARGS__); break; \
             case -4: hipLaunchKernelGGL((KERNEL<4, true>), grid, dim3(64), l.lds, s, __VA_ARGS__); break; \
+            case -8: hipLaunchKernelGGL((KERNEL<8, true>), grid, dim3(64), l.lds, s, __VA_ARGS__); break; \
             default: return hipErrorInvalidValue;                                         \
         }                                                                                 \
         const hipError_t le = hipGetLastError();                                          \
@@ -952,6 +957,7 @@ hipError_t launch_leaf(const Params& p, const uint32_t* groups, uint32_t ngroups
         case 1: LF_LEAF_LAUNCH(1, false, grid); break;
         case 2: LF_LEAF_LAUNCH(2, false, grid); break;
         case 4: LF_LEAF_LAUNCH(4, false, grid); break;
+        case 8: LF_LEAF_LAUNCH(8, false, grid); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -968,6 +974,7 @@ hipError_t launch_leaf_features(const Params& p, const uint32_t* groups,
         case 1: LF_LEAF_LAUNCH(1, true, nwork); break;
         case 2: LF_LEAF_LAUNCH(2, true, nwork); break;
         case 4: LF_LEAF_LAUNCH(4, true, nwork); break;
+        case 8: LF_LEAF_LAUNCH(8, true, nwork); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1373,6 +1380,8 @@ hipError_t launch_leaf_corners(const Params& p, bool fast, const uint32_t* group
         case -1: LF_CORNERS(1, true); break;
         case -2: LF_CORNERS(2, true); break;
         case -4: LF_CORNERS(4, true); break;
+        case 8:
+        case -8: LF_CORNERS(8, false); break;     // calls cost nothing next to spills
         default: return hipErrorInvalidValue;
     }
 #undef LF_CORNERS
@@ -1393,6 +1402,8 @@ hipError_t launch_leaf_edges(const Params& p, bool fast, const uint32_t* groups,
         case -1: LF_EDGES(1, true); break;
         case -2: LF_EDGES(2, true); break;
         case -4: LF_EDGES(4, true); break;
+        case 8:
+        case -8: LF_EDGES(8, false); break;
         default: return hipErrorInvalidValue;
     }
 #undef LF_EDGES
@@ -1564,6 +1575,7 @@ hipError_t launch_collapse(const Params& p, int level, const LevelView& cur,
         case 1: hipLaunchKernelGGL(k_collapse<1>, dim3(cur.n), dim3(64), 0, s, p, level, cur, below); break;
         case 2: hipLaunchKernelGGL(k_collapse<2>, dim3(cur.n), dim3(64), 0, s, p, level, cur, below); break;
         case 4: hipLaunchKernelGGL(k_collapse<4>, dim3(cur.n), dim3(64), 0, s, p, level, cur, below); break;
+        case 8: hipLaunchKernelGGL(k_collapse<8>, dim3(cur.n), dim3(64), 0, s, p, level, cur, below); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

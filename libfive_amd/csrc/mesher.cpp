@@ -266,9 +266,9 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
     }
     const DeviceTape base = compile_deck(deck, &var_values);
     const uint32_t nregs = base.num_regs;
-    if (nregs > 128) {
+    if (nregs > 256) {
         throw Error("render_mesh: the tape needs " + std::to_string(nregs) +
-                    " registers; the mesher supports 128");
+                    " registers; the mesher supports 256");
     }
     // The call-free leaf kernels serve tapes of the common opcodes only
     // (tape::fast_opcode; pe_value_op in point_eval.hpp handles the same set).

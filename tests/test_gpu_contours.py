@@ -106,6 +106,12 @@ def test_four_register_banks_match_oracle():
     compare(s, (-1, -1), (1, 1), 0.1, 2 / 128)
 
 
+def test_eight_register_banks_match_oracle():
+    s = wide_registers(195)
+    assert s.tape_info()[1] > 128
+    compare(s, (-1, -1), (1, 1), 0.1, 2 / 64)
+
+
 @pytest.mark.parametrize("name,mk,lo,hi,z,mf", CASES[:9], ids=[c[0] for c in CASES[:9]])
 def test_chained_contours_match_oracle(name, mk, lo, hi, z, mf):
     """Both walks visit the quadtree in the same depth-first order, so the

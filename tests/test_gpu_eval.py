@@ -91,6 +91,8 @@ SHAPES = {
                         + Shape.binary("nanfill", (X() - 0.5).sqrt(), Y())
                         + Shape.unary("recip", Z() + 3)),
     "wide_75_registers": wide_registers,
+    # 200 registers: eight banks, part of each file outside the VGPRs
+    "wide_200_registers": lambda: wide_registers(195),
 }
 
 

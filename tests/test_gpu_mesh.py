@@ -134,6 +134,26 @@ def test_four_register_banks_match_oracle(gpu):
     compare(s, -1.0, 1.0, 2.0 / 16)
 
 
+def test_eight_register_banks_match_oracle(gpu):
+    """200 registers: the kernels' register files spill out of the VGPRs
+    (eight banks), with the same mesh."""
+    s = wide_registers(195)
+    assert s.tape_info()[1] > 128
+    compare(s, -1.0, 1.0, 2.0 / 16)
+
+
+def test_more_than_256_registers_is_refused(gpu):
+    s = wide_registers(260)
+    assert s.tape_info()[1] > 256
+    with pytest.raises(Exception, match="256"):
+        s.render_mesh([-1.0] * 3, [1.0] * 3, min_feature=0.25)
+
+
+def test_more_than_20_levels_is_refused(gpu):
+    with pytest.raises(Exception, match="20 levels"):
+        W.sphere(0.5).render_mesh([-1.0] * 3, [1.0] * 3, min_feature=2.0 / 2 ** 21)
+
+
 def test_sliced_box_matches_oracle(gpu):
     """Box faces on the grid: exact zeros on tied MIN/MAX go through the
     feature evaluation."""

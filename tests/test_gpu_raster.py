@@ -78,6 +78,14 @@ def test_four_register_banks_match_oracle():
     assert g.any() and not g.all()
 
 
+def test_eight_register_banks_match_oracle():
+    s = wide_registers(195)
+    assert s.tape_info()[1] > 128
+    g = s.render_pixels((-1, -1), (1, 1), 0.1, 40.0)
+    o = od(s).render_pixels((-1, -1), (1, 1), 0.1, 40.0)
+    assert (g == o).all()
+
+
 def test_empty_full_and_single_pixel():
     far = W.sphere(0.5, (10, 10, 10))
     assert not far.render_pixels((-1, -1), (1, 1), 0, 50).any()

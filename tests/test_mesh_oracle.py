@@ -160,6 +160,101 @@ def test_find_vertex_truncated_pseudo_inverse():
         assert err == pytest.approx(v @ ata @ v - 2 * v @ atb + btb, abs=1e-9)
 
 
+def _np_find_vertex(ata, atb, btb, mass):
+    """findVertex (dc_tree.inl:740-813) written with numpy.linalg.eigh:
+    truncated pseudo-inverse with |w| < 0.1 dropped, solved about the mass
+    point; returns (vertex, count of kept eigenvalues, error)."""
+    w, u = np.linalg.eigh(ata)
+    keep = np.abs(w) >= 0.1
+    d = np.where(keep, 1.0 / np.where(keep, w, 1.0), 0.0)
+    c = mass[:3] / mass[3]
+    v = u @ np.diag(d) @ u.T @ (atb - ata @ c) + c
+    return v, int(keep.sum()), float(v @ ata @ v - 2 * v @ atb + btb)
+
+
+# The QEF tolerance: the product's Jacobi solve and LAPACK agree to a few
+# ulp of the largest eigenvalue; vertices are compared to 1e-9 of the
+# problem's scale (|c| + |A^+ A^T b| stay below 10 here), errors to 1e-9.
+QEF_TOL = 1e-9
+
+
+@pytest.mark.parametrize("kind", ["full", "deficient", "near_cutoff", "tiny", "samples"])
+def test_find_vertex_many_random_qefs(kind):
+    """3000 QEFs per kind against the numpy restatement: full-rank and
+    rank-deficient systems, eigenvalues placed 1e-7 either side of the 0.1
+    cutoff (the rank must come out as numpy's), near-zero matrices, and
+    systems accumulated by push_sample from random planes."""
+    rng = np.random.default_rng({"full": 11, "deficient": 12, "near_cutoff": 13,
+                                 "tiny": 14, "samples": 15}[kind])
+    for k in range(3000):
+        q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        if kind == "full":
+            w = rng.uniform(0.2, 6.0, 3)
+        elif kind == "deficient":
+            w = rng.uniform(0.2, 6.0, 3)
+            w[rng.integers(0, 3, rng.integers(1, 3))] = 0.0
+        elif kind == "near_cutoff":
+            w = rng.uniform(0.2, 6.0, 3)
+            w[k % 3] = 0.1 * (1 + (1e-7 if k % 2 else -1e-7))
+        elif kind == "tiny":
+            w = rng.uniform(0.0, 1e-6, 3)
+        if kind == "samples":
+            m = int(rng.integers(1, 7))
+            acc = np.zeros(14)
+            for _ in range(m):
+                oracle.push_sample(acc, rng.uniform(-1, 1, 3), rng.standard_normal(3) * 3,
+                                   rng.uniform(-0.05, 0.05))
+            ata = acc[[4, 5, 6, 5, 7, 8, 6, 8, 9]].reshape(3, 3)
+            atb, btb, mass = acc[10:13], acc[13], acc[:4]
+        else:
+            ata = q @ np.diag(w) @ q.T
+            ata = (ata + ata.T) / 2
+            atb = rng.uniform(-3, 3, 3)
+            btb = float(rng.uniform(0, 5))
+            mass = np.append(rng.uniform(-2, 2, 3), rng.integers(1, 9)).astype(np.float64)
+        v, rank, err = oracle.find_vertex(ata, atb, btb, mass)
+        wv, wrank, werr = _np_find_vertex(ata, atb, btb, mass)
+        if kind == "samples":
+            # sample sums can put an eigenvalue within rounding of the cutoff
+            w_np = np.linalg.eigvalsh(ata)
+            if np.min(np.abs(np.abs(w_np) - 0.1)) < 1e-12:
+                continue
+        assert rank == wrank, (kind, k)
+        assert np.allclose(v, wv, rtol=0, atol=QEF_TOL), (kind, k, v, wv)
+        assert err == pytest.approx(werr, abs=QEF_TOL * max(1.0, abs(werr))), (kind, k)
+
+
+def test_push_sample_many_random_against_numpy():
+    """Intersection::push (intersection.hpp:32-59) restated with numpy on
+    random samples, including zero, tiny and non-finite derivatives."""
+    rng = np.random.default_rng(16)
+    for k in range(2000):
+        acc = np.zeros(14)
+        want = np.zeros(14)
+        for _ in range(int(rng.integers(1, 6))):
+            pos = rng.uniform(-2, 2, 3)
+            d = rng.standard_normal(3) * 10 ** rng.uniform(-3, 3)
+            if k % 7 == 0:
+                d = d * 1e-14
+            if k % 11 == 0:
+                d[int(rng.integers(0, 3))] = np.inf
+            val = rng.uniform(-1, 1)
+            oracle.push_sample(acc, pos, d, val)
+            want[:3] += pos
+            want[3] += 1
+            nrm = np.sqrt(d @ d)
+            with np.errstate(invalid="ignore"):
+                dn, vn = d / nrm, val / nrm
+            if nrm <= 1e-12 or not np.isfinite(dn).all():
+                continue
+            want[4:10] += [dn[0] * dn[0], dn[0] * dn[1], dn[0] * dn[2],
+                           dn[1] * dn[1], dn[1] * dn[2], dn[2] * dn[2]]
+            b = dn @ pos - vn
+            want[10:13] += dn * b
+            want[13] += b * b
+        assert np.allclose(acc, want, rtol=1e-13, atol=1e-13), k
+
+
 def test_push_sample_normalises_and_skips_bad_normals():
     """libfive/include/libfive/render/brep/dc/intersection.hpp:32-59"""
     acc = np.zeros(14)

@@ -139,17 +139,26 @@ def profiled_traffic(kernel):
     return None
 
 
-def render_ms(shape, lo, hi, res, steps, warmup=1):
-    """Wall-clock ms of libfive_tree_render_mesh (median of steps)."""
+def render_ms(shape, lo, hi, res, steps, warmup=1, precision=0):
+    """Wall-clock ms of libfive_tree_render_mesh (median of steps); with
+    precision 1, of libfive_tree_render_mesh_settings with the same
+    min_feature and the compensated-FP32 QEF sums."""
     from libfive_amd import ffi
     L = ffi.lib()
     R = ffi.libfive_region3(ffi.libfive_interval(lo, hi),
                             ffi.libfive_interval(lo, hi),
                             ffi.libfive_interval(lo, hi))
+    st = ffi.libfive_mesh_settings()
+    L.libfive_mesh_settings_default(ctypes.byref(st))
+    st.min_feature = float(np.float32(1) / np.float32(res))
+    st.precision = precision
     times, counts = [], None
     for i in range(warmup + steps):
         t0 = time.perf_counter()
-        m = L.libfive_tree_render_mesh(shape.ptr, R, float(res))
+        if precision:
+            m = L.libfive_tree_render_mesh_settings(shape.ptr, R, ctypes.byref(st))
+        else:
+            m = L.libfive_tree_render_mesh(shape.ptr, R, float(res))
         dt = time.perf_counter() - t0
         if not m:
             raise RuntimeError(ffi.last_error())
@@ -158,6 +167,34 @@ def render_ms(shape, lo, hi, res, steps, warmup=1):
         if i >= warmup:
             times.append(dt * 1e3)
     return float(np.median(times)), counts
+
+
+def vertex_error_record(mesh, r64):
+    """Vertex error of a GPU mesh against the FP64 error oracle's mesh:
+    vertices matched by key; the maximum, the 99.99th percentile, the
+    median and the share within 1e-5."""
+    import oracle
+    d, matched, nverts = oracle.vertex_errors(mesh.verts, mesh.keys, r64)
+    return {"max_abs": float(d.max()) if d.size else 0.0,
+            "p9999": float(np.quantile(d, 0.9999)) if d.size else 0.0,
+            "median": float(np.median(d)) if d.size else 0.0,
+            "within_1e-5": float((d <= 1e-5).mean()) if d.size else 1.0,
+            "matched": matched, "verts": nverts}
+
+
+def plain_vs_precision(a, b):
+    """Per-vertex difference between two GPU meshes of one configuration,
+    matched by key: the maximum and the share of bit-identical vertices."""
+    import oracle
+    ka = oracle._pack_keys(np.asarray(a.keys)[1:])
+    kb = oracle._pack_keys(np.asarray(b.keys)[1:])
+    common, ia, ib = np.intersect1d(ka, kb, return_indices=True)
+    va = np.asarray(a.verts)[1:][ia]
+    vb = np.asarray(b.verts)[1:][ib]
+    d = np.abs(va.astype(np.float64) - vb).max(axis=1) if len(common) else np.zeros(0)
+    return {"matched": int(len(common)), "verts": int(len(ka)),
+            "max_abs": float(d.max()) if d.size else 0.0,
+            "identical": float((va == vb).all(axis=1).mean()) if len(common) else 1.0}
 
 
 def mesh_measurements(args):
@@ -195,10 +232,25 @@ def mesh_measurements(args):
         del om
         r64 = od.render_mesh([lo32] * 3, [hi32] * 3, mf512, threads=threads,
                              fp64=True)
-        err, matched, nverts = oracle.vertex_error(d.verts, d.keys, r64)
-        out["vertex_error_vs_fp64"] = {"max_abs": err, "matched": matched,
-                                       "verts": nverts}
+        out["vertex_error_vs_fp64"] = vertex_error_record(d, r64)
+    # The precision form (compensated-FP32 QEF sums, FP64 eigensolve) of the
+    # same configuration: time, and vertex error against the FP64 oracle
+    # and against the plain GPU form.
+    p_ms, _ = render_ms(gy, lo, hi, res / (hi - lo), args.mesh_steps, precision=1)
+    dp = gy.render_mesh([lo] * 3, [hi] * 3, res=res / (hi - lo), precision=1,
+                        details=True)
+    prec = {"workload": "as above, compensated-FP32 QEF accumulation",
+            "ms": round(p_ms, 2), "vs_plain_form_speed": round(ms / p_ms, 3),
+            "vs_plain_form": plain_vs_precision(d, dp)}
+    if not args.no_cpu:
+        prec["vs_oracle"] = oracle.compare_meshes(
+            dp.verts, dp.tris, dp.keys,
+            od.render_mesh([lo32] * 3, [hi32] * 3, mf512, threads=threads,
+                           qef_f32c=True))
+        prec["vertex_error_vs_fp64"] = vertex_error_record(dp, r64)
         del r64
+    out["precision_form"] = prec
+    del dp
     # CSG configuration at resolution 16 (64^3 finest cells)
     c1 = W.csg_c1()
     c1_ms, (c1v, c1t) = render_ms(c1, -2.0, 2.0, W.C1_RES, 5)
@@ -267,9 +319,7 @@ def mesh_measurements(args):
         del osm
         r64 = ods.render_mesh([slo32] * 3, [shi32] * 3, mf512,
                               threads=threads, fp64=True)
-        err, matched, nverts = oracle.vertex_error(sd.verts, sd.keys, r64)
-        out["sphere_union_512"]["vertex_error_vs_fp64"] = {
-            "max_abs": err, "matched": matched, "verts": nverts}
+        out["sphere_union_512"]["vertex_error_vs_fp64"] = vertex_error_record(sd, r64)
         del r64, sd
     # Many independent models (256 seeded random CSG trees, 10-2000 ops,
     # resolutions 32..256 over [-1,1]^3): one libfive_tree_render_mesh

@@ -351,7 +351,8 @@ int libfive_tree_deck(libfive_tree t, uint32_t* clauses, size_t* n_clauses,
 typedef struct libfive_mesh_settings {
     double min_feature;     /* default 0.1 */
     double max_err;         /* default 1e-8; < 0 disables collapse */
-    uint32_t precision;     /* QEF accumulation; only 0 (FP64) is built */
+    uint32_t precision;     /* QEF accumulation: 0 FP64 (default), 1
+                               compensated FP32; FP64 eigensolve in both */
     uint32_t collect_stats; /* non-zero: record per-level statistics */
     volatile const int* cancel;   /* optional; non-zero requests cancel */
 } libfive_mesh_settings;

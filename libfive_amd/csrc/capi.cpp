@@ -684,9 +684,9 @@ static libfive_mesh* render(libfive_tree tree, libfive_region3 R,
         ms.min_feature = s.min_feature;
         ms.max_err = s.max_err;
         ms.precision = s.precision;
-        if (s.precision != 0) {
+        if (s.precision > 1) {
             throw Error("precision " + std::to_string(s.precision) +
-                        " is not available (only 0, FP64 QEF accumulation)");
+                        " is not available (0: FP64 QEF accumulation, 1: compensated FP32)");
         }
         ms.collect_stats = s.collect_stats != 0;
         ms.cancel = s.cancel;

@@ -865,25 +865,32 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
             if (lane < ec) {
                 double acc[14];
                 for (int q = 0; q < 14; ++q) acc[q] = 0.0;
+                qef::AccF2 af;
+                qef::f2_zero(af);
+                const bool f2 = P.qef_f32c != 0;
                 bool exists = false;
                 for (int end = 0; end < 2; ++end) {
                     const int k = 2 * lane + end;
                     double pos[3];
                     for (int q = 0; q < 3; ++q) pos[q] = end ? s_ts[lane][q] : s_tf[lane][q];
+                    const float posf[3] = {float(pos[0]), float(pos[1]), float(pos[2])};
                     if (!(s_fl[k] & PT_AMBIGUOUS)) {
                         const double d[3] = {double(s_g[k][0]), double(s_g[k][1]),
                                              double(s_g[k][2])};
-                        qef::push_sample(acc, pos, d, double(s_v[k]));
+                        if (f2) qef::push_sample_f2(af, posf, s_g[k], s_v[k]);
+                        else qef::push_sample(acc, pos, d, double(s_v[k]));
                         exists = true;
                     } else if (FEAT) {
                         for (int f = 0; f < s_nfeat[k]; ++f) {
                             const double d[3] = {double(s_feat[k][f][0]), double(s_feat[k][f][1]),
                                                  double(s_feat[k][f][2])};
-                            qef::push_sample(acc, pos, d, double(s_v[k]));
+                            if (f2) qef::push_sample_f2(af, posf, s_feat[k][f], s_v[k]);
+                            else qef::push_sample(acc, pos, d, double(s_v[k]));
                             exists = true;
                         }
                     }
                 }
+                if (f2) qef::f2_widen(af, acc);
                 int r = -1;
                 if (exists) {
                     double ata[3][3];
@@ -909,9 +916,19 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
                         mass[3] = mass[3] + 1.0;
                     }
                 }
-                for (int e = 0; e < ec; ++e) {
-                    if (s_rank[e] < 0) continue;
-                    for (int q = 4; q < 14; ++q) sum[q] = sum[q] + s_acc[e][q];
+                if (P.qef_f32c) {
+                    qef::AccF2 af;
+                    qef::f2_zero(af);
+                    for (int e = 0; e < ec; ++e) {
+                        if (s_rank[e] < 0) continue;
+                        qef::f2_add_acc(af, s_acc[e]);
+                    }
+                    qef::f2_widen(af, sum);
+                } else {
+                    for (int e = 0; e < ec; ++e) {
+                        if (s_rank[e] < 0) continue;
+                        for (int q = 4; q < 14; ++q) sum[q] = sum[q] + s_acc[e][q];
+                    }
                 }
                 double ata[3][3];
                 qef::unpack_ata(sum, ata);
@@ -1282,13 +1299,25 @@ void k_leaf_edges(Params P, const uint32_t* __restrict__ groups, LevelView l1,
     }
     if (lane < 4 && 4u * j + uint32_t(lane) < ne) {
         EdgeRec out;
-        for (int q = 0; q < 14; ++q) out.acc[q] = 0.0;
-        for (int end = 0; end < 2; ++end) {
-            const int k = 2 * lane + end;
-            double pos[3];
-            for (int q = 0; q < 3; ++q) pos[q] = end ? s_ts[lane][q] : s_tf[lane][q];
-            const double d[3] = {double(s_g[k][0]), double(s_g[k][1]), double(s_g[k][2])};
-            qef::push_sample(out.acc, pos, d, double(s_v[k]));
+        if (P.qef_f32c) {
+            qef::AccF2 a;
+            qef::f2_zero(a);
+            for (int end = 0; end < 2; ++end) {
+                const int k = 2 * lane + end;
+                float pos[3];
+                for (int q = 0; q < 3; ++q) pos[q] = float(end ? s_ts[lane][q] : s_tf[lane][q]);
+                qef::push_sample_f2(a, pos, s_g[k], s_v[k]);
+            }
+            qef::f2_widen(a, out.acc);
+        } else {
+            for (int q = 0; q < 14; ++q) out.acc[q] = 0.0;
+            for (int end = 0; end < 2; ++end) {
+                const int k = 2 * lane + end;
+                double pos[3];
+                for (int q = 0; q < 3; ++q) pos[q] = end ? s_ts[lane][q] : s_tf[lane][q];
+                const double d[3] = {double(s_g[k][0]), double(s_g[k][1]), double(s_g[k][2])};
+                qef::push_sample(out.acc, pos, d, double(s_v[k]));
+            }
         }
         double ata[3][3];
         qef::unpack_ata(out.acc, ata);
@@ -1343,10 +1372,21 @@ void k_leaf_solve(Params P, const uint32_t* __restrict__ groups, LevelView l1,
                 mass[3] = mass[3] + 1.0;
             }
         }
-        for (int e = 0; e < ec; ++e) {
-            const EdgeRec& er = edges[idx[e]];
-            if (er.rank < 0) continue;
-            for (int q = 4; q < 14; ++q) sum[q] = sum[q] + er.acc[q];
+        if (P.qef_f32c) {
+            qef::AccF2 a;
+            qef::f2_zero(a);
+            for (int e = 0; e < ec; ++e) {
+                const EdgeRec& er = edges[idx[e]];
+                if (er.rank < 0) continue;
+                qef::f2_add_acc(a, er.acc);
+            }
+            qef::f2_widen(a, sum);
+        } else {
+            for (int e = 0; e < ec; ++e) {
+                const EdgeRec& er = edges[idx[e]];
+                if (er.rank < 0) continue;
+                for (int q = 4; q < 14; ++q) sum[q] = sum[q] + er.acc[q];
+            }
         }
         double ata[3][3];
         qef::unpack_ata(sum, ata);

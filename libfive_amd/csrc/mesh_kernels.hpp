@@ -92,6 +92,7 @@ struct Params {
     uint32_t nregs;
     uint32_t root_reg;
     uint32_t slot_x, slot_y, slot_z;
+    uint32_t qef_f32c;           // QEF sums as compensated float pairs
     Counters* counters;
     const LevelView* levels;     // device copy, indexed by level
 };
@@ -105,8 +106,9 @@ constexpr uint32_t kPushExtra = 9;
 // The interval passes keep two per-clause work arrays (3 bytes per clause)
 // in LDS while they fit in kIntervalLdsMax bytes together with the
 // per-register tables; longer tapes use a global workspace of
-// interval_workspace_bytes() instead, and then at most kIntervalWaves
-// workgroups, each looping over the groups.  No tape length is refused.
+// interval_workspace_bytes() instead, and are launched over at most
+// kIntervalWaves groups at a time (one workspace slot per workgroup).  No
+// tape length is refused.
 constexpr size_t kIntervalLdsMax = 65536;
 constexpr uint32_t kIntervalWaves = 2048;
 __host__ __device__ inline size_t interval_ws_stride(uint32_t max_len) {

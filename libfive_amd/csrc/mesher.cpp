@@ -318,6 +318,7 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
     P.slot_x = deck.X;
     P.slot_y = deck.Y;
     P.slot_z = deck.Z;
+    P.qef_f32c = settings.precision == 1 ? 1u : 0u;
     P.counters = counters.p;
     auto refresh = [&]() {
         P.pool = pool.p;

@@ -19,7 +19,10 @@ namespace amd {
 struct MeshSettings {
     double min_feature = 0.1;     // settings.hpp:34-43 defaults
     double max_err = 1e-8;
-    uint32_t precision = 0;       // QEF accumulation: only 0 (FP64) is built
+    // QEF accumulation: 0 in double (the reference's arithmetic), 1 as
+    // compensated float pairs (qef::push_sample_f2); the eigensolve is in
+    // double either way.
+    uint32_t precision = 0;
     bool collect_stats = false;   // per-level cell counts and tape lengths
     volatile const int* cancel = nullptr;           // C API flag
     const std::atomic<bool>* cancel_flag = nullptr; // BRepSettings::cancel

@@ -174,6 +174,93 @@ LF_HD void push_sample(double acc[14], const double pos[3],
     acc[13] = acc[13] + b * b;
 }
 
+// Compensated single-precision accumulation (the mesher's precision form,
+// libfive_mesh_settings::precision = 1): each of the 14 sums is a pair of
+// floats (hi, lo) whose exact sum carries the value.  Terms are added with
+// the error-free transformations TwoSum (Knuth, TAOCP vol. 2, 4.2.2) and
+// TwoProduct (one fused multiply-add), the errors collected in lo, as in
+// the Sum2 / Dot2 algorithms of Ogita, Rump and Oishi ("Accurate sum and
+// dot product", SIAM J. Sci. Comput. 26(6), 2005).  The pairs are widened
+// to double (exactly, in practice) for the rank test and the eigensolve.
+struct AccF2 {
+    float hi[14];
+    float lo[14];
+};
+
+LF_HD void f2_zero(AccF2& a) {
+    for (int q = 0; q < 14; ++q) {
+        a.hi[q] = 0.0f;
+        a.lo[q] = 0.0f;
+    }
+}
+
+// hi + lo += x (TwoSum: s + e == hi + x exactly)
+LF_HD void f2_add(float& hi, float& lo, float x) {
+    const float s = hi + x;
+    const float bp = s - hi;
+    const float e = (hi - (s - bp)) + (x - bp);
+    hi = s;
+    lo = lo + e;
+}
+
+// hi + lo += a * b (TwoProduct: p + pe == a * b exactly)
+LF_HD void f2_add_prod(float& hi, float& lo, float a, float b) {
+    const float p = a * b;
+    const float pe = __builtin_fmaf(a, b, -p);
+    f2_add(hi, lo, p);
+    lo = lo + pe;
+}
+
+// Intersection::push (intersection.hpp:32-59) with the sums as float
+// pairs: single-precision normalisation, then A^T A, A^T b and b^T b from
+// the normal and b = n . pos - value / |d| (itself a float pair).
+LF_HD void push_sample_f2(AccF2& a, const float pos[3], const float deriv[3],
+                          float value) {
+    f2_add(a.hi[0], a.lo[0], pos[0]);
+    f2_add(a.hi[1], a.lo[1], pos[1]);
+    f2_add(a.hi[2], a.lo[2], pos[2]);
+    a.hi[3] = a.hi[3] + 1.0f;
+    const float norm = sqrtf((deriv[0] * deriv[0] + deriv[1] * deriv[1]) +
+                             deriv[2] * deriv[2]);
+    const float n[3] = {deriv[0] / norm, deriv[1] / norm, deriv[2] / norm};
+    const float val = value / norm;
+    const bool finite = __builtin_isfinite(n[0]) && __builtin_isfinite(n[1]) &&
+                        __builtin_isfinite(n[2]);
+    if (norm <= 1e-12f || !finite) return;
+    f2_add_prod(a.hi[4], a.lo[4], n[0], n[0]);
+    f2_add_prod(a.hi[5], a.lo[5], n[0], n[1]);
+    f2_add_prod(a.hi[6], a.lo[6], n[0], n[2]);
+    f2_add_prod(a.hi[7], a.lo[7], n[1], n[1]);
+    f2_add_prod(a.hi[8], a.lo[8], n[1], n[2]);
+    f2_add_prod(a.hi[9], a.lo[9], n[2], n[2]);
+    float bh = 0.0f, bl = 0.0f;
+    f2_add_prod(bh, bl, n[0], pos[0]);
+    f2_add_prod(bh, bl, n[1], pos[1]);
+    f2_add_prod(bh, bl, n[2], pos[2]);
+    f2_add(bh, bl, -val);
+    for (int j = 0; j < 3; ++j) {
+        f2_add_prod(a.hi[10 + j], a.lo[10 + j], n[j], bh);
+        a.lo[10 + j] = a.lo[10 + j] + n[j] * bl;
+    }
+    f2_add_prod(a.hi[13], a.lo[13], bh, bh);
+    a.lo[13] = a.lo[13] + 2.0f * (bh * bl);
+}
+
+// Adds a widened accumulator's A^T A, A^T b and b^T b (entries 4..13),
+// each split back into a float pair.
+LF_HD void f2_add_acc(AccF2& a, const double acc[14]) {
+    for (int q = 4; q < 14; ++q) {
+        const float xh = float(acc[q]);
+        const float xl = float(acc[q] - double(xh));
+        f2_add(a.hi[q], a.lo[q], xh);
+        a.lo[q] = a.lo[q] + xl;
+    }
+}
+
+LF_HD void f2_widen(const AccF2& a, double acc[14]) {
+    for (int q = 0; q < 14; ++q) acc[q] = double(a.hi[q]) + double(a.lo[q]);
+}
+
 LF_HD void unpack_ata(const double acc[14], double ata[3][3]) {
     ata[0][0] = acc[4]; ata[0][1] = acc[5]; ata[0][2] = acc[6];
     ata[1][0] = acc[5]; ata[1][1] = acc[7]; ata[1][2] = acc[8];

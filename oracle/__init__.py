@@ -71,6 +71,9 @@ _SIGS = {
     "oracle_find_vertex": (ctypes.c_double, [_dp, _dp, ctypes.c_double, _dp,
                                              _dp, ctypes.POINTER(ctypes.c_int)]),
     "oracle_push_sample": (None, [_dp, _dp, _dp, ctypes.c_double]),
+    "oracle_push_sample_f2": (None, [_fp, _fp, _fp, ctypes.c_float]),
+    "oracle_f2_add_acc": (None, [_fp, _dp]),
+    "oracle_f2_widen": (None, [_fp, _dp]),
     "oracle_resolution_level": (ctypes.c_int, [_dp, _dp, ctypes.c_double]),
     "oracle_render_pixels": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_float] * 6
                              + [_u8, _u32, _u32]),
@@ -227,16 +230,19 @@ class OracleDeck:
         return out[:min(n, 64)]
 
     def render_mesh(self, lo, hi, min_feature, max_err=1e-8, threads=1,
-                    fp64=False):
+                    fp64=False, qef_f32c=False):
         """Dual-contouring mesh: a MeshResult (see below).  fp64=True gives
         the FP64 error oracle (double evaluation of values, partials and
-        the collapse test); its double vertices are in .verts64."""
+        the collapse test); its double vertices are in .verts64.
+        qef_f32c=True accumulates the QEF sums as compensated float pairs
+        (the product's precision form, mesh setting precision = 1)."""
         lo = np.ascontiguousarray(lo, np.float64)
         hi = np.ascontiguousarray(hi, np.float64)
         L = lib()
         m = L.oracle_render_mesh_ex(self.ptr, lo.ctypes.data_as(_dp),
                                     hi.ctypes.data_as(_dp), min_feature, max_err,
-                                    int(threads), 1 if fp64 else 0)
+                                    int(threads),
+                                    (1 if fp64 else 0) | (2 if qef_f32c else 0))
         try:
             sz = [ctypes.c_size_t() for _ in range(4)]
             L.oracle_mesh_sizes(m, *[ctypes.byref(x) for x in sz])
@@ -396,6 +402,25 @@ def push_sample(acc, pos, deriv, value):
                              deriv.ctypes.data_as(_dp), float(value))
 
 
+def push_sample_f2(state, pos, deriv, value):
+    """Compensated-FP32 push into state (28 float32: 14 hi, 14 lo), in
+    place."""
+    pos = np.ascontiguousarray(pos, np.float32)
+    deriv = np.ascontiguousarray(deriv, np.float32)
+    lib().oracle_push_sample_f2(_f(state), _f(pos), _f(deriv), float(value))
+
+
+def f2_add_acc(state, acc):
+    acc = np.ascontiguousarray(acc, np.float64)
+    lib().oracle_f2_add_acc(_f(state), acc.ctypes.data_as(_dp))
+
+
+def f2_widen(state):
+    out = np.empty(14)
+    lib().oracle_f2_widen(_f(state), out.ctypes.data_as(_dp))
+    return out
+
+
 def resolution_level(lo, hi, min_feature):
     lo = np.ascontiguousarray(lo, np.float64)
     hi = np.ascontiguousarray(hi, np.float64)
@@ -421,6 +446,13 @@ def vertex_error(verts, keys, ref):
     """Vertex error of a mesh against the FP64 error oracle's mesh `ref`:
     vertices are matched by key (owning cell and patch); returns
     (max |v - v64| over matched vertices, matched count, vertex count)."""
+    d, matched, n = vertex_errors(verts, keys, ref)
+    return (float(d.max()) if d.size else 0.0), matched, n
+
+
+def vertex_errors(verts, keys, ref):
+    """Per matched vertex, max over axes of |v - v64| (see vertex_error);
+    returns (errors, matched count, vertex count)."""
     a = _pack_keys(np.asarray(keys)[1:])
     b = _pack_keys(np.asarray(ref.keys)[1:])
     order = np.argsort(b)
@@ -429,8 +461,7 @@ def vertex_error(verts, keys, ref):
     hit = (bs[pos] == a) if len(bs) else np.zeros(len(a), bool)
     j = order[pos[hit]] + 1
     d = np.abs(np.asarray(verts)[1:][hit].astype(np.float64) - ref.verts64[j])
-    err = float(d.max()) if d.size else 0.0
-    return err, int(hit.sum()), len(a)
+    return (d.max(axis=1) if d.size else np.zeros(0)), int(hit.sum()), len(a)
 
 
 def _canonical(verts, tris, keys):

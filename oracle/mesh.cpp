@@ -245,6 +245,10 @@ struct Mesher {
     // double value is exactly zero fall back to the float path (ambiguity,
     // isInside and features stay in float).
     bool fp64 = false;
+    // Precision form: QEF sums as compensated float pairs
+    // (qef::push_sample_f2 per intersection, qef::f2_add_acc per patch),
+    // widened to double for the rank test and the eigensolve.
+    bool f32c = false;
     std::vector<double> sc, sd;     // FP64 scratch: values, values + partials
     std::vector<std::array<float, 3>> verts{{0, 0, 0}};
     std::vector<std::array<double, 3>> verts64{{0, 0, 0}};
@@ -257,8 +261,10 @@ struct Mesher {
     uint64_t collapsed = 0;
     std::vector<uint32_t> cells;   // x, y, z, level, state of every evaluated cell
 
-    Mesher(Deck& d, double me, bool f64 = false)
-        : deck(d), ie(d), de(d), fe(d), max_err(me), fp64(f64) {}
+    // mode: bit 0 the FP64 error oracle, bit 1 the compensated-FP32 QEF
+    Mesher(Deck& d, double me, int mode = 0)
+        : deck(d), ie(d), de(d), fe(d), max_err(me), fp64(mode & 1),
+          f32c((mode & 2) != 0) {}
 
     double f64(const double* p, const Tape& t) {
         return eval_point_f64(deck, t, p[0], p[1], p[2], sc);
@@ -465,8 +471,10 @@ struct Mesher {
             }
             uint8_t amb[24];
             de.ambiguous(2 * ec, *tape, amb);
+            std::map<int, libfive::qef::AccF2> inter2;
             for (int i = 0; i < 2 * ec; ++i) {
                 const double* p = (i & 1) ? ts[i / 2] : tf[i / 2];
+                const float pf[3] = {float(p[0]), float(p[1]), float(p[2])};
                 auto acc = [&]() -> double* {
                     auto it = inter.find(edges[i / 2]);
                     if (it == inter.end()) {
@@ -474,17 +482,35 @@ struct Mesher {
                     }
                     return it->second.data();
                 };
+                auto acc2 = [&]() -> libfive::qef::AccF2& {
+                    acc();          // the intersection exists from its first sample
+                    auto it = inter2.find(edges[i / 2]);
+                    if (it == inter2.end()) {
+                        libfive::qef::AccF2 z;
+                        libfive::qef::f2_zero(z);
+                        it = inter2.emplace(edges[i / 2], z).first;
+                    }
+                    return it->second;
+                };
+                auto push = [&](const double d[3]) {
+                    if (f32c) {
+                        const float df[3] = {float(d[0]), float(d[1]), float(d[2])};
+                        libfive::qef::push_sample_f2(acc2(), pf, df, float(ds[i][3]));
+                    } else {
+                        libfive::qef::push_sample(acc(), p, d, double(ds[i][3]));
+                    }
+                };
                 if (!amb[i]) {
                     const double d[3] = {ds[i][0], ds[i][1], ds[i][2]};
-                    libfive::qef::push_sample(acc(), p, d, double(ds[i][3]));
+                    push(d);
                 } else {
-                    const float pf[3] = {float(p[0]), float(p[1]), float(p[2])};
                     for (const auto& f : fe.features(pf, tape)) {
                         const double d[3] = {f[0], f[1], f[2]};
-                        libfive::qef::push_sample(acc(), p, d, double(ds[i][3]));
+                        push(d);
                     }
                 }
             }
+            for (const auto& kv : inter2) libfive::qef::f2_widen(kv.second, inter[kv.first].data());
             // mass point from the max-rank intersections (dc_tree.inl:481-502)
             int max_rank = 0;
             int ranks[12];
@@ -508,11 +534,15 @@ struct Mesher {
                 }
             }
             double sum[14] = {};
+            libfive::qef::AccF2 sum2;
+            libfive::qef::f2_zero(sum2);
             for (int i = 0; i < ec; ++i) {
                 auto it = inter.find(edges[i]);
                 if (it == inter.end()) continue;
-                for (int k = 4; k < 14; ++k) sum[k] = sum[k] + it->second[k];
+                if (f32c) libfive::qef::f2_add_acc(sum2, it->second.data());
+                else for (int k = 4; k < 14; ++k) sum[k] = sum[k] + it->second[k];
             }
+            if (f32c) libfive::qef::f2_widen(sum2, sum);
             double ata[3][3];
             libfive::qef::unpack_ata(sum, ata);
             std::memcpy(leaf->mass, mass, sizeof(mass));
@@ -819,7 +849,7 @@ oracle_mesh* oracle_render_mesh_ex(oracle_deck* d, const double lo[3],
                                    const double hi[3], double min_feature,
                                    double max_err, int threads, int fp64) {
     Deck& deck = d->deck;
-    Mesher m(deck, max_err, fp64 != 0);
+    Mesher m(deck, max_err, fp64);
     auto root = std::make_unique<Cell>();
     for (int j = 0; j < 3; ++j) {
         root->lo[j] = lo[j];
@@ -837,8 +867,7 @@ oracle_mesh* oracle_render_mesh_ex(oracle_deck* d, const double lo[3],
         std::vector<std::unique_ptr<Mesher>> workers;
         for (int t = 0; t < threads; ++t) {
             decks.push_back(std::make_unique<Deck>(deck));
-            workers.push_back(std::make_unique<Mesher>(*decks.back(), max_err,
-                                                       fp64 != 0));
+            workers.push_back(std::make_unique<Mesher>(*decks.back(), max_err, fp64));
         }
         std::vector<std::thread> pool;
         for (int t = 0; t < threads; ++t) {
@@ -971,6 +1000,39 @@ double oracle_find_vertex(const double* ata9, const double* atb,
     double a[3][3];
     for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) a[i][j] = ata9[3 * i + j];
     return libfive::qef::find_vertex(a, atb, btb, mass, vert, rank);
+}
+
+static libfive::qef::AccF2 load_f2(const float* st) {
+    libfive::qef::AccF2 a;
+    for (int q = 0; q < 14; ++q) {
+        a.hi[q] = st[q];
+        a.lo[q] = st[14 + q];
+    }
+    return a;
+}
+
+static void store_f2(const libfive::qef::AccF2& a, float* st) {
+    for (int q = 0; q < 14; ++q) {
+        st[q] = a.hi[q];
+        st[14 + q] = a.lo[q];
+    }
+}
+
+void oracle_push_sample_f2(float* state, const float* pos, const float* deriv,
+                           float value) {
+    libfive::qef::AccF2 a = load_f2(state);
+    libfive::qef::push_sample_f2(a, pos, deriv, value);
+    store_f2(a, state);
+}
+
+void oracle_f2_add_acc(float* state, const double* acc) {
+    libfive::qef::AccF2 a = load_f2(state);
+    libfive::qef::f2_add_acc(a, acc);
+    store_f2(a, state);
+}
+
+void oracle_f2_widen(const float* state, double* acc) {
+    libfive::qef::f2_widen(load_f2(state), acc);
 }
 
 void oracle_push_sample(double* acc, const double* pos, const double* deriv,

@@ -96,7 +96,7 @@ oracle_mesh* oracle_render_mesh(oracle_deck* d, const double lo[3],
 oracle_mesh* oracle_render_mesh_mt(oracle_deck* d, const double lo[3],
                                    const double hi[3], double min_feature,
                                    double max_err, int threads);
-/* With fp64 != 0: the FP64 error oracle.  The same octree algorithm, but
+/* fp64 bit 0: the FP64 error oracle.  The same octree algorithm, but
  * field values, partial derivatives and the collapse test's |f(v)| are
  * evaluated in double on the unpruned tape at double positions (points
  * whose double value is exactly zero use the float ambiguity and feature
@@ -104,6 +104,14 @@ oracle_mesh* oracle_render_mesh_mt(oracle_deck* d, const double lo[3],
 oracle_mesh* oracle_render_mesh_ex(oracle_deck* d, const double lo[3],
                                    const double hi[3], double min_feature,
                                    double max_err, int threads, int fp64);
+/* (fp64 bit 1: QEF sums as compensated float pairs, the product's
+ * precision form.) */
+/* The compensated-FP32 QEF primitives (qef.hpp, shared with the product):
+ * state = 14 hi floats then 14 lo floats. */
+void oracle_push_sample_f2(float* state, const float* pos, const float* deriv,
+                           float value);
+void oracle_f2_add_acc(float* state, const double* acc);
+void oracle_f2_widen(const float* state, double* acc);
 /* Emitted vertices in double (same order and count as the float ones). */
 size_t oracle_mesh_verts64(const oracle_mesh* m, double* out);
 void oracle_mesh_sizes(const oracle_mesh* m, size_t* n_verts, size_t* n_tris,

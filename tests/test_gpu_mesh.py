@@ -25,16 +25,18 @@ def canonical(verts, tris, keys):
     return vmap, tri
 
 
-def compare(shape, lo, hi, mf, max_err=1e-8, threads=1):
+def compare(shape, lo, hi, mf, max_err=1e-8, threads=1, precision=0):
     # The C API takes the region as floats (libfive_region3), so the oracle
     # gets the same float-rounded bounds.
     lo3 = [float(np.float32(lo))] * 3
     hi3 = [float(np.float32(hi))] * 3
-    print(f"mesh parity: lo={lo} hi={hi} min_feature={mf}", flush=True)
+    print(f"mesh parity: lo={lo} hi={hi} min_feature={mf} precision={precision}",
+          flush=True)
     g = shape.render_mesh(lo3, hi3, min_feature=mf, max_err=max_err,
-                          details=True)
+                          precision=precision, details=True)
     o = oracle.OracleDeck(shape.deck()).render_mesh(lo3, hi3, mf, max_err,
-                                                    threads=threads)
+                                                    threads=threads,
+                                                    qef_f32c=precision == 1)
     assert g.level_counts.shape == o.level_counts.shape
     assert (g.level_counts == o.level_counts).all()
     # identical interval / leaf classification, cell by cell
@@ -103,6 +105,45 @@ def test_long_tape_sphere_union_matches_oracle(gpu):
     assert s.deck()[0].shape[0] > 25000
     g = compare(s, -1.2, 1.2, 2.4 / 32, threads=16)
     assert max(g.tape_lengths.tolist()) > 25000
+
+
+PRECISION_CASES = [
+    ("sphere", lambda: W.sphere(0.5), -1.0, 1.0, 0.1),
+    ("gyroid64", W.gyroid_c4, -1.2, 1.2, 2.4 / 64),
+    ("csg_c1", W.csg_c1, -2.0, 2.0, 0.25),
+    ("sphere_union", lambda: W.sphere_union(n=60), -1.2, 1.2, 2.4 / 64),
+    ("sliced_box", lambda: W.box((0, 0, 0), (1, 1, 1)).max(X() + Y() + Z() - 1.3),
+     -2.0, 2.0, 0.1),
+]
+
+
+@pytest.mark.parametrize("name,mk,lo,hi,mf", PRECISION_CASES,
+                         ids=[c[0] for c in PRECISION_CASES])
+def test_precision_form_matches_oracle(gpu, name, mk, lo, hi, mf):
+    """precision = 1 (compensated-FP32 QEF sums, FP64 eigensolve): the same
+    octree, vertices and triangles as the oracle's precision form."""
+    compare(mk(), lo, hi, mf, precision=1)
+
+
+def test_precision_form_vertex_error(gpu):
+    """Both QEF forms against the FP64 error oracle on the gyroid at 128^3:
+    vertices within 1e-5 apart from rank flips, which the compensated sums
+    may cause where an eigenvalue sits at the 0.1 cutoff."""
+    s = W.gyroid_c4()
+    lo3, hi3 = [float(np.float32(-1.2))] * 3, [float(np.float32(1.2))] * 3
+    mf = float(np.float32(1) / np.float32(128 / 2.4))
+    r64 = oracle.OracleDeck(s.deck()).render_mesh(lo3, hi3, mf, threads=16, fp64=True)
+    out = {}
+    for p in (0, 1):
+        g = s.render_mesh(lo3, hi3, min_feature=mf, precision=p, details=True)
+        d, matched, n = oracle.vertex_errors(g.verts, g.keys, r64)
+        out[p] = d
+        print(f"precision {p}: {matched}/{n} matched, max {d.max():.3g}, "
+              f"99.99% {np.quantile(d, 0.9999):.3g}, median {np.median(d):.3g}",
+              flush=True)
+        assert matched > 0.999 * n
+        assert np.quantile(d, 0.9999) < 1e-5
+    assert np.median(out[1]) < 1e-6
 
 
 def test_large_trig_arguments_match_oracle(gpu):

@@ -255,6 +255,64 @@ def test_push_sample_many_random_against_numpy():
         assert np.allclose(acc, want, rtol=1e-13, atol=1e-13), k
 
 
+def test_compensated_push_is_exact_to_float_pair_rounding():
+    """push_sample_f2 (the precision form's accumulation) against exact
+    arithmetic: every sum is carried by its float pair to within a few
+    units of 2^-44 of the terms' magnitude, where a plain float sum drifts
+    by 2^-24 per term.  The reference values come from fractions (exact
+    rational arithmetic) on the float-rounded normals, so a dropped error
+    term or a wrong product fails."""
+    from fractions import Fraction as Fr
+    rng = np.random.default_rng(17)
+    for k in range(60):
+        st = np.zeros(28, np.float32)
+        m = int(rng.integers(1, 40))
+        exact = [Fr(0)] * 14
+        for _ in range(m):
+            pos = rng.uniform(-2, 2, 3).astype(np.float32)
+            d = (rng.standard_normal(3) * 10 ** rng.uniform(-2, 2)).astype(np.float32)
+            val = np.float32(rng.uniform(-0.1, 0.1))
+            oracle.push_sample_f2(st, pos, d, val)
+            # the normal and scaled value as the routine rounds them
+            nrm = np.sqrt(np.float32(np.float32(d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]),
+                          dtype=np.float32)
+            n = (d / nrm).astype(np.float32)
+            v = np.float32(val / nrm)
+            for q in range(3):
+                exact[q] += Fr(float(pos[q]))
+            exact[3] += 1
+            nf = [Fr(float(x)) for x in n]
+            pf = [Fr(float(x)) for x in pos]
+            b = nf[0] * pf[0] + nf[1] * pf[1] + nf[2] * pf[2] - Fr(float(v))
+            for q, (i, j) in enumerate([(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]):
+                exact[4 + q] += nf[i] * nf[j]
+            for j in range(3):
+                exact[10 + j] += nf[j] * b
+            exact[13] += b * b
+        got = oracle.f2_widen(st)
+        for q in range(14):
+            scale = max(1.0, m * 16.0)
+            assert abs(Fr(got[q]) - exact[q]) <= Fr(scale) * Fr(2) ** -42, (k, q)
+
+
+def test_compensated_sum_of_accumulators():
+    """f2_add_acc sums widened accumulators (entries 4..13) as float pairs:
+    within 2^-42 relative of the exact sum, mass-point entries untouched."""
+    from fractions import Fraction as Fr
+    rng = np.random.default_rng(18)
+    for k in range(200):
+        st = np.zeros(28, np.float32)
+        accs = rng.standard_normal((int(rng.integers(1, 13)), 14)) * 10 ** rng.uniform(-3, 3)
+        for a in accs:
+            oracle.f2_add_acc(st, a)
+        got = oracle.f2_widen(st)
+        assert (got[:4] == 0).all()
+        for q in range(4, 14):
+            ex = sum(Fr(float(a[q])) for a in accs)
+            mag = sum(abs(Fr(float(a[q]))) for a in accs)
+            assert abs(Fr(got[q]) - ex) <= mag * Fr(2) ** -42, (k, q)
+
+
 def test_push_sample_normalises_and_skips_bad_normals():
     """libfive/include/libfive/render/brep/dc/intersection.hpp:32-59"""
     acc = np.zeros(14)

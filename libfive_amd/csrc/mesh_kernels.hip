@@ -1067,11 +1067,15 @@ __device__ __forceinline__ void defer_group(const Params& P, LeafGroupRec* grec,
 template <int NV, bool FAST>
 __global__ __launch_bounds__(64)
 void k_leaf_corners(Params P, const uint32_t* __restrict__ groups, uint32_t g0,
-                    uint32_t ngroups, LevelView l1, LevelView l0,
-                    LeafGroupRec* __restrict__ grec, uint64_t* __restrict__ tasks,
-                    uint32_t* __restrict__ leaves, uint32_t* __restrict__ deferred)
+                    uint32_t ngroups, uint32_t nall, LevelView l1, LevelView l0,
+                    LeafGroupRec* __restrict__ grec, uint32_t* __restrict__ cnt,
+                    uint32_t* __restrict__ deferred)
 {
-    // This launch covers groups g0 .. g0 + ngroups - 1 of the level.
+    // This launch covers groups g0 .. g0 + ngroups - 1 of the level (nall
+    // in all).  cnt[g], cnt[nall + g], cnt[2 nall + g]: the group's lattice
+    // edges, edge-search tasks and ambiguous leaves, turned into list
+    // offsets by scans and k_leaf_scatter (no atomics on shared counters:
+    // a million waves adding to three addresses serialise there).
     const int lane = threadIdx.x;
     if (blockIdx.x >= ngroups) return;
     const uint32_t g = g0 + blockIdx.x;
@@ -1116,6 +1120,9 @@ void k_leaf_corners(Params P, const uint32_t* __restrict__ groups, uint32_t g0,
             r.flags = 0;
             r.pad = 0;
             grec[g] = r;
+            cnt[g] = 0;
+            cnt[nall + g] = 0;
+            cnt[2 * nall + g] = 0;
             __threadfence();
             defer_group(P, grec, g, deferred);
         }
@@ -1160,25 +1167,46 @@ void k_leaf_corners(Params P, const uint32_t* __restrict__ groups, uint32_t g0,
         emask |= uint64_t(lo) | (uint64_t(hi) << 32);
     }
     const uint32_t ambm = uint32_t(ballot(amb)) & 0xffu;
-    uint32_t lbase = 0;
     if (lane == 0) {
         const uint32_t ne = uint32_t(__popcll(emask));
         LeafGroupRec r;
         r.edge_mask = emask;
-        r.edge_base = ne ? atomicAdd(&P.counters->edges, ne) : 0u;
+        r.edge_base = 0;             // set by k_leaf_scatter
         r.inside = in27;
         r.flags = 0;
-        r.pad = 0;
+        r.pad = ambm;
         grec[g] = r;
-        const uint32_t nt = (ne + 3u) / 4u;
-        if (nt) {
-            const uint32_t tb = atomicAdd(&P.counters->tasks, nt);
-            for (uint32_t j = 0; j < nt; ++j) tasks[tb + j] = (uint64_t(g) << 32) | j;
-        }
-        if (ambm) lbase = atomicAdd(&P.counters->leaves, uint32_t(__popc(ambm)));
+        cnt[g] = ne;
+        cnt[nall + g] = (ne + 3u) / 4u;
+        cnt[2 * nall + g] = uint32_t(__popc(ambm));
     }
-    lbase = __shfl(lbase, 0);
-    if (amb) leaves[lbase + uint32_t(__popc(ambm & ((1u << lane) - 1u)))] = 8u * g + oct;
+}
+
+// Offsets from the scanned counts: each group's first edge record, its
+// edge-search tasks and its ambiguous leaves (one thread per group).  The
+// thread of the last group stores the totals in the counters.
+__global__ __launch_bounds__(256)
+void k_leaf_scatter(Params P, uint32_t nall, LeafGroupRec* __restrict__ grec,
+                    const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ base,
+                    uint64_t* __restrict__ tasks, uint32_t* __restrict__ leaves)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nall) return;
+    const uint32_t eb = base[g], tb = base[nall + g], lb = base[2 * nall + g];
+    const uint32_t nt = cnt[nall + g];
+    grec[g].edge_base = eb;
+    for (uint32_t j = 0; j < nt; ++j) tasks[tb + j] = (uint64_t(g) << 32) | j;
+    uint32_t ambm = cnt[2 * nall + g] ? grec[g].pad : 0u;
+    for (uint32_t k = 0; ambm; ++k) {
+        const uint32_t oct = uint32_t(__builtin_ctz(ambm));
+        ambm &= ambm - 1u;
+        leaves[lb + k] = 8u * g + oct;
+    }
+    if (g == nall - 1) {
+        P.counters->edges = eb + cnt[g];
+        P.counters->tasks = tb + nt;
+        P.counters->leaves = lb + cnt[2 * nall + g];
+    }
 }
 
 // Edge search, partials and per-edge QEF (dc_tree.inl:339-502 for one
@@ -1406,13 +1434,12 @@ void k_leaf_solve(Params P, const uint32_t* __restrict__ groups, LevelView l1,
 }
 
 hipError_t launch_leaf_corners(const Params& p, bool fast, const uint32_t* groups,
-                               uint32_t g0, uint32_t ngroups, const LevelView& l1,
-                               const LevelView& l0, LeafGroupRec* grec,
-                               uint64_t* tasks, uint32_t* leaves,
-                               uint32_t* deferred, hipStream_t s) {
+                               uint32_t g0, uint32_t ngroups, uint32_t nall,
+                               const LevelView& l1, const LevelView& l0, LeafGroupRec* grec,
+                               uint32_t* cnt, uint32_t* deferred, hipStream_t s) {
     if (ngroups == 0) return hipSuccess;
 #define LF_CORNERS(NV, F) hipLaunchKernelGGL((k_leaf_corners<NV, F>), dim3(ngroups), dim3(64), 0, s, \
-                                             p, groups, g0, ngroups, l1, l0, grec, tasks, leaves, deferred)
+                                             p, groups, g0, ngroups, nall, l1, l0, grec, cnt, deferred)
     switch (banks(p.nregs) * (fast ? -1 : 1)) {
         case 1: LF_CORNERS(1, false); break;
         case 2: LF_CORNERS(2, false); break;
@@ -1425,6 +1452,15 @@ hipError_t launch_leaf_corners(const Params& p, bool fast, const uint32_t* group
         default: return hipErrorInvalidValue;
     }
 #undef LF_CORNERS
+    return hipGetLastError();
+}
+
+hipError_t launch_leaf_scatter(const Params& p, uint32_t nall, LeafGroupRec* grec,
+                               const uint32_t* cnt, const uint32_t* base, uint64_t* tasks,
+                               uint32_t* leaves, hipStream_t s) {
+    if (nall == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_leaf_scatter, dim3((nall + 255) / 256), dim3(256), 0, s, p, nall,
+                       grec, cnt, base, tasks, leaves);
     return hipGetLastError();
 }
 

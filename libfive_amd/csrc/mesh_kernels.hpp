@@ -192,7 +192,7 @@ struct LeafGroupRec {
     uint32_t edge_base;  // first of the parent's edges in the edge records
     uint32_t inside;     // bit i: lattice point i is inside
     uint32_t flags;      // bit 0: deferred to the feature pass
-    uint32_t pad;
+    uint32_t pad;        // bit i: child i is an ambiguous leaf
 };
 struct EdgeRec {
     double acc[14];      // Intersection sums (qef::push_sample layout)
@@ -206,11 +206,17 @@ constexpr uint32_t kLeafTasksPerGroup = 14;   // ceil(54 / 4)
 // feature pass.
 // Corners of groups g0 .. g0 + ngroups - 1 (a launch may cover part of a
 // level); the edge kernel runs the tasks it is given.
+// cnt: 3 * nall counts (edges, tasks, leaves per group, see k_leaf_corners);
+// after their exclusive scans (base, same layout) launch_leaf_scatter fills
+// the edge bases, the task list and the leaf list and sets
+// Counters::edges, tasks and leaves.
 hipError_t launch_leaf_corners(const Params& p, bool fast, const uint32_t* groups,
-                               uint32_t g0, uint32_t ngroups, const LevelView& l1,
-                               const LevelView& l0, LeafGroupRec* grec,
-                               uint64_t* tasks, uint32_t* leaves,
-                               uint32_t* deferred, hipStream_t s);
+                               uint32_t g0, uint32_t ngroups, uint32_t nall,
+                               const LevelView& l1, const LevelView& l0, LeafGroupRec* grec,
+                               uint32_t* cnt, uint32_t* deferred, hipStream_t s);
+hipError_t launch_leaf_scatter(const Params& p, uint32_t nall, LeafGroupRec* grec,
+                               const uint32_t* cnt, const uint32_t* base, uint64_t* tasks,
+                               uint32_t* leaves, hipStream_t s);
 hipError_t launch_leaf_edges(const Params& p, bool fast, const uint32_t* groups,
                              const LevelView& l1, LeafGroupRec* grec,
                              const uint64_t* tasks, uint32_t ntasks,

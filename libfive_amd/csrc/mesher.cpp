@@ -143,8 +143,10 @@ void run_leaves(const Params& P, const uint32_t* groups, uint32_t ngroups,
     if (per_group == 8) {
         // corners per parent, then the parents' lattice edges, then one
         // vertex solve per ambiguous leaf
-        hip_check(hipMemsetAsync(&counters.p->edges, 0, 3 * sizeof(uint32_t), s), "leaf counters");
         PoolBuf<LeafGroupRec> grec(ngroups);
+        // per-group counts and their scans (edges, tasks, leaves)
+        PoolBuf<uint32_t> cnt(3 * size_t(ngroups)), base(3 * size_t(ngroups)),
+            scan_tmp(scan_scratch_words(ngroups));
         PoolBuf<uint64_t> tasks(size_t(ngroups) * kLeafTasksPerGroup);
         PoolBuf<uint32_t> leaves(cells);
         // With a cancel flag the kernels are launched in chunks of at most
@@ -155,14 +157,20 @@ void run_leaves(const Params& P, const uint32_t* groups, uint32_t ngroups,
         const uint32_t chunk = poll ? kLeafChunk : ngroups;
         for (uint32_t g0 = 0; g0 < ngroups; g0 += chunk) {
             const uint32_t ng = std::min(chunk, ngroups - g0);
-            hip_check(launch_leaf_corners(P, fast, groups, g0, ng, l1, l0, grec.p,
-                                          tasks.p, leaves.p, deferred.p, s),
+            hip_check(launch_leaf_corners(P, fast, groups, g0, ng, ngroups, l1, l0, grec.p,
+                                          cnt.p, deferred.p, s),
                       "leaf corner kernel");
             if (poll) {
                 wait(s, "leaf corner kernel");
                 if (settings.cancelled()) return;     // the caller checks too
             }
         }
+        for (int k = 0; k < 3; ++k) {
+            hip_check(exclusive_scan(cnt.p + size_t(k) * ngroups, base.p + size_t(k) * ngroups,
+                                     ngroups, scan_tmp.p, s), "leaf scan");
+        }
+        hip_check(launch_leaf_scatter(P, ngroups, grec.p, cnt.p, base.p, tasks.p, leaves.p, s),
+                  "leaf scatter kernel");
         hip_check(hipMemcpyAsync(&hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost, s),
                   "counters");
         wait(s, "leaf corner kernel");

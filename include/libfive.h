@@ -111,6 +111,12 @@ libfive_tree libfive_tree_unary(int op, libfive_tree a);
 libfive_tree libfive_tree_binary(int op, libfive_tree a, libfive_tree b);
 const void* libfive_tree_id(libfive_tree t);
 void libfive_tree_delete(libfive_tree ptr);
+/* libfive.h:290-293 / libfive.cpp:186-189: q(x, y, z) = p(x', y', z').
+ * The substitution is applied at once (the tree the reference's remap node
+ * becomes after Tree::flatten, tree.cpp:389-520), so printing the result
+ * shows the substituted expression rather than a "(remap ...)" form. */
+libfive_tree libfive_tree_remap(libfive_tree p, libfive_tree x,
+                                libfive_tree y, libfive_tree z);
 /* libfive.h:306 / libfive.cpp:190-193 */
 libfive_tree libfive_tree_optimized(libfive_tree t);
 /* libfive.h:311-316 / libfive.cpp:216-231: prefix print, freed with

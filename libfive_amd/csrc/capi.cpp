@@ -142,6 +142,13 @@ void libfive_tree_delete(libfive_tree ptr) {
     if (ptr) Tree::reclaim(D(ptr));
 }
 
+libfive_tree libfive_tree_remap(libfive_tree p, libfive_tree x,
+                                libfive_tree y, libfive_tree z) {
+    return (p && x && y && z)
+        ? H(Tree(D(p)).remap(Tree(D(x)), Tree(D(y)), Tree(D(z))).release())
+        : nullptr;
+}
+
 libfive_tree libfive_tree_optimized(libfive_tree t) {
     return H(Tree(D(t)).optimized().release());
 }

@@ -7,6 +7,8 @@
 #include <cassert>
 #include <cmath>
 #include <cstring>
+#include <exception>
+#include <pthread.h>
 #include <iostream>
 #include <map>
 #include <sstream>
@@ -706,11 +708,91 @@ public:
     std::unordered_map<NodeKey, Tree, NodeKeyHash> canonical;
 };
 
+// The optimiser recurses once per tree level; a chain tens of thousands of
+// nodes deep would exhaust an ordinary thread's stack, so the pass runs on a
+// helper thread with a 512 MiB stack (address space only: pages are touched
+// as the recursion reaches them).
+namespace {
+struct OptimizeJob {
+    const TreeData* root;
+    Tree out;
+    std::exception_ptr error;
+};
+void* optimize_entry(void* arg) {
+    auto job = static_cast<OptimizeJob*>(arg);
+    try {
+        Optimizer opt;
+        job->out = opt.visit(job->root, 1.0f, Optimizer::Ctx());
+    } catch (...) {
+        job->error = std::current_exception();
+    }
+    return nullptr;
+}
+}   // namespace
+
 Tree Tree::optimized() const {
     if (flags & TREE_FLAG_IS_OPTIMIZED) return *this;
-    Optimizer opt;
-    Tree out = opt.visit(ptr, 1.0f, Optimizer::Ctx());
-    return out.with_flags(TREE_FLAG_IS_OPTIMIZED);
+    OptimizeJob job{ptr, Tree(), nullptr};
+    pthread_attr_t attr;
+    pthread_t th;
+    bool ran = false;
+    if (pthread_attr_init(&attr) == 0) {
+        if (pthread_attr_setstacksize(&attr, size_t(512) << 20) == 0 &&
+            pthread_create(&th, &attr, optimize_entry, &job) == 0) {
+            pthread_join(th, nullptr);
+            ran = true;
+        }
+        pthread_attr_destroy(&attr);
+    }
+    if (!ran) optimize_entry(&job);   // no thread available: this stack
+    if (job.error) std::rethrow_exception(job.error);
+    return job.out.with_flags(TREE_FLAG_IS_OPTIMIZED);
+}
+
+////////////////////////////////////////////////////////////////////////////////
+// Remap (tree.cpp:366-380) with the substitution of Tree::flatten
+// (tree.cpp:389-520) done at once.  Nodes without x, y or z are kept; a node
+// whose children changed is rebuilt through unary()/binary(), so constant
+// folding and the local rewrites apply to the substituted tree exactly as
+// they do in flatten.  Shared subtrees are substituted once (memoised); the
+// walk is an explicit stack, so trees tens of thousands deep are fine.
+Tree Tree::remap(const Tree& rx, const Tree& ry, const Tree& rz) const {
+    if (rx == X() && ry == Y() && rz == Z()) return *this;
+    if (!(ptr->flags & TreeData::TREE_FLAG_HAS_XYZ)) return *this;
+    std::unordered_map<const Data*, Tree> done;
+    done.emplace(X().get(), rx);
+    done.emplace(Y().get(), ry);
+    done.emplace(Z().get(), rz);
+    std::vector<std::pair<const Data*, bool>> todo{{ptr, false}};
+    while (!todo.empty()) {
+        const auto [d, up] = todo.back();
+        todo.pop_back();
+        if (done.count(d)) continue;
+        const int n = Opcode::args(d->op());
+        if (n == 0 || !(d->flags & TreeData::TREE_FLAG_HAS_XYZ)) {
+            done.emplace(d, Tree(d));
+            continue;
+        }
+        if (!up) {
+            // children first; the right operand is substituted first, as in
+            // flatten's task stack
+            todo.push_back({d, true});
+            todo.push_back({d->lhs().get(), false});
+            if (n == 2) todo.push_back({d->rhs().get(), false});
+            continue;
+        }
+        const Tree a = done.at(d->lhs().get());
+        if (n == 1) {
+            done.emplace(d, a.get() == d->lhs().get() ? Tree(d)
+                                                      : Tree::unary(d->op(), a));
+        } else {
+            const Tree b = done.at(d->rhs().get());
+            const bool same = a.get() == d->lhs().get() &&
+                              b.get() == d->rhs().get();
+            done.emplace(d, same ? Tree(d) : Tree::binary(d->op(), a, b));
+        }
+    }
+    return done.at(ptr);
 }
 
 }   // namespace libfive

@@ -64,6 +64,10 @@ public:
 
     // Hash-consed, affine-collapsed, commutative-flattened copy.
     Tree optimized() const;
+    // q(x, y, z) = this(X(x, y, z), Y(x, y, z), Z(x, y, z)), substituted at
+    // once: the result is what the reference's remap node becomes after
+    // Tree::flatten (tree.cpp:366-380, 389-520).
+    Tree remap(const Tree& X, const Tree& Y, const Tree& Z) const;
     // Number of unique nodes reachable from this tree.
     size_t size() const;
     // Leaves-to-root topological order (tree.cpp:530-594).

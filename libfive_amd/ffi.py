@@ -117,6 +117,7 @@ SIGNATURES = {
     "libfive_tree_binary": (tree_t, [ctypes.c_int, tree_t, tree_t]),
     "libfive_tree_id": (ctypes.c_void_p, [tree_t]),
     "libfive_tree_delete": (None, [tree_t]),
+    "libfive_tree_remap": (tree_t, [tree_t, tree_t, tree_t, tree_t]),
     "libfive_tree_optimized": (tree_t, [tree_t]),
     "libfive_tree_print": (ctypes.c_void_p, [tree_t]),
     "libfive_free_str": (None, [ctypes.c_void_p]),

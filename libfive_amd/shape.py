@@ -112,6 +112,11 @@ class Shape:
     def optimized(self):
         return Shape(ffi.lib().libfive_tree_optimized(self.ptr))
 
+    def remap(self, x, y, z):
+        """self(x', y', z') (libfive/src/libfive.cpp:186-189)."""
+        x, y, z = (Shape.wrap(v) for v in (x, y, z))
+        return Shape(ffi.lib().libfive_tree_remap(self.ptr, x.ptr, y.ptr, z.ptr))
+
     def deck(self):
         """Reference-format clause list: (clauses[n,4] root first,
         constants {id: value}, (X, Y, Z) ids, slot count)."""

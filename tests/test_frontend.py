@@ -44,6 +44,55 @@ def test_print_unoptimized():
     assert str(X() + 5) == "(+ x 5)"
 
 
+def test_remap_known_answers():
+    # libfive/test/tree.cpp:72-156 (each result compared after flatten,
+    # which libfive_tree_remap applies at once)
+    x, y, z = X(), Y(), Z()
+    L = ffi.lib()
+    assert L.libfive_tree_id(x.remap(y, x, x).ptr) == L.libfive_tree_id(y.ptr)
+    assert str(x.remap(12, x, x)) == "12"
+    assert str((x + 5).remap(3, x, x)) == "8"          # folds while remapping
+    f = x * 2 + x * 3 + x
+    assert str(f.remap(y, y, z)) == "(+ (* y 2) (* y 3) y)"
+    y_plus_two = x.remap(y + 2, z, x)
+    assert str((y + z).remap(y, y_plus_two, y_plus_two)) == "(+ y 2 y 2)"
+    c = (x * x + y * y).sqrt()
+    c = c.remap(x / 2, y, z)
+    c = c.remap(x - -0.5, y, z - 0.25)
+    assert str(c) == "(sqrt (+ (square (/ (- x -0.5) 2)) (square y)))"
+    # identity remap and a tree without x, y, z return the tree itself
+    assert L.libfive_tree_id(f.remap(x, y, z).ptr) == L.libfive_tree_id(f.ptr)
+    k = Shape.constant(3.0) + 0
+    assert L.libfive_tree_id(k.remap(y, z, x).ptr) == L.libfive_tree_id(k.ptr)
+
+
+def test_remap_deep_chain_and_composition():
+    import oracle
+    # "Deep remapping" (tree.cpp:96-129): a chain 32768 additions deep is
+    # substituted, optimised and compiled without exhausting the stack
+    x, y, z = X(), Y(), Z()
+    t = x
+    for i in range(32768):
+        t = t + y * i
+    r = t.remap(z, x, y)
+    # (y * 0 and y * 1 were simplified to 0 and y when t was built)
+    assert str(r)[:22] == "(+ z x (* x 2) (* x 3)"
+    clauses = r.deck()[0]
+    assert len(clauses) == len(t.deck()[0])
+    # composition: g = f(X', Y', Z') equals f evaluated at the remapped
+    # coordinates (the optimiser may merge constants across the seam, so
+    # the comparison allows a few float32 roundings)
+    rng = np.random.default_rng(7)
+    p = rng.uniform(-1, 1, (3, 4096)).astype(np.float32)
+    f = x * 2 + y.sin() * z - x.min(z)
+    X2, Y2, Z2 = y * 3, z - x, (x + 1).cos()
+    got = oracle.OracleDeck(f.remap(X2, Y2, Z2).deck()).values(*p)
+    q = [oracle.OracleDeck(c.deck()).values(*p) for c in (X2, Y2, Z2)]
+    want = oracle.OracleDeck(f.deck()).values(*q)
+    np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-5)
+    assert not np.allclose(got, oracle.OracleDeck(f.deck()).values(*p))
+
+
 def n_slots(t):
     return t.deck()[3]
 

@@ -67,7 +67,33 @@ def parse():
     p.add_argument("--sweep", action="store_true",
                    help="also time the value kernel at 1, 2 and 4 points "
                         "per lane")
+    p.add_argument("--mesh-only", choices=("gyroid", "union", "union512"),
+                   default=None,
+                   help="only render this mesh workload, --mesh-steps times "
+                        "after one warm-up, and print the times (for "
+                        "profiling: the gyroid at --mesh-res, the sphere "
+                        "union at depth 8, or the union at --mesh-res)")
     return p.parse_args()
+
+
+def mesh_only(args):
+    """Renders one mesh workload 1 + --mesh-steps times and prints the
+    wall-clock times; nothing else runs, so a profiler's dispatch list
+    holds exactly these renders."""
+    from libfive_amd import workloads as W
+    if args.mesh_only == "gyroid":
+        shape = W.gyroid_c4()
+        lo, hi = W.C4_REGION[0][0], W.C4_REGION[1][0]
+        res = args.mesh_res / (hi - lo)
+    else:
+        shape = W.sphere_union()
+        lo, hi = W.C3_REGION[0][0], W.C3_REGION[1][0]
+        cells = 2 ** W.C3_DEPTH if args.mesh_only == "union" else args.mesh_res
+        res = cells / (hi - lo)
+    ms, (nv, nt) = render_ms(shape, lo, hi, res, args.mesh_steps)
+    print(json.dumps({"mesh_only": args.mesh_only, "renders": 1 + args.mesh_steps,
+                      "median_ms": round(ms, 2), "verts": int(nv) - 1,
+                      "tris": int(nt)}))
 
 
 def host_cpu():
@@ -438,6 +464,9 @@ def main():
 
     if not libfive_amd.gpu_available():
         raise SystemExit("no usable GPU: " + ffi.last_error())
+    if args.mesh_only:
+        mesh_only(args)
+        return
     L = ffi.lib()
     shape = W.random_dag()
     deck = shape.deck()

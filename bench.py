@@ -67,12 +67,13 @@ def parse():
     p.add_argument("--sweep", action="store_true",
                    help="also time the value kernel at 1, 2 and 4 points "
                         "per lane")
-    p.add_argument("--mesh-only", choices=("gyroid", "union", "union512"),
+    p.add_argument("--mesh-only", choices=("gyroid", "union", "union512", "batch"),
                    default=None,
                    help="only render this mesh workload, --mesh-steps times "
                         "after one warm-up, and print the times (for "
                         "profiling: the gyroid at --mesh-res, the sphere "
-                        "union at depth 8, or the union at --mesh-res)")
+                        "union at depth 8, or the union at --mesh-res; "
+                        "batch: only the many-model leg)")
     return p.parse_args()
 
 
@@ -81,6 +82,9 @@ def mesh_only(args):
     wall-clock times; nothing else runs, so a profiler's dispatch list
     holds exactly these renders."""
     from libfive_amd import workloads as W
+    if args.mesh_only == "batch":
+        print(json.dumps({"many_models": many_models(args)}))
+        return
     if args.mesh_only == "gyroid":
         shape = W.gyroid_c4()
         lo, hi = W.C4_REGION[0][0], W.C4_REGION[1][0]
@@ -223,6 +227,65 @@ def plain_vs_precision(a, b):
             "identical": float((va == vb).all(axis=1).mean()) if len(common) else 1.0}
 
 
+def many_models(args):
+    """Many independent models (seeded random CSG trees of 10-2000 ops,
+    resolutions 32..256 over [-1,1]^3): one libfive_tree_render_mesh call
+    per model (plain), one batched call that runs 8 models at a time on
+    their own streams, and one merged call (all models through one set of
+    passes)."""
+    from libfive_amd import ffi, render_mesh_batch
+    from libfive_amd import workloads as W
+    L = ffi.lib()
+    models = W.csg_batch(args.batch_models)
+    shapes = [m for m, _ in models]
+    res = [r / 2.0 for _, r in models]
+    regions = [((-1.0,) * 3, (1.0,) * 3)] * len(shapes)
+    R = ffi.libfive_region3(ffi.libfive_interval(-1, 1),
+                            ffi.libfive_interval(-1, 1),
+                            ffi.libfive_interval(-1, 1))
+    render_mesh_batch(shapes[:8], regions[:8], res[:8], keep=False)
+    t0 = time.perf_counter()
+    tris_plain = 0
+    for sh, r in zip(shapes, res):
+        m = L.libfive_tree_render_mesh(sh.ptr, R, float(r))
+        if not m:
+            raise RuntimeError(ffi.last_error())
+        tris_plain += m.contents.tri_count
+        L.libfive_mesh_delete(m)
+    plain_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    counts = render_mesh_batch(shapes, regions, res, streams=8, keep=False)
+    batch_ms = (time.perf_counter() - t0) * 1e3
+    tris_batch = sum(int(t) for _, t in counts)
+    # the merged form: one set of passes over all models (median of 3
+    # after a warm-up)
+    render_mesh_batch(shapes, regions, res, keep=False, merged=True)
+    mts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        mcounts = render_mesh_batch(shapes, regions, res, keep=False, merged=True)
+        mts.append((time.perf_counter() - t0) * 1e3)
+    merged_ms = float(np.median(mts))
+    tris_merged = sum(int(t) for _, t in mcounts)
+    # untimed: the merged meshes equal the per-stream ones bit for bit
+    a_ = render_mesh_batch(shapes, regions, res, streams=8, merged=False)
+    b_ = render_mesh_batch(shapes, regions, res, merged=True)
+    same = all(va.shape == vb.shape and ta.shape == tb.shape and
+               (va.view(np.uint32) == vb.view(np.uint32)).all() and
+               (ta == tb).all() for (va, ta), (vb, tb) in zip(a_, b_))
+    del a_, b_
+    return {
+        "models": len(shapes), "plain_ms": round(plain_ms, 1),
+        "batched_ms": round(batch_ms, 1),
+        "batched_over_plain": round(plain_ms / batch_ms, 2),
+        "merged_ms": round(merged_ms, 1),
+        "merged_over_plain": round(plain_ms / merged_ms, 2),
+        "merged_over_batched": round(batch_ms / merged_ms, 2),
+        "tris": tris_batch, "same_tri_count": tris_batch == tris_plain == tris_merged,
+        "merged_meshes_identical": bool(same),
+    }
+
+
 def mesh_measurements(args):
     from libfive_amd import workloads as W
     import oracle
@@ -347,40 +410,8 @@ def mesh_measurements(args):
                               threads=threads, fp64=True)
         out["sphere_union_512"]["vertex_error_vs_fp64"] = vertex_error_record(sd, r64)
         del r64, sd
-    # Many independent models (256 seeded random CSG trees, 10-2000 ops,
-    # resolutions 32..256 over [-1,1]^3): one libfive_tree_render_mesh
-    # call per model (plain) against one batched call that runs 8 models
-    # at a time on their own streams.
     if args.batch_models:
-        from libfive_amd import ffi, render_mesh_batch
-        L = ffi.lib()
-        models = W.csg_batch(args.batch_models)
-        shapes = [m for m, _ in models]
-        res = [r / 2.0 for _, r in models]
-        regions = [((-1.0,) * 3, (1.0,) * 3)] * len(shapes)
-        R = ffi.libfive_region3(ffi.libfive_interval(-1, 1),
-                                ffi.libfive_interval(-1, 1),
-                                ffi.libfive_interval(-1, 1))
-        render_mesh_batch(shapes[:8], regions[:8], res[:8], keep=False)
-        t0 = time.perf_counter()
-        tris_plain = 0
-        for sh, r in zip(shapes, res):
-            m = L.libfive_tree_render_mesh(sh.ptr, R, float(r))
-            if not m:
-                raise RuntimeError(ffi.last_error())
-            tris_plain += m.contents.tri_count
-            L.libfive_mesh_delete(m)
-        plain_ms = (time.perf_counter() - t0) * 1e3
-        t0 = time.perf_counter()
-        counts = render_mesh_batch(shapes, regions, res, streams=8, keep=False)
-        batch_ms = (time.perf_counter() - t0) * 1e3
-        tris_batch = sum(int(t) for _, t in counts)
-        out["many_models"] = {
-            "models": len(shapes), "plain_ms": round(plain_ms, 1),
-            "batched_ms": round(batch_ms, 1),
-            "batched_over_plain": round(plain_ms / batch_ms, 2),
-            "tris": tris_batch, "same_tri_count": tris_batch == tris_plain,
-        }
+        out["many_models"] = many_models(args)
 
     # Z-slice occupancy raster (libfive_tree_render_pixels) of the gyroid
     # at 4096 x 4096 pixels

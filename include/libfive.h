@@ -317,6 +317,20 @@ int libfive_trees_render_mesh_batch(const libfive_tree* trees,
                                     const float* res, size_t n, int streams,
                                     libfive_mesh** out);
 
+/* The same batch through one set of level-synchronous GPU passes (no
+ * counterpart in the reference): every octree level, the leaf pass, the
+ * collapse and the triangle emission run as one launch over the cells of
+ * all n models, so the host waits once per level for the whole batch
+ * instead of once per level per model.  Model i's mesh is the one
+ * libfive_tree_render_mesh(trees[i], regions[i], res[i]) returns, bit for
+ * bit.  out[i] receives the mesh or NULL; returns the number of models
+ * that failed (all of them when a GPU step fails; the error is in
+ * libfive_last_error()). */
+int libfive_trees_render_mesh_merged(const libfive_tree* trees,
+                                     const libfive_region3* regions,
+                                     const float* res, size_t n,
+                                     libfive_mesh** out);
+
 /* Tape specialisation (no counterpart in the reference, whose evaluators
  * interpret the tape on the CPU: libfive/src/eval/eval_array.cpp:107-118).
  * Compiles the tape into straight-line value and derivative kernels with

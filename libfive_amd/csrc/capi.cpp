@@ -776,6 +776,60 @@ int libfive_trees_render_mesh_batch(const libfive_tree* trees,
     }
 }
 
+int libfive_trees_render_mesh_merged(const libfive_tree* trees,
+                                     const libfive_region3* regions,
+                                     const float* res, size_t n, libfive_mesh** out) {
+    for (size_t i = 0; i < n; ++i) out[i] = nullptr;
+    if (n == 0) return 0;
+    std::vector<std::unique_ptr<libfive_mesh>> meshes(n);
+    auto free_arrays = [&]() {
+        for (auto& m : meshes) {
+            if (m) {
+                delete[] m->verts;
+                delete[] m->tris;
+                m.reset();
+            }
+        }
+    };
+    try {
+        std::vector<Tree> tr;
+        tr.reserve(n);
+        for (size_t i = 0; i < n; ++i) tr.emplace_back(D(trees[i]));
+        std::vector<BatchItem> items(n);
+        std::vector<HostMesh> hms(n);
+        for (size_t i = 0; i < n; ++i) {
+            const libfive_region3& R = regions[i];
+            items[i].tree = &tr[i];
+            items[i].lo[0] = R.X.lower; items[i].lo[1] = R.Y.lower; items[i].lo[2] = R.Z.lower;
+            items[i].hi[0] = R.X.upper; items[i].hi[1] = R.Y.upper; items[i].hi[2] = R.Z.upper;
+            items[i].min_feature = 1 / res[i];      // float division, as libfive.cpp:343
+            meshes[i].reset(new libfive_mesh{nullptr, nullptr, 0, 0});
+            hms[i].want_keys = false;
+            libfive_mesh* m = meshes[i].get();
+            hms[i].alloc = [m](size_t nv, size_t nt, float** v, uint32_t** t) {
+                m->verts = new libfive_vec3[nv];
+                m->tris = new libfive_tri[nt ? nt : 1];
+                m->vert_count = uint32_t(nv);
+                m->tri_count = uint32_t(nt);
+                *v = &m->verts[0].x;
+                *t = &m->tris[0].a;
+            };
+        }
+        Context& ctx = context();
+        std::lock_guard<std::mutex> lock(ctx.mu);
+        render_meshes_merged(items.data(), n, hms.data(), ctx.stream);
+        for (size_t i = 0; i < n; ++i) {
+            if (!meshes[i]->verts) throw Error("model " + std::to_string(i) + " was not meshed");
+        }
+        for (size_t i = 0; i < n; ++i) out[i] = meshes[i].release();
+        return 0;
+    } catch (const std::exception& e) {
+        free_arrays();
+        set_error("libfive_trees_render_mesh_merged", e);
+        return int(n);
+    }
+}
+
 libfive_mesh* libfive_tree_render_mesh(libfive_tree tree, libfive_region3 R,
                                        float res) {
     libfive_mesh_settings s;

@@ -77,6 +77,27 @@ __device__ __forceinline__ void cell_bounds(const Params& P, const uint32_t key[
 
 __device__ __forceinline__ uint64_t ballot(bool b) { return __ballot(b); }
 
+// The parameters of one model: P's own when P.models is null (one model),
+// else those of model m.
+__device__ __forceinline__ Params model_view(const Params& P, uint32_t m) {
+    Params Q = P;
+    if (P.models != nullptr) {
+        const ModelRec& M = P.models[m];
+        for (int a = 0; a < 3; ++a) {
+            Q.lo[a] = M.lo[a];
+            Q.hi[a] = M.hi[a];
+        }
+        Q.top = M.top;
+        Q.root_reg = M.root_reg;
+        Q.slot_x = M.slot_x;
+        Q.slot_y = M.slot_y;
+        Q.slot_z = M.slot_z;
+        Q.root_cell = M.root_cell;
+        Q.slot0 = M.slot0;
+    }
+    return Q;
+}
+
 ////////////////////////////////////////////////////////////////////////////////
 // Interval classification and pruning
 
@@ -369,7 +390,7 @@ __device__ __forceinline__ uint32_t interval_push(const Params& P,
                 r.box[2] = flo[1]; r.box[3] = fhi[1];
                 r.box[4] = flo[2]; r.box[5] = fhi[2];
                 r.ref_len = refc;
-                r.pad = 0;
+                r.model = tr.model;
                 P.tapes[id] = r;
                 child_tape = id;
             }
@@ -420,14 +441,16 @@ __global__ __launch_bounds__(64)
 void k_interval(Params P, const Clause* __restrict__ pool_in,
                 const TapeRec* __restrict__ recs_in, int level,
                 const uint32_t* __restrict__ groups, uint32_t g0, LevelView parent,
-                LevelView cur, uint32_t max_len, uint8_t* ws)
+                LevelView cur, uint32_t max_len, uint8_t* ws, uint32_t first_root)
 {
     HIP_DYNAMIC_SHARED(uint32_t, lds)
     const IntervalWork w = interval_work<GLOBAL>(lds, P, max_len, ws);
     const int lane = threadIdx.x;
     const bool at_root = (groups == nullptr);
     auto run = [&](uint32_t g) {
-        uint32_t tape_id = 0, first = 0, pkey[3] = {0, 0, 0};
+        // At the roots, group g is model first_root + g, whose base tape is
+        // record first_root + g.
+        uint32_t tape_id = first_root + g, first = 0, pkey[3] = {0, 0, 0};
         if (!at_root) {
             const uint32_t p = groups[g];
             tape_id = parent.tape[p];
@@ -435,6 +458,8 @@ void k_interval(Params P, const Clause* __restrict__ pool_in,
             for (int a = 0; a < 3; ++a) pkey[a] = parent.key[3 * p + a];
         }
         tape_id = __builtin_amdgcn_readfirstlane(tape_id);
+        const Params Q = model_view(P, __builtin_amdgcn_readfirstlane(recs_in[tape_id].model));
+        if (at_root) first = Q.root_cell;
         const uint32_t nchild = at_root ? 1u : 8u;
         const bool active = uint32_t(lane) < nchild;
         const uint32_t octant = active ? uint32_t(lane) : 0u;
@@ -444,7 +469,7 @@ void k_interval(Params P, const Clause* __restrict__ pool_in,
             key[a] = pkey[a] + (at_root ? 0u : (((octant >> a) & 1u) << level));
         }
         double lo[3], hi[3];
-        cell_bounds(P, key, level, lo, hi);
+        cell_bounds(Q, key, level, lo, hi);
         float flo[3], fhi[3];
         for (int a = 0; a < 3; ++a) {
             flo[a] = float(lo[a]);
@@ -452,7 +477,7 @@ void k_interval(Params P, const Clause* __restrict__ pool_in,
         }
         float rlo, rhi;
         bool rnan;
-        const uint32_t child_tape = interval_push<NV>(P, pool_in, recs_in, tape_id, active,
+        const uint32_t child_tape = interval_push<NV>(Q, pool_in, recs_in, tape_id, active,
                                                       flo, fhi, false,
                                                       w.choice, w.lv, w.need, w.bytes,
                                                       rlo, rhi, rnan);
@@ -494,11 +519,12 @@ void k_interval_boxes(Params P, const Clause* __restrict__ pool_in,
         const uint32_t count = __builtin_amdgcn_readfirstlane(G.count);
         const bool active = uint32_t(lane) < count;
         const uint32_t b = G.first + (active ? uint32_t(lane) : 0u);
+        const Params Q = model_view(P, __builtin_amdgcn_readfirstlane(recs_in[tape_id].model));
         const float flo[3] = {boxes[6 * b], boxes[6 * b + 2], boxes[6 * b + 4]};
         const float fhi[3] = {boxes[6 * b + 1], boxes[6 * b + 3], boxes[6 * b + 5]};
         float rlo, rhi;
         bool rnan;
-        const uint32_t child_tape = interval_push<NV>(P, pool_in, recs_in, tape_id, active,
+        const uint32_t child_tape = interval_push<NV>(Q, pool_in, recs_in, tape_id, active,
                                                       flo, fhi, push_nan != 0, w.choice, w.lv,
                                                       w.need, w.bytes, rlo, rhi, rnan);
         if (active) {
@@ -580,13 +606,13 @@ hipError_t interval_launch(uint32_t nregs, uint32_t max_len, uint32_t ngroups,
 hipError_t launch_interval(const Params& p, int level, const uint32_t* groups,
                            uint32_t ngroups, const LevelView& parent,
                            const LevelView& cur, uint32_t max_len, uint8_t* ws,
-                           size_t ws_bytes, hipStream_t s) {
+                           size_t ws_bytes, hipStream_t s, uint32_t first_root) {
     if (ngroups == 0) return hipSuccess;
     IntervalLaunch l;
     const hipError_t e = interval_launch(p.nregs, max_len, ngroups, ws, ws_bytes, l);
     if (e != hipSuccess) return e;
     LF_INTERVAL_SWITCH(k_interval, p, p.pool, p.tapes, level, groups, g0, parent, cur,
-                       max_len, ws)
+                       max_len, ws, first_root)
     return hipSuccess;
 }
 
@@ -663,7 +689,6 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
     __shared__ int s_nfeat[FEAT ? 24 : 1];
     const int lane = threadIdx.x;
     const uint32_t total = ngroups * per_group;
-    const uint32_t root = P.root_reg;
     uint32_t* err = &P.counters->error;
 
     const uint32_t w = FEAT ? work[blockIdx.x] : blockIdx.x;
@@ -681,8 +706,10 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
         const TapeRec tr = P.tapes[tape_id];
         const uint32_t n = __builtin_amdgcn_readfirstlane(tr.len);
         const Clause* __restrict__ t = P.pool + __builtin_amdgcn_readfirstlane(tr.off);
+        const Params Q = model_view(P, __builtin_amdgcn_readfirstlane(tr.model));
+        const uint32_t root = Q.root_reg;
         double lo[3], hi[3];
-        cell_bounds(P, key, 0, lo, hi);
+        cell_bounds(Q, key, 0, lo, hi);
         feat::Workspace W{};
         if (FEAT) W = feat::carve(ws_base + size_t(blockIdx.x) * ws_bytes, n, P.nregs);
 
@@ -715,7 +742,7 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
                     const float q[3] = {__shfl(cp[0], l), __shfl(cp[1], l), __shfl(cp[2], l)};
                     int r = 0;
                     if (lane == 0) {
-                        r = feat::is_inside(t, n, P.nregs, root, P.slot_x, P.slot_y, P.slot_z,
+                        r = feat::is_inside(t, n, P.nregs, root, Q.slot_x, Q.slot_y, Q.slot_z,
                                             q, W, err) ? 1 : 0;
                     }
                     r = __shfl(r, 0);
@@ -786,8 +813,8 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
                                                     __shfl(fp[2], l)};
                                 int r = 0;
                                 if (lane == 0) {
-                                    r = feat::is_inside(t, n, P.nregs, root, P.slot_x, P.slot_y,
-                                                        P.slot_z, q, W, err) ? 1 : 0;
+                                    r = feat::is_inside(t, n, P.nregs, root, Q.slot_x, Q.slot_y,
+                                                        Q.slot_z, q, W, err) ? 1 : 0;
                                 }
                                 r = __shfl(r, 0);
                                 if (lane == l) zero_out = (r == 0);
@@ -849,8 +876,8 @@ void k_leaf(Params P, const uint32_t* __restrict__ groups, uint32_t ngroups,
                         for (int c = 0; c < 3; ++c) q[c] = float((l & 1) ? s_ts[e][c] : s_tf[e][c]);
                         if (lane == 0) {
                             float fo[feat::kMaxOut][3];
-                            const int nf = feat::features(t, n, P.nregs, root, P.slot_x,
-                                                          P.slot_y, P.slot_z, q, W, fo, err);
+                            const int nf = feat::features(t, n, P.nregs, root, Q.slot_x,
+                                                          Q.slot_y, Q.slot_z, q, W, fo, err);
                             s_nfeat[l] = nf;
                             for (int f = 0; f < nf; ++f) {
                                 s_feat[l][f][0] = fo[f][0];
@@ -1084,11 +1111,12 @@ void k_leaf_corners(Params P, const uint32_t* __restrict__ groups, uint32_t g0,
     const TapeRec tr = P.tapes[tape_id];
     const uint32_t n = __builtin_amdgcn_readfirstlane(tr.len);
     const Clause* __restrict__ t = P.pool + __builtin_amdgcn_readfirstlane(tr.off);
-    const uint32_t root = P.root_reg;
+    const Params Q = model_view(P, __builtin_amdgcn_readfirstlane(tr.model));
+    const uint32_t root = Q.root_reg;
     uint32_t pkey[3];
     for (int a = 0; a < 3; ++a) pkey[a] = l1.key[3 * p + a];
     double plo[3], phi[3];
-    cell_bounds(P, pkey, 1, plo, phi);
+    cell_bounds(Q, pkey, 1, plo, phi);
 
     // Lanes 0..26: one lattice point each.
     const bool pt = lane < 27;
@@ -1233,11 +1261,12 @@ void k_leaf_edges(Params P, const uint32_t* __restrict__ groups, LevelView l1,
     const TapeRec tr = P.tapes[tape_id];
     const uint32_t n = __builtin_amdgcn_readfirstlane(tr.len);
     const Clause* __restrict__ t = P.pool + __builtin_amdgcn_readfirstlane(tr.off);
-    const uint32_t root = P.root_reg;
+    const Params Q = model_view(P, __builtin_amdgcn_readfirstlane(tr.model));
+    const uint32_t root = Q.root_reg;
     uint32_t pkey[3];
     for (int a = 0; a < 3; ++a) pkey[a] = l1.key[3 * p + a];
     double plo[3], phi[3];
-    cell_bounds(P, pkey, 1, plo, phi);
+    cell_bounds(Q, pkey, 1, plo, phi);
 
     const uint32_t ne = uint32_t(__popcll(emask));
     const int e = lane >> 4, smp = lane & 15;
@@ -1612,8 +1641,9 @@ void k_collapse(Params P, int level, LevelView cur, LevelView below)
     const double err = qef::find_vertex(ata, sum + 10, sum[13], mass, vert, nullptr);
     uint32_t key[3];
     for (int a = 0; a < 3; ++a) key[a] = cur.key[3 * cell + a];
+    const Params Q = model_view(P, __builtin_amdgcn_readfirstlane(P.tapes[cur.tape[cell]].model));
     double lo[3], hi[3];
-    cell_bounds(P, key, level, lo, hi);
+    cell_bounds(Q, key, level, lo, hi);
     const double eps = 1e-6;
     const bool inside = vert[0] >= lo[0] - eps && vert[1] >= lo[1] - eps &&
                         vert[2] >= lo[2] - eps && vert[0] <= hi[0] + eps &&
@@ -1626,7 +1656,7 @@ void k_collapse(Params P, int level, LevelView cur, LevelView below)
     const TapeRec tr = P.tapes[tid];
     uint32_t fl;
     const float fv = pe_value<NV>(P.pool + __builtin_amdgcn_readfirstlane(tr.off),
-                                  __builtin_amdgcn_readfirstlane(tr.len), P.root_reg,
+                                  __builtin_amdgcn_readfirstlane(tr.len), Q.root_reg,
                                   vf[0], vf[1], vf[2], &fl);
     if (!(fabsf(fv) < P.max_err)) return;
     if (threadIdx.x == 0) {
@@ -1667,7 +1697,7 @@ __device__ bool locate(const Params& P, int64_t px, int64_t py, int64_t pz,
     const int64_t lim = int64_t(2) << P.top;
     if (px < 0 || py < 0 || pz < 0 || px >= lim || py >= lim || pz >= lim) return false;
     int l = P.top;
-    uint32_t i = 0;
+    uint32_t i = P.root_cell;
     for (;;) {
         const int32_t ch = P.levels[l].child[i];
         if (ch < 0 || l == 0) break;
@@ -1691,10 +1721,13 @@ __global__ void k_quads(Params P, LevelView cur, int pass, uint32_t* tri_count,
 {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= cur.n) return;
+    const uint32_t model = P.tapes[cur.tape[c]].model;
+    const uint32_t gc = cur.cbase[model] + c;       // global cell number
     if (cur.type[c] != AMBIGUOUS || cur.child[c] >= 0) {
-        if (pass == 0) tri_count[cur.cell_base + c] = 0;
+        if (pass == 0) tri_count[gc] = 0;
         return;
     }
+    const Params MP = model_view(P, model);
     const int my_level = int((cur.leaf[c].flags >> 8) & 0xffu);
     int64_t key[3];
     for (int a = 0; a < 3; ++a) key[a] = cur.key[3 * c + a];
@@ -1705,15 +1738,17 @@ __global__ void k_quads(Params P, LevelView cur, int pass, uint32_t* tri_count,
         // centre must be the cell itself.
         int cl;
         uint32_t ci;
-        const bool found = locate(P, 2 * key[0] + size, 2 * key[1] + size,
+        const bool found = locate(MP, 2 * key[0] + size, 2 * key[1] + size,
                                   2 * key[2] + size, &cl, &ci);
         if (!found || cl != my_level || ci != c) {
-            if (pass == 0) tri_count[cur.cell_base + c] = 0;
+            if (pass == 0) tri_count[gc] = 0;
             return;
         }
     }
     uint32_t count = 0;
-    uint32_t out = (pass == 1) ? tri_off[cur.cell_base + c] : 0u;
+    uint32_t out = (pass == 1) ? tri_off[gc] : 0u;
+    // vertex numbers are per model, after the model's dummy vertex 0
+    const uint32_t vbase = (pass == 1) ? vid[MP.slot0] : 0u;
     for (int A = 1; A <= 4; A <<= 1) {
         const int Q = axis_q(A), R = axis_r(A);
         const int ia = bit_index(A), iq = bit_index(Q), ir = bit_index(R);
@@ -1729,7 +1764,7 @@ __global__ void k_quads(Params P, LevelView cur, int pass, uint32_t* tri_count,
                 pt[ia] = 2 * key[ia] + size;
                 pt[iq] = 2 * (key[iq] + q * size) + ((k & 1) ? 1 : -1);
                 pt[ir] = 2 * (key[ir] + r * size) + ((k & 2) ? 1 : -1);
-                ok = locate(P, pt[0], pt[1], pt[2], &lv[k], &ix[k]);
+                ok = locate(MP, pt[0], pt[1], pt[2], &lv[k], &ix[k]);
             }
             if (!ok) continue;
             // The segment belongs to the smallest of the four cells, and
@@ -1758,7 +1793,7 @@ __global__ void k_quads(Params P, LevelView cur, int pass, uint32_t* tri_count,
                 const LevelView& L = P.levels[lv[i]];
                 const LeafRec& lf = L.leaf[ix[i]];
                 const int vi = lv[i] > 0 ? 0 : c_mp[lf.mask * 24 + es];
-                slots[i] = 4u * (L.cell_base + ix[i]) + uint32_t(vi < 0 ? 0 : vi);
+                slots[i] = 4u * (L.cbase[model] + ix[i]) + uint32_t(vi < 0 ? 0 : vi);
                 for (int q2 = 0; q2 < 3; ++q2) vp[i][q2] = float(lf.vert[vi < 0 ? 0 : vi][q2]);
             }
             if (!D) {
@@ -1800,13 +1835,15 @@ __global__ void k_quads(Params P, LevelView cur, int pass, uint32_t* tri_count,
                 if (pass == 0) {
                     for (int q2 = 0; q2 < 3; ++q2) vslot[tv[s][q2]] = 1u;
                 } else {
-                    for (int q2 = 0; q2 < 3; ++q2) tris[3 * (out + count) + q2] = vid[tv[s][q2]] + 1u;
+                    for (int q2 = 0; q2 < 3; ++q2) {
+                        tris[3 * (out + count) + q2] = vid[tv[s][q2]] - vbase + 1u;
+                    }
                 }
                 ++count;
             }
         }
     }
-    if (pass == 0) tri_count[cur.cell_base + c] = count;
+    if (pass == 0) tri_count[gc] = count;
 }
 
 hipError_t launch_quads(const Params& p, const LevelView& cur, int pass,
@@ -1819,15 +1856,24 @@ hipError_t launch_quads(const Params& p, const LevelView& cur, int pass,
     return hipGetLastError();
 }
 
-__global__ void k_verts(LevelView cur, int level, const uint32_t* vslot,
+__global__ void k_verts(Params P, LevelView cur, int level, const uint32_t* vslot,
                         const uint32_t* vid, float* verts, uint32_t* keys)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= 4u * cur.n) return;
     const uint32_t c = i >> 2, vi = i & 3u;
-    const uint32_t slot = 4u * cur.cell_base + i;
+    const uint32_t m = P.tapes[cur.tape[c]].model;
+    const uint32_t slot = 4u * (cur.cbase[m] + c) + vi;
+    if (vi == 0 && P.models != nullptr && slot == P.models[m].slot0) {
+        // the model's dummy vertex (and key), before its first vertex
+        const uint32_t d = vid[slot] + m;
+        for (int q = 0; q < 3; ++q) verts[3 * d + q] = 0.0f;
+        if (keys) {
+            for (int q = 0; q < 5; ++q) keys[5 * d + q] = 0u;
+        }
+    }
     if (!vslot[slot]) return;
-    const uint32_t id = vid[slot] + 1u;
+    const uint32_t id = vid[slot] + 1u + m;
     const LeafRec& l = cur.leaf[c];
     for (int q = 0; q < 3; ++q) verts[3 * id + q] = float(l.vert[vi][q]);
     if (keys) {
@@ -1837,12 +1883,75 @@ __global__ void k_verts(LevelView cur, int level, const uint32_t* vslot,
     }
 }
 
-hipError_t launch_verts(const LevelView& cur, int level, const uint32_t* vslot,
-                        const uint32_t* vid, float* verts, uint32_t* keys,
-                        hipStream_t s) {
+hipError_t launch_verts(const Params& p, const LevelView& cur, int level,
+                        const uint32_t* vslot, const uint32_t* vid, float* verts,
+                        uint32_t* keys, hipStream_t s) {
     if (cur.n == 0) return hipSuccess;
     const uint32_t n = 4u * cur.n;
-    hipLaunchKernelGGL(k_verts, dim3((n + 255) / 256), dim3(256), 0, s, cur, level, vslot, vid, verts, keys);
+    hipLaunchKernelGGL(k_verts, dim3((n + 255) / 256), dim3(256), 0, s, p, cur, level, vslot, vid,
+                       verts, keys);
+    return hipGetLastError();
+}
+
+__global__ void k_model_bases(const ModelRec* models, uint32_t nmodels,
+                              const uint32_t* tri_off, const uint32_t* tri_count,
+                              uint32_t ncells, const uint32_t* vid, const uint32_t* vslot,
+                              uint32_t nslots, uint32_t* out)
+{
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m > nmodels) return;
+    if (m < nmodels) {
+        const uint32_t s0 = models[m].slot0;
+        out[m] = tri_off[s0 / 4u];
+        out[nmodels + 1 + m] = vid[s0];
+    } else {
+        out[m] = tri_off[ncells - 1] + tri_count[ncells - 1];
+        out[nmodels + 1 + m] = vid[nslots - 1] + vslot[nslots - 1];
+    }
+}
+
+hipError_t launch_model_bases(const ModelRec* models, uint32_t nmodels,
+                              const uint32_t* tri_off, const uint32_t* tri_count,
+                              uint32_t ncells, const uint32_t* vid, const uint32_t* vslot,
+                              uint32_t nslots, uint32_t* out, hipStream_t s) {
+    if (ncells == 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_model_bases, dim3((nmodels + 256) / 256), dim3(256), 0, s, models,
+                       nmodels, tri_off, tri_count, ncells, vid, vslot, nslots, out);
+    return hipGetLastError();
+}
+
+// See launch_child_ranges.  upper is null at the top level of the batch.
+__global__ void k_child_ranges(const ModelRec* models, uint32_t nmodels, int level,
+                               const uint32_t* upper, const uint32_t* upper_base,
+                               const uint32_t* upper_flags, uint32_t upper_n,
+                               uint32_t* ranges)
+{
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmodels) return;
+    uint32_t b = 0, e = 0;
+    if (models[m].top == level) {
+        b = models[m].root_cell;
+        e = b + 1;
+    } else if (upper != nullptr && upper_n > 0) {
+        // ambiguous cells of the level above before index i, times 8
+        auto before = [&](uint32_t i) {
+            return i < upper_n ? upper_base[i]
+                               : upper_base[upper_n - 1] + upper_flags[upper_n - 1];
+        };
+        b = 8u * before(upper[m]);
+        e = 8u * before(upper[nmodels + m]);
+    }
+    ranges[m] = b;
+    ranges[nmodels + m] = e;
+}
+
+hipError_t launch_child_ranges(const ModelRec* models, uint32_t nmodels, int level,
+                               const uint32_t* upper, const uint32_t* upper_base,
+                               const uint32_t* upper_flags, uint32_t upper_n,
+                               uint32_t* ranges, hipStream_t s) {
+    if (nmodels == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_child_ranges, dim3((nmodels + 255) / 256), dim3(256), 0, s, models,
+                       nmodels, level, upper, upper_base, upper_flags, upper_n, ranges);
     return hipGetLastError();
 }
 

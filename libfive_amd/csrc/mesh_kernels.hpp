@@ -35,7 +35,7 @@ struct TapeRec {
     uint32_t flags;      // TAPE_* bits
     float box[6];        // xlo, xhi, ylo, yhi, zlo, zhi of an interval tape
     uint32_t ref_len;    // clauses that are not COPY / LOADI
-    uint32_t pad;
+    uint32_t model;      // model the tape belongs to (index into Params::models)
 };
 enum : uint32_t { TAPE_INTERVAL = 1u, TAPE_TERMINAL = 2u };
 
@@ -54,6 +54,10 @@ struct LeafRec {
     uint32_t flags;      // bit 0 manifold; bits 8-15 level of the vertex
 };
 
+// A level's cells are ordered by model (a batch renders several models
+// through the same passes); the global number of cell i of model m is
+// cbase[m] + i, which numbers the cells model by model, and level by level
+// within a model.
 struct LevelView {
     uint32_t* key;       // x, y, z of the lower corner in finest-cell units
     uint8_t* type;
@@ -61,7 +65,20 @@ struct LevelView {
     uint32_t* tape;      // tape of the cell's children (or of its leaf eval)
     LeafRec* leaf;       // valid when the cell is an ambiguous leaf
     uint32_t n;
-    uint32_t cell_base;  // first global cell number of this level
+    uint32_t pad;
+    const uint32_t* cbase;   // per model: global cell number of index 0
+};
+
+// One model of a render: its region, octree depth and tape layout.  The
+// base tape of model m is tape record m, and every pushed tape carries its
+// parent's model.
+struct ModelRec {
+    double lo[3], hi[3];
+    int32_t top;         // level of the model's root cell
+    uint32_t root_reg;
+    uint32_t slot_x, slot_y, slot_z;
+    uint32_t root_cell;  // index of the root in the level-`top` arrays
+    uint32_t slot0;      // first vertex slot of the model (4 x its first cell)
 };
 
 struct Counters {
@@ -81,9 +98,14 @@ enum : uint32_t {
     ERR_LOCATE = 8u,     // point location inconsistency
 };
 
+// lo, hi, top, root_reg, slot_*, root_cell and slot0 describe one model;
+// a kernel that serves several models takes them from `models` (when not
+// null) for the model of the tape it evaluates (model_view).
 struct Params {
     double lo[3], hi[3];
     int top;                     // level of the root cell
+    uint32_t root_cell;
+    uint32_t slot0;
     double max_err;
     tape::Clause* pool;
     uint32_t pool_cap;
@@ -95,6 +117,7 @@ struct Params {
     uint32_t qef_f32c;           // QEF sums as compensated float pairs
     Counters* counters;
     const LevelView* levels;     // device copy, indexed by level
+    const ModelRec* models;      // null: one model, described by the fields above
 };
 
 hipError_t upload_tables(hipStream_t s);
@@ -118,13 +141,15 @@ size_t interval_lds_bytes(uint32_t nregs, uint32_t max_len);
 size_t interval_workspace_bytes(uint32_t nregs, uint32_t max_len);
 
 // Interval classification and pruning of the cells of one level: one wave
-// per parent (8 children), or the root alone when groups is null.
+// per parent (8 children), or, when groups is null, the roots of models
+// first_root .. first_root + ngroups - 1 (each evaluated with its base tape
+// and stored at its ModelRec::root_cell).
 // max_len bounds the length of every tape the pass reads; ws holds
 // interval_workspace_bytes(p.nregs, max_len) bytes (null when that is 0).
 hipError_t launch_interval(const Params& p, int level, const uint32_t* groups,
                            uint32_t ngroups, const LevelView& parent,
                            const LevelView& cur, uint32_t max_len, uint8_t* ws,
-                           size_t ws_bytes, hipStream_t s);
+                           size_t ws_bytes, hipStream_t s, uint32_t first_root = 0);
 
 // Interval evaluation of listed boxes (xlo, xhi, ylo, yhi, zlo, zhi per
 // box), grouped by the tape they are evaluated with: group g covers boxes
@@ -238,9 +263,30 @@ hipError_t launch_quads(const Params& p, const LevelView& cur, int pass,
                         const uint32_t* tri_off, const uint32_t* vid,
                         uint32_t* tris, hipStream_t s);
 // Vertex positions (and keys: x, y, z, level, patch) of the used slots.
-hipError_t launch_verts(const LevelView& cur, int level, const uint32_t* vslot,
-                        const uint32_t* vid, float* verts, uint32_t* keys,
-                        hipStream_t s);
+// Model m's vertices go to vid + 1 + m: each model's block of the vertex
+// array starts with its own dummy vertex (written as zeros here).
+hipError_t launch_verts(const Params& p, const LevelView& cur, int level,
+                        const uint32_t* vslot, const uint32_t* vid, float* verts,
+                        uint32_t* keys, hipStream_t s);
+
+// Per model m (and m = nmodels for the totals): first triangle (tri_off at
+// the model's first cell, ModelRec::slot0 / 4) and first vertex (vid at its
+// first slot), from the scans of the emit pass; out holds 2 (nmodels + 1)
+// words, the triangle offsets first.
+hipError_t launch_model_bases(const ModelRec* models, uint32_t nmodels,
+                              const uint32_t* tri_off,
+                              const uint32_t* tri_count, uint32_t ncells,
+                              const uint32_t* vid, const uint32_t* vslot, uint32_t nslots,
+                              uint32_t* out, hipStream_t s);
+
+// Per-model cell ranges of a level ([begin, end) in the level's arrays),
+// derived from the ranges of the level above and its scanned ambiguity
+// flags: the children of model m's ambiguous cells, then, for models whose
+// root is at this level, the root.  ranges: 2 nmodels words, begin then end.
+hipError_t launch_child_ranges(const ModelRec* models, uint32_t nmodels, int level,
+                               const uint32_t* upper, const uint32_t* upper_base,
+                               const uint32_t* upper_flags, uint32_t upper_n,
+                               uint32_t* ranges, hipStream_t s);
 
 // Exclusive prefix sum of n uint32 values (device-wide), using
 // scan_scratch_words(n) words of device scratch.

@@ -18,6 +18,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
+#include <exception>
 #include <mutex>
 #include <thread>
 
@@ -61,7 +63,7 @@ struct LevelStore {
         tape.alloc(count);
         leaf.alloc(count);
     }
-    LevelView view(uint32_t base) {
+    LevelView view(const uint32_t* cbase = nullptr) {
         LevelView v;
         v.key = key.p;
         v.type = type.p;
@@ -69,7 +71,8 @@ struct LevelStore {
         v.tape = tape.p;
         v.leaf = leaf.p;
         v.n = n;
-        v.cell_base = base;
+        v.pad = 0;
+        v.cbase = cbase;
         return v;
     }
 };
@@ -233,7 +236,7 @@ bool render_mesh(const Tree& tree, const double lo[3], const double hi[3],
 
 // The lookup tables live in __constant__ memory: uploaded once per process
 // and completed before any render uses them.
-static void ensure_tables() {
+static void ensure_tables_once() {
     static std::once_flag once;
     std::call_once(once, [] {
         hipStream_t s = context().stream;
@@ -262,7 +265,7 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
                         const MeshSettings& settings, HostMesh& out, hipStream_t s) {
     const auto t_start = std::chrono::steady_clock::now();
     g_stats = MeshStats();
-    ensure_tables();
+    ensure_tables_once();
 
     const Deck deck(tree);
     std::vector<float> var_values(deck.vars.size(), 0.0f);
@@ -357,13 +360,13 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
         hip_check(hipMemcpyAsync(virt.tape.p, &tape0, sizeof(tape0), hipMemcpyHostToDevice, s), "root");
         groups[1].alloc(1);
         hip_check(hipMemsetAsync(groups[1].p, 0, sizeof(uint32_t), s), "root");
-        run_leaves(P, groups[1].p, 1, 1, virt.view(0), levels[0].view(0), nbase,
+        run_leaves(P, groups[1].p, 1, 1, virt.view(), levels[0].view(), nbase,
                    fast, settings, counters, hc, s);
     } else {
         hip_check(hipMemsetAsync(&counters.p->max_len, 0, sizeof(uint32_t), s), "max_len");
         iws[top].alloc(interval_workspace_bytes(P.nregs, nbase));
-        hip_check(launch_interval(P, top, nullptr, 1, levels[top].view(0),
-                                  levels[top].view(0), nbase, iws[top].p, iws[top].n, s),
+        hip_check(launch_interval(P, top, nullptr, 1, levels[top].view(),
+                                  levels[top].view(), nbase, iws[top].p, iws[top].n, s),
                   "interval kernel");
         for (int L = top; L >= 1; --L) {
             LevelStore& cur = levels[L];
@@ -371,9 +374,9 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
             basei[L].alloc(cur.n);
             scan_tmp[L].alloc(scan_scratch_words(cur.n));
             groups[L].alloc(cur.n);
-            hip_check(launch_flags(cur.view(0), flags[L].p, s), "flags");
+            hip_check(launch_flags(cur.view(), flags[L].p, s), "flags");
             hip_check(exclusive_scan(flags[L].p, basei[L].p, cur.n, scan_tmp[L].p, s), "scan");
-            hip_check(launch_link(cur.view(0), basei[L].p, groups[L].p, s), "link");
+            hip_check(launch_link(cur.view(), basei[L].p, groups[L].p, s), "link");
             uint32_t last[2] = {0, 0};
             hip_check(hipMemcpyAsync(&last[0], basei[L].p + cur.n - 1, 4, hipMemcpyDeviceToHost, s), "count");
             hip_check(hipMemcpyAsync(&last[1], flags[L].p + cur.n - 1, 4, hipMemcpyDeviceToHost, s), "count");
@@ -396,13 +399,13 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
                 refresh();
                 hip_check(hipMemsetAsync(&counters.p->max_len, 0, sizeof(uint32_t), s), "max_len");
                 iws[L - 1].alloc(interval_workspace_bytes(P.nregs, max_len));
-                hip_check(launch_interval(P, L - 1, groups[L].p, uint32_t(namb), cur.view(0),
-                                          levels[L - 1].view(0), max_len, iws[L - 1].p,
+                hip_check(launch_interval(P, L - 1, groups[L].p, uint32_t(namb), cur.view(),
+                                          levels[L - 1].view(), max_len, iws[L - 1].p,
                                           iws[L - 1].n, s),
                           "interval kernel");
             } else {
-                run_leaves(P, groups[1].p, uint32_t(namb), 8, cur.view(0),
-                           levels[0].view(0), std::max<uint32_t>(hc.max_len, 1),
+                run_leaves(P, groups[1].p, uint32_t(namb), 8, cur.view(),
+                           levels[0].view(), std::max<uint32_t>(hc.max_len, 1),
                            fast, settings, counters, hc, s);
             }
             if (settings.cancelled()) {
@@ -425,6 +428,9 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
         total_cells += levels[L].n;
     }
     if (total_cells * 4 > 0xffffffffull) throw Error("render_mesh: too many cells");
+    PoolBuf<uint32_t> d_cbase(top + 1);
+    hip_check(hipMemcpyAsync(d_cbase.p, cell_base.data(), cell_base.size() * sizeof(uint32_t),
+                             hipMemcpyHostToDevice, s), "cell numbering");
 
     if (settings.collect_stats) {
         // Per-level state counts and pruned tape lengths, before collapse.
@@ -465,7 +471,7 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
     // ---- collapse -----------------------------------------------------------
     const auto t_collapse = std::chrono::steady_clock::now();
     std::vector<LevelView> views(top + 1);
-    for (int L = 0; L <= top; ++L) views[L] = levels[L].view(cell_base[L]);
+    for (int L = 0; L <= top; ++L) views[L] = levels[L].view(d_cbase.p + L);
     for (int L = 1; L <= top; ++L) {
         if (levels[L].n && levels[L - 1].n) {
             hip_check(launch_collapse(P, L, views[L], views[L - 1], s), "collapse kernel");
@@ -522,7 +528,7 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
     for (int L = 0; L <= top; ++L) {
         hip_check(launch_quads(P, views[L], 1, tri_count.p, vslot.p, tri_off.p, vid.p,
                                d_tris.p, s), "quad write");
-        hip_check(launch_verts(views[L], L, vslot.p, vid.p, d_verts.p, d_keys.p, s),
+        hip_check(launch_verts(P, views[L], L, vslot.p, vid.p, d_verts.p, d_keys.p, s),
                   "vertex gather");
     }
     if (settings.cancelled()) {
@@ -567,6 +573,414 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
     g_stats.ms_emit = ms_since(t_emit);
     g_stats.ms_total = ms_since(t_start);
     return true;
+}
+
+// ---- many models through one set of passes -----------------------------------
+
+namespace {
+
+struct MergedModel {
+    DeviceTape tape;
+    uint32_t slot_x = 0, slot_y = 0, slot_z = 0;
+    int top = 0;
+    size_t index = 0;            // position in the caller's list
+};
+
+// Runs f(i) for i in [0, n) on up to 16 host threads.
+template <typename F>
+void parallel_for(size_t n, F f) {
+    const size_t k = std::min<size_t>(n, std::max<unsigned>(1u, std::min(16u,
+                                      std::thread::hardware_concurrency())));
+    if (k <= 1) {
+        for (size_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::exception_ptr err;
+    std::mutex mu;
+    std::vector<std::thread> th;
+    for (size_t w = 0; w < k; ++w) {
+        th.emplace_back([&]() {
+            for (;;) {
+                const size_t i = next.fetch_add(1);
+                if (i >= n) break;
+                try {
+                    f(i);
+                } catch (...) {
+                    std::lock_guard<std::mutex> lock(mu);
+                    if (!err) err = std::current_exception();
+                }
+            }
+        });
+    }
+    for (auto& t : th) t.join();
+    if (err) std::rethrow_exception(err);
+}
+
+void merged_body(const BatchItem* items, const std::vector<size_t>& which, HostMesh* outs,
+                 hipStream_t s) {
+    ensure_tables_once();
+    const size_t nm = which.size();
+    // Front end per model (Deck, register allocation), on host threads.
+    std::vector<MergedModel> mm(nm);
+    parallel_for(nm, [&](size_t k) {
+        const BatchItem& it = items[which[k]];
+        const Deck deck(*it.tree);
+        std::vector<float> var_values(deck.vars.size(), 0.0f);
+        mm[k].tape = compile_deck(deck, &var_values);
+        mm[k].slot_x = deck.X;
+        mm[k].slot_y = deck.Y;
+        mm[k].slot_z = deck.Z;
+        mm[k].top = resolution_level(it.lo, it.hi, it.min_feature);
+        mm[k].index = which[k];
+    });
+    // Roots first by level, highest first: every level's cells are then in
+    // model order (children of earlier models, then the roots that join).
+    std::stable_sort(mm.begin(), mm.end(), [](const MergedModel& a, const MergedModel& b) {
+        return a.top > b.top;
+    });
+    uint32_t nregs = tape::kFirstFreeReg;
+    bool fast = true;
+    size_t pool_clauses = 0;
+    for (const MergedModel& m : mm) {
+        if (m.tape.num_regs > 256) {
+            throw Error("render_mesh: the tape needs " + std::to_string(m.tape.num_regs) +
+                        " registers; the mesher supports 256");
+        }
+        if (m.top > 20) throw Error("render_mesh: resolution too fine (more than 20 levels)");
+        nregs = std::max(nregs, m.tape.num_regs);
+        for (const tape::Clause& c : m.tape.clauses) fast = fast && tape::fast_opcode(tape::op(c));
+        pool_clauses += tape::padded_length(uint32_t(m.tape.clauses.size()));
+    }
+    const int top = mm.front().top;
+
+    // Base tapes (record k = model k) and model records.
+    PoolBuf<tape::Clause> pool(std::max<size_t>(pool_clauses * 64, 1u << 20));
+    PoolBuf<TapeRec> tapes(std::max<size_t>(1u << 16, 4 * nm));
+    std::vector<ModelRec> hmodels(nm);
+    {
+        std::vector<tape::Clause> h(pool_clauses, tape::Clause{0, 0, 0, 0});
+        std::vector<TapeRec> recs(nm);
+        uint32_t off = 0;
+        for (size_t k = 0; k < nm; ++k) {
+            const MergedModel& m = mm[k];
+            std::copy(m.tape.clauses.begin(), m.tape.clauses.end(), h.begin() + off);
+            TapeRec r{};
+            r.off = off;
+            r.len = uint32_t(m.tape.clauses.size());
+            r.parent = kNoTape;
+            r.ref_len = m.tape.ref_length;
+            r.model = uint32_t(k);
+            recs[k] = r;
+            off += tape::padded_length(r.len);
+            const BatchItem& it = items[m.index];
+            ModelRec& M = hmodels[k];
+            for (int a = 0; a < 3; ++a) {
+                M.lo[a] = it.lo[a];
+                M.hi[a] = it.hi[a];
+            }
+            M.top = m.top;
+            M.root_reg = m.tape.root_reg;
+            M.slot_x = m.slot_x;
+            M.slot_y = m.slot_y;
+            M.slot_z = m.slot_z;
+            M.root_cell = 0;
+            M.slot0 = 0;
+        }
+        hip_check(hipMemcpyAsync(pool.p, h.data(), h.size() * sizeof(tape::Clause),
+                                 hipMemcpyHostToDevice, s), "upload tapes");
+        hip_check(hipMemcpyAsync(tapes.p, recs.data(), recs.size() * sizeof(TapeRec),
+                                 hipMemcpyHostToDevice, s), "upload tape records");
+        wait(s, "upload tapes");
+    }
+    PoolBuf<ModelRec> d_models(nm);
+    PoolBuf<Counters> counters(1);
+    Counters hc{};
+    hc.pool_top = uint32_t(pool_clauses);
+    hc.tape_count = uint32_t(nm);
+    hip_check(hipMemcpyAsync(counters.p, &hc, sizeof(hc), hipMemcpyHostToDevice, s), "counters");
+
+    Params P{};
+    const ModelRec& M0 = hmodels[0];
+    for (int a = 0; a < 3; ++a) {
+        P.lo[a] = M0.lo[a];
+        P.hi[a] = M0.hi[a];
+    }
+    P.top = M0.top;
+    P.max_err = MeshSettings().max_err;
+    P.nregs = nregs;
+    P.root_reg = M0.root_reg;
+    P.slot_x = M0.slot_x;
+    P.slot_y = M0.slot_y;
+    P.slot_z = M0.slot_z;
+    P.qef_f32c = 0;
+    P.counters = counters.p;
+    P.models = d_models.p;
+    auto refresh = [&]() {
+        P.pool = pool.p;
+        P.pool_cap = uint32_t(std::min<size_t>(pool.n, 0xffffffffu));
+        P.tapes = tapes.p;
+        P.tape_cap = uint32_t(tapes.n);
+    };
+    refresh();
+
+    std::vector<LevelStore> levels(top + 1);
+    std::vector<PoolBuf<uint32_t>> groups(top + 2), flags(top + 1), basei(top + 1),
+        scan_tmp(top + 1), ranges(top + 1);
+    std::vector<PoolBuf<uint8_t>> iws(top + 1);
+    std::vector<uint32_t> namb(top + 2, 0);
+    uint32_t max_len_above = 0;
+    const MeshSettings settings;
+
+    // ---- build ------------------------------------------------------------
+    size_t next_root = 0;            // models are in decreasing order of top
+    for (int L = top; L >= 1; --L) {
+        const size_t first_root = next_root;
+        uint32_t root_len = 0;
+        while (next_root < nm && mm[next_root].top == L) {
+            hmodels[next_root].root_cell = 8u * namb[L + 1] + uint32_t(next_root - first_root);
+            root_len = std::max<uint32_t>(root_len, uint32_t(mm[next_root].tape.clauses.size()));
+            ++next_root;
+        }
+        const uint32_t nroots = uint32_t(next_root - first_root);
+        const uint64_t ncells = 8ull * namb[L + 1] + nroots;
+        if (ncells > 0xffffffffull / 4) throw Error("render_mesh: too many cells");
+        levels[L].alloc(uint32_t(ncells));
+        ranges[L].alloc(2 * nm);
+        if (nroots) {
+            hip_check(hipMemcpyAsync(d_models.p + first_root, &hmodels[first_root],
+                                     nroots * sizeof(ModelRec), hipMemcpyHostToDevice, s),
+                      "model records");
+        }
+        if (ncells) {
+            // room for the pushes of this level's evaluations
+            grow(pool, hc.pool_top, size_t(hc.pool_top) +
+                 size_t(namb[L + 1]) * 8u * (max_len_above + kPushExtra) +
+                 size_t(nroots) * (root_len + kPushExtra), s);
+            grow(tapes, hc.tape_count, size_t(hc.tape_count) + ncells, s);
+            refresh();
+            hip_check(hipMemsetAsync(&counters.p->max_len, 0, sizeof(uint32_t), s), "max_len");
+            const uint32_t wl = std::max(max_len_above, root_len);
+            iws[L].alloc(interval_workspace_bytes(P.nregs, wl));
+            if (namb[L + 1]) {
+                hip_check(launch_interval(P, L, groups[L + 1].p, namb[L + 1], levels[L + 1].view(),
+                                          levels[L].view(), max_len_above, iws[L].p, iws[L].n, s),
+                          "interval kernel");
+            }
+            if (nroots) {
+                hip_check(launch_interval(P, L, nullptr, nroots, levels[L].view(), levels[L].view(),
+                                          root_len, iws[L].p, iws[L].n, s, uint32_t(first_root)),
+                          "root interval kernel");
+            }
+        }
+        const bool has_upper = L < top && levels[L + 1].n > 0;
+        hip_check(launch_child_ranges(d_models.p, uint32_t(nm), L,
+                                      has_upper ? ranges[L + 1].p : nullptr,
+                                      has_upper ? basei[L + 1].p : nullptr,
+                                      has_upper ? flags[L + 1].p : nullptr,
+                                      has_upper ? levels[L + 1].n : 0u, ranges[L].p, s),
+                  "cell ranges");
+        LevelStore& cur = levels[L];
+        if (cur.n == 0) {
+            namb[L] = 0;
+            max_len_above = 0;
+            continue;
+        }
+        flags[L].alloc(cur.n);
+        basei[L].alloc(cur.n);
+        scan_tmp[L].alloc(scan_scratch_words(cur.n));
+        groups[L].alloc(cur.n);
+        hip_check(launch_flags(cur.view(), flags[L].p, s), "flags");
+        hip_check(exclusive_scan(flags[L].p, basei[L].p, cur.n, scan_tmp[L].p, s), "scan");
+        hip_check(launch_link(cur.view(), basei[L].p, groups[L].p, s), "link");
+        uint32_t last[2] = {0, 0};
+        hip_check(hipMemcpyAsync(&last[0], basei[L].p + cur.n - 1, 4, hipMemcpyDeviceToHost, s), "count");
+        hip_check(hipMemcpyAsync(&last[1], flags[L].p + cur.n - 1, 4, hipMemcpyDeviceToHost, s), "count");
+        hip_check(hipMemcpyAsync(&hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost, s), "counters");
+        wait(s, "interval pass");
+        check_errors(hc.error);
+        const uint64_t na = uint64_t(last[0]) + last[1];
+        if (na > cur.n) throw Error("render_mesh: inconsistent cell count");
+        if (na * 8u > 0xffffffffull / 4) throw Error("render_mesh: too many cells");
+        namb[L] = uint32_t(na);
+        max_len_above = hc.max_len;
+        if (na && (max_len_above == 0 || max_len_above > hc.pool_top)) {
+            throw Error("render_mesh: inconsistent tape length");
+        }
+    }
+    if (next_root != nm) throw Error("render_mesh: a model's root level was not reached");
+    // level 0: the children of the ambiguous level-1 cells
+    levels[0].alloc(8u * namb[1]);
+    ranges[0].alloc(2 * nm);
+    if (namb[1]) {
+        run_leaves(P, groups[1].p, namb[1], 8, levels[1].view(), levels[0].view(),
+                   std::max<uint32_t>(max_len_above, 1), fast, settings, counters, hc, s);
+    }
+    {
+        const bool has_upper = levels[1].n > 0;
+        hip_check(launch_child_ranges(d_models.p, uint32_t(nm), 0,
+                                      has_upper ? ranges[1].p : nullptr,
+                                      has_upper ? basei[1].p : nullptr,
+                                      has_upper ? flags[1].p : nullptr,
+                                      has_upper ? levels[1].n : 0u, ranges[0].p, s),
+                  "cell ranges");
+    }
+
+    // Global numbering: model by model, level 0 first within a model.
+    std::vector<uint32_t> hr(size_t(top + 1) * 2 * nm);
+    for (int L = 0; L <= top; ++L) {
+        hip_check(hipMemcpyAsync(hr.data() + size_t(L) * 2 * nm, ranges[L].p,
+                                 2 * nm * sizeof(uint32_t), hipMemcpyDeviceToHost, s), "ranges");
+    }
+    wait(s, "cell ranges");
+    std::vector<uint32_t> cbase(size_t(top + 1) * nm);
+    uint64_t total_cells = 0;
+    for (size_t k = 0; k < nm; ++k) {
+        hmodels[k].slot0 = uint32_t(4 * total_cells);
+        for (int L = 0; L <= top; ++L) {
+            const uint32_t b = hr[size_t(L) * 2 * nm + k], e = hr[size_t(L) * 2 * nm + nm + k];
+            if (e < b || e > levels[L].n) throw Error("render_mesh: inconsistent cell ranges");
+            // index i of the level maps to total_cells + (i - b)
+            cbase[size_t(L) * nm + k] = uint32_t(total_cells - b);
+            total_cells += e - b;
+        }
+        if (total_cells * 4 > 0xffffffffull) throw Error("render_mesh: too many cells");
+    }
+    {
+        uint64_t all = 0;
+        for (int L = 0; L <= top; ++L) all += levels[L].n;
+        if (all != total_cells) throw Error("render_mesh: cell ranges do not cover the levels");
+    }
+    PoolBuf<uint32_t> d_cbase(cbase.size());
+    hip_check(hipMemcpyAsync(d_cbase.p, cbase.data(), cbase.size() * sizeof(uint32_t),
+                             hipMemcpyHostToDevice, s), "cell numbering");
+    hip_check(hipMemcpyAsync(d_models.p, hmodels.data(), nm * sizeof(ModelRec),
+                             hipMemcpyHostToDevice, s), "model records");
+
+    // ---- collapse -----------------------------------------------------------
+    std::vector<LevelView> views(top + 1);
+    for (int L = 0; L <= top; ++L) views[L] = levels[L].view(d_cbase.p + size_t(L) * nm);
+    for (int L = 1; L <= top; ++L) {
+        if (levels[L].n && levels[L - 1].n) {
+            hip_check(launch_collapse(P, L, views[L], views[L - 1], s), "collapse kernel");
+        }
+    }
+
+    // ---- emit -----------------------------------------------------------------
+    PoolBuf<LevelView> d_views(top + 1);
+    hip_check(hipMemcpyAsync(d_views.p, views.data(), views.size() * sizeof(LevelView),
+                             hipMemcpyHostToDevice, s), "views");
+    P.levels = d_views.p;
+    PoolBuf<uint32_t> tri_count(total_cells), tri_off(total_cells);
+    PoolBuf<uint32_t> vslot(4 * total_cells), vid(4 * total_cells);
+    PoolBuf<uint32_t> scan_a(scan_scratch_words(total_cells));
+    PoolBuf<uint32_t> scan_b(scan_scratch_words(4 * total_cells));
+    hip_check(hipMemsetAsync(vslot.p, 0, vslot.n * sizeof(uint32_t), s), "vslot");
+    for (int L = 0; L <= top; ++L) {
+        hip_check(launch_quads(P, views[L], 0, tri_count.p, vslot.p, nullptr, nullptr,
+                               nullptr, s), "quad count");
+    }
+    hip_check(exclusive_scan(tri_count.p, tri_off.p, total_cells, scan_a.p, s), "scan");
+    hip_check(exclusive_scan(vslot.p, vid.p, vslot.n, scan_b.p, s), "scan");
+    PoolBuf<uint32_t> d_bases(2 * (nm + 1));
+    hip_check(launch_model_bases(d_models.p, uint32_t(nm), tri_off.p, tri_count.p,
+                                 uint32_t(total_cells), vid.p, vslot.p, uint32_t(vslot.n),
+                                 d_bases.p, s), "model bases");
+    std::vector<uint32_t> bases(2 * (nm + 1));
+    hip_check(hipMemcpyAsync(bases.data(), d_bases.p, bases.size() * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, s), "model bases");
+    hip_check(hipMemcpyAsync(&hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost, s), "counters");
+    wait(s, "quad count");
+    check_errors(hc.error);
+    const uint32_t* tb = bases.data();
+    const uint32_t* vb = bases.data() + nm + 1;
+    const uint64_t ntris = tb[nm], nverts = vb[nm];
+    if (ntris > 24 * total_cells || nverts > 4 * total_cells) {
+        throw Error("render_mesh: inconsistent mesh size");
+    }
+    for (size_t k = 0; k < nm; ++k) {
+        if (tb[k + 1] < tb[k] || vb[k + 1] < vb[k]) throw Error("render_mesh: inconsistent mesh size");
+    }
+    const size_t nv_all = size_t(nverts) + nm;            // a dummy vertex per model
+    PoolBuf<uint32_t> d_tris(3 * std::max<size_t>(ntris, 1));
+    PoolBuf<float> d_verts(3 * nv_all);
+    bool want_keys = false;
+    for (size_t k = 0; k < nm; ++k) want_keys = want_keys || outs[mm[k].index].want_keys;
+    PoolBuf<uint32_t> d_keys(want_keys ? 5 * nv_all : 0);
+    for (int L = 0; L <= top; ++L) {
+        hip_check(launch_quads(P, views[L], 1, tri_count.p, vslot.p, tri_off.p, vid.p,
+                               d_tris.p, s), "quad write");
+        hip_check(launch_verts(P, views[L], L, vslot.p, vid.p, d_verts.p, d_keys.p, s),
+                  "vertex gather");
+    }
+    const size_t vbytes = 3 * nv_all * sizeof(float);
+    const size_t tbytes = 3 * size_t(ntris) * sizeof(uint32_t);
+    const size_t kbytes = want_keys ? 5 * nv_all * sizeof(uint32_t) : 0;
+    size_t granted = 0;
+    char* stage = static_cast<char*>(pinned_alloc(vbytes + tbytes + kbytes, &granted));
+    struct StageRelease {
+        char* p;
+        size_t n;
+        ~StageRelease() { pinned_free(p, n); }
+    } stage_release{stage, granted};
+    hip_check(hipMemcpyAsync(stage, d_verts.p, vbytes, hipMemcpyDeviceToHost, s), "copy verts");
+    if (ntris) {
+        hip_check(hipMemcpyAsync(stage + vbytes, d_tris.p, tbytes, hipMemcpyDeviceToHost, s),
+                  "copy tris");
+    }
+    if (kbytes) {
+        hip_check(hipMemcpyAsync(stage + vbytes + tbytes, d_keys.p, kbytes,
+                                 hipMemcpyDeviceToHost, s), "copy keys");
+    }
+    wait(s, "mesh copy");
+    check_errors(hc.error);
+    // Each model's block: its dummy vertex and vertices at vb[k] + k, its
+    // triangles (already numbered within the model) at tb[k].
+    const float* sv = reinterpret_cast<const float*>(stage);
+    const uint32_t* st = reinterpret_cast<const uint32_t*>(stage + vbytes);
+    const uint32_t* sk = reinterpret_cast<const uint32_t*>(stage + vbytes + tbytes);
+    parallel_for(nm, [&](size_t k) {
+        HostMesh& out = outs[mm[k].index];
+        const size_t nv = size_t(vb[k + 1] - vb[k]) + 1, nt = tb[k + 1] - tb[k];
+        const size_t v0 = size_t(vb[k]) + k;
+        float* vdst = nullptr;
+        uint32_t* tdst = nullptr;
+        if (out.alloc) {
+            out.alloc(nv, nt, &vdst, &tdst);
+        } else {
+            out.verts.resize(3 * nv);
+            out.tris.resize(3 * nt);
+            vdst = out.verts.data();
+            tdst = out.tris.data();
+        }
+        std::memcpy(vdst, sv + 3 * v0, 3 * nv * sizeof(float));
+        if (nt) std::memcpy(tdst, st + 3 * size_t(tb[k]), 3 * nt * sizeof(uint32_t));
+        if (out.want_keys) out.keys.assign(sk + 5 * v0, sk + 5 * (v0 + nv));
+    });
+}
+
+}   // namespace
+
+void render_meshes_merged(const BatchItem* items, size_t n, HostMesh* outs, hipStream_t s) {
+    std::vector<size_t> merged;
+    for (size_t i = 0; i < n; ++i) {
+        if (resolution_level(items[i].lo, items[i].hi, items[i].min_feature) >= 1) {
+            merged.push_back(i);
+        } else {
+            MeshSettings ms;
+            ms.min_feature = items[i].min_feature;
+            render_mesh_on(*items[i].tree, items[i].lo, items[i].hi, ms, outs[i], s);
+        }
+    }
+    if (merged.empty()) return;
+    pool_defer_begin();
+    try {
+        merged_body(items, merged, outs, s);
+        pool_defer_end(s);
+    } catch (...) {
+        pool_defer_end(s);
+        throw;
+    }
 }
 
 }   // namespace amd

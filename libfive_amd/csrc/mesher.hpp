@@ -73,6 +73,25 @@ bool render_mesh(const Tree& t, const double lo[3], const double hi[3],
 bool render_mesh_on(const Tree& t, const double lo[3], const double hi[3],
                     const MeshSettings& s, HostMesh& out, hipStream_t stream);
 
+// One model of a merged batch: a tree meshed over [lo, hi] with
+// min_feature (default max_err, FP64 QEF sums).
+struct BatchItem {
+    const Tree* tree;
+    double lo[3], hi[3];
+    double min_feature;
+};
+
+// Meshes many models through one set of level-synchronous passes: every
+// pass (interval and pruning, leaf corners, edges and solves, collapse,
+// quads, vertex gather) is one launch over the cells of all models, and the
+// host waits once per level for all of them instead of once per level per
+// model.  A model's cells and tapes carry its index (ModelRec), so each
+// model's mesh is the one render_mesh gives for it, bit for bit.  outs[i]
+// receives model i (through its alloc callback, or its vectors; keys when
+// want_keys).  Models whose root is a leaf (resolution level 0) are meshed
+// by render_mesh_on on the same stream.
+void render_meshes_merged(const BatchItem* items, size_t n, HostMesh* outs, hipStream_t s);
+
 const MeshStats& last_mesh_stats();
 
 // Region::withResolution (region.hpp:226-231), evaluated in float.

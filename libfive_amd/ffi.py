@@ -170,6 +170,10 @@ SIGNATURES = {
         ctypes.c_int, [ctypes.POINTER(tree_t), ctypes.POINTER(libfive_region3),
                        fp, ctypes.c_size_t, ctypes.c_int,
                        ctypes.POINTER(ctypes.POINTER(libfive_mesh))]),
+    "libfive_trees_render_mesh_merged": (
+        ctypes.c_int, [ctypes.POINTER(tree_t), ctypes.POINTER(libfive_region3),
+                       fp, ctypes.c_size_t,
+                       ctypes.POINTER(ctypes.POINTER(libfive_mesh))]),
     "libfive_tree_render_mesh_coords": (ctypes.POINTER(libfive_mesh_coords),
                                         [tree_t, libfive_region3, ctypes.c_float]),
     "libfive_mesh_coords_delete": (None, [ctypes.POINTER(libfive_mesh_coords)]),

@@ -345,8 +345,10 @@ def _mesh_arrays(m):
     return verts, tris
 
 
-def render_mesh_batch(shapes, regions, res, streams=8, keep=True):
-    """Meshes many shapes in one call (libfive_trees_render_mesh_batch).
+def render_mesh_batch(shapes, regions, res, streams=8, keep=True, merged=False):
+    """Meshes many shapes in one call: libfive_trees_render_mesh_batch
+    (`streams` models at a time on their own streams) or, with merged,
+    libfive_trees_render_mesh_merged (one set of passes over all models).
     regions[i] = (lo, hi) 3-vectors, res[i] = cells per unit.  Returns a
     list of (verts, tris), or of (vert_count, tri_count) with keep=False."""
     L = ffi.lib()
@@ -359,8 +361,12 @@ def render_mesh_batch(shapes, regions, res, streams=8, keep=True):
                                    ffi.libfive_interval(lo[2], hi[2]))
     rr = np.ascontiguousarray(res, np.float32)
     out = (ctypes.POINTER(ffi.libfive_mesh) * n)()
-    failed = L.libfive_trees_render_mesh_batch(
-        trees, R, rr.ctypes.data_as(ffi.fp), n, int(streams), out)
+    if merged:
+        failed = L.libfive_trees_render_mesh_merged(
+            trees, R, rr.ctypes.data_as(ffi.fp), n, out)
+    else:
+        failed = L.libfive_trees_render_mesh_batch(
+            trees, R, rr.ctypes.data_as(ffi.fp), n, int(streams), out)
     try:
         if failed:
             raise GpuError(ffi.last_error())

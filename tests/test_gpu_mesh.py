@@ -300,6 +300,47 @@ def test_batch_matches_single_renders(gpu):
         assert (t == t1).all()
 
 
+def test_merged_batch_matches_single_renders(gpu):
+    """libfive_trees_render_mesh_merged runs all models through one set of
+    passes; each model's mesh is still libfive_tree_render_mesh's, bit for
+    bit.  The models differ in region, depth (root levels 0 to 8, in no
+    particular order), tape length, register count (one, two and four
+    banks), features (exact zeros on tied MIN/MAX) and emptiness."""
+    from libfive_amd import render_mesh_batch
+    models = []
+    for s, r in W.csg_batch(12, seed=W.SEED_BASE + 43):
+        models.append((s, ((-1.0,) * 3, (1.0,) * 3), r / 2.0))
+    models += [
+        (W.sphere(0.7), ((-2.0,) * 3, (2.0,) * 3), 4.0),
+        (W.gyroid_c4(), ((-1.2,) * 3, (1.2,) * 3), 32 / 2.4),
+        (W.box((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5)), ((-1.0,) * 3, (1.0,) * 3), 8.0),
+        (W.random_dag() - 0.3, ((-1.0,) * 3, (1.0,) * 3), 8.0),
+        (wide_registers(70), ((-1.0,) * 3, (1.0,) * 3), 8.0),
+        (W.sphere(0.2, (3.0, 3.0, 3.0)), ((-1.0,) * 3, (1.0,) * 3), 16.0),   # empty
+        (W.sphere(0.5), ((-1.0,) * 3, (1.0,) * 3), 0.25),                    # root leaf
+        (W.sphere_union(60), ((-1.2,) * 3, (1.2,) * 3), 64 / 2.4),
+        (W.csg_c1(), ((-2.0, -2.0, -2.0), (2.0, 2.0, 2.0)), 8.0),
+        (W.sphere(0.6), ((-0.9, -1.1, -0.7), (0.8, 1.2, 0.75)), 20.0),    # not a cube
+    ]
+    order = np.random.default_rng(7).permutation(len(models))
+    models = [models[i] for i in order]
+    shapes = [m[0] for m in models]
+    regions = [m[1] for m in models]
+    res = [m[2] for m in models]
+    print("merged batch of", len(shapes), "models", flush=True)
+    got = render_mesh_batch(shapes, regions, res, merged=True)
+    for i, (s, (lo, hi), r) in enumerate(models):
+        v1, t1 = s.render_mesh(list(lo), list(hi), res=r)
+        v, t = got[i]
+        assert v.shape == v1.shape and t.shape == t1.shape, i
+        assert (v.view(np.uint32) == v1.view(np.uint32)).all(), i
+        assert (t == t1).all(), i
+    # a second call on the same library state gives the same meshes
+    again = render_mesh_batch(shapes, regions, res, merged=True)
+    for (v, t), (v2, t2) in zip(got, again):
+        assert (v.view(np.uint32) == v2.view(np.uint32)).all() and (t == t2).all()
+
+
 def _region(lo, hi):
     from libfive_amd import ffi
     return ffi.libfive_region3(ffi.libfive_interval(lo, hi),

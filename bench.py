@@ -67,13 +67,15 @@ def parse():
     p.add_argument("--sweep", action="store_true",
                    help="also time the value kernel at 1, 2 and 4 points "
                         "per lane")
-    p.add_argument("--mesh-only", choices=("gyroid", "union", "union512", "batch"),
+    p.add_argument("--mesh-only", choices=("gyroid", "union", "union512", "batch",
+                                           "merged"),
                    default=None,
                    help="only render this mesh workload, --mesh-steps times "
                         "after one warm-up, and print the times (for "
                         "profiling: the gyroid at --mesh-res, the sphere "
                         "union at depth 8, or the union at --mesh-res; "
-                        "batch: only the many-model leg)")
+                        "batch: only the many-model leg; merged: only the "
+                        "merged many-model call)")
     return p.parse_args()
 
 
@@ -84,6 +86,20 @@ def mesh_only(args):
     from libfive_amd import workloads as W
     if args.mesh_only == "batch":
         print(json.dumps({"many_models": many_models(args)}))
+        return
+    if args.mesh_only == "merged":
+        from libfive_amd import render_mesh_batch
+        models = W.csg_batch(args.batch_models)
+        shapes = [m for m, _ in models]
+        res = [r / 2.0 for _, r in models]
+        regions = [((-1.0,) * 3, (1.0,) * 3)] * len(shapes)
+        ts = []
+        for _ in range(1 + args.mesh_steps):
+            t0 = time.perf_counter()
+            render_mesh_batch(shapes, regions, res, keep=False, merged=True)
+            ts.append((time.perf_counter() - t0) * 1e3)
+        print(json.dumps({"mesh_only": "merged", "models": len(shapes),
+                          "ms": [round(t, 1) for t in ts]}))
         return
     if args.mesh_only == "gyroid":
         shape = W.gyroid_c4()

@@ -487,8 +487,20 @@ void k_interval(Params P, const Clause* __restrict__ pool_in,
         else if (rlo > 0.0f) state = EMPTY;
         else if (rhi < 0.0f) state = FILLED;
         else state = AMBIGUOUS;
-        if (active && state == AMBIGUOUS) {
-            atomicMax(&P.counters->max_len, P.tapes[child_tape].len);
+        // The longest tape and the push budget of the ambiguous children,
+        // reduced over the wave's 8 lanes and added by lane 0.
+        uint32_t len = (active && state == AMBIGUOUS) ? P.tapes[child_tape].len : 0u;
+        unsigned long long need = (active && state == AMBIGUOUS)
+            ? 8ull * (len + kPushExtra) : 0ull;
+        for (int d = 1; d < 8; d <<= 1) {
+            const uint32_t ol = __shfl_xor(len, d);
+            len = ol > len ? ol : len;
+            need += (unsigned long long)__shfl_xor((uint32_t)need, d) |
+                    ((unsigned long long)__shfl_xor((uint32_t)(need >> 32), d) << 32);
+        }
+        if (lane == 0 && len) {
+            atomicMax(&P.counters->max_len, len);
+            atomicAdd(&P.counters->push_need, need);
         }
         if (active) {
             cur.type[cidx] = state;

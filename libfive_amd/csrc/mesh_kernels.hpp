@@ -90,6 +90,10 @@ struct Counters {
     uint32_t edges;      // lattice edges searched by the leaf pass
     uint32_t tasks;      // edge-search tasks (four edges each)
     uint32_t leaves;     // ambiguous leaves solved by the leaf pass
+    uint32_t pad;
+    // Clauses the pushes of the next level may take: over the ambiguous
+    // cells of the last level, 8 x (tape length + kPushExtra).
+    unsigned long long push_need;
 };
 enum : uint32_t {
     ERR_POOL = 1u,       // clause pool exhausted

@@ -117,6 +117,22 @@ void wait(hipStream_t s, const char* what) {
     }
 }
 
+// Clears the per-level maxima and sums the interval pass accumulates
+// (Counters::max_len, Counters::push_need).
+void reset_level_counters(Counters* c, hipStream_t s) {
+    hip_check(hipMemsetAsync(&c->max_len, 0, sizeof(uint32_t), s), "max_len");
+    hip_check(hipMemsetAsync(&c->push_need, 0, sizeof(c->push_need), s), "push_need");
+}
+
+// The pushes of a level take at most need clauses (Counters::push_need);
+// need is also bounded by namb x 8 x (max_len + kPushExtra).
+size_t push_budget(const Counters& hc, uint64_t namb) {
+    const unsigned long long bound =
+        (unsigned long long)namb * 8ull * (hc.max_len + kPushExtra);
+    if (hc.push_need > bound) throw Error("render_mesh: inconsistent push budget");
+    return size_t(hc.push_need);
+}
+
 void check_errors(uint32_t err) {
     if (err & ERR_FEATURE) {
         throw Error("render_mesh: a point on a min/max feature has more "
@@ -363,7 +379,7 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
         run_leaves(P, groups[1].p, 1, 1, virt.view(), levels[0].view(), nbase,
                    fast, settings, counters, hc, s);
     } else {
-        hip_check(hipMemsetAsync(&counters.p->max_len, 0, sizeof(uint32_t), s), "max_len");
+        reset_level_counters(counters.p, s);
         iws[top].alloc(interval_workspace_bytes(P.nregs, nbase));
         hip_check(launch_interval(P, top, nullptr, 1, levels[top].view(),
                                   levels[top].view(), nbase, iws[top].p, iws[top].n, s),
@@ -393,11 +409,10 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
                 if (max_len == 0 || max_len > hc.pool_top) {
                     throw Error("render_mesh: inconsistent tape length");
                 }
-                grow(pool, hc.pool_top,
-                     size_t(hc.pool_top) + size_t(namb) * 8u * (max_len + kPushExtra), s);
+                grow(pool, hc.pool_top, size_t(hc.pool_top) + push_budget(hc, namb), s);
                 grow(tapes, hc.tape_count, size_t(hc.tape_count) + size_t(namb) * 8u, s);
                 refresh();
-                hip_check(hipMemsetAsync(&counters.p->max_len, 0, sizeof(uint32_t), s), "max_len");
+                reset_level_counters(counters.p, s);
                 iws[L - 1].alloc(interval_workspace_bytes(P.nregs, max_len));
                 hip_check(launch_interval(P, L - 1, groups[L].p, uint32_t(namb), cur.view(),
                                           levels[L - 1].view(), max_len, iws[L - 1].p,
@@ -619,6 +634,8 @@ void parallel_for(size_t n, F f) {
 
 void merged_body(const BatchItem* items, const std::vector<size_t>& which, HostMesh* outs,
                  hipStream_t s) {
+    const auto t_start = std::chrono::steady_clock::now();
+    g_stats = MeshStats();
     ensure_tables_once();
     const size_t nm = which.size();
     // Front end per model (Deck, register allocation), on host threads.
@@ -653,6 +670,7 @@ void merged_body(const BatchItem* items, const std::vector<size_t>& which, HostM
         pool_clauses += tape::padded_length(uint32_t(m.tape.clauses.size()));
     }
     const int top = mm.front().top;
+    const double ms_front = ms_since(t_start);
 
     // Base tapes (record k = model k) and model records.
     PoolBuf<tape::Clause> pool(std::max<size_t>(pool_clauses * 64, 1u << 20));
@@ -730,6 +748,7 @@ void merged_body(const BatchItem* items, const std::vector<size_t>& which, HostM
     std::vector<PoolBuf<uint8_t>> iws(top + 1);
     std::vector<uint32_t> namb(top + 2, 0);
     uint32_t max_len_above = 0;
+    size_t push_above = 0;           // push budget of the level above's cells
     const MeshSettings settings;
 
     // ---- build ------------------------------------------------------------
@@ -754,12 +773,11 @@ void merged_body(const BatchItem* items, const std::vector<size_t>& which, HostM
         }
         if (ncells) {
             // room for the pushes of this level's evaluations
-            grow(pool, hc.pool_top, size_t(hc.pool_top) +
-                 size_t(namb[L + 1]) * 8u * (max_len_above + kPushExtra) +
+            grow(pool, hc.pool_top, size_t(hc.pool_top) + push_above +
                  size_t(nroots) * (root_len + kPushExtra), s);
             grow(tapes, hc.tape_count, size_t(hc.tape_count) + ncells, s);
             refresh();
-            hip_check(hipMemsetAsync(&counters.p->max_len, 0, sizeof(uint32_t), s), "max_len");
+            reset_level_counters(counters.p, s);
             const uint32_t wl = std::max(max_len_above, root_len);
             iws[L].alloc(interval_workspace_bytes(P.nregs, wl));
             if (namb[L + 1]) {
@@ -784,6 +802,7 @@ void merged_body(const BatchItem* items, const std::vector<size_t>& which, HostM
         if (cur.n == 0) {
             namb[L] = 0;
             max_len_above = 0;
+            push_above = 0;
             continue;
         }
         flags[L].alloc(cur.n);
@@ -807,6 +826,7 @@ void merged_body(const BatchItem* items, const std::vector<size_t>& which, HostM
         if (na && (max_len_above == 0 || max_len_above > hc.pool_top)) {
             throw Error("render_mesh: inconsistent tape length");
         }
+        push_above = push_budget(hc, na);
     }
     if (next_root != nm) throw Error("render_mesh: a model's root level was not reached");
     // level 0: the children of the ambiguous level-1 cells
@@ -857,7 +877,10 @@ void merged_body(const BatchItem* items, const std::vector<size_t>& which, HostM
     hip_check(hipMemcpyAsync(d_models.p, hmodels.data(), nm * sizeof(ModelRec),
                              hipMemcpyHostToDevice, s), "model records");
 
+    g_stats.ms_build = ms_since(t_start);
+
     // ---- collapse -----------------------------------------------------------
+    const auto t_collapse = std::chrono::steady_clock::now();
     std::vector<LevelView> views(top + 1);
     for (int L = 0; L <= top; ++L) views[L] = levels[L].view(d_cbase.p + size_t(L) * nm);
     for (int L = 1; L <= top; ++L) {
@@ -866,7 +889,11 @@ void merged_body(const BatchItem* items, const std::vector<size_t>& which, HostM
         }
     }
 
+    wait(s, "collapse");
+    g_stats.ms_collapse = ms_since(t_collapse);
+
     // ---- emit -----------------------------------------------------------------
+    const auto t_emit = std::chrono::steady_clock::now();
     PoolBuf<LevelView> d_views(top + 1);
     hip_check(hipMemcpyAsync(d_views.p, views.data(), views.size() * sizeof(LevelView),
                              hipMemcpyHostToDevice, s), "views");
@@ -934,6 +961,7 @@ void merged_body(const BatchItem* items, const std::vector<size_t>& which, HostM
     }
     wait(s, "mesh copy");
     check_errors(hc.error);
+    const double ms_device = ms_since(t_emit);
     // Each model's block: its dummy vertex and vertices at vb[k] + k, its
     // triangles (already numbered within the model) at tb[k].
     const float* sv = reinterpret_cast<const float*>(stage);
@@ -957,6 +985,14 @@ void merged_body(const BatchItem* items, const std::vector<size_t>& which, HostM
         if (nt) std::memcpy(tdst, st + 3 * size_t(tb[k]), 3 * nt * sizeof(uint32_t));
         if (out.want_keys) out.keys.assign(sk + 5 * v0, sk + 5 * (v0 + nv));
     });
+    g_stats.ms_emit = ms_since(t_emit);
+    g_stats.ms_total = ms_since(t_start);
+    if (std::getenv("LIBFIVE_MESH_TRACE") != nullptr) {
+        std::fprintf(stderr, "merged render: %zu models, front end %.1f ms, build %.1f ms, "
+                     "collapse %.1f ms, emit %.1f ms (device part %.1f), total %.1f ms\n",
+                     nm, ms_front, g_stats.ms_build - ms_front, g_stats.ms_collapse,
+                     g_stats.ms_emit, ms_device, g_stats.ms_total);
+    }
 }
 
 }   // namespace

@@ -656,6 +656,9 @@ def main():
     L.libfive_tape_delete(tape)
     if not args.no_mesh:
         result["mesh"] = mesh_measurements(args)
+    # launch schedules the autotuner picked during the warm-ups (candidate
+    # times in ms, each on a slice of the real work)
+    result["autotune"] = libfive_amd.autotune_report()
     print(json.dumps(result))
 
 

@@ -331,6 +331,22 @@ int libfive_trees_render_mesh_merged(const libfive_tree* trees,
                                      const float* res, size_t n,
                                      libfive_mesh** out);
 
+/* Launch-schedule autotuner (no counterpart in the reference, whose CPU
+ * evaluators have no launch schedules).  The first large evaluation call of
+ * a tape and the first leaf pass of each register-bank class time every
+ * launch schedule the kernel offers (workgroup shape, points per lane,
+ * waves per workgroup; all give the same results) on a slice of the real
+ * work and keep the fastest for the process.  libfive_autotune_enable(0)
+ * turns tuning off (default schedules); libfive_autotune_report writes the
+ * decisions made so far as JSON text into buf (at most cap bytes, NUL
+ * terminated) and returns the length needed including the NUL. */
+void libfive_autotune_enable(int on);
+size_t libfive_autotune_report(char* buf, size_t cap);
+/* Forgets every decision; makes `choice` the decision for `key` (a key of
+ * the report) without timing, so that each schedule can be exercised. */
+void libfive_autotune_reset(void);
+void libfive_autotune_force(const char* key, int choice);
+
 /* Tape specialisation (no counterpart in the reference, whose evaluators
  * interpret the tape on the CPU: libfive/src/eval/eval_array.cpp:107-118).
  * Compiles the tape into straight-line value and derivative kernels with
@@ -365,6 +381,12 @@ int libfive_tree_deck(libfive_tree t, uint32_t* clauses, size_t* n_clauses,
                       uint32_t* const_ids, float* const_values,
                       size_t* n_consts, uint32_t* xyz_ids,
                       uint32_t* num_slots);
+/* The free variables of a tree's Deck: slot_ids[k] is the slot of the
+ * variable whose libfive_tree_id is var_ids[k], in the Deck's order
+ * (evaluator.hpp:41-49 loads each variable's value into its slot).  Call
+ * with NULL buffers to get the count in n.  Returns 0 on success. */
+int libfive_tree_deck_vars(libfive_tree t, uint32_t* slot_ids, const void** var_ids,
+                           size_t* n);
 
 /* Mesh settings beyond the reference's two render entry points
  * (BRepSettings, libfive/include/libfive/render/brep/settings.hpp:27-74). */

@@ -11,6 +11,7 @@
 #include <string>
 
 #include "../../include/libfive.h"
+#include "autotune.hpp"
 #include "eval_kernels.hpp"
 #include "contours.hpp"
 #include "heightmap.hpp"
@@ -388,6 +389,18 @@ void libfive_tape_delete(libfive_tape t) { delete t; }
 uint32_t libfive_tape_clauses(libfive_tape t) { return t->t->host.ref_length; }
 uint32_t libfive_tape_registers(libfive_tape t) { return t->t->host.num_regs; }
 
+// Evaluation calls of at least kTuneMin points tune their launch schedule
+// on their first kTuneSlice points (autotune.hpp).
+static constexpr size_t kTuneMin = size_t(1) << 20;
+static constexpr size_t kTuneSlice = size_t(1) << 22;
+static const unsigned kBlocks[3] = {256, 128, 64};
+static const std::vector<std::string> kBlockNames = {
+    "256 lanes per workgroup", "128 lanes per workgroup", "64 lanes per workgroup"};
+
+static std::string spec_key(const char* kernel, const SpecializedTape& k) {
+    return std::string(kernel) + "/tape=" + std::to_string(k.serial);
+}
+
 int libfive_tape_eval_values_device(libfive_tape t, const float* x,
                                     const float* y, const float* z, size_t n,
                                     float* out, void* stream) {
@@ -395,13 +408,33 @@ int libfive_tape_eval_values_device(libfive_tape t, const float* x,
         Context& c = context();
         hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c.stream;
         const auto& h = t->t->host;
+        // Launch schedules are tuned on the first large call (a slice of
+        // its own points; every schedule gives the same values).
+        const size_t slice = std::min<size_t>(n, kTuneSlice);
         if (t->spec && t->use_spec) {
-            hip_check(launch_specialized_values(*t->spec, x, y, z, out, n, s),
+            const SpecializedTape& k = *t->spec;
+            const int c = n < kTuneMin ? std::max(0, tune::known(spec_key("lf_values", k)))
+                : tune::pick(spec_key("lf_values", k), kBlockNames, [&](int b) {
+                      return launch_specialized_values(k, x, y, z, out, slice, s, kBlocks[b]);
+                  }, s);
+            hip_check(launch_specialized_values(k, x, y, z, out, n, s, kBlocks[c % 3]),
                       "eval_values (specialised)");
             return 0;
         }
-        hip_check(launch_eval_values(t->t->d.p, t->t->size(), h.root_reg,
-                                     h.num_regs, x, y, z, out, n, s),
+        int ppl = default_value_points(h.num_regs);
+        const std::vector<int> opts = value_points_options(h.num_regs);
+        if (!value_points_forced() && opts.size() > 1) {
+            std::vector<std::string> names;
+            for (int p : opts) names.push_back(std::to_string(p) + " points per lane");
+            const std::string key = "eval_values/banks=" + std::to_string((h.num_regs + 31) / 32);
+            const int c = n < kTuneMin ? tune::known(key) : tune::pick(key, names, [&](int k) {
+                return launch_eval_values_ppl(t->t->d.p, t->t->size(), h.root_reg, h.num_regs,
+                                              x, y, z, out, slice, s, opts[k]);
+            }, s);
+            if (c >= 0 && c < int(opts.size())) ppl = opts[c];
+        }
+        hip_check(launch_eval_values_ppl(t->t->d.p, t->t->size(), h.root_reg,
+                                         h.num_regs, x, y, z, out, n, s, ppl),
                   "eval_values");
         return 0;
     } catch (const std::exception& e) {
@@ -419,8 +452,15 @@ int libfive_tape_eval_derivs_device(libfive_tape t, const float* x,
         hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c.stream;
         const auto& h = t->t->host;
         if (t->spec && t->use_spec) {
-            hip_check(launch_specialized_derivs(*t->spec, x, y, z, value, dx, dy,
-                                                dz, n, s),
+            const SpecializedTape& k = *t->spec;
+            const size_t slice = std::min<size_t>(n, kTuneSlice);
+            const int c = n < kTuneMin ? std::max(0, tune::known(spec_key("lf_derivs", k)))
+                : tune::pick(spec_key("lf_derivs", k), kBlockNames, [&](int b) {
+                      return launch_specialized_derivs(k, x, y, z, value, dx, dy, dz, slice, s,
+                                                       kBlocks[b]);
+                  }, s);
+            hip_check(launch_specialized_derivs(k, x, y, z, value, dx, dy, dz, n, s,
+                                                kBlocks[c % 3]),
                       "eval_derivs (specialised)");
             return 0;
         }
@@ -602,6 +642,24 @@ int libfive_tree_deck(libfive_tree t, uint32_t* clauses, size_t* n_clauses,
         return 0;
     } catch (const std::exception& e) {
         set_error("libfive_tree_deck", e);
+        return -1;
+    }
+}
+
+int libfive_tree_deck_vars(libfive_tree t, uint32_t* slot_ids, const void** var_ids,
+                           size_t* n) {
+    try {
+        Deck deck{Tree(D(t))};
+        if (slot_ids && var_ids) {
+            for (size_t i = 0; i < deck.vars.size(); ++i) {
+                slot_ids[i] = deck.vars[i];
+                var_ids[i] = deck.var_nodes[i];
+            }
+        }
+        if (n) *n = deck.vars.size();
+        return 0;
+    } catch (const std::exception& e) {
+        set_error("libfive_tree_deck_vars", e);
         return -1;
     }
 }
@@ -828,6 +886,38 @@ int libfive_trees_render_mesh_merged(const libfive_tree* trees,
         set_error("libfive_trees_render_mesh_merged", e);
         return int(n);
     }
+}
+
+void libfive_autotune_enable(int on) { tune::set_enabled(on != 0); }
+
+void libfive_autotune_reset() { tune::reset(); }
+
+void libfive_autotune_force(const char* key, int choice) {
+    if (key) tune::force(key, choice);
+}
+
+size_t libfive_autotune_report(char* buf, size_t cap) {
+    std::string out = "[";
+    const auto ds = tune::decisions();
+    for (size_t i = 0; i < ds.size(); ++i) {
+        const auto& d = ds[i];
+        out += i ? ", " : "";
+        out += "{\"key\": \"" + d.key + "\", \"choice\": \"" + d.names[size_t(d.choice)] +
+               "\", \"candidates_ms\": {";
+        for (size_t c = 0; c < d.names.size(); ++c) {
+            char num[32];
+            std::snprintf(num, sizeof(num), "%.4f", d.ms[c]);
+            out += (c ? ", \"" : "\"") + d.names[c] + "\": " + num;
+        }
+        out += "}}";
+    }
+    out += "]";
+    if (buf && cap) {
+        const size_t k = std::min(cap - 1, out.size());
+        std::memcpy(buf, out.data(), k);
+        buf[k] = '\0';
+    }
+    return out.size() + 1;
 }
 
 libfive_mesh* libfive_tree_render_mesh(libfive_tree tree, libfive_region3 R,

@@ -320,14 +320,34 @@ static void launch_v(const Clause* t, uint32_t n, uint32_t root,
     k_eval_values<NV, P><<<grid, block, 0, s>>>(t, n, root, x, y, z, out, npts);
 }
 
+std::vector<int> value_points_options(uint32_t nregs) {
+    const int nv = banks_for(nregs);
+    if (nv == 1) return {1, 2, 4};
+    if (nv == 2) return {1, 2};
+    return {1};
+}
+
+int default_value_points(uint32_t nregs) { return points_for(banks_for(nregs)); }
+
+bool value_points_forced() { return g_points_per_lane > 0; }
+
 hipError_t launch_eval_values(const Clause* t, uint32_t n, uint32_t root,
                               uint32_t nregs, const float* x, const float* y,
                               const float* z, float* out, size_t npts,
                               hipStream_t s)
 {
+    return launch_eval_values_ppl(t, n, root, nregs, x, y, z, out, npts, s,
+                                  points_for(banks_for(nregs)));
+}
+
+hipError_t launch_eval_values_ppl(const Clause* t, uint32_t n, uint32_t root,
+                                  uint32_t nregs, const float* x, const float* y,
+                                  const float* z, float* out, size_t npts,
+                                  hipStream_t s, int p)
+{
     if (npts == 0) return hipSuccess;
     const int nv = banks_for(nregs);
-    const int p = points_for(nv);
+    if (!(p == 1 || (p == 2 && nv <= 2) || (p == 4 && nv == 1))) return hipErrorInvalidValue;
     if (nv == 1 && p == 1) launch_v<1, 1>(t, n, root, x, y, z, out, npts, s);
     else if (nv == 1 && p == 2) launch_v<1, 2>(t, n, root, x, y, z, out, npts, s);
     else if (nv == 1) launch_v<1, 4>(t, n, root, x, y, z, out, npts, s);

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 #include "tape.hpp"
 
@@ -15,6 +16,18 @@ hipError_t launch_eval_values(const tape::Clause* t, uint32_t n, uint32_t root,
                               uint32_t nregs, const float* x, const float* y,
                               const float* z, float* out, size_t npts,
                               hipStream_t s);
+
+// The same with p points per lane (1, 2 or 4; 4 only for tapes of one
+// register bank, 2 for at most two); value_points_options() lists the
+// counts a tape allows, default_value_points() the one launch_eval_values
+// uses (libfive_set_points_per_lane's value when set, value_points_forced).
+hipError_t launch_eval_values_ppl(const tape::Clause* t, uint32_t n, uint32_t root,
+                                  uint32_t nregs, const float* x, const float* y,
+                                  const float* z, float* out, size_t npts,
+                                  hipStream_t s, int p);
+std::vector<int> value_points_options(uint32_t nregs);
+int default_value_points(uint32_t nregs);
+bool value_points_forced();
 
 hipError_t launch_eval_derivs(const tape::Clause* t, uint32_t n, uint32_t root,
                               uint32_t nregs, const float* x, const float* y,

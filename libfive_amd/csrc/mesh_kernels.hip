@@ -558,6 +558,8 @@ static int banks(uint32_t nregs) {
     return b <= 1 ? 1 : b <= 2 ? 2 : b <= 4 ? 4 : b <= 8 ? 8 : 0;
 }
 
+int register_banks(uint32_t nregs) { return banks(nregs); }
+
 size_t interval_lds_bytes(uint32_t nregs, uint32_t max_len) {
     const uint32_t W = (nregs + 31) / 32;
     return 32u * W + 16u * size_t(nregs) + 4u * ((3u * size_t(max_len) + 3u) / 4u);
@@ -1102,9 +1104,11 @@ __device__ __forceinline__ void defer_group(const Params& P, LeafGroupRec* grec,
     }
 }
 
-// Corners (dc_tree.inl:131-239 for the 8 children at once).
-template <int NV, bool FAST>
-__global__ __launch_bounds__(64)
+// Corners (dc_tree.inl:131-239 for the 8 children at once).  WG waves per
+// workgroup, one parent per wave (a launch schedule: the results do not
+// depend on it).
+template <int NV, bool FAST, int WG>
+__global__ __launch_bounds__(64 * WG)
 void k_leaf_corners(Params P, const uint32_t* __restrict__ groups, uint32_t g0,
                     uint32_t ngroups, uint32_t nall, LevelView l1, LevelView l0,
                     LeafGroupRec* __restrict__ grec, uint32_t* __restrict__ cnt,
@@ -1115,9 +1119,11 @@ void k_leaf_corners(Params P, const uint32_t* __restrict__ groups, uint32_t g0,
     // edges, edge-search tasks and ambiguous leaves, turned into list
     // offsets by scans and k_leaf_scatter (no atomics on shared counters:
     // a million waves adding to three addresses serialise there).
-    const int lane = threadIdx.x;
-    if (blockIdx.x >= ngroups) return;
-    const uint32_t g = g0 + blockIdx.x;
+    const int lane = int(threadIdx.x & 63u);
+    const uint32_t task = blockIdx.x * uint32_t(WG) +
+                          __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (task >= ngroups) return;
+    const uint32_t g = g0 + task;
     const uint32_t p = groups[g];
     const uint32_t tape_id = __builtin_amdgcn_readfirstlane(l1.tape[p]);
     const TapeRec tr = P.tapes[tape_id];
@@ -1157,7 +1163,7 @@ void k_leaf_corners(Params P, const uint32_t* __restrict__ groups, uint32_t g0,
             r.edge_mask = 0;
             r.edge_base = 0;
             r.inside = in27;
-            r.flags = 0;
+            r.flags = grec[g].flags & 1u;    // listed already by an earlier run
             r.pad = 0;
             grec[g] = r;
             cnt[g] = 0;
@@ -1251,19 +1257,28 @@ void k_leaf_scatter(Params P, uint32_t nall, LeafGroupRec* __restrict__ grec,
 
 // Edge search, partials and per-edge QEF (dc_tree.inl:339-502 for one
 // lattice edge, shared by the children whose patches cross it).
-template <int NV, bool FAST>
-__global__ __launch_bounds__(64)
+// WG waves per workgroup, one task per wave, each with its own LDS slots
+// (a launch schedule: the results do not depend on it).
+template <int NV, bool FAST, int WG>
+__global__ __launch_bounds__(64 * WG)
 void k_leaf_edges(Params P, const uint32_t* __restrict__ groups, LevelView l1,
                   LeafGroupRec* __restrict__ grec, const uint64_t* __restrict__ tasks,
                   uint32_t ntasks, EdgeRec* __restrict__ edges,
                   uint32_t* __restrict__ deferred)
 {
-    __shared__ double s_tf[4][3], s_ts[4][3];
-    __shared__ float s_v[8], s_g[8][3];
-    __shared__ uint32_t s_fl[8];
-    const int lane = threadIdx.x;
-    if (blockIdx.x >= ntasks) return;
-    const uint64_t task = tasks[blockIdx.x];
+    __shared__ double s_tf_w[WG][4][3], s_ts_w[WG][4][3];
+    __shared__ float s_v_w[WG][8], s_g_w[WG][8][3];
+    __shared__ uint32_t s_fl_w[WG][8];
+    const int lane = int(threadIdx.x & 63u);
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t ti = blockIdx.x * uint32_t(WG) + wv;
+    if (ti >= ntasks) return;
+    double (&s_tf)[4][3] = s_tf_w[wv];
+    double (&s_ts)[4][3] = s_ts_w[wv];
+    float (&s_v)[8] = s_v_w[wv];
+    float (&s_g)[8][3] = s_g_w[wv];
+    uint32_t (&s_fl)[8] = s_fl_w[wv];
+    const uint64_t task = tasks[ti];
     const uint32_t g = uint32_t(task >> 32), j = uint32_t(task & 0xffffffffu);
     const uint64_t emask = grec[g].edge_mask;
     const uint32_t ebase = grec[g].edge_base;
@@ -1477,10 +1492,20 @@ void k_leaf_solve(Params P, const uint32_t* __restrict__ groups, LevelView l1,
 hipError_t launch_leaf_corners(const Params& p, bool fast, const uint32_t* groups,
                                uint32_t g0, uint32_t ngroups, uint32_t nall,
                                const LevelView& l1, const LevelView& l0, LeafGroupRec* grec,
-                               uint32_t* cnt, uint32_t* deferred, hipStream_t s) {
+                               uint32_t* cnt, uint32_t* deferred, hipStream_t s, int waves) {
     if (ngroups == 0) return hipSuccess;
-#define LF_CORNERS(NV, F) hipLaunchKernelGGL((k_leaf_corners<NV, F>), dim3(ngroups), dim3(64), 0, s, \
-                                             p, groups, g0, ngroups, nall, l1, l0, grec, cnt, deferred)
+    if (waves != 1 && waves != 4) return hipErrorInvalidValue;
+    const uint32_t nb = (ngroups + uint32_t(waves) - 1) / uint32_t(waves);
+#define LF_CORNERS(NV, F)                                                                      \
+    do {                                                                                       \
+        if (waves == 4) {                                                                      \
+            hipLaunchKernelGGL((k_leaf_corners<NV, F, 4>), dim3(nb), dim3(256), 0, s, p,       \
+                               groups, g0, ngroups, nall, l1, l0, grec, cnt, deferred);        \
+        } else {                                                                               \
+            hipLaunchKernelGGL((k_leaf_corners<NV, F, 1>), dim3(nb), dim3(64), 0, s, p,        \
+                               groups, g0, ngroups, nall, l1, l0, grec, cnt, deferred);        \
+        }                                                                                      \
+    } while (0)
     switch (banks(p.nregs) * (fast ? -1 : 1)) {
         case 1: LF_CORNERS(1, false); break;
         case 2: LF_CORNERS(2, false); break;
@@ -1489,7 +1514,10 @@ hipError_t launch_leaf_corners(const Params& p, bool fast, const uint32_t* group
         case -2: LF_CORNERS(2, true); break;
         case -4: LF_CORNERS(4, true); break;
         case 8:
-        case -8: LF_CORNERS(8, false); break;     // calls cost nothing next to spills
+        case -8:                                  // calls cost nothing next to spills
+            hipLaunchKernelGGL((k_leaf_corners<8, false, 1>), dim3(ngroups), dim3(64), 0, s, p,
+                               groups, g0, ngroups, nall, l1, l0, grec, cnt, deferred);
+            break;
         default: return hipErrorInvalidValue;
     }
 #undef LF_CORNERS
@@ -1508,10 +1536,20 @@ hipError_t launch_leaf_scatter(const Params& p, uint32_t nall, LeafGroupRec* gre
 hipError_t launch_leaf_edges(const Params& p, bool fast, const uint32_t* groups,
                              const LevelView& l1, LeafGroupRec* grec,
                              const uint64_t* tasks, uint32_t ntasks,
-                             EdgeRec* edges, uint32_t* deferred, hipStream_t s) {
+                             EdgeRec* edges, uint32_t* deferred, hipStream_t s, int waves) {
     if (ntasks == 0) return hipSuccess;
-#define LF_EDGES(NV, F) hipLaunchKernelGGL((k_leaf_edges<NV, F>), dim3(ntasks), dim3(64), 0, s, \
-                                           p, groups, l1, grec, tasks, ntasks, edges, deferred)
+    if (waves != 1 && waves != 4) return hipErrorInvalidValue;
+    const uint32_t nb = (ntasks + uint32_t(waves) - 1) / uint32_t(waves);
+#define LF_EDGES(NV, F)                                                                        \
+    do {                                                                                       \
+        if (waves == 4) {                                                                      \
+            hipLaunchKernelGGL((k_leaf_edges<NV, F, 4>), dim3(nb), dim3(256), 0, s, p, groups, \
+                               l1, grec, tasks, ntasks, edges, deferred);                      \
+        } else {                                                                               \
+            hipLaunchKernelGGL((k_leaf_edges<NV, F, 1>), dim3(nb), dim3(64), 0, s, p, groups,  \
+                               l1, grec, tasks, ntasks, edges, deferred);                      \
+        }                                                                                      \
+    } while (0)
     switch (banks(p.nregs) * (fast ? -1 : 1)) {
         case 1: LF_EDGES(1, false); break;
         case 2: LF_EDGES(2, false); break;
@@ -1520,7 +1558,10 @@ hipError_t launch_leaf_edges(const Params& p, bool fast, const uint32_t* groups,
         case -2: LF_EDGES(2, true); break;
         case -4: LF_EDGES(4, true); break;
         case 8:
-        case -8: LF_EDGES(8, false); break;
+        case -8:
+            hipLaunchKernelGGL((k_leaf_edges<8, false, 1>), dim3(ntasks), dim3(64), 0, s, p, groups,
+                               l1, grec, tasks, ntasks, edges, deferred);
+            break;
         default: return hipErrorInvalidValue;
     }
 #undef LF_EDGES

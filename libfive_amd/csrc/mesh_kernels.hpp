@@ -239,17 +239,24 @@ constexpr uint32_t kLeafTasksPerGroup = 14;   // ceil(54 / 4)
 // after their exclusive scans (base, same layout) launch_leaf_scatter fills
 // the edge bases, the task list and the leaf list and sets
 // Counters::edges, tasks and leaves.
+// waves: waves per workgroup (1 or 4; one parent or task per wave), a
+// launch schedule picked by the autotuner; tapes of 8 register banks use 1.
 hipError_t launch_leaf_corners(const Params& p, bool fast, const uint32_t* groups,
                                uint32_t g0, uint32_t ngroups, uint32_t nall,
                                const LevelView& l1, const LevelView& l0, LeafGroupRec* grec,
-                               uint32_t* cnt, uint32_t* deferred, hipStream_t s);
+                               uint32_t* cnt, uint32_t* deferred, hipStream_t s,
+                               int waves = 1);
 hipError_t launch_leaf_scatter(const Params& p, uint32_t nall, LeafGroupRec* grec,
                                const uint32_t* cnt, const uint32_t* base, uint64_t* tasks,
                                uint32_t* leaves, hipStream_t s);
 hipError_t launch_leaf_edges(const Params& p, bool fast, const uint32_t* groups,
                              const LevelView& l1, LeafGroupRec* grec,
                              const uint64_t* tasks, uint32_t ntasks,
-                             EdgeRec* edges, uint32_t* deferred, hipStream_t s);
+                             EdgeRec* edges, uint32_t* deferred, hipStream_t s,
+                             int waves = 1);
+// Register banks (1, 2, 4 or 8) of the kernels' register files for a tape
+// of nregs registers; 0 if the tape has more than 256.
+int register_banks(uint32_t nregs);
 hipError_t launch_leaf_solve(const Params& p, const uint32_t* groups,
                              const LevelView& l1, const LevelView& l0,
                              const LeafGroupRec* grec, const uint32_t* leaves,

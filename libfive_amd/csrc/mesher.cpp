@@ -23,6 +23,7 @@
 #include <mutex>
 #include <thread>
 
+#include "autotune.hpp"
 #include "deck.hpp"
 #include "mesh_kernels.hpp"
 #include "runtime.hpp"
@@ -174,10 +175,23 @@ void run_leaves(const Params& P, const uint32_t* groups, uint32_t ngroups,
         // chunk's run time.
         const bool poll = settings.cancel || settings.cancel_flag;
         const uint32_t chunk = poll ? kLeafChunk : ngroups;
+        // The corner kernel keeps a group's deferred bit when it runs again
+        // (the autotuner reruns a slice), so the bits start cleared.
+        hip_check(hipMemsetAsync(grec.p, 0, size_t(ngroups) * sizeof(LeafGroupRec), s), "grec");
+        const int nv = register_banks(P.nregs);
+        const std::string tag = "/banks=" + std::to_string(nv) + "/fast=" + (fast ? "1" : "0");
+        const std::vector<std::string> wave_names = {"1 wave per workgroup",
+                                                     "4 waves per workgroup"};
+        auto waves_of = [](int c) { return c == 1 ? 4 : 1; };
+        const int cw = nv == 8 ? 0 : tune::pick(
+            "leaf_corners" + tag, wave_names, [&](int c) {
+                return launch_leaf_corners(P, fast, groups, 0, std::min(ngroups, 16384u), ngroups,
+                                           l1, l0, grec.p, cnt.p, deferred.p, s, waves_of(c));
+            }, s);
         for (uint32_t g0 = 0; g0 < ngroups; g0 += chunk) {
             const uint32_t ng = std::min(chunk, ngroups - g0);
             hip_check(launch_leaf_corners(P, fast, groups, g0, ng, ngroups, l1, l0, grec.p,
-                                          cnt.p, deferred.p, s),
+                                          cnt.p, deferred.p, s, waves_of(cw)),
                       "leaf corner kernel");
             if (poll) {
                 wait(s, "leaf corner kernel");
@@ -201,10 +215,16 @@ void run_leaves(const Params& P, const uint32_t* groups, uint32_t ngroups,
         }
         PoolBuf<EdgeRec> edges(std::max<uint32_t>(hc.edges, 1u));
         const uint32_t tchunk = poll ? 4 * kLeafChunk : std::max<uint32_t>(hc.tasks, 1u);
+        const int ew = (nv == 8 || hc.tasks == 0) ? 0 : tune::pick(
+            "leaf_edges" + tag, wave_names, [&](int c) {
+                return launch_leaf_edges(P, fast, groups, l1, grec.p, tasks.p,
+                                         std::min(hc.tasks, 65536u), edges.p, deferred.p, s,
+                                         waves_of(c));
+            }, s);
         for (uint32_t t0 = 0; t0 < hc.tasks; t0 += tchunk) {
             const uint32_t nt = std::min(tchunk, hc.tasks - t0);
             hip_check(launch_leaf_edges(P, fast, groups, l1, grec.p, tasks.p + t0, nt, edges.p,
-                                        deferred.p, s), "leaf edge kernel");
+                                        deferred.p, s, waves_of(ew)), "leaf edge kernel");
             if (poll) {
                 wait(s, "leaf edge kernel");
                 if (settings.cancelled()) return;

@@ -1,6 +1,8 @@
 // Tape specialisation with hipRTC (see tape_jit.hpp).
 #include "tape_jit.hpp"
 
+#include <atomic>
+
 #include <hip/hiprtc.h>
 
 #include <chrono>
@@ -171,7 +173,7 @@ std::string specialized_source(const DeviceTape& t) {
         "void lf_values(const float* __restrict__ xs, const float* __restrict__ ys,\n"
         "               const float* __restrict__ zs, float* __restrict__ out,\n"
         "               unsigned long long n) {\n"
-        "    const unsigned long long i = (unsigned long long)blockIdx.x * 256ull + threadIdx.x;\n"
+        "    const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
         "    if (i >= n) return;\n"
         "    const float px = xs[i], py = ys[i], pz = zs[i];\n";
     value_body(t, s);
@@ -181,7 +183,7 @@ std::string specialized_source(const DeviceTape& t) {
          "               const float* __restrict__ zs, float* __restrict__ ov,\n"
          "               float* __restrict__ odx, float* __restrict__ ody,\n"
          "               float* __restrict__ odz, unsigned long long n) {\n"
-         "    const unsigned long long i = (unsigned long long)blockIdx.x * 256ull + threadIdx.x;\n"
+         "    const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
          "    if (i >= n) return;\n"
          "    const float px = xs[i], py = ys[i], pz = zs[i];\n";
     deriv_body(t, s);
@@ -239,31 +241,39 @@ std::unique_ptr<SpecializedTape> specialize(const DeviceTape& t) {
               "hipModuleGetFunction(lf_values)");
     hip_check(hipModuleGetFunction(&k->derivs, k->module, "lf_derivs"),
               "hipModuleGetFunction(lf_derivs)");
+    static std::atomic<uint64_t> serial{0};
+    k->serial = ++serial;
     k->code_bytes = code.size();
     k->compile_ms = std::chrono::duration<double, std::milli>(
                         std::chrono::steady_clock::now() - t0).count();
     return k;
 }
 
+static bool valid_block(unsigned block) {
+    return block == 64 || block == 128 || block == 256;
+}
+
 hipError_t launch_specialized_values(const SpecializedTape& k, const float* x,
                                      const float* y, const float* z,
-                                     float* out, size_t n, hipStream_t s) {
+                                     float* out, size_t n, hipStream_t s, unsigned block) {
     if (n == 0) return hipSuccess;
+    if (!valid_block(block)) return hipErrorInvalidValue;
     unsigned long long nn = n;
     void* args[] = {&x, &y, &z, &out, &nn};
-    const unsigned grid = unsigned((n + 255) / 256);
-    return hipModuleLaunchKernel(k.values, grid, 1, 1, 256, 1, 1, 0, s, args, nullptr);
+    const unsigned grid = unsigned((n + block - 1) / block);
+    return hipModuleLaunchKernel(k.values, grid, 1, 1, block, 1, 1, 0, s, args, nullptr);
 }
 
 hipError_t launch_specialized_derivs(const SpecializedTape& k, const float* x,
                                      const float* y, const float* z, float* v,
                                      float* dx, float* dy, float* dz, size_t n,
-                                     hipStream_t s) {
+                                     hipStream_t s, unsigned block) {
     if (n == 0) return hipSuccess;
+    if (!valid_block(block)) return hipErrorInvalidValue;
     unsigned long long nn = n;
     void* args[] = {&x, &y, &z, &v, &dx, &dy, &dz, &nn};
-    const unsigned grid = unsigned((n + 255) / 256);
-    return hipModuleLaunchKernel(k.derivs, grid, 1, 1, 256, 1, 1, 0, s, args, nullptr);
+    const unsigned grid = unsigned((n + block - 1) / block);
+    return hipModuleLaunchKernel(k.derivs, grid, 1, 1, block, 1, 1, 0, s, args, nullptr);
 }
 
 }   // namespace amd

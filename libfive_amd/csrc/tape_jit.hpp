@@ -32,6 +32,7 @@ struct SpecializedTape {
     hipFunction_t derivs = nullptr;    // (x, y, z, v, dx, dy, dz, n)
     double compile_ms = 0;
     size_t code_bytes = 0;
+    uint64_t serial = 0;               // distinct for every specialised tape
     ~SpecializedTape();
 };
 
@@ -44,13 +45,16 @@ std::vector<char> compile_specialized(const DeviceTape& t);
 // Compiles and loads the kernels on the current device; throws Error.
 std::unique_ptr<SpecializedTape> specialize(const DeviceTape& t);
 
+// block: workgroup size (64, 128 or 256; one point per lane), a launch
+// schedule the autotuner picks per tape.
 hipError_t launch_specialized_values(const SpecializedTape& k, const float* x,
                                      const float* y, const float* z,
-                                     float* out, size_t n, hipStream_t s);
+                                     float* out, size_t n, hipStream_t s,
+                                     unsigned block = 256);
 hipError_t launch_specialized_derivs(const SpecializedTape& k, const float* x,
                                      const float* y, const float* z, float* v,
                                      float* dx, float* dy, float* dz, size_t n,
-                                     hipStream_t s);
+                                     hipStream_t s, unsigned block = 256);
 
 }   // namespace amd
 }   // namespace libfive

@@ -209,6 +209,13 @@ SIGNATURES = {
     "libfive_tree_specialized_code_size": (ctypes.c_size_t, [tree_t]),
     "libfive_set_points_per_lane": (None, [ctypes.c_int]),
     "libfive_tree_tape_info": (ctypes.c_int, [tree_t, u32p, u32p]),
+    "libfive_autotune_enable": (None, [ctypes.c_int]),
+    "libfive_autotune_reset": (None, []),
+    "libfive_autotune_force": (None, [ctypes.c_char_p, ctypes.c_int]),
+    "libfive_autotune_report": (ctypes.c_size_t, [ctypes.c_char_p, ctypes.c_size_t]),
+    "libfive_tree_deck_vars": (ctypes.c_int, [tree_t, ctypes.POINTER(ctypes.c_uint32),
+                                              ctypes.POINTER(ctypes.c_void_p),
+                                              ctypes.POINTER(ctypes.c_size_t)]),
     "libfive_tree_deck": (ctypes.c_int, [tree_t, u32p, szp, u32p, fp, szp,
                                          u32p, u32p]),
     "libfive_mesh_vertex_keys": (u32p, [ctypes.POINTER(libfive_mesh)]),

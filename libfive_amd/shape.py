@@ -117,9 +117,13 @@ class Shape:
         x, y, z = (Shape.wrap(v) for v in (x, y, z))
         return Shape(ffi.lib().libfive_tree_remap(self.ptr, x.ptr, y.ptr, z.ptr))
 
-    def deck(self):
+    def deck(self, vars=None):
         """Reference-format clause list: (clauses[n,4] root first,
-        constants {id: value}, (X, Y, Z) ids, slot count)."""
+        constants {id: value}, (X, Y, Z) ids, slot count).  vars maps free
+        variables (Shape.var() shapes) to values: their slots are then
+        listed with the constants, holding those values (the reference's
+        evaluators load a variable's value into its slot; a variable not
+        given is 0, as in the GPU compile)."""
         L = ffi.lib()
         n = ctypes.c_size_t()
         nc = ctypes.c_size_t()
@@ -136,6 +140,17 @@ class Shape:
                             _f32p(cv), ctypes.byref(nc),
                             xyz.ctypes.data_as(ffi.u32p), ctypes.byref(ns))
         consts = {int(i): float(v) for i, v in zip(ci[:nc.value], cv[:nc.value])}
+        nv = ctypes.c_size_t()
+        if L.libfive_tree_deck_vars(self.ptr, None, None, ctypes.byref(nv)):
+            raise GpuError(ffi.last_error())
+        if nv.value:
+            slots = (ctypes.c_uint32 * nv.value)()
+            ids = (ctypes.c_void_p * nv.value)()
+            L.libfive_tree_deck_vars(self.ptr, slots, ids, ctypes.byref(nv))
+            given = {L.libfive_tree_id(v.ptr): float(np.float32(x))
+                     for v, x in (vars or {}).items()}
+            for k in range(nv.value):
+                consts[int(slots[k])] = given.get(ids[k], 0.0)
         return cl[:n.value].copy(), consts, tuple(int(v) for v in xyz), ns.value
 
     def tape_info(self):
@@ -343,6 +358,17 @@ def _mesh_arrays(m):
     else:
         tris = np.zeros((0, 3), np.uint32)
     return verts, tris
+
+
+def autotune_report():
+    """The launch-schedule decisions made so far in this process
+    (libfive_autotune_report): a list of {key, choice, candidates_ms}."""
+    import json
+    L = ffi.lib()
+    n = L.libfive_autotune_report(None, 0)
+    buf = ctypes.create_string_buffer(n)
+    L.libfive_autotune_report(buf, n)
+    return json.loads(buf.value.decode())
 
 
 def render_mesh_batch(shapes, regions, res, streams=8, keep=True, merged=False):

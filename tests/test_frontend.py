@@ -266,3 +266,18 @@ def test_evaluator_update_vars():
         assert not L.libfive_evaluator_update_vars(ev, _vars([(wid, 5.0)]))
     finally:
         L.libfive_evaluator_delete(ev)
+
+
+def test_autotune_report_without_decisions():
+    """The autotuner's report is valid JSON before anything has been tuned
+    (no GPU needed), and enabling, disabling and resetting are plain
+    calls."""
+    from libfive_amd import autotune_report, ffi
+    L = ffi.lib()
+    L.libfive_autotune_reset()
+    assert autotune_report() == []
+    L.libfive_autotune_enable(0)
+    L.libfive_autotune_enable(1)
+    L.libfive_autotune_force(b"leaf_edges/banks=1/fast=1", 1)
+    assert autotune_report() == []          # forcing records no timing
+    L.libfive_autotune_reset()

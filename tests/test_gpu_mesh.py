@@ -444,6 +444,47 @@ def test_evaluator_save_mesh_uses_variables(gpu, tmp_path):
         L.libfive_evaluator_delete(ev)
 
 
+def test_variables_match_oracle(gpu, tmp_path):
+    """Free variables (libfive.cpp:408-496) against the oracle with each
+    variable's value loaded into its slot, as the reference's evaluators do
+    (evaluator.hpp:41-49): the mesh libfive_evaluator_save_mesh writes has
+    the oracle mesh's triangles, vertex coordinates bit for bit, at two sets
+    of values (after libfive_evaluator_update_vars)."""
+    import ctypes
+    from libfive_amd import ffi
+    L = ffi.lib()
+    r, cx = Shape.var(), Shape.var()
+    sph = ((X() - cx).square() + Y().square() + Z().square()).sqrt() - r
+    t = Shape.binary("max", sph, Z() - 0.3)          # a flat cut: features
+    ids = [L.libfive_tree_id(v.ptr) for v in (r, cx)]
+
+    def vars_(vals):
+        return ffi.libfive_vars((ctypes.c_void_p * 2)(*ids), (ctypes.c_float * 2)(*vals), 2)
+
+    lo3 = [float(np.float32(-1.0))] * 3
+    hi3 = [float(np.float32(1.0))] * 3
+    ev = L.libfive_tree_evaluator(t.ptr, vars_((0.6, 0.05)))
+    try:
+        for k, vals in enumerate(((0.6, 0.05), (0.45, -0.2))):
+            if k:
+                assert L.libfive_evaluator_update_vars(ev, vars_(vals))
+            path = str(tmp_path / f"v{k}.stl")
+            assert L.libfive_evaluator_save_mesh(ev, _region(-1, 1), path.encode()), \
+                ffi.last_error()
+            _, n, rec = _read_stl(path)
+            got = sorted(tuple(tri.view(np.uint32).ravel().tolist()) for tri in rec["v"])
+            o = oracle.OracleDeck(t.deck(vars={r: vals[0], cx: vals[1]})).render_mesh(
+                lo3, hi3, 0.1, 1e-8)
+            ov = np.asarray(o.verts, np.float32)
+            want = sorted(tuple(ov[list(tr)].view(np.uint32).ravel().tolist())
+                          for tr in o.tris.tolist())
+            print("variables", vals, "triangles", n, len(want), flush=True)
+            assert n == len(want) and n > 0
+            assert got == want
+    finally:
+        L.libfive_evaluator_delete(ev)
+
+
 def test_cpp_mesh_api(gpu):
     """The C++ entry (libfive::Mesh::render, both overloads, saveSTL) and
     the reference's cancellation case, in the test program built with the

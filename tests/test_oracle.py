@@ -410,3 +410,21 @@ def test_f64_twin_close_to_f32():
     v32 = d.values(*p)
     v64 = d.values_f64(*p)
     assert np.abs(v32 - v64).max() < 1e-5
+
+
+def test_free_variables_load_into_slots():
+    """Shape.deck(vars=...) lists each free variable's slot with its value,
+    as the reference's evaluators load variable values into their slots
+    (evaluator.hpp:41-49); a variable not given is 0.  The oracle's values
+    then equal the closed form of the sphere at those values."""
+    from libfive_amd import Shape, X, Y, Z
+    r, cx = Shape.var(), Shape.var()
+    t = ((X() - cx).square() + Y().square() + Z().square()).sqrt() - r
+    rng = np.random.default_rng(5)
+    p = rng.uniform(-1, 1, (3, 500)).astype(np.float32)
+    got = oracle.OracleDeck(t.deck(vars={r: 0.6, cx: 0.25})).values(*p)
+    want = np.sqrt((p[0] - 0.25) ** 2 + p[1] ** 2 + p[2] ** 2) - 0.6
+    assert np.abs(got - want).max() < 1e-6
+    got0 = oracle.OracleDeck(t.deck(vars={r: 0.6})).values(*p)       # cx = 0
+    want0 = np.sqrt(p[0] ** 2 + p[1] ** 2 + p[2] ** 2) - 0.6
+    assert np.abs(got0 - want0).max() < 1e-6

@@ -27,6 +27,9 @@
 #include "feature_eval.hpp"
 #include "qef.hpp"
 #include "regfile.hpp"
+#include "autotune.hpp"
+
+#include <algorithm>
 
 namespace libfive {
 namespace amd {
@@ -565,9 +568,38 @@ size_t interval_lds_bytes(uint32_t nregs, uint32_t max_len) {
     return 32u * W + 16u * size_t(nregs) + 4u * ((3u * size_t(max_len) + 3u) / 4u);
 }
 
-size_t interval_workspace_bytes(uint32_t nregs, uint32_t max_len) {
+__host__ __device__ size_t interval_wide_stride(uint32_t max_len);
+
+// The wide kernel serves levels of at most kWideMaxGroups parents whose
+// longest tape has at least kWideMinLen clauses (and at most kWideMaxLen,
+// the done bits' LDS): there the one-wave walks are few and long, and the
+// level's time is the longest walk.  With more parents the walks fill the
+// device anyway.  The "interval_wide" decision of the autotuner overrides
+// the rule for tests (0: never, 1: for every level up to kWideMaxLen).
+constexpr uint32_t kWideMinLen = 1024;
+constexpr uint32_t kWideMaxLen = 1u << 16;
+constexpr uint32_t kWideMaxGroups = 1024;
+constexpr uint32_t kWideChunk = 256;       // workgroups per launch (workspace slots)
+
+bool interval_uses_wide(uint32_t max_len, uint32_t ngroups) {
+    if (max_len > kWideMaxLen) return false;
+    const int forced = tune::known("interval_wide");
+    if (forced == 0) return false;
+    if (forced == 1) return true;
+    return max_len >= kWideMinLen && ngroups <= kWideMaxGroups;
+}
+
+size_t interval_boxes_workspace_bytes(uint32_t nregs, uint32_t max_len) {
     if (interval_lds_bytes(nregs, max_len) <= kIntervalLdsMax) return 0;
     return size_t(kIntervalWaves) * interval_ws_stride(max_len);
+}
+
+size_t interval_workspace_bytes(uint32_t nregs, uint32_t max_len, uint32_t ngroups) {
+    size_t b = interval_boxes_workspace_bytes(nregs, max_len);
+    if (interval_uses_wide(max_len, ngroups)) {
+        b = std::max(b, size_t(std::min(ngroups, kWideChunk)) * interval_wide_stride(max_len));
+    }
+    return b;
 }
 
 namespace {
@@ -589,7 +621,7 @@ hipError_t interval_launch(uint32_t nregs, uint32_t max_len, uint32_t ngroups,
     l.lds = l.global ? interval_lds_bytes(nregs, 0) : full;
     l.chunk = ngroups;
     if (l.global) {
-        if (ws == nullptr || ws_bytes < interval_workspace_bytes(nregs, max_len)) {
+        if (ws == nullptr || ws_bytes < interval_boxes_workspace_bytes(nregs, max_len)) {
             return hipErrorInvalidValue;
         }
         l.chunk = kIntervalWaves;
@@ -598,6 +630,618 @@ hipError_t interval_launch(uint32_t nregs, uint32_t max_len, uint32_t ngroups,
 }
 
 }   // namespace
+
+////////////////////////////////////////////////////////////////////////////////
+// Interval classification and pruning of long tapes
+//
+// k_interval walks a parent's tape with one wave: 8 lanes (the children)
+// step through the clauses one at a time, so a tape of n clauses costs n
+// dependent steps however few parents a level has.  At the top levels of a
+// long tape (a few parents, thousands of clauses) that serial walk is the
+// whole cost of the level.  k_interval_wide gives each parent a workgroup
+// of kWideWaves waves and works on the tape in dependency order:
+//   1. sources: the clause that defines each operand (the SSA form of the
+//      register-allocated tape); each wave renames one chunk of the tape
+//      and the chunks are joined through their last definitions;
+//   2. values: each clause is evaluated for the 8 children (8 lanes per
+//      clause, 64 clauses in flight) as soon as its sources are done, and
+//      the keep decisions of MIN/MAX are recorded per child;
+//   3. liveness: from the root towards the leaves, each clause passes its
+//      children's needs to its sources once all of its users are done;
+//   4. emission: each pushing child's tape is written by its own wave with
+//      interval_push's walk (live clauses in order, operands renamed
+//      through decided clauses, aliases copied before their register is
+//      overwritten).
+// Each value is computed by the same operations on the same operands as in
+// interval_push, and the emission walk is the same, so the bounds, the
+// decisions and the pushed tapes are the same.  Every wait is bounded: a
+// loop that makes no progress for too long sets ERR_WIDE and ends.
+
+constexpr int kWideWaves = 8;
+constexpr uint32_t kWideThreads = 64u * kWideWaves;
+constexpr int32_t kSrcX = -1, kSrcY = -2, kSrcZ = -3;   // registers 0, 1, 2
+constexpr int32_t kSrcImm = -4;                         // the clause's immediate
+constexpr int32_t kSrcNone = -5;                        // no operand
+constexpr int32_t kSrcUnres = -16;                      // kSrcUnres - r: register r, not yet found
+constexpr uint32_t kWideIdleCap = 1u << 22;             // idle iterations before ERR_WIDE
+
+__host__ __device__ inline size_t wide_align(size_t b) { return (b + 255) & ~size_t(255); }
+
+__host__ __device__ size_t interval_wide_stride(uint32_t max_len) {
+    const size_t n = max_len;
+    return wide_align(64 * n) + wide_align(8 * n) + 3 * wide_align(4 * n) +
+           wide_align(32 * n) + wide_align(2 * n) + wide_align(n);
+}
+
+struct WideWork {
+    float2* val;        // [n][8]: interval of each clause's value per child
+    int32_t* src;       // [n][2]: source of operands a and b
+    uint32_t* rslot;    // [n]: reference slot of the value (the keep rule's a == b)
+    uint32_t* live;     // [n]: children for which the clause is live
+    uint32_t* pend;     // [n]: users not yet through the liveness pass
+    uint32_t* list;     // [8][n]: live clauses of each child, in order
+    uint16_t* choice;   // [n]: keep-A children | keep-B children << 8
+    uint8_t* vnan;      // [n]: children whose value may be NaN
+};
+
+__device__ inline WideWork wide_work(uint8_t* base, uint32_t max_len) {
+    const size_t n = max_len;
+    WideWork w;
+    size_t o = 0;
+    w.val = reinterpret_cast<float2*>(base + o); o += wide_align(64 * n);
+    w.src = reinterpret_cast<int32_t*>(base + o); o += wide_align(8 * n);
+    w.rslot = reinterpret_cast<uint32_t*>(base + o); o += wide_align(4 * n);
+    w.live = reinterpret_cast<uint32_t*>(base + o); o += wide_align(4 * n);
+    w.pend = reinterpret_cast<uint32_t*>(base + o); o += wide_align(4 * n);
+    w.list = reinterpret_cast<uint32_t*>(base + o); o += wide_align(32 * n);
+    w.choice = reinterpret_cast<uint16_t*>(base + o); o += wide_align(2 * n);
+    w.vnan = base + o;
+    return w;
+}
+
+// Dynamic LDS of k_interval_wide: done bits of the clauses, the waves'
+// last-definition tables, and per child the emission walk's renaming and
+// alias counts.
+size_t interval_wide_lds(uint32_t nregs, uint32_t max_len) {
+    return 4u * ((size_t(max_len) + 31u) / 32u) + 4u * kWideWaves * size_t(nregs) +
+           2u * 8u * size_t(nregs);
+}
+
+// One register-file table distributed over the 64 lanes of a wave: entry r
+// lives in lane r % 64, slot r / 64 (up to 256 entries).
+struct LaneTable {
+    int32_t v[4];
+};
+__device__ __forceinline__ int32_t lt_get(const LaneTable& t, uint32_t r) {
+    const uint32_t l = r & 63u;
+    switch (r >> 6) {
+        case 0: return __builtin_amdgcn_readlane(t.v[0], l);
+        case 1: return __builtin_amdgcn_readlane(t.v[1], l);
+        case 2: return __builtin_amdgcn_readlane(t.v[2], l);
+        default: return __builtin_amdgcn_readlane(t.v[3], l);
+    }
+}
+__device__ __forceinline__ void lt_set(LaneTable& t, uint32_t r, int32_t x, uint32_t lane) {
+    if (lane == (r & 63u)) {
+        switch (r >> 6) {
+            case 0: t.v[0] = x; break;
+            case 1: t.v[1] = x; break;
+            case 2: t.v[2] = x; break;
+            default: t.v[3] = x; break;
+        }
+    }
+}
+
+__device__ __forceinline__ bool wide_done(const uint32_t* done, int32_t s) {
+    if (s < 0) return true;
+    const uint32_t w = __hip_atomic_load(&done[uint32_t(s) >> 5], __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP);
+    return (w >> (uint32_t(s) & 31u)) & 1u;
+}
+
+__global__ __launch_bounds__(512)
+void k_interval_wide(Params P, const Clause* __restrict__ pool_in,
+                     const TapeRec* __restrict__ recs_in, int level,
+                     const uint32_t* __restrict__ groups, uint32_t g0, LevelView parent,
+                     LevelView cur, uint32_t max_len, uint8_t* ws, uint32_t first_root)
+{
+    HIP_DYNAMIC_SHARED(uint32_t, lds)
+    __shared__ float s_lo[8][3], s_hi[8][3];
+    __shared__ uint32_t s_key[8][3];
+    __shared__ uint32_t s_cidx[8], s_base[8], s_child_tape[8];
+    __shared__ uint8_t s_state[8];
+    __shared__ uint32_t s_anyab, s_dopush, s_changed, s_nonterm, s_err;
+    __shared__ int32_t s_root;
+    __shared__ uint32_t s_stage[kWideWaves][64][5];
+
+    const uint32_t t = threadIdx.x, lane = t & 63u;
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(t >> 6);
+    const uint32_t g = g0 + blockIdx.x;
+    const bool at_root = (groups == nullptr);
+    const WideWork W = wide_work(ws + size_t(blockIdx.x) * interval_wide_stride(max_len), max_len);
+    const uint32_t nregs = P.nregs;
+    uint32_t* done = lds;
+    const uint32_t nwords = (max_len + 31u) / 32u;
+    int32_t* defs = reinterpret_cast<int32_t*>(lds + nwords);        // [kWideWaves][nregs]
+    uint8_t* rc_base = reinterpret_cast<uint8_t*>(defs + kWideWaves * nregs);
+
+    uint32_t tape_id = first_root + g, first = 0, pkey[3] = {0, 0, 0};
+    if (!at_root) {
+        const uint32_t p = groups[g];
+        tape_id = parent.tape[p];
+        first = uint32_t(parent.child[p]);
+        for (int a = 0; a < 3; ++a) pkey[a] = parent.key[3 * p + a];
+    }
+    tape_id = __builtin_amdgcn_readfirstlane(tape_id);
+    const TapeRec tr = recs_in[tape_id];
+    const Params Q = model_view(P, __builtin_amdgcn_readfirstlane(tr.model));
+    if (at_root) first = Q.root_cell;
+    const uint32_t off = __builtin_amdgcn_readfirstlane(tr.off);
+    const uint32_t n = __builtin_amdgcn_readfirstlane(tr.len);
+    const uint32_t tflags = __builtin_amdgcn_readfirstlane(tr.flags);
+    const uint32_t nchild = at_root ? 1u : 8u;
+    const uint32_t root = Q.root_reg;
+
+    if (t < 8) {
+        const uint32_t octant = t < nchild ? t : 0u;
+        uint32_t key[3];
+        for (int a = 0; a < 3; ++a) {
+            key[a] = pkey[a] + (at_root ? 0u : (((octant >> a) & 1u) << level));
+            s_key[t][a] = key[a];
+        }
+        double lo[3], hi[3];
+        cell_bounds(Q, key, level, lo, hi);
+        for (int a = 0; a < 3; ++a) {
+            s_lo[t][a] = float(lo[a]);
+            s_hi[t][a] = float(hi[a]);
+        }
+        s_cidx[t] = first + octant;
+        s_child_tape[t] = tape_id;
+        s_base[t] = 0;
+    }
+    if (t == 0) {
+        s_anyab = 0;
+        s_dopush = 0;
+        s_changed = 0;
+        s_nonterm = 0;
+        s_err = 0;
+        s_root = kSrcNone;
+    }
+    for (uint32_t i = t; i < nwords; i += kWideThreads) done[i] = 0u;
+    if (n > max_len) {
+        // the host sized the workspace by max_len; a longer tape is an error
+        if (t == 0) atomicOr(&P.counters->error, ERR_WIDE);
+    }
+    const uint32_t nn = n > max_len ? 0u : n;
+    __syncthreads();
+
+    // ---- 1. sources ---------------------------------------------------------
+    const uint32_t csz = (nn + kWideWaves - 1) / kWideWaves;
+    const uint32_t c0 = wv * csz, c1 = min(nn, c0 + csz);
+    {
+        LaneTable def;
+        for (int q = 0; q < 4; ++q) def.v[q] = -1;
+        for (uint32_t kb = c0; kb < c1; kb += 64) {
+            const uint32_t k = kb + lane;
+            uint32_t w0 = 0, w1 = 0;
+            if (k < c1) {
+                const Clause cl = pool_in[off + k];
+                w0 = cl.w0;
+                w1 = cl.w1;
+            }
+            int32_t my_a = kSrcNone, my_b = kSrcNone;
+            const uint32_t cnt = min(64u, c1 - kb);
+            for (uint32_t j = 0; j < cnt; ++j) {
+                Clause cl;
+                cl.w0 = __builtin_amdgcn_readlane(w0, j);
+                cl.w1 = __builtin_amdgcn_readlane(w1, j);
+                const uint32_t o = tape::op(cl), m = mode(cl);
+                auto reg_src = [&](uint32_t r) -> int32_t {
+                    if (r < kFirstFreeReg) return -int32_t(r) - 1;
+                    const int32_t d = lt_get(def, r);
+                    return d >= 0 ? d : kSrcUnres - int32_t(r);
+                };
+                int32_t sa = kSrcNone, sb = kSrcNone;
+                if (m == MODE_A_IMM || o == OP_LOADI) sa = kSrcImm;
+                else if (nargs(o) >= 1) sa = reg_src(ra(cl));
+                if (is_binary(cl)) sb = (m == MODE_B_IMM) ? kSrcImm : reg_src(rb(cl));
+                if (lane == j) {
+                    my_a = sa;
+                    my_b = sb;
+                }
+                lt_set(def, dst(cl), int32_t(kb + j), lane);
+            }
+            if (k < c1) {
+                W.src[2 * k] = my_a;
+                W.src[2 * k + 1] = my_b;
+            }
+        }
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t r = lane + 64u * uint32_t(q);
+            if (r < nregs) defs[wv * nregs + r] = def.v[q];
+        }
+    }
+    __syncthreads();
+    // operands defined in an earlier chunk; the root's source
+    auto resolve = [&](int32_t s, uint32_t w) -> int32_t {
+        if (s > kSrcUnres) return s;
+        const uint32_t r = uint32_t(kSrcUnres - s);
+        for (uint32_t w2 = w; w2-- > 0;) {
+            const int32_t d = defs[w2 * nregs + r];
+            if (d >= 0) return d;
+        }
+        atomicOr(&s_err, 1u);               // a register read before any write
+        return kSrcNone;
+    };
+    for (uint32_t k = t; k < nn; k += kWideThreads) {
+        const uint32_t w = k / csz;
+        const int32_t a = W.src[2 * k], b = W.src[2 * k + 1];
+        if (a <= kSrcUnres) W.src[2 * k] = resolve(a, w);
+        if (b <= kSrcUnres) W.src[2 * k + 1] = resolve(b, w);
+        W.pend[k] = 0u;
+        W.live[k] = 0u;
+    }
+    if (t == 0) {
+        s_root = root < kFirstFreeReg ? -int32_t(root) - 1 : resolve(kSrcUnres - int32_t(root),
+                                                                      kWideWaves);
+    }
+    __syncthreads();
+    // users of each clause (the operands the liveness pass will follow)
+    for (uint32_t k = t; k < nn; k += kWideThreads) {
+        const int32_t a = W.src[2 * k], b = W.src[2 * k + 1];
+        if (a >= 0) atomicAdd(&W.pend[a], 1u);
+        if (b >= 0) atomicAdd(&W.pend[b], 1u);
+    }
+    __syncthreads();
+
+    // ---- 2. values ------------------------------------------------------------
+    {
+        const uint32_t slot_g = t >> 3, c = t & 7u;
+        const bool cact = c < nchild;
+        uint32_t k = slot_g;
+        bool have = false;
+        Clause cl = Clause{0, 0, 0, 0};
+        int32_t sa = kSrcNone, sb = kSrcNone;
+        uint32_t idle = 0;
+        auto input = [&](int a) -> Iv { return make_checked(s_lo[c][a], s_hi[c][a]); };
+        auto operand = [&](int32_t s, float im) -> Iv {
+            if (s >= 0) {
+                const float2 v = W.val[8u * uint32_t(s) + c];
+                return {v.x, v.y, ((W.vnan[s] >> c) & 1u) != 0};
+            }
+            if (s == kSrcX) return input(0);
+            if (s == kSrcY) return input(1);
+            if (s == kSrcZ) return input(2);
+            if (s == kSrcImm) return make_checked(im, im);
+            return {0.0f, 0.0f, false};
+        };
+        auto rslot_of = [&](int32_t s) -> uint32_t {
+            if (s >= 0) return W.rslot[s];
+            if (s == kSrcX) return Q.slot_x;
+            if (s == kSrcY) return Q.slot_y;
+            if (s == kSrcZ) return Q.slot_z;
+            return 0u;
+        };
+        for (;;) {
+            const bool mine = k < nn;
+            if (ballot(mine) == 0) break;
+            if (mine && !have) {
+                cl = pool_in[off + k];
+                sa = W.src[2 * k];
+                sb = W.src[2 * k + 1];
+                have = true;
+            }
+            const bool ready = mine && wide_done(done, sa) && wide_done(done, sb);
+            uint32_t keep = KEEP_BOTH;
+            bool onan = false;
+            uint32_t rs = 0;
+            if (ready) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                const uint32_t o = tape::op(cl), m = mode(cl);
+                const float im = imm(cl);
+                Iv a = operand(sa, im);
+                Iv b = {0.0f, 0.0f, false};
+                if (is_binary(cl)) b = operand(sb, im);
+                if (o == OC::OP_MIN || o == OC::OP_MAX) {
+                    const bool same = (m == MODE_RR) && rslot_of(sa) == rslot_of(sb);
+                    if (same) keep = KEEP_A;
+                    else if (a.lo > b.hi) keep = (o == OC::OP_MAX) ? KEEP_A : KEEP_B;
+                    else if (b.lo > a.hi) keep = (o == OC::OP_MAX) ? KEEP_B : KEEP_A;
+                }
+                const Iv out = iv_apply(o, a, b, im);
+                onan = out.nan;
+                rs = (o == OP_COPY) ? rslot_of(sa) : slot(cl);
+                W.val[8u * k + c] = make_float2(out.lo, out.hi);
+            }
+            const uint64_t rm = ballot(ready);
+            const uint64_t ka = ballot(ready && cact && keep == KEEP_A);
+            const uint64_t kbm = ballot(ready && cact && keep == KEEP_B);
+            const uint64_t nm = ballot(ready && onan);
+            if (ready && cact && (keep == KEEP_A || keep == KEEP_B)) atomicOr(&s_anyab, 1u << c);
+            if (ready && c == 0) {
+                const uint32_t sh = lane & ~7u;
+                W.choice[k] = uint16_t(((ka >> sh) & 0xffu) | (((kbm >> sh) & 0xffu) << 8));
+                W.vnan[k] = uint8_t((nm >> sh) & 0xffu);
+                W.rslot[k] = rs;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            if (ready && c == 0) {
+                __hip_atomic_fetch_or(&done[k >> 5], 1u << (k & 31u), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            if (ready) {
+                k += 64;
+                have = false;
+            }
+            if (rm == 0) {
+                if (++idle > kWideIdleCap) {
+                    if (lane == 0) atomicOr(&s_err, 2u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- the children's bounds and states; push decisions -----------------------
+    if (t < nchild) {
+        Iv r;
+        const int32_t rsrc = s_root;
+        if (rsrc >= 0) {
+            const float2 v = W.val[8u * uint32_t(rsrc) + t];
+            r = {v.x, v.y, ((W.vnan[rsrc] >> t) & 1u) != 0};
+        } else if (rsrc >= kSrcZ) {
+            const int a = -rsrc - 1;
+            r = make_checked(s_lo[t][a], s_hi[t][a]);
+        } else {
+            r = {0.0f, 0.0f, true};
+        }
+        uint8_t state;
+        if (r.nan) state = AMBIGUOUS;
+        else if (r.lo > 0.0f) state = EMPTY;
+        else if (r.hi < 0.0f) state = FILLED;
+        else state = AMBIGUOUS;
+        s_state[t] = state;
+        bool dopush = !(r.lo > 0.0f) && !(r.hi < 0.0f) && !r.nan &&
+                      !(tflags & TAPE_TERMINAL) && ((s_anyab >> t) & 1u) && s_err == 0;
+        if (dopush) {
+            const uint32_t base = atomicAdd(&P.counters->pool_top, n + kPushExtra);
+            if (base + n + kPushExtra > P.pool_cap) {
+                atomicOr(&P.counters->error, ERR_POOL);
+                dopush = false;
+            }
+            s_base[t] = base;
+        }
+        if (dopush) atomicOr(&s_dopush, 1u << t);
+    }
+    __syncthreads();
+
+    if (s_dopush != 0u) {
+        // ---- 3. liveness ----------------------------------------------------------
+        if (t == 0 && s_root >= 0) W.live[s_root] = s_dopush;
+        __syncthreads();
+        {
+            int32_t k = int32_t(nn) - 1 - int32_t(t);
+            bool have = false;
+            Clause cl = Clause{0, 0, 0, 0};
+            int32_t sa = kSrcNone, sb = kSrcNone;
+            uint32_t idle = 0, changed = 0, nonterm = 0;
+            for (;;) {
+                const bool mine = k >= 0;
+                if (ballot(mine) == 0) break;
+                if (mine && !have) {
+                    cl = pool_in[off + uint32_t(k)];
+                    sa = W.src[2 * k];
+                    sb = W.src[2 * k + 1];
+                    have = true;
+                }
+                bool ready = false;
+                if (mine) {
+                    ready = __hip_atomic_load(&W.pend[k], __ATOMIC_ACQUIRE,
+                                              __HIP_MEMORY_SCOPE_WORKGROUP) == 0u;
+                }
+                if (ready) {
+                    const uint32_t lv = __hip_atomic_load(&W.live[k], __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_WORKGROUP);
+                    const uint32_t o = tape::op(cl);
+                    uint32_t na = lv, nb = lv;
+                    if (lv && (o == OC::OP_MIN || o == OC::OP_MAX)) {
+                        const uint32_t ch = W.choice[k];
+                        const uint32_t kam = ch & 0xffu, kbm = ch >> 8;
+                        na = lv & ~kbm;
+                        nb = lv & ~kam;
+                        changed |= lv & (kam | kbm);
+                        nonterm |= lv & ~(kam | kbm);
+                    }
+                    if (sa >= 0) {
+                        if (na) {
+                            __hip_atomic_fetch_or(&W.live[sa], na, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                        __hip_atomic_fetch_sub(&W.pend[sa], 1u, __ATOMIC_RELEASE,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                    if (sb >= 0) {
+                        if (nb) {
+                            __hip_atomic_fetch_or(&W.live[sb], nb, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                        __hip_atomic_fetch_sub(&W.pend[sb], 1u, __ATOMIC_RELEASE,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                    k -= int32_t(kWideThreads);
+                    have = false;
+                }
+                if (ballot(ready) == 0) {
+                    if (++idle > kWideIdleCap) {
+                        if (lane == 0) atomicOr(&s_err, 4u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            if (changed) atomicOr(&s_changed, changed);
+            if (nonterm) atomicOr(&s_nonterm, nonterm);
+        }
+        __syncthreads();
+
+        // ---- 4. emission: wave c writes child c's tape ----------------------------
+        const uint32_t emit = s_err ? 0u : (s_dopush & s_changed);
+        if (wv < nchild && ((emit >> wv) & 1u)) {
+            const uint32_t c = wv;
+            uint8_t* ren = rc_base + 2u * nregs * c;
+            uint8_t* cnt = ren + nregs;
+            uint32_t* lst = W.list + size_t(c) * max_len;
+            uint32_t nl = 0;
+            for (uint32_t b = 0; b < nn; b += 64) {
+                const uint32_t k = b + lane;
+                const bool bit = k < nn && ((W.live[k] >> c) & 1u);
+                const uint64_t m = ballot(bit);
+                if (bit) lst[nl + uint32_t(__popcll(m & ((uint64_t(1) << lane) - 1)))] = k;
+                nl += uint32_t(__popcll(m));
+            }
+            for (uint32_t r = lane; r < nregs; r += 64) {
+                ren[r] = uint8_t(r);
+                cnt[r] = 0;
+            }
+            __threadfence_block();
+            uint32_t pos = s_base[c], refc = 0;
+            uint32_t (*stage)[5] = s_stage[wv];
+            for (uint32_t jb = 0; jb < nl; jb += 64) {
+                // this batch's clauses and decisions, staged in LDS
+                if (jb + lane < nl) {
+                    const uint32_t k = lst[jb + lane];
+                    const Clause cc = pool_in[off + k];
+                    stage[lane][0] = cc.w0;
+                    stage[lane][1] = cc.w1;
+                    stage[lane][2] = cc.w2;
+                    stage[lane][3] = cc.w3;
+                    stage[lane][4] = W.choice[k];
+                }
+                __threadfence_block();
+                if (lane == 0) {
+                    const uint32_t cntb = min(64u, nl - jb);
+                    auto will_write = [&](uint32_t d) {
+                        for (uint32_t y = 0; cnt[d] != 0 && y < nregs; ++y) {
+                            if (y != d && ren[y] == d) {
+                                P.pool[pos++] = make(OP_COPY, MODE_RR, y, d, 0, 0.0f, 0);
+                                ren[y] = uint8_t(y);
+                                cnt[d]--;
+                            }
+                        }
+                        if (ren[d] != d) {
+                            cnt[ren[d]]--;
+                            ren[d] = uint8_t(d);
+                        }
+                    };
+                    for (uint32_t q = 0; q < cntb; ++q) {
+                        Clause c2;
+                        c2.w0 = stage[q][0];
+                        c2.w1 = stage[q][1];
+                        c2.w2 = stage[q][2];
+                        c2.w3 = stage[q][3];
+                        const uint32_t ch = stage[q][4];
+                        const uint32_t o = tape::op(c2);
+                        const uint32_t m = mode(c2);
+                        const uint32_t d = dst(c2);
+                        bool alias = (o == OP_COPY);
+                        bool ka = true;
+                        if (o == OC::OP_MIN || o == OC::OP_MAX) {
+                            ka = (ch >> c) & 1u;
+                            alias = ka || ((ch >> (8 + c)) & 1u);
+                        }
+                        if (alias) {
+                            const bool from_imm = (o != OP_COPY) &&
+                                                  (ka ? (m == MODE_A_IMM) : (m == MODE_B_IMM));
+                            if (from_imm) {
+                                will_write(d);
+                                P.pool[pos++] = make(OP_LOADI, MODE_A_IMM, d, 0, 0, imm(c2),
+                                                     slot(c2));
+                                continue;
+                            }
+                            const uint32_t r = ren[ka ? ra(c2) : rb(c2)];
+                            if (r == d) continue;
+                            if (cnt[d] != 0) {
+                                will_write(d);
+                                P.pool[pos++] = make(OP_COPY, MODE_RR, d, r, 0, 0.0f, slot(c2));
+                            } else {
+                                if (ren[d] != d) cnt[ren[d]]--;
+                                ren[d] = uint8_t(r);
+                                cnt[r]++;
+                            }
+                            continue;
+                        }
+                        const bool use_a = nargs(o) >= 1 && m != MODE_A_IMM && o != OP_LOADI;
+                        const bool use_b = is_binary(c2) && m != MODE_B_IMM;
+                        const uint32_t a2 = use_a ? ren[ra(c2)] : ra(c2);
+                        const uint32_t b2 = use_b ? ren[rb(c2)] : rb(c2);
+                        will_write(d);
+                        Clause outc = c2;
+                        outc.w1 = (a2 & 0xffffu) | (b2 << 16);
+                        P.pool[pos++] = outc;
+                        if (o != OP_LOADI) ++refc;
+                    }
+                }
+                __threadfence_block();
+            }
+            if (lane == 0) {
+                if (ren[root] != root) {
+                    P.pool[pos++] = make(OP_COPY, MODE_RR, root, ren[root], 0, 0.0f, 0);
+                }
+                for (uint32_t k = 0; k < 8u; ++k) P.pool[pos + k] = Clause{0, 0, 0, 0};
+                const uint32_t id = atomicAdd(&P.counters->tape_count, 1u);
+                if (id >= P.tape_cap) {
+                    atomicOr(&P.counters->error, ERR_TAPES);
+                } else {
+                    TapeRec r;
+                    const uint32_t base = s_base[c];
+                    r.off = base;
+                    r.len = pos - base;
+                    r.parent = tape_id;
+                    r.flags = TAPE_INTERVAL | (((s_nonterm >> c) & 1u) ? 0u : TAPE_TERMINAL);
+                    r.box[0] = s_lo[c][0]; r.box[1] = s_hi[c][0];
+                    r.box[2] = s_lo[c][1]; r.box[3] = s_hi[c][1];
+                    r.box[4] = s_lo[c][2]; r.box[5] = s_hi[c][2];
+                    r.ref_len = refc;
+                    r.model = tr.model;
+                    P.tapes[id] = r;
+                    s_child_tape[c] = id;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (t == 0 && s_err) atomicOr(&P.counters->error, ERR_WIDE);
+
+    // ---- the children's cell records, as k_interval --------------------------------
+    if (wv == 0) {
+        const bool active = lane < nchild;
+        const uint32_t ci = active ? lane : 0u;
+        const uint32_t child_tape = s_child_tape[ci];
+        const uint8_t state = s_state[ci];
+        uint32_t len = (active && state == AMBIGUOUS) ? P.tapes[child_tape].len : 0u;
+        unsigned long long need = (active && state == AMBIGUOUS)
+            ? 8ull * (len + kPushExtra) : 0ull;
+        for (int d = 1; d < 8; d <<= 1) {
+            const uint32_t ol = __shfl_xor(len, d);
+            len = ol > len ? ol : len;
+            need += (unsigned long long)__shfl_xor((uint32_t)need, d) |
+                    ((unsigned long long)__shfl_xor((uint32_t)(need >> 32), d) << 32);
+        }
+        if (lane == 0 && len) {
+            atomicMax(&P.counters->max_len, len);
+            atomicAdd(&P.counters->push_need, need);
+        }
+        if (active) {
+            const uint32_t cidx = s_cidx[ci];
+            cur.type[cidx] = state;
+            cur.tape[cidx] = child_tape;
+            cur.child[cidx] = -1;
+            for (int a = 0; a < 3; ++a) cur.key[3 * cidx + a] = s_key[ci][a];
+        }
+    }
+}
 
 #define LF_INTERVAL_SWITCH(KERNEL, ...)                                                   \
     for (uint32_t g0 = 0; g0 < ngroups; g0 += l.chunk) {                                  \
@@ -622,6 +1266,21 @@ hipError_t launch_interval(const Params& p, int level, const uint32_t* groups,
                            const LevelView& cur, uint32_t max_len, uint8_t* ws,
                            size_t ws_bytes, hipStream_t s, uint32_t first_root) {
     if (ngroups == 0) return hipSuccess;
+    if (banks(p.nregs) != 0 && interval_uses_wide(max_len, ngroups)) {
+        const size_t stride = interval_wide_stride(max_len);
+        if (ws == nullptr || ws_bytes < size_t(std::min(ngroups, kWideChunk)) * stride) {
+            return hipErrorInvalidValue;
+        }
+        const size_t lds = interval_wide_lds(p.nregs, max_len);
+        for (uint32_t g0 = 0; g0 < ngroups; g0 += kWideChunk) {
+            const uint32_t ng = std::min(kWideChunk, ngroups - g0);
+            hipLaunchKernelGGL(k_interval_wide, dim3(ng), dim3(kWideThreads), lds, s, p, p.pool,
+                               p.tapes, level, groups, g0, parent, cur, max_len, ws, first_root);
+            const hipError_t le = hipGetLastError();
+            if (le != hipSuccess) return le;
+        }
+        return hipSuccess;
+    }
     IntervalLaunch l;
     const hipError_t e = interval_launch(p.nregs, max_len, ngroups, ws, ws_bytes, l);
     if (e != hipSuccess) return e;

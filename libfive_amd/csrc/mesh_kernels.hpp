@@ -100,6 +100,7 @@ enum : uint32_t {
     ERR_TAPES = 2u,      // tape records exhausted
     ERR_FEATURE = 4u,    // a feature evaluation exceeded its fixed capacity
     ERR_LOCATE = 8u,     // point location inconsistency
+    ERR_WIDE = 16u,      // the wide interval pass met an invalid tape or stalled
 };
 
 // lo, hi, top, root_reg, slot_*, root_cell and slot0 describe one model;
@@ -142,14 +143,23 @@ __host__ __device__ inline size_t interval_ws_stride(uint32_t max_len) {
     return (3u * size_t(max_len) + 255u) & ~size_t(255);
 }
 size_t interval_lds_bytes(uint32_t nregs, uint32_t max_len);
-size_t interval_workspace_bytes(uint32_t nregs, uint32_t max_len);
+size_t interval_workspace_bytes(uint32_t nregs, uint32_t max_len, uint32_t ngroups);
+// The same for launch_interval_boxes (which never uses the wide kernel).
+size_t interval_boxes_workspace_bytes(uint32_t nregs, uint32_t max_len);
+// Whether launch_interval runs a level of ngroups parents whose longest
+// tape has max_len clauses with k_interval_wide (a workgroup of 8 waves
+// per parent working on the tape in dependency order) instead of
+// k_interval.
+bool interval_uses_wide(uint32_t max_len, uint32_t ngroups);
+size_t interval_wide_lds(uint32_t nregs, uint32_t max_len);
 
 // Interval classification and pruning of the cells of one level: one wave
 // per parent (8 children), or, when groups is null, the roots of models
 // first_root .. first_root + ngroups - 1 (each evaluated with its base tape
 // and stored at its ModelRec::root_cell).
 // max_len bounds the length of every tape the pass reads; ws holds
-// interval_workspace_bytes(p.nregs, max_len) bytes (null when that is 0).
+// interval_workspace_bytes(p.nregs, max_len, ngroups) bytes (null when that
+// is 0).
 hipError_t launch_interval(const Params& p, int level, const uint32_t* groups,
                            uint32_t ngroups, const LevelView& parent,
                            const LevelView& cur, uint32_t max_len, uint8_t* ws,

@@ -142,6 +142,10 @@ void check_errors(uint32_t err) {
     if (err & (ERR_POOL | ERR_TAPES)) {
         throw Error("render_mesh: tape storage exhausted");
     }
+    if (err & ERR_WIDE) {
+        throw Error("render_mesh: the interval pass of a long tape met an invalid tape "
+                    "or made no progress");
+    }
     if (err) throw Error("render_mesh: internal error " + std::to_string(err));
 }
 
@@ -400,7 +404,7 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
                    fast, settings, counters, hc, s);
     } else {
         reset_level_counters(counters.p, s);
-        iws[top].alloc(interval_workspace_bytes(P.nregs, nbase));
+        iws[top].alloc(interval_workspace_bytes(P.nregs, nbase, 1));
         hip_check(launch_interval(P, top, nullptr, 1, levels[top].view(),
                                   levels[top].view(), nbase, iws[top].p, iws[top].n, s),
                   "interval kernel");
@@ -433,7 +437,7 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
                 grow(tapes, hc.tape_count, size_t(hc.tape_count) + size_t(namb) * 8u, s);
                 refresh();
                 reset_level_counters(counters.p, s);
-                iws[L - 1].alloc(interval_workspace_bytes(P.nregs, max_len));
+                iws[L - 1].alloc(interval_workspace_bytes(P.nregs, max_len, uint32_t(namb)));
                 hip_check(launch_interval(P, L - 1, groups[L].p, uint32_t(namb), cur.view(),
                                           levels[L - 1].view(), max_len, iws[L - 1].p,
                                           iws[L - 1].n, s),
@@ -798,8 +802,8 @@ void merged_body(const BatchItem* items, const std::vector<size_t>& which, HostM
             grow(tapes, hc.tape_count, size_t(hc.tape_count) + ncells, s);
             refresh();
             reset_level_counters(counters.p, s);
-            const uint32_t wl = std::max(max_len_above, root_len);
-            iws[L].alloc(interval_workspace_bytes(P.nregs, wl));
+            iws[L].alloc(std::max(interval_workspace_bytes(P.nregs, max_len_above, namb[L + 1]),
+                                  interval_workspace_bytes(P.nregs, root_len, nroots)));
             if (namb[L + 1]) {
                 hip_check(launch_interval(P, L, groups[L + 1].p, namb[L + 1], levels[L + 1].view(),
                                           levels[L].view(), max_len_above, iws[L].p, iws[L].n, s),

@@ -95,7 +95,7 @@ void SliceTapes::eval(const std::vector<float>& boxes, const std::vector<uint32_
 
     PoolBuf<mesh::BoxGroup> dg(groups.size());
     PoolBuf<float> db(6 * nb), dbo(2 * nb);
-    PoolBuf<uint8_t> dn(nb), ws(mesh::interval_workspace_bytes(P_.nregs, max_len_));
+    PoolBuf<uint8_t> dn(nb), ws(mesh::interval_boxes_workspace_bytes(P_.nregs, max_len_));
     PoolBuf<uint32_t> dt(nb);
     hip_check(hipMemcpyAsync(dg.p, groups.data(), groups.size() * sizeof(mesh::BoxGroup),
                              hipMemcpyHostToDevice, s_), "copy box groups");

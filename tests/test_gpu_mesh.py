@@ -97,13 +97,20 @@ def test_c3_sphere_union_depth8_matches_oracle(gpu):
     assert min(g.tape_lengths.tolist()) < 20
 
 
-def test_long_tape_sphere_union_matches_oracle(gpu):
-    """A 2500-sphere union (27 499 clauses): the interval passes keep their
-    per-clause work arrays in a global workspace instead of LDS while the
-    pruned tapes are that long, with the same octree and mesh."""
+@pytest.mark.parametrize("wide", [-1, 0])
+def test_long_tape_sphere_union_matches_oracle(gpu, wide):
+    """A 2500-sphere union (27 499 clauses), with the wide interval kernel
+    (the default for tapes this long) and with the one-wave walk, which
+    then keeps its per-clause work arrays in a global workspace instead of
+    LDS: the same octree and mesh as the oracle."""
+    from libfive_amd import ffi
     s = W.sphere_union(2500)
     assert s.deck()[0].shape[0] > 25000
-    g = compare(s, -1.2, 1.2, 2.4 / 32, threads=16)
+    ffi.lib().libfive_autotune_force(b"interval_wide", wide)
+    try:
+        g = compare(s, -1.2, 1.2, 2.4 / 32, threads=16)
+    finally:
+        ffi.lib().libfive_autotune_force(b"interval_wide", -1)
     assert max(g.tape_lengths.tolist()) > 25000
 
 
@@ -298,6 +305,46 @@ def test_batch_matches_single_renders(gpu):
         assert v.shape == v1.shape and t.shape == t1.shape
         assert (v.view(np.uint32) == v1.view(np.uint32)).all()
         assert (t == t1).all()
+
+
+def _wide_shapes():
+    return {
+        "gyroid": (W.gyroid_c4(), 1.2, 2.4 / 64),
+        "csg_c1": (W.csg_c1(), 2.0, 4.0 / 64),
+        "union60": (W.sphere_union(60), 1.2, 2.4 / 64),
+        "dag37": (W.random_dag() - 0.3, 1.0, 2.0 / 16),
+        "wide75": (wide_registers(70), 1.0, 2.0 / 16),
+        "box": (W.box((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5)), 1.0, 2.0 / 16),
+    }
+
+
+@pytest.mark.parametrize("name", ["gyroid", "csg_c1", "union60", "dag37", "wide75", "box"])
+def test_wide_interval_pass_matches_walk(gpu, name):
+    """The wide interval kernel (a workgroup of 8 waves per parent working
+    on the tape in dependency order) gives the one-wave walk's cell states,
+    pruned tape lengths and mesh bit for bit, forced onto shapes whose
+    tapes are too short to get it by default (one, two and four register
+    banks; a box whose faces meet the grid)."""
+    from libfive_amd import ffi
+    L = ffi.lib()
+    shape, half, mf = _wide_shapes()[name]
+    lo3 = [float(np.float32(-half))] * 3
+    hi3 = [float(np.float32(half))] * 3
+    out = []
+    try:
+        for choice in (0, 1):
+            L.libfive_autotune_force(b"interval_wide", choice)
+            print(name, "interval_wide", choice, flush=True)
+            out.append(shape.render_mesh(lo3, hi3, min_feature=mf, details=True))
+    finally:
+        L.libfive_autotune_force(b"interval_wide", -1)
+    a, b = out
+    assert (a.level_counts == b.level_counts).all()
+    assert a.cells.tolist() == b.cells.tolist()
+    assert sorted(a.tape_lengths.tolist()) == sorted(b.tape_lengths.tolist())
+    assert a.verts.shape == b.verts.shape and a.tris.shape == b.tris.shape
+    assert (a.verts.view(np.uint32) == b.verts.view(np.uint32)).all()
+    assert (a.tris == b.tris).all() and (a.keys == b.keys).all()
 
 
 def test_merged_batch_matches_single_renders(gpu):

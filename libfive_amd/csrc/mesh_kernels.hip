@@ -639,29 +639,35 @@ hipError_t interval_launch(uint32_t nregs, uint32_t max_len, uint32_t ngroups,
 // dependent steps however few parents a level has.  At the top levels of a
 // long tape (a few parents, thousands of clauses) that serial walk is the
 // whole cost of the level.  k_interval_wide gives each parent a workgroup
-// of kWideWaves waves and works on the tape in dependency order:
-//   1. sources: the clause that defines each operand (the SSA form of the
-//      register-allocated tape); each wave renames one chunk of the tape
-//      and the chunks are joined through their last definitions;
-//   2. values: each clause is evaluated for the 8 children (8 lanes per
-//      clause, 64 clauses in flight) as soon as its sources are done, and
-//      the keep decisions of MIN/MAX are recorded per child;
-//   3. liveness: from the root towards the leaves, each clause passes its
-//      children's needs to its sources once all of its users are done;
-//   4. emission: each pushing child's tape is written by its own wave with
-//      interval_push's walk (live clauses in order, operands renamed
-//      through decided clauses, aliases copied before their register is
-//      overwritten).
+// of kWideWaves waves:
+//   1. sources: for every operand, the clause that defines the value it
+//      reads (the SSA form of the register-allocated tape); each wave
+//      renames one chunk of the tape and the chunks are joined through
+//      their last definitions;
+//   2. values: the clauses run out of order, 8 lanes per clause (one per
+//      child) and 64 clauses in flight, each as soon as the clauses that
+//      define its operands are done; every clause's value has its own slot
+//      in the workspace (the register file's reuse would order clauses
+//      that are independent).  The smallest pending clause can always run,
+//      so the pass always progresses.  The keep decisions of MIN/MAX are
+//      recorded per child;
+//   3. liveness: each pushing child's wave walks the tape backwards with
+//      its set of needed registers spread over its lanes, as
+//      interval_push does, listing the live clauses;
+//   4. emission: the same wave writes the child's tape with interval_push's
+//      walk over that list (operands renamed through decided clauses,
+//      aliases copied before their register is overwritten).
 // Each value is computed by the same operations on the same operands as in
-// interval_push, and the emission walk is the same, so the bounds, the
-// decisions and the pushed tapes are the same.  Every wait is bounded: a
-// loop that makes no progress for too long sets ERR_WIDE and ends.
+// interval_push, and the liveness and emission walks are the same, so the
+// bounds, the decisions and the pushed tapes are the same.  Every wait is
+// bounded: a loop that makes no progress for too long sets ERR_WIDE and
+// ends.
 
 constexpr int kWideWaves = 8;
 constexpr uint32_t kWideThreads = 64u * kWideWaves;
 constexpr int32_t kSrcX = -1, kSrcY = -2, kSrcZ = -3;   // registers 0, 1, 2
 constexpr int32_t kSrcImm = -4;                         // the clause's immediate
-constexpr int32_t kSrcNone = -5;                        // no operand
+constexpr int32_t kSrcNone = -5;                        // no operand / no definition
 constexpr int32_t kSrcUnres = -16;                      // kSrcUnres - r: register r, not yet found
 constexpr uint32_t kWideIdleCap = 1u << 22;             // idle iterations before ERR_WIDE
 
@@ -669,17 +675,15 @@ __host__ __device__ inline size_t wide_align(size_t b) { return (b + 255) & ~siz
 
 __host__ __device__ size_t interval_wide_stride(uint32_t max_len) {
     const size_t n = max_len;
-    return wide_align(64 * n) + wide_align(8 * n) + 3 * wide_align(4 * n) +
-           wide_align(32 * n) + wide_align(2 * n) + wide_align(n);
+    return wide_align(64 * n) + wide_align(8 * n) + wide_align(4 * n) + wide_align(32 * n) +
+           wide_align(2 * n) + wide_align(n);
 }
 
 struct WideWork {
     float2* val;        // [n][8]: interval of each clause's value per child
     int32_t* src;       // [n][2]: source of operands a and b
     uint32_t* rslot;    // [n]: reference slot of the value (the keep rule's a == b)
-    uint32_t* live;     // [n]: children for which the clause is live
-    uint32_t* pend;     // [n]: users not yet through the liveness pass
-    uint32_t* list;     // [8][n]: live clauses of each child, in order
+    uint32_t* list;     // [8][n]: live clauses of each child, last first
     uint16_t* choice;   // [n]: keep-A children | keep-B children << 8
     uint8_t* vnan;      // [n]: children whose value may be NaN
 };
@@ -691,45 +695,71 @@ __device__ inline WideWork wide_work(uint8_t* base, uint32_t max_len) {
     w.val = reinterpret_cast<float2*>(base + o); o += wide_align(64 * n);
     w.src = reinterpret_cast<int32_t*>(base + o); o += wide_align(8 * n);
     w.rslot = reinterpret_cast<uint32_t*>(base + o); o += wide_align(4 * n);
-    w.live = reinterpret_cast<uint32_t*>(base + o); o += wide_align(4 * n);
-    w.pend = reinterpret_cast<uint32_t*>(base + o); o += wide_align(4 * n);
     w.list = reinterpret_cast<uint32_t*>(base + o); o += wide_align(32 * n);
     w.choice = reinterpret_cast<uint16_t*>(base + o); o += wide_align(2 * n);
     w.vnan = base + o;
     return w;
 }
 
-// Dynamic LDS of k_interval_wide: done bits of the clauses, the waves'
-// last-definition tables, and per child the emission walk's renaming and
-// alias counts.
+// Tapes of at most this many clauses keep the liveness pass in LDS.
+constexpr uint32_t kWideLdsLive = 12288;
+
+// Dynamic LDS of k_interval_wide: the waves' last-definition tables (4 B x
+// kWideWaves per register) and the clauses' done bits; later the liveness
+// pass's live bits and pending user counts (3 B per clause) in the same
+// space.
 size_t interval_wide_lds(uint32_t nregs, uint32_t max_len) {
-    return 4u * ((size_t(max_len) + 31u) / 32u) + 4u * kWideWaves * size_t(nregs) +
-           2u * 8u * size_t(nregs);
+    const size_t a = size_t(nregs) * 4u * kWideWaves + 4u * ((size_t(max_len) + 31u) / 32u);
+    const size_t ml = std::min<size_t>(max_len, kWideLdsLive);
+    const size_t b = 4u * ((ml + 3u) / 4u + (ml + 1u) / 2u);
+    return std::max(a, b);
 }
 
-// One register-file table distributed over the 64 lanes of a wave: entry r
-// lives in lane r % 64, slot r / 64 (up to 256 entries).
+// One table over the registers, spread over the 64 lanes of a wave: entry
+// r lives in lane r % 64, slot r / 64 (up to 256 entries).  The slots are
+// named registers, never an array: an indexed array would be placed in
+// scratch memory.
 struct LaneTable {
-    int32_t v[4];
+    int32_t v0, v1, v2, v3;
 };
+__device__ __forceinline__ LaneTable lt_make(int32_t a, int32_t b, int32_t c, int32_t d) {
+    LaneTable t;
+    t.v0 = a;
+    t.v1 = b;
+    t.v2 = c;
+    t.v3 = d;
+    return t;
+}
+// this lane's entry in slot q
+__device__ __forceinline__ int32_t lt_slot(const LaneTable& t, uint32_t q) {
+    return q == 0 ? t.v0 : q == 1 ? t.v1 : q == 2 ? t.v2 : t.v3;
+}
 __device__ __forceinline__ int32_t lt_get(const LaneTable& t, uint32_t r) {
     const uint32_t l = r & 63u;
     switch (r >> 6) {
-        case 0: return __builtin_amdgcn_readlane(t.v[0], l);
-        case 1: return __builtin_amdgcn_readlane(t.v[1], l);
-        case 2: return __builtin_amdgcn_readlane(t.v[2], l);
-        default: return __builtin_amdgcn_readlane(t.v[3], l);
+        case 0: return __builtin_amdgcn_readlane(t.v0, l);
+        case 1: return __builtin_amdgcn_readlane(t.v1, l);
+        case 2: return __builtin_amdgcn_readlane(t.v2, l);
+        default: return __builtin_amdgcn_readlane(t.v3, l);
     }
 }
 __device__ __forceinline__ void lt_set(LaneTable& t, uint32_t r, int32_t x, uint32_t lane) {
     if (lane == (r & 63u)) {
         switch (r >> 6) {
-            case 0: t.v[0] = x; break;
-            case 1: t.v[1] = x; break;
-            case 2: t.v[2] = x; break;
-            default: t.v[3] = x; break;
+            case 0: t.v0 = x; break;
+            case 1: t.v1 = x; break;
+            case 2: t.v2 = x; break;
+            default: t.v3 = x; break;
         }
     }
+}
+
+// A set of registers spread the same way: lane r % 64 holds bit r / 64.
+__device__ __forceinline__ bool bs_get(uint32_t bits, uint32_t r) {
+    return (__builtin_amdgcn_readlane(bits, r & 63u) >> (r >> 6)) & 1u;
+}
+__device__ __forceinline__ void bs_put(uint32_t& bits, uint32_t r, bool on, uint32_t lane) {
+    if (lane == (r & 63u)) bits = on ? (bits | (1u << (r >> 6))) : (bits & ~(1u << (r >> 6)));
 }
 
 __device__ __forceinline__ bool wide_done(const uint32_t* done, int32_t s) {
@@ -748,11 +778,10 @@ void k_interval_wide(Params P, const Clause* __restrict__ pool_in,
     HIP_DYNAMIC_SHARED(uint32_t, lds)
     __shared__ float s_lo[8][3], s_hi[8][3];
     __shared__ uint32_t s_key[8][3];
-    __shared__ uint32_t s_cidx[8], s_base[8], s_child_tape[8];
+    __shared__ uint32_t s_cidx[8], s_base[8], s_child_tape[8], s_nlive[8];
     __shared__ uint8_t s_state[8];
     __shared__ uint32_t s_anyab, s_dopush, s_changed, s_nonterm, s_err;
     __shared__ int32_t s_root;
-    __shared__ uint32_t s_stage[kWideWaves][64][5];
 
     const uint32_t t = threadIdx.x, lane = t & 63u;
     const uint32_t wv = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -760,10 +789,10 @@ void k_interval_wide(Params P, const Clause* __restrict__ pool_in,
     const bool at_root = (groups == nullptr);
     const WideWork W = wide_work(ws + size_t(blockIdx.x) * interval_wide_stride(max_len), max_len);
     const uint32_t nregs = P.nregs;
-    uint32_t* done = lds;
+    // LDS carve (see interval_wide_lds)
+    int32_t* defs = reinterpret_cast<int32_t*>(lds);                       // [kWideWaves][nregs]
+    uint32_t* done = reinterpret_cast<uint32_t*>(defs + kWideWaves * nregs);
     const uint32_t nwords = (max_len + 31u) / 32u;
-    int32_t* defs = reinterpret_cast<int32_t*>(lds + nwords);        // [kWideWaves][nregs]
-    uint8_t* rc_base = reinterpret_cast<uint8_t*>(defs + kWideWaves * nregs);
 
     uint32_t tape_id = first_root + g, first = 0, pkey[3] = {0, 0, 0};
     if (!at_root) {
@@ -798,6 +827,7 @@ void k_interval_wide(Params P, const Clause* __restrict__ pool_in,
         s_cidx[t] = first + octant;
         s_child_tape[t] = tape_id;
         s_base[t] = 0;
+        s_nlive[t] = 0;
     }
     if (t == 0) {
         s_anyab = 0;
@@ -808,10 +838,7 @@ void k_interval_wide(Params P, const Clause* __restrict__ pool_in,
         s_root = kSrcNone;
     }
     for (uint32_t i = t; i < nwords; i += kWideThreads) done[i] = 0u;
-    if (n > max_len) {
-        // the host sized the workspace by max_len; a longer tape is an error
-        if (t == 0) atomicOr(&P.counters->error, ERR_WIDE);
-    }
+    if (n > max_len && t == 0) atomicOr(&s_err, 8u);   // the host sized the workspace by max_len
     const uint32_t nn = n > max_len ? 0u : n;
     __syncthreads();
 
@@ -819,8 +846,7 @@ void k_interval_wide(Params P, const Clause* __restrict__ pool_in,
     const uint32_t csz = (nn + kWideWaves - 1) / kWideWaves;
     const uint32_t c0 = wv * csz, c1 = min(nn, c0 + csz);
     {
-        LaneTable def;
-        for (int q = 0; q < 4; ++q) def.v[q] = -1;
+        LaneTable def = lt_make(-1, -1, -1, -1);
         for (uint32_t kb = c0; kb < c1; kb += 64) {
             const uint32_t k = kb + lane;
             uint32_t w0 = 0, w1 = 0;
@@ -858,7 +884,7 @@ void k_interval_wide(Params P, const Clause* __restrict__ pool_in,
         }
         for (int q = 0; q < 4; ++q) {
             const uint32_t r = lane + 64u * uint32_t(q);
-            if (r < nregs) defs[wv * nregs + r] = def.v[q];
+            if (r < nregs) defs[wv * nregs + r] = lt_slot(def, uint32_t(q));
         }
     }
     __syncthreads();
@@ -878,22 +904,12 @@ void k_interval_wide(Params P, const Clause* __restrict__ pool_in,
         const int32_t a = W.src[2 * k], b = W.src[2 * k + 1];
         if (a <= kSrcUnres) W.src[2 * k] = resolve(a, w);
         if (b <= kSrcUnres) W.src[2 * k + 1] = resolve(b, w);
-        W.pend[k] = 0u;
-        W.live[k] = 0u;
     }
     if (t == 0) {
         s_root = root < kFirstFreeReg ? -int32_t(root) - 1 : resolve(kSrcUnres - int32_t(root),
                                                                       kWideWaves);
     }
     __syncthreads();
-    // users of each clause (the operands the liveness pass will follow)
-    for (uint32_t k = t; k < nn; k += kWideThreads) {
-        const int32_t a = W.src[2 * k], b = W.src[2 * k + 1];
-        if (a >= 0) atomicAdd(&W.pend[a], 1u);
-        if (b >= 0) atomicAdd(&W.pend[b], 1u);
-    }
-    __syncthreads();
-
     // ---- 2. values ------------------------------------------------------------
     {
         const uint32_t slot_g = t >> 3, c = t & 7u;
@@ -1017,179 +1033,311 @@ void k_interval_wide(Params P, const Clause* __restrict__ pool_in,
     }
     __syncthreads();
 
-    if (s_dopush != 0u) {
-        // ---- 3. liveness ----------------------------------------------------------
-        if (t == 0 && s_root >= 0) W.live[s_root] = s_dopush;
+    // ---- 3. liveness ----------------------------------------------------------------
+    // Tapes of at most kWideLdsLive clauses: all threads, from the root back
+    // to the leaves, each clause passing its children's needs to its
+    // sources once all of its users are through (live bits and pending user
+    // counts packed in LDS, over the sources' and done bits' space).  Longer
+    // tapes: each pushing child's wave walks the tape backwards with its set
+    // of needed registers spread over its lanes.  Both give interval_push's
+    // live clauses, changed and nonterminal flags.
+    const bool lds_live = nn <= kWideLdsLive;
+    if (lds_live && s_dopush != 0u) {
+        uint32_t* lw = lds;                                  // live bits, 4 clauses a word
+        uint32_t* pw = lds + (nn + 3u) / 4u;                 // pending users, 2 clauses a word
+        const uint32_t nlw = (nn + 3u) / 4u, npw = (nn + 1u) / 2u;
+        for (uint32_t i = t; i < nlw + npw; i += kWideThreads) lds[i] = 0u;
         __syncthreads();
-        {
-            int32_t k = int32_t(nn) - 1 - int32_t(t);
-            bool have = false;
-            Clause cl = Clause{0, 0, 0, 0};
-            int32_t sa = kSrcNone, sb = kSrcNone;
-            uint32_t idle = 0, changed = 0, nonterm = 0;
-            for (;;) {
-                const bool mine = k >= 0;
-                if (ballot(mine) == 0) break;
-                if (mine && !have) {
-                    cl = pool_in[off + uint32_t(k)];
-                    sa = W.src[2 * k];
-                    sb = W.src[2 * k + 1];
-                    have = true;
-                }
-                bool ready = false;
-                if (mine) {
-                    ready = __hip_atomic_load(&W.pend[k], __ATOMIC_ACQUIRE,
-                                              __HIP_MEMORY_SCOPE_WORKGROUP) == 0u;
-                }
-                if (ready) {
-                    const uint32_t lv = __hip_atomic_load(&W.live[k], __ATOMIC_RELAXED,
-                                                          __HIP_MEMORY_SCOPE_WORKGROUP);
-                    const uint32_t o = tape::op(cl);
-                    uint32_t na = lv, nb = lv;
-                    if (lv && (o == OC::OP_MIN || o == OC::OP_MAX)) {
-                        const uint32_t ch = W.choice[k];
-                        const uint32_t kam = ch & 0xffu, kbm = ch >> 8;
-                        na = lv & ~kbm;
-                        nb = lv & ~kam;
-                        changed |= lv & (kam | kbm);
-                        nonterm |= lv & ~(kam | kbm);
-                    }
-                    if (sa >= 0) {
-                        if (na) {
-                            __hip_atomic_fetch_or(&W.live[sa], na, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
-                        }
-                        __hip_atomic_fetch_sub(&W.pend[sa], 1u, __ATOMIC_RELEASE,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-                    }
-                    if (sb >= 0) {
-                        if (nb) {
-                            __hip_atomic_fetch_or(&W.live[sb], nb, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
-                        }
-                        __hip_atomic_fetch_sub(&W.pend[sb], 1u, __ATOMIC_RELEASE,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-                    }
-                    k -= int32_t(kWideThreads);
-                    have = false;
-                }
-                if (ballot(ready) == 0) {
-                    if (++idle > kWideIdleCap) {
-                        if (lane == 0) atomicOr(&s_err, 4u);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                }
-            }
-            if (changed) atomicOr(&s_changed, changed);
-            if (nonterm) atomicOr(&s_nonterm, nonterm);
+        for (uint32_t k = t; k < nn; k += kWideThreads) {
+            const int32_t a = W.src[2 * k], b = W.src[2 * k + 1];
+            if (a >= 0) atomicAdd(&pw[uint32_t(a) >> 1], 1u << (16u * (uint32_t(a) & 1u)));
+            if (b >= 0) atomicAdd(&pw[uint32_t(b) >> 1], 1u << (16u * (uint32_t(b) & 1u)));
+        }
+        if (t == 0 && s_root >= 0) {
+            atomicOr(&lw[uint32_t(s_root) >> 2], s_dopush << (8u * (uint32_t(s_root) & 3u)));
         }
         __syncthreads();
-
-        // ---- 4. emission: wave c writes child c's tape ----------------------------
-        const uint32_t emit = s_err ? 0u : (s_dopush & s_changed);
-        if (wv < nchild && ((emit >> wv) & 1u)) {
+        int32_t k = int32_t(nn) - 1 - int32_t(t);
+        bool have = false;
+        int32_t sa = kSrcNone, sb = kSrcNone;
+        uint32_t o = 0, ch = 0, idle = 0, changed = 0, nonterm = 0;
+        for (;;) {
+            const bool mine = k >= 0;
+            if (ballot(mine) == 0) break;
+            if (mine && !have) {
+                o = tape::op(pool_in[off + uint32_t(k)]);
+                sa = W.src[2 * k];
+                sb = W.src[2 * k + 1];
+                ch = W.choice[k];
+                have = true;
+            }
+            bool ready = false;
+            if (mine) {
+                const uint32_t pwv = __hip_atomic_load(&pw[uint32_t(k) >> 1], __ATOMIC_ACQUIRE,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                ready = ((pwv >> (16u * (uint32_t(k) & 1u))) & 0xffffu) == 0u;
+            }
+            if (ready) {
+                const uint32_t lwv = __hip_atomic_load(&lw[uint32_t(k) >> 2], __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                const uint32_t lv = (lwv >> (8u * (uint32_t(k) & 3u))) & 0xffu;
+                uint32_t na = lv, nb = lv;
+                if (lv && (o == OC::OP_MIN || o == OC::OP_MAX)) {
+                    const uint32_t kam = ch & 0xffu, kbm = ch >> 8;
+                    na = lv & ~kbm;
+                    nb = lv & ~kam;
+                    changed |= lv & (kam | kbm);
+                    nonterm |= lv & ~(kam | kbm);
+                }
+                auto pass = [&](int32_t src, uint32_t needs) {
+                    const uint32_t u = uint32_t(src);
+                    if (needs) {
+                        __hip_atomic_fetch_or(&lw[u >> 2], needs << (8u * (u & 3u)),
+                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                    __hip_atomic_fetch_sub(&pw[u >> 1], 1u << (16u * (u & 1u)), __ATOMIC_RELEASE,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                };
+                if (sa >= 0) pass(sa, na);
+                if (sb >= 0) pass(sb, nb);
+                k -= int32_t(kWideThreads);
+                have = false;
+            }
+            if (ballot(ready) == 0) {
+                if (++idle > kWideIdleCap) {
+                    if (lane == 0) atomicOr(&s_err, 4u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        if (changed) atomicOr(&s_changed, changed);
+        if (nonterm) atomicOr(&s_nonterm, nonterm);
+        __syncthreads();
+        // each pushing child's live clauses, in tape order
+        if (wv < nchild && ((s_dopush >> wv) & 1u)) {
             const uint32_t c = wv;
-            uint8_t* ren = rc_base + 2u * nregs * c;
-            uint8_t* cnt = ren + nregs;
             uint32_t* lst = W.list + size_t(c) * max_len;
             uint32_t nl = 0;
             for (uint32_t b = 0; b < nn; b += 64) {
-                const uint32_t k = b + lane;
-                const bool bit = k < nn && ((W.live[k] >> c) & 1u);
+                const uint32_t kk = b + lane;
+                const bool bit = kk < nn && ((lw[kk >> 2] >> (8u * (kk & 3u) + c)) & 1u);
                 const uint64_t m = ballot(bit);
-                if (bit) lst[nl + uint32_t(__popcll(m & ((uint64_t(1) << lane) - 1)))] = k;
+                if (bit) lst[nl + uint32_t(__popcll(m & ((uint64_t(1) << lane) - 1)))] = kk;
                 nl += uint32_t(__popcll(m));
             }
-            for (uint32_t r = lane; r < nregs; r += 64) {
-                ren[r] = uint8_t(r);
-                cnt[r] = 0;
-            }
-            __threadfence_block();
-            uint32_t pos = s_base[c], refc = 0;
-            uint32_t (*stage)[5] = s_stage[wv];
-            for (uint32_t jb = 0; jb < nl; jb += 64) {
-                // this batch's clauses and decisions, staged in LDS
-                if (jb + lane < nl) {
-                    const uint32_t k = lst[jb + lane];
+            if (lane == 0) s_nlive[c] = nl;
+        }
+    }
+
+    // ---- 4. emission: wave c for child c -------------------------------------------
+    if (wv < nchild && ((s_dopush >> wv) & 1u)) {
+        const uint32_t c = wv;
+        uint32_t* lst = W.list + size_t(c) * max_len;
+        uint32_t nl = 0;
+        bool changed = false;
+        if (lds_live) {
+            nl = s_nlive[c];
+            changed = (s_changed >> c) & 1u;
+        } else {
+            // the backward walk with the needed registers spread over the lanes
+            uint32_t need = 0;
+            bs_put(need, root, true, lane);
+            bool nonterm = false;
+            for (uint32_t kb = nn; kb > 0;) {
+                const uint32_t cntb = min(64u, kb);
+                kb -= cntb;
+                const uint32_t k = kb + lane;
+                uint32_t w0 = 0, w1 = 0, chv = 0;
+                if (lane < cntb) {
                     const Clause cc = pool_in[off + k];
-                    stage[lane][0] = cc.w0;
-                    stage[lane][1] = cc.w1;
-                    stage[lane][2] = cc.w2;
-                    stage[lane][3] = cc.w3;
-                    stage[lane][4] = W.choice[k];
+                    w0 = cc.w0;
+                    w1 = cc.w1;
+                    chv = W.choice[k];
                 }
-                __threadfence_block();
-                if (lane == 0) {
-                    const uint32_t cntb = min(64u, nl - jb);
-                    auto will_write = [&](uint32_t d) {
-                        for (uint32_t y = 0; cnt[d] != 0 && y < nregs; ++y) {
-                            if (y != d && ren[y] == d) {
-                                P.pool[pos++] = make(OP_COPY, MODE_RR, y, d, 0, 0.0f, 0);
-                                ren[y] = uint8_t(y);
-                                cnt[d]--;
-                            }
+                uint32_t my_live = 0;
+                for (uint32_t j = cntb; j-- > 0;) {
+                    Clause c2;
+                    c2.w0 = __builtin_amdgcn_readlane(w0, j);
+                    c2.w1 = __builtin_amdgcn_readlane(w1, j);
+                    const uint32_t d = dst(c2);
+                    if (!bs_get(need, d)) continue;
+                    bs_put(need, d, false, lane);
+                    const uint32_t o = tape::op(c2), m = mode(c2);
+                    bool use_a = nargs(o) >= 1 && m != MODE_A_IMM && o != OP_LOADI;
+                    bool use_b = is_binary(c2) && m != MODE_B_IMM;
+                    if (o == OC::OP_MIN || o == OC::OP_MAX) {
+                        const uint32_t ch = __builtin_amdgcn_readlane(chv, j);
+                        const bool ka = (ch >> c) & 1u, kb2 = (ch >> (8 + c)) & 1u;
+                        if (ka || kb2) {
+                            changed = true;
+                            use_a = use_a && ka;
+                            use_b = use_b && kb2;
+                        } else {
+                            nonterm = true;
                         }
-                        if (ren[d] != d) {
-                            cnt[ren[d]]--;
-                            ren[d] = uint8_t(d);
-                        }
-                    };
-                    for (uint32_t q = 0; q < cntb; ++q) {
-                        Clause c2;
-                        c2.w0 = stage[q][0];
-                        c2.w1 = stage[q][1];
-                        c2.w2 = stage[q][2];
-                        c2.w3 = stage[q][3];
-                        const uint32_t ch = stage[q][4];
-                        const uint32_t o = tape::op(c2);
-                        const uint32_t m = mode(c2);
-                        const uint32_t d = dst(c2);
-                        bool alias = (o == OP_COPY);
-                        bool ka = true;
-                        if (o == OC::OP_MIN || o == OC::OP_MAX) {
-                            ka = (ch >> c) & 1u;
-                            alias = ka || ((ch >> (8 + c)) & 1u);
-                        }
-                        if (alias) {
-                            const bool from_imm = (o != OP_COPY) &&
-                                                  (ka ? (m == MODE_A_IMM) : (m == MODE_B_IMM));
-                            if (from_imm) {
-                                will_write(d);
-                                P.pool[pos++] = make(OP_LOADI, MODE_A_IMM, d, 0, 0, imm(c2),
-                                                     slot(c2));
-                                continue;
-                            }
-                            const uint32_t r = ren[ka ? ra(c2) : rb(c2)];
-                            if (r == d) continue;
-                            if (cnt[d] != 0) {
-                                will_write(d);
-                                P.pool[pos++] = make(OP_COPY, MODE_RR, d, r, 0, 0.0f, slot(c2));
-                            } else {
-                                if (ren[d] != d) cnt[ren[d]]--;
-                                ren[d] = uint8_t(r);
-                                cnt[r]++;
-                            }
-                            continue;
-                        }
-                        const bool use_a = nargs(o) >= 1 && m != MODE_A_IMM && o != OP_LOADI;
-                        const bool use_b = is_binary(c2) && m != MODE_B_IMM;
-                        const uint32_t a2 = use_a ? ren[ra(c2)] : ra(c2);
-                        const uint32_t b2 = use_b ? ren[rb(c2)] : rb(c2);
-                        will_write(d);
-                        Clause outc = c2;
-                        outc.w1 = (a2 & 0xffffu) | (b2 << 16);
-                        P.pool[pos++] = outc;
-                        if (o != OP_LOADI) ++refc;
                     }
+                    if (use_a) bs_put(need, ra(c2), true, lane);
+                    if (use_b) bs_put(need, rb(c2), true, lane);
+                    if (lane == j) my_live = 1;
                 }
-                __threadfence_block();
+                // this batch's live clauses at the end of the list, last last
+                const uint64_t lm = ballot(my_live != 0);
+                const uint32_t hi_rank = uint32_t(__popcll(lm & ~((uint64_t(2) << lane) - 1)));
+                if (my_live) lst[max_len - 1 - (nl + hi_rank)] = k;
+                nl += uint32_t(__popcll(lm));
             }
+            lst += max_len - nl;                       // now in tape order
             if (lane == 0) {
-                if (ren[root] != root) {
-                    P.pool[pos++] = make(OP_COPY, MODE_RR, root, ren[root], 0, 0.0f, 0);
+                s_nlive[c] = nl;
+                if (changed) atomicOr(&s_changed, 1u << c);
+                if (nonterm) atomicOr(&s_nonterm, 1u << c);
+            }
+        }
+        __threadfence_block();
+
+        if (changed && s_err == 0) {
+            // interval_push's emission walk, uniform over the wave: the
+            // renaming (ren) and alias counts (cnt) are tables spread over
+            // the lanes; lane 0 stores the clauses
+            LaneTable ren = lt_make(int32_t(lane), int32_t(lane + 64u), int32_t(lane + 128u),
+                                    int32_t(lane + 192u));
+            LaneTable cnt = lt_make(0, 0, 0, 0);
+            uint32_t pos = s_base[c], refc = 0;
+            auto put = [&](const Clause& x) {
+                if (lane == 0) P.pool[pos] = x;
+                ++pos;
+            };
+            auto will_write = [&](uint32_t d) {
+                if (lt_get(cnt, d) != 0) {
+                    // every register aliased to d becomes a copy of d, in
+                    // register order
+                    for (int q = 0; q < 4; ++q) {
+                        const uint32_t y = lane + 64u * uint32_t(q);
+                        uint64_t m = ballot(y < nregs && y != d &&
+                                            uint32_t(lt_slot(ren, uint32_t(q))) == d);
+                        while (m) {
+                            const uint32_t l = uint32_t(__builtin_ctzll(m));
+                            m &= m - 1;
+                            const uint32_t yy = l + 64u * uint32_t(q);
+                            put(make(OP_COPY, MODE_RR, yy, d, 0, 0.0f, 0));
+                            lt_set(ren, yy, int32_t(yy), lane);
+                        }
+                    }
+                    lt_set(cnt, d, 0, lane);
                 }
-                for (uint32_t k = 0; k < 8u; ++k) P.pool[pos + k] = Clause{0, 0, 0, 0};
+                const uint32_t rd = uint32_t(lt_get(ren, d));
+                if (rd != d) {
+                    lt_set(cnt, rd, lt_get(cnt, rd) - 1, lane);
+                    lt_set(ren, d, int32_t(d), lane);
+                }
+            };
+            // registers that are aliases or aliased (S, four 64-bit words over
+            // the register numbers): a clause that reads or writes none of
+            // them and is no alias itself is emitted as it stands
+            uint64_t sp0 = 0, sp1 = 0, sp2 = 0, sp3 = 0;
+            auto recompute = [&]() {
+                auto word = [&](uint32_t q) {
+                    const uint32_t y = lane + 64u * q;
+                    return ballot(y < nregs && (uint32_t(lt_slot(ren, q)) != y ||
+                                                lt_slot(cnt, q) != 0));
+                };
+                sp0 = word(0);
+                sp1 = word(1);
+                sp2 = word(2);
+                sp3 = word(3);
+            };
+            // interval_push's step for one clause (uniform over the wave)
+            auto serial = [&](const Clause& c2, uint32_t ch) {
+                const uint32_t o = tape::op(c2);
+                const uint32_t m = mode(c2);
+                const uint32_t d = dst(c2);
+                bool alias = (o == OP_COPY);
+                bool ka = true;
+                if (o == OC::OP_MIN || o == OC::OP_MAX) {
+                    ka = (ch >> c) & 1u;
+                    alias = ka || ((ch >> (8 + c)) & 1u);
+                }
+                if (alias) {
+                    const bool from_imm = (o != OP_COPY) &&
+                                          (ka ? (m == MODE_A_IMM) : (m == MODE_B_IMM));
+                    if (from_imm) {
+                        will_write(d);
+                        put(make(OP_LOADI, MODE_A_IMM, d, 0, 0, imm(c2), slot(c2)));
+                        return;
+                    }
+                    const uint32_t r = uint32_t(lt_get(ren, ka ? ra(c2) : rb(c2)));
+                    if (r == d) return;
+                    if (lt_get(cnt, d) != 0) {
+                        will_write(d);
+                        put(make(OP_COPY, MODE_RR, d, r, 0, 0.0f, slot(c2)));
+                    } else {
+                        const uint32_t rd = uint32_t(lt_get(ren, d));
+                        if (rd != d) lt_set(cnt, rd, lt_get(cnt, rd) - 1, lane);
+                        lt_set(ren, d, int32_t(r), lane);
+                        lt_set(cnt, r, lt_get(cnt, r) + 1, lane);
+                    }
+                    return;
+                }
+                const bool use_a = nargs(o) >= 1 && m != MODE_A_IMM && o != OP_LOADI;
+                const bool use_b = is_binary(c2) && m != MODE_B_IMM;
+                const uint32_t a2 = use_a ? uint32_t(lt_get(ren, ra(c2))) : ra(c2);
+                const uint32_t b2 = use_b ? uint32_t(lt_get(ren, rb(c2))) : rb(c2);
+                will_write(d);
+                Clause outc = c2;
+                outc.w1 = (a2 & 0xffffu) | (b2 << 16);
+                put(outc);
+                if (o != OP_LOADI) ++refc;
+            };
+            for (uint32_t jb = 0; jb < nl; jb += 64) {
+                const uint32_t cntb = min(64u, nl - jb);
+                Clause mine = Clause{0, 0, 0, 0};
+                uint32_t chv = 0;
+                if (lane < cntb) {
+                    const uint32_t k = lst[jb + lane];
+                    mine = pool_in[off + k];
+                    chv = W.choice[k];
+                }
+                const uint32_t lo = tape::op(mine), lm = mode(mine);
+                bool lal = (lo == OP_COPY);
+                if (lo == OC::OP_MIN || lo == OC::OP_MAX) {
+                    lal = (((chv >> c) | (chv >> (8 + c))) & 1u) != 0;
+                }
+                const bool lua = nargs(lo) >= 1 && lm != MODE_A_IMM && lo != OP_LOADI;
+                const bool lub = is_binary(mine) && lm != MODE_B_IMM;
+                for (uint32_t q = 0; q < cntb;) {
+                    bool special = lal;
+                    if ((sp0 | sp1 | sp2 | sp3) != 0) {
+                        auto in_s = [&](uint32_t r) {
+                            const uint64_t w = r < 64u ? sp0 : r < 128u ? sp1 : r < 192u ? sp2 : sp3;
+                            return ((w >> (r & 63u)) & 1u) != 0;
+                        };
+                        special = special || (lua && in_s(ra(mine))) || (lub && in_s(rb(mine))) ||
+                                  in_s(dst(mine));
+                    }
+                    const bool ahead = lane >= q && lane < cntb;
+                    const uint64_t sm = ballot(ahead && special);
+                    const uint32_t e = sm ? uint32_t(__builtin_ctzll(sm)) : cntb;
+                    // clauses q .. e-1 as they stand
+                    const bool run = lane >= q && lane < e;
+                    if (run) P.pool[pos + lane - q] = mine;
+                    refc += uint32_t(__popcll(ballot(run && lo != OP_LOADI)));
+                    pos += e - q;
+                    if (e >= cntb) break;
+                    Clause c2;
+                    c2.w0 = __builtin_amdgcn_readlane(mine.w0, e);
+                    c2.w1 = __builtin_amdgcn_readlane(mine.w1, e);
+                    c2.w2 = __builtin_amdgcn_readlane(mine.w2, e);
+                    c2.w3 = __builtin_amdgcn_readlane(mine.w3, e);
+                    serial(c2, __builtin_amdgcn_readlane(chv, e));
+                    recompute();
+                    q = e + 1;
+                }
+            }
+            const uint32_t rr = uint32_t(lt_get(ren, root));
+            if (rr != root) put(make(OP_COPY, MODE_RR, root, rr, 0, 0.0f, 0));
+            if (lane < 8u) P.pool[pos + lane] = Clause{0, 0, 0, 0};
+            if (lane == 0) {
                 const uint32_t id = atomicAdd(&P.counters->tape_count, 1u);
                 if (id >= P.tape_cap) {
                     atomicOr(&P.counters->error, ERR_TAPES);

@@ -5,7 +5,8 @@
 #include <algorithm>
 #include <cassert>
 #include <stdexcept>
-#include <unordered_map>
+
+#include "flat_map.hpp"
 
 namespace libfive {
 
@@ -13,7 +14,7 @@ Deck::Deck(const Tree& root_) {
     tree = root_.optimized();
     const auto flat = tree.walk();
 
-    std::unordered_map<const TreeData*, uint32_t> ids;
+    FlatMap<const TreeData*, uint32_t> ids(flat.size() + 4);
     ids[nullptr] = 0;
     uint32_t id = uint32_t(flat.size());
     std::vector<RefClause> rev;
@@ -38,8 +39,8 @@ Deck::Deck(const Tree& root_) {
             default: {
                 const int n = Opcode::args(op);
                 rev.push_back({uint32_t(op), id,
-                               n >= 1 ? ids.at(m->lhs().get()) : 0u,
-                               n >= 2 ? ids.at(m->rhs().get()) : 0u});
+                               n >= 1 ? *ids.find(m->lhs().get()) : 0u,
+                               n >= 2 ? *ids.find(m->rhs().get()) : 0u});
                 break;
             }
         }
@@ -50,16 +51,16 @@ Deck::Deck(const Tree& root_) {
 
     const Tree axes[3] = {Tree::X(), Tree::Y(), Tree::Z()};
     for (const auto& a : axes) {
-        if (ids.find(a.get()) == ids.end()) {
+        if (ids.find(a.get()) == nullptr) {
             const uint32_t next = uint32_t(ids.size());
             ids[a.get()] = next;
         }
     }
     num_clauses = ids.size() - 1;
-    X = ids.at(axes[0].get());
-    Y = ids.at(axes[1].get());
-    Z = ids.at(axes[2].get());
-    root = ids.at(tree.get());
+    X = *ids.find(axes[0].get());
+    Y = *ids.find(axes[1].get());
+    Z = *ids.find(axes[2].get());
+    root = *ids.find(tree.get());
     assert(root == 1);
 }
 

@@ -100,7 +100,12 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
 // Waits for the stream with a time limit, so that a GPU step that does not
 // finish is reported by name instead of blocking the caller forever.
 void wait(hipStream_t s, const char* what) {
-    static const bool trace = std::getenv("LIBFIVE_MESH_TRACE") != nullptr;
+    // LIBFIVE_MESH_TRACE=wait names every wait; any other value prints the
+    // phase times of each render.
+    static const bool trace = [] {
+        const char* e = std::getenv("LIBFIVE_MESH_TRACE");
+        return e != nullptr && std::string(e) == "wait";
+    }();
     if (trace) {
         std::fprintf(stderr, "render_mesh: waiting for %s\n", what);
         std::fflush(stderr);
@@ -316,6 +321,7 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
         }
     }
     const DeviceTape base = compile_deck(deck, &var_values);
+    const double ms_front = ms_since(t_start);
     const uint32_t nregs = base.num_regs;
     if (nregs > 256) {
         throw Error("render_mesh: the tape needs " + std::to_string(nregs) +
@@ -611,6 +617,11 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
     if (ntris) parallel_memcpy(tdst, stage + vbytes, tbytes);
     g_stats.ms_emit = ms_since(t_emit);
     g_stats.ms_total = ms_since(t_start);
+    if (std::getenv("LIBFIVE_MESH_TRACE") != nullptr) {
+        std::fprintf(stderr, "render: front end %.1f ms, build %.1f ms, collapse %.1f ms, "
+                     "emit %.1f ms, total %.1f ms\n", ms_front, g_stats.ms_build - ms_front,
+                     g_stats.ms_collapse, g_stats.ms_emit, g_stats.ms_total);
+    }
     return true;
 }
 

@@ -15,6 +15,7 @@
 #include <stdexcept>
 #include <unordered_set>
 
+#include "flat_map.hpp"
 #include "fmath.hpp"
 
 namespace libfive {
@@ -400,7 +401,7 @@ size_t Tree::size() const {
 // (tree.cpp:530-594): count incoming edges, then emit root-first and release
 // a child once its last parent has been emitted; the list is reversed.
 std::vector<const Tree::Data*> Tree::walk() const {
-    std::unordered_map<const Data*, unsigned> count;
+    FlatMap<const Data*, unsigned> count(1024);
     std::vector<const Data*> todo{ptr};
     while (!todo.empty()) {
         auto d = todo.back();
@@ -416,8 +417,8 @@ std::vector<const Tree::Data*> Tree::walk() const {
         todo.pop_back();
         flat.push_back(d);
         const int n = Opcode::args(d->op());
-        if (n >= 1 && --count.at(d->lhs().get()) == 0) todo.push_back(d->lhs().get());
-        if (n >= 2 && --count.at(d->rhs().get()) == 0) todo.push_back(d->rhs().get());
+        if (n >= 1 && --*count.find(d->lhs().get()) == 0) todo.push_back(d->lhs().get());
+        if (n >= 2 && --*count.find(d->rhs().get()) == 0) todo.push_back(d->rhs().get());
     }
     std::reverse(flat.begin(), flat.end());
     return flat;
@@ -449,11 +450,10 @@ struct NodeKey {
 };
 struct NodeKeyHash {
     size_t operator()(const NodeKey& k) const {
-        size_t h = std::hash<int>()(k.op);
-        h = h * 1000003u ^ std::hash<uint32_t>()(k.value_bits);
-        h = h * 1000003u ^ std::hash<const void*>()(k.a);
-        h = h * 1000003u ^ std::hash<const void*>()(k.b);
-        return h;
+        uint64_t h = uint64_t(uint32_t(k.op)) * 0x9e3779b97f4a7c15ull ^ k.value_bits;
+        h = h * 0x100000001b3ull ^ uint64_t(uintptr_t(k.a));
+        h = h * 0x100000001b3ull ^ uint64_t(uintptr_t(k.b));
+        return mix_hash(h);
     }
 };
 
@@ -480,9 +480,9 @@ public:
 
     Tree uniq(const Tree& t) {
         auto k = key_of(t.get());
-        auto it = canonical.find(k);
-        if (it != canonical.end()) return it->second;
-        canonical.emplace(k, t);
+        if (const TreeData* const* hit = canonical.find(k)) return Tree(*hit);
+        canonical.emplace(k, t.get());
+        owned.push_back(t);
         return t;
     }
 
@@ -490,10 +490,26 @@ public:
     // reference iterates an unordered_map, then sorts; the sort makes the
     // iteration order irrelevant except for exact ties, which cannot occur
     // because keys are unique).
+    // Most maps hold a handful of terms: they are found by a linear scan,
+    // and an index is built only for large maps.
     struct Affine {
+        static constexpr size_t kScan = 16;
         std::vector<std::pair<Tree, float>> terms;
         std::unordered_map<const TreeData*, size_t> index;
         void add(const Tree& t, float s) {
+            if (terms.size() < kScan) {
+                for (auto& p : terms) {
+                    if (p.first.get() == t.get()) {
+                        p.second += s;
+                        return;
+                    }
+                }
+                terms.push_back({t, 0.0f + s});
+                return;
+            }
+            if (index.empty()) {
+                for (size_t i = 0; i < terms.size(); ++i) index.emplace(terms[i].first.get(), i);
+            }
             auto it = index.find(t.get());
             if (it == index.end()) {
                 index.emplace(t.get(), terms.size());
@@ -565,10 +581,18 @@ public:
     // commutative group root.  The reference re-traverses shared subtrees
     // every time they are reached; reusing the first result gives the same
     // canonical nodes without the exponential re-traversal of shared DAGs.
-    std::unordered_map<const TreeData*, Tree> memo_plain, memo_affine,
-        memo_comm;
+    // The tables hold plain node pointers (cheap to rehash); every node in
+    // them is canonical and kept alive by `owned`.
+    FlatMap<const TreeData*, const TreeData*> memo_plain{1024}, memo_affine{1024},
+        memo_comm{256};
+
+    // Recursion guard: when set, visiting below this stack address throws
+    // TooDeep (the caller then reruns the pass on a larger stack).
+    struct TooDeep {};
+    uintptr_t stack_floor = 0;
 
     Tree visit(const TreeData* d, float scale, Ctx parent) {
+        if (stack_floor && uintptr_t(__builtin_frame_address(0)) < stack_floor) throw TooDeep{};
         float sl = 1.0f, sr = 1.0f;
         int which = 0;
         if (is_affine(d, &sl, &sr, scale, &which)) {
@@ -578,13 +602,12 @@ public:
                 if (which & 1) visit(d->lhs().get(), sl, parent);
                 return Tree();
             }
-            auto hit = memo_affine.find(d);
-            if (hit != memo_affine.end()) {
+            if (const TreeData* const* hit = memo_affine.find(d)) {
                 if (parent.comm) {
-                    parent.comm->list.push_back(hit->second);
+                    parent.comm->list.push_back(Tree(*hit));
                     return Tree();
                 }
-                return hit->second;
+                return Tree(*hit);
             }
             Affine map;
             Ctx c;
@@ -592,7 +615,7 @@ public:
             if (which & 2) visit(d->rhs().get(), sr, c);
             if (which & 1) visit(d->lhs().get(), sl, c);
             Tree t = collapse_affine(map);
-            memo_affine.emplace(d, t);
+            memo_affine.emplace(d, t.get());
             // An affine root always arrives with scale 1: its parent is
             // either a plain node or a commutative list.
             if (parent.comm) {
@@ -607,20 +630,22 @@ public:
                 visit(d->lhs().get(), 1.0f, parent);
                 return Tree();
             }
-            auto hit = memo_comm.find(d);
-            if (hit != memo_comm.end()) return deliver(hit->second, scale, parent);
+            if (const TreeData* const* hit = memo_comm.find(d)) {
+                return deliver(Tree(*hit), scale, parent);
+            }
             Comm list{d->op(), {}};
             Ctx c;
             c.comm = &list;
             visit(d->rhs().get(), 1.0f, c);
             visit(d->lhs().get(), 1.0f, c);
             Tree t = fold_comm(list);
-            memo_comm.emplace(d, t);
+            memo_comm.emplace(d, t.get());
             return deliver(t, scale, parent);
         }
         // Plain node.
-        auto hit = memo_plain.find(d);
-        if (hit != memo_plain.end()) return deliver(hit->second, scale, parent);
+        if (const TreeData* const* hit = memo_plain.find(d)) {
+            return deliver(Tree(*hit), scale, parent);
+        }
         const int n = Opcode::args(d->op());
         Tree self;
         if (n == 1) {
@@ -636,7 +661,7 @@ public:
         } else {
             self = uniq(Tree(d));
         }
-        memo_plain.emplace(d, self);
+        memo_plain.emplace(d, self.get());
         return deliver(self, scale, parent);
     }
 
@@ -705,7 +730,8 @@ public:
         return out;
     }
 
-    std::unordered_map<NodeKey, Tree, NodeKeyHash> canonical;
+    FlatMap<NodeKey, const TreeData*, NodeKeyHash> canonical{1024};
+    std::vector<Tree> owned;
 };
 
 // The optimiser recurses once per tree level; a chain tens of thousands of
@@ -732,6 +758,18 @@ void* optimize_entry(void* arg) {
 
 Tree Tree::optimized() const {
     if (flags & TREE_FLAG_IS_OPTIMIZED) return *this;
+    // Most trees are shallow: the pass runs on the calling thread with at
+    // most 1 MiB of its stack, and moves to the helper thread only when the
+    // recursion goes deeper.
+    {
+        Optimizer opt;
+        opt.stack_floor = uintptr_t(__builtin_frame_address(0)) - (uintptr_t(1) << 20);
+        try {
+            Tree t = opt.visit(ptr, 1.0f, Optimizer::Ctx());
+            return t.with_flags(TREE_FLAG_IS_OPTIMIZED);
+        } catch (const Optimizer::TooDeep&) {
+        }
+    }
     OptimizeJob job{ptr, Tree(), nullptr};
     pthread_attr_t attr;
     pthread_t th;

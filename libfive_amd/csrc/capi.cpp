@@ -70,14 +70,66 @@ const char* libfive_last_error(void) { return g_last_error.c_str(); }
 
 int libfive_gpu_available(void) { return gpu_available() ? 1 : 0; }
 
+// Mesh arrays.  A large mesh lives in one pooled page-locked block (the
+// vertices, then the triangles at the next 256-byte boundary) that the
+// device copies into directly and libfive_mesh_delete hands back to the
+// pool; a small one uses new[], as the reference does.
+static std::mutex g_blocks_mu;
+static std::map<const libfive_mesh*, std::pair<void*, size_t>> g_blocks;
+constexpr size_t kPinnedMeshMin = size_t(4) << 20;
+
+static bool alloc_mesh_arrays(libfive_mesh* m, size_t nv, size_t nt, float** v,
+                              uint32_t** t) {
+    const size_t vb = nv * sizeof(libfive_vec3);
+    const size_t tb = std::max<size_t>(nt, 1) * sizeof(libfive_tri);
+    const size_t toff = (vb + 255) & ~size_t(255);
+    bool pinned = false;
+    if (toff + tb >= kPinnedMeshMin) {
+        size_t granted = 0;
+        char* p = static_cast<char*>(pinned_alloc(toff + tb, &granted));
+        {
+            std::lock_guard<std::mutex> lock(g_blocks_mu);
+            g_blocks[m] = {p, granted};
+        }
+        m->verts = reinterpret_cast<libfive_vec3*>(p);
+        m->tris = reinterpret_cast<libfive_tri*>(p + toff);
+        pinned = true;
+    } else {
+        m->verts = new libfive_vec3[nv];
+        m->tris = new libfive_tri[nt ? nt : 1];
+    }
+    m->vert_count = uint32_t(nv);
+    m->tri_count = uint32_t(nt);
+    *v = &m->verts[0].x;
+    *t = &m->tris[0].a;
+    return pinned;
+}
+
+static void free_mesh_arrays(libfive_mesh* m) {
+    {
+        std::lock_guard<std::mutex> lock(g_blocks_mu);
+        auto it = g_blocks.find(m);
+        if (it != g_blocks.end()) {
+            pinned_free(it->second.first, it->second.second);
+            g_blocks.erase(it);
+            m->verts = nullptr;
+            m->tris = nullptr;
+            return;
+        }
+    }
+    delete[] m->verts;
+    delete[] m->tris;
+    m->verts = nullptr;
+    m->tris = nullptr;
+}
+
 void libfive_mesh_delete(libfive_mesh* m) {
     if (!m) return;
     {
         std::lock_guard<std::mutex> lock(g_keys_mu);
         g_keys.erase(m);
     }
-    delete[] m->verts;
-    delete[] m->tris;
+    free_mesh_arrays(m);
     delete m;
 }
 
@@ -705,9 +757,9 @@ void libfive_last_mesh_times(double* ms4) {
     ms4[3] = st.ms_total;
 }
 
-// Renders into a new libfive_mesh whose arrays (new[], as
-// libfive_mesh_delete expects) receive the copy from the device directly.
-// Returns NULL when cancelled.
+// Renders into a new libfive_mesh whose arrays (alloc_mesh_arrays, released
+// by libfive_mesh_delete) receive the copy from the device.  Returns NULL
+// when cancelled.
 static libfive_mesh* render_to_c(const Tree& tree, const double lo[3],
                                  const double hi[3], const MeshSettings& ms,
                                  bool keys, hipStream_t* stream) {
@@ -715,21 +767,13 @@ static libfive_mesh* render_to_c(const Tree& tree, const double lo[3],
     struct Arrays {
         libfive_mesh* m;
         ~Arrays() {
-            if (m) {
-                delete[] m->verts;
-                delete[] m->tris;
-            }
+            if (m) free_mesh_arrays(m);
         }
     } arrays{out.get()};
     HostMesh hm;
     hm.want_keys = keys;
     hm.alloc = [&](size_t nv, size_t nt, float** v, uint32_t** t) {
-        out->verts = new libfive_vec3[nv];
-        out->tris = new libfive_tri[nt ? nt : 1];
-        out->vert_count = uint32_t(nv);
-        out->tri_count = uint32_t(nt);
-        *v = &out->verts[0].x;
-        *t = &out->tris[0].a;
+        return alloc_mesh_arrays(out.get(), nv, nt, v, t);
     };
     const bool done = stream ? render_mesh_on(tree, lo, hi, ms, hm, *stream)
                              : render_mesh(tree, lo, hi, ms, hm);
@@ -843,8 +887,7 @@ int libfive_trees_render_mesh_merged(const libfive_tree* trees,
     auto free_arrays = [&]() {
         for (auto& m : meshes) {
             if (m) {
-                delete[] m->verts;
-                delete[] m->tris;
+                free_mesh_arrays(m.get());
                 m.reset();
             }
         }
@@ -865,12 +908,7 @@ int libfive_trees_render_mesh_merged(const libfive_tree* trees,
             hms[i].want_keys = false;
             libfive_mesh* m = meshes[i].get();
             hms[i].alloc = [m](size_t nv, size_t nt, float** v, uint32_t** t) {
-                m->verts = new libfive_vec3[nv];
-                m->tris = new libfive_tri[nt ? nt : 1];
-                m->vert_count = uint32_t(nv);
-                m->tri_count = uint32_t(nt);
-                *v = &m->verts[0].x;
-                *t = &m->tris[0].a;
+                return alloc_mesh_arrays(m, nv, nt, v, t);
             };
         }
         Context& ctx = context();

@@ -39,6 +39,7 @@ std::unique_ptr<Mesh> render_impl(const Tree& t,
             ntris = nt;
             *v = out->verts[0].v;
             *tr = out->branes[0].v;
+            return false;
         };
         if (!amd::render_mesh(t, lo, hi, ms, hm)) return nullptr;
         out->branes.resize(ntris);

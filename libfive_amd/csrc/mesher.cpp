@@ -555,6 +555,7 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
     hip_check(hipMemcpyAsync(&tail[3], vslot.p + vslot.n - 1, 4, hipMemcpyDeviceToHost, s), "tail");
     hip_check(hipMemcpyAsync(&hc, counters.p, sizeof(hc), hipMemcpyDeviceToHost, s), "counters");
     wait(s, "quad count");
+    const double ms_counted = ms_since(t_emit);
     check_errors(hc.error);
     if (settings.cancelled()) return false;
     const uint64_t ntris64 = uint64_t(tail[0]) + tail[1];
@@ -582,30 +583,36 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
     }
     float* vdst = nullptr;
     uint32_t* tdst = nullptr;
+    bool dst_pinned = false;
     if (out.alloc) {
-        out.alloc(nverts, ntris, &vdst, &tdst);
+        dst_pinned = out.alloc(nverts, ntris, &vdst, &tdst);
     } else {
         out.verts.resize(3 * size_t(nverts));
         out.tris.resize(3 * size_t(ntris));
         vdst = out.verts.data();
         tdst = out.tris.data();
     }
-    // Device -> pinned staging at full link speed, then host threads copy
-    // into the caller's (pageable) arrays.
+    const double ms_alloced = ms_since(t_emit);
+    // Device -> page-locked memory at full link speed: the caller's arrays
+    // when they are page-locked, else a staging block that host threads
+    // then copy into the caller's (pageable) arrays.
     const size_t vbytes = 3 * size_t(nverts) * sizeof(float);
     const size_t tbytes = 3 * size_t(ntris) * sizeof(uint32_t);
     size_t granted = 0;
-    char* stage = static_cast<char*>(pinned_alloc(vbytes + tbytes, &granted));
+    char* stage = dst_pinned ? nullptr
+                             : static_cast<char*>(pinned_alloc(vbytes + tbytes, &granted));
     struct StageRelease {
         char* p;
         size_t n;
-        ~StageRelease() { pinned_free(p, n); }
+        ~StageRelease() {
+            if (p) pinned_free(p, n);
+        }
     } stage_release{stage, granted};
-    hip_check(hipMemcpyAsync(stage, d_verts.p, vbytes, hipMemcpyDeviceToHost, s),
-              "copy verts");
+    char* vto = dst_pinned ? reinterpret_cast<char*>(vdst) : stage;
+    char* tto = dst_pinned ? reinterpret_cast<char*>(tdst) : stage + vbytes;
+    hip_check(hipMemcpyAsync(vto, d_verts.p, vbytes, hipMemcpyDeviceToHost, s), "copy verts");
     if (ntris) {
-        hip_check(hipMemcpyAsync(stage + vbytes, d_tris.p, tbytes,
-                                 hipMemcpyDeviceToHost, s), "copy tris");
+        hip_check(hipMemcpyAsync(tto, d_tris.p, tbytes, hipMemcpyDeviceToHost, s), "copy tris");
     }
     if (out.want_keys) {
         out.keys.resize(5 * size_t(nverts));
@@ -613,14 +620,19 @@ static bool render_body(const Tree& tree, const double lo[3], const double hi[3]
                                  hipMemcpyDeviceToHost, s), "copy keys");
     }
     wait(s, "mesh copy");
-    parallel_memcpy(vdst, stage, vbytes);
-    if (ntris) parallel_memcpy(tdst, stage + vbytes, tbytes);
+    const double ms_copied = ms_since(t_emit);
+    if (!dst_pinned) {
+        parallel_memcpy(vdst, stage, vbytes);
+        if (ntris) parallel_memcpy(tdst, stage + vbytes, tbytes);
+    }
     g_stats.ms_emit = ms_since(t_emit);
     g_stats.ms_total = ms_since(t_start);
     if (std::getenv("LIBFIVE_MESH_TRACE") != nullptr) {
         std::fprintf(stderr, "render: front end %.1f ms, build %.1f ms, collapse %.1f ms, "
-                     "emit %.1f ms, total %.1f ms\n", ms_front, g_stats.ms_build - ms_front,
-                     g_stats.ms_collapse, g_stats.ms_emit, g_stats.ms_total);
+                     "emit %.1f ms (counted %.1f, arrays %.1f, copied %.1f; %u verts, %u tris), "
+                     "total %.1f ms\n", ms_front, g_stats.ms_build - ms_front,
+                     g_stats.ms_collapse, g_stats.ms_emit, ms_counted, ms_alloced, ms_copied,
+                     nverts, ntris, g_stats.ms_total);
     }
     return true;
 }

@@ -44,8 +44,9 @@ struct HostMesh {
     // When set, called once with the vertex count (dummy included) and the
     // triangle count; it returns the host buffers (3 floats per vertex,
     // 3 indices per triangle) the mesh is copied into, and verts / tris
-    // stay empty.
-    std::function<void(size_t, size_t, float**, uint32_t**)> alloc;
+    // stay empty.  It returns true when both buffers are page-locked host
+    // memory, which the device then writes directly.
+    std::function<bool(size_t, size_t, float**, uint32_t**)> alloc;
     bool want_keys = true;          // fill keys
 };
 

@@ -89,6 +89,7 @@ void pool_free(void* p, size_t granted) {
 namespace {
 std::mutex g_pinned_mu;
 std::multimap<size_t, void*> g_pinned;
+size_t g_pinned_bytes = 0;               // bytes held in g_pinned
 }
 
 void* pinned_alloc(size_t bytes, size_t* granted) {
@@ -101,6 +102,7 @@ void* pinned_alloc(size_t bytes, size_t* granted) {
         if (it != g_pinned.end()) {
             void* p = it->second;
             g_pinned.erase(it);
+            g_pinned_bytes -= c;
             return p;
         }
     }
@@ -115,7 +117,12 @@ void pinned_free(void* p, size_t granted) {
         return;
     }
     std::lock_guard<std::mutex> lock(g_pinned_mu);
+    if (g_pinned_bytes + granted > kPinnedKeep) {
+        (void)hipHostFree(p);
+        return;
+    }
     g_pinned.emplace(granted, p);
+    g_pinned_bytes += granted;
 }
 
 void parallel_memcpy(void* dst, const void* src, size_t bytes) {

@@ -112,6 +112,9 @@ struct PoolBuf {
 };
 
 // Pinned host staging blocks (hipHostMalloc), cached like pool_alloc's.
+// Page-locked host blocks (power-of-two size classes from 1 MiB), pooled
+// like the device blocks; at most kPinnedKeep bytes stay cached.
+constexpr size_t kPinnedKeep = size_t(8) << 30;
 void* pinned_alloc(size_t bytes, size_t* granted);
 void pinned_free(void* p, size_t granted);
 

@@ -758,7 +758,13 @@ void merged_body(const BatchItem* items, const std::vector<size_t>& which, HostM
                                  hipMemcpyHostToDevice, s), "upload tape records");
         wait(s, "upload tapes");
     }
+    // Every record is on the device from the start (k_child_ranges reads
+    // the top level of all models at every level); each level's pass then
+    // re-uploads the records of the models rooted there, with their root
+    // cells set.
     PoolBuf<ModelRec> d_models(nm);
+    hip_check(hipMemcpyAsync(d_models.p, hmodels.data(), nm * sizeof(ModelRec),
+                             hipMemcpyHostToDevice, s), "model records");
     PoolBuf<Counters> counters(1);
     Counters hc{};
     hc.pool_top = uint32_t(pool_clauses);
@@ -906,7 +912,11 @@ void merged_body(const BatchItem* items, const std::vector<size_t>& which, HostM
         hmodels[k].slot0 = uint32_t(4 * total_cells);
         for (int L = 0; L <= top; ++L) {
             const uint32_t b = hr[size_t(L) * 2 * nm + k], e = hr[size_t(L) * 2 * nm + nm + k];
-            if (e < b || e > levels[L].n) throw Error("render_mesh: inconsistent cell ranges");
+            if (e < b || e > levels[L].n) {
+                throw Error("render_mesh: inconsistent cell ranges (level " + std::to_string(L) +
+                            ", model " + std::to_string(k) + ": " + std::to_string(b) + ".." +
+                            std::to_string(e) + " of " + std::to_string(levels[L].n) + ")");
+            }
             // index i of the level maps to total_cells + (i - b)
             cbase[size_t(L) * nm + k] = uint32_t(total_cells - b);
             total_cells += e - b;

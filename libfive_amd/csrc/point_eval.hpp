@@ -33,6 +33,26 @@ struct Clause4x {
     tape::Clause c[tape::kFetch];
 };
 
+// Clause group g of a tape, read through the constant address space: no
+// kernel writes a tape it evaluates, and the tape pointer is wave-uniform,
+// so the reads are scalar loads (through a generic pointer the compiler
+// must assume the kernel's own stores may alias the tape, and reads it
+// with vector loads).  Tapes are followed by at least kFetch clauses of
+// padding, so group g + 1 of a tape can always be read ahead.
+typedef const __attribute__((address_space(4))) uint32_t* ConstWordPtr;
+__device__ __forceinline__ Clause4x pe_fetch4(const tape::Clause* t, uint32_t g) {
+    const ConstWordPtr w = (ConstWordPtr)(t) + 4u * tape::kFetch * g;
+    Clause4x r;
+#pragma unroll
+    for (uint32_t k = 0; k < tape::kFetch; ++k) {
+        r.c[k].w0 = w[4u * k];
+        r.c[k].w1 = w[4u * k + 1u];
+        r.c[k].w2 = w[4u * k + 2u];
+        r.c[k].w3 = w[4u * k + 3u];
+    }
+    return r;
+}
+
 template <bool FAST = false>
 __device__ __forceinline__ float pe_value_op(uint32_t o, float a, float b,
                                              float im, uint32_t& flags) {
@@ -126,13 +146,14 @@ __device__ __forceinline__ float pe_value(const tape::Clause* __restrict__ t,
     LF_RF_SET(r, NV, tape::kRegY, y);
     LF_RF_SET(r, NV, tape::kRegZ, z);
     uint32_t flags = 0;
-    const Clause4x* t4 = reinterpret_cast<const Clause4x*>(t);
+    Clause4x cur = pe_fetch4(t, 0);
     for (uint32_t k = 0; k < n; k += tape::kFetch) {
-        const Clause4x cur = t4[k / tape::kFetch];
+        const Clause4x nxt = pe_fetch4(t, k / tape::kFetch + 1);
         LF_PE_VALUE_STEP(cur.c[0]);
         LF_PE_VALUE_STEP(cur.c[1]);
         LF_PE_VALUE_STEP(cur.c[2]);
         LF_PE_VALUE_STEP(cur.c[3]);
+        cur = nxt;
     }
     *out_flags = flags;
     return LF_RF_GET(r, NV, root);
@@ -413,13 +434,14 @@ __device__ __forceinline__ float pe_deriv(const tape::Clause* __restrict__ t,
     LF_RF_SET(gy, NV, tape::kRegY, 1.0f);
     LF_RF_SET(gz, NV, tape::kRegZ, 1.0f);
     uint32_t flags = 0;
-    const Clause4x* t4 = reinterpret_cast<const Clause4x*>(t);
+    Clause4x cur = pe_fetch4(t, 0);
     for (uint32_t k = 0; k < n; k += tape::kFetch) {
-        const Clause4x cur = t4[k / tape::kFetch];
+        const Clause4x nxt = pe_fetch4(t, k / tape::kFetch + 1);
         LF_PE_DERIV_STEP(cur.c[0]);
         LF_PE_DERIV_STEP(cur.c[1]);
         LF_PE_DERIV_STEP(cur.c[2]);
         LF_PE_DERIV_STEP(cur.c[3]);
+        cur = nxt;
     }
     *out_flags = flags;
     g[0] = LF_RF_GET(gx, NV, root);
@@ -478,13 +500,14 @@ __device__ __forceinline__ float pe_deriv1(const tape::Clause* __restrict__ t,
     LF_RF_SET(gd, NV, tape::kRegY, axis == 1 ? 1.0f : 0.0f);
     LF_RF_SET(gd, NV, tape::kRegZ, axis == 2 ? 1.0f : 0.0f);
     uint32_t flags = 0;
-    const Clause4x* t4 = reinterpret_cast<const Clause4x*>(t);
+    Clause4x cur = pe_fetch4(t, 0);
     for (uint32_t k = 0; k < n; k += tape::kFetch) {
-        const Clause4x cur = t4[k / tape::kFetch];
+        const Clause4x nxt = pe_fetch4(t, k / tape::kFetch + 1);
         LF_PE_DERIV1_STEP(cur.c[0]);
         LF_PE_DERIV1_STEP(cur.c[1]);
         LF_PE_DERIV1_STEP(cur.c[2]);
         LF_PE_DERIV1_STEP(cur.c[3]);
+        cur = nxt;
     }
     *out_flags = flags;
     *g = LF_RF_GET(gd, NV, root);

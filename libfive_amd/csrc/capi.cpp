@@ -233,7 +233,7 @@ static float eval_point(libfive_tree t, libfive_vec3 p, float grad[3]) {
     const auto& h = gt->host;
     if (grad) {
         hip_check(launch_eval_derivs(gt->d.p, gt->size(), h.root_reg,
-                                     h.num_regs, d, d + 1, d + 2, d + 4, d + 5,
+                                     h.num_regs, tape_is_fast(h), d, d + 1, d + 2, d + 4, d + 5,
                                      d + 6, d + 7, 1, c.stream),
                   "eval_d");
     } else {
@@ -325,7 +325,7 @@ int libfive_tree_eval_deriv_array(libfive_tree t, const float* x,
         hip_check(hipMemcpyAsync(p + n, y, b, hipMemcpyHostToDevice, c.stream), "copy y");
         hip_check(hipMemcpyAsync(p + 2 * n, z, b, hipMemcpyHostToDevice, c.stream), "copy z");
         hip_check(launch_eval_derivs(gt->d.p, gt->size(), gt->host.root_reg,
-                                     gt->host.num_regs, p, p + n, p + 2 * n,
+                                     gt->host.num_regs, tape_is_fast(gt->host), p, p + n, p + 2 * n,
                                      p + 3 * n, p + 4 * n, p + 5 * n, p + 6 * n,
                                      n, c.stream),
                   "eval_derivs");
@@ -517,7 +517,7 @@ int libfive_tape_eval_derivs_device(libfive_tape t, const float* x,
             return 0;
         }
         hip_check(launch_eval_derivs(t->t->d.p, t->t->size(), h.root_reg,
-                                     h.num_regs, x, y, z, value, dx, dy, dz, n,
+                                     h.num_regs, tape_is_fast(h), x, y, z, value, dx, dy, dz, n,
                                      s),
                   "eval_derivs");
         return 0;

@@ -47,6 +47,15 @@ struct DeviceTape {
     uint32_t ref_length = 0;             // clause count excluding COPY/LOADI
 };
 
+// True when every clause has one of the common opcodes (tape::fast_opcode),
+// which the call-free device evaluators handle.
+inline bool tape_is_fast(const DeviceTape& t) {
+    for (const tape::Clause& c : t.clauses) {
+        if (!tape::fast_opcode(tape::op(c))) return false;
+    }
+    return true;
+}
+
 // Compiles a Deck: renames slots to registers (backward pass).
 DeviceTape compile_deck(const Deck& deck,
                         const std::vector<float>* var_values = nullptr);

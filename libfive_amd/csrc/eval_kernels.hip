@@ -20,6 +20,7 @@
 #include "regfile.hpp"
 #include "tape.hpp"
 #include "clause_ops.hpp"
+#include "point_eval.hpp"
 
 namespace libfive {
 namespace amd {
@@ -116,46 +117,27 @@ void k_eval_values(const Clause* __restrict__ t, uint32_t n, uint32_t root,
     }
 }
 
-#define LF_DERIV_STEP(C)                                                     \
-    do {                                                                     \
-        const Clause c_ = (C);                                               \
-        const uint32_t o_ = tape::op(c_);                                    \
-        if (o_ != OP_NOP) {                                                  \
-            const uint32_t m_ = mode(c_);                                    \
-            const float im_ = imm(c_);                                       \
-            float av_ = im_, ad_[3] = {0.0f, 0.0f, 0.0f};                    \
-            float bv_ = 0.0f, bd_[3] = {0.0f, 0.0f, 0.0f};                   \
-            if (m_ != MODE_A_IMM && o_ != OP_LOADI) {                        \
-                const uint32_t a_ = ra(c_);                                  \
-                av_ = LF_RF_GET(v, NV, a_);                                  \
-                ad_[0] = LF_RF_GET(gx, NV, a_);                              \
-                ad_[1] = LF_RF_GET(gy, NV, a_);                              \
-                ad_[2] = LF_RF_GET(gz, NV, a_);                              \
-            }                                                                \
-            if (is_binary(c_)) {                                             \
-                if (m_ == MODE_B_IMM) {                                      \
-                    bv_ = im_;                                               \
-                } else {                                                     \
-                    const uint32_t b_ = rb(c_);                              \
-                    bv_ = LF_RF_GET(v, NV, b_);                              \
-                    bd_[0] = LF_RF_GET(gx, NV, b_);                          \
-                    bd_[1] = LF_RF_GET(gy, NV, b_);                          \
-                    bd_[2] = LF_RF_GET(gz, NV, b_);                          \
-                }                                                            \
-            }                                                                \
-            float ov_, od_[3];                                               \
-            apply_deriv(o_, av_, ad_, bv_, bd_, im_, ov_, od_);              \
-            const uint32_t d_ = dst(c_);                                     \
-            LF_RF_SET(v, NV, d_, ov_);                                       \
-            LF_RF_SET(gx, NV, d_, od_[0]);                                   \
-            LF_RF_SET(gy, NV, d_, od_[1]);                                   \
-            LF_RF_SET(gz, NV, d_, od_[2]);                                   \
-        }                                                                    \
-    } while (0)
+// Values and one partial derivative per launch row (blockIdx.y is the
+// axis): the register file is two files (values, that partial) instead of
+// four, so that it stays in VGPRs with several waves per SIMD; the values
+// are computed in every row and written by row 0.  Each component is
+// bit-identical to a pass carrying all three partials (the rules of
+// eval_deriv_array.cpp:119-256 are per component).
+//
+// FAST: the tape has only the common opcodes, and the kernel calls no
+// function (no stack, no registers held around calls).  For a point it
+// cannot evaluate (a sin / cos argument of 2^17 or above; every row sees
+// the same values, so every row agrees), row 0 writes kRedoMark as the
+// value, and k_eval_deriv1_redo evaluates the marked points again with
+// the general rules.  A point whose true value has the marker's bits is
+// evaluated twice, with the same result.  With two banks the fast form
+// fits in 168 VGPRs (three waves per SIMD) when asked to, without spilling.
+constexpr uint32_t kRedoMark = 0x7fa5a5a5u;    // a signalling NaN: no arithmetic result
 
-template <int NV>
+template <int NV, bool FAST>
 __global__ __launch_bounds__(256)
-void k_eval_derivs(const Clause* __restrict__ t, uint32_t n, uint32_t root,
+__attribute__((amdgpu_waves_per_eu((FAST && NV == 2) ? 3 : 1)))
+void k_eval_deriv1(const Clause* __restrict__ t, uint32_t n, uint32_t root,
                    const float* __restrict__ xs, const float* __restrict__ ys,
                    const float* __restrict__ zs, float* __restrict__ ov_out,
                    float* __restrict__ odx, float* __restrict__ ody,
@@ -163,32 +145,45 @@ void k_eval_derivs(const Clause* __restrict__ t, uint32_t n, uint32_t root,
 {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool ok = i < npts;
-    LF_RF_DECL(v);
-    LF_RF_DECL(gx);
-    LF_RF_DECL(gy);
-    LF_RF_DECL(gz);
-    LF_RF_SET(v, NV, kRegX, ok ? xs[i] : 0.0f);
-    LF_RF_SET(v, NV, kRegY, ok ? ys[i] : 0.0f);
-    LF_RF_SET(v, NV, kRegZ, ok ? zs[i] : 0.0f);
-    LF_RF_SET(gx, NV, kRegX, 1.0f);
-    LF_RF_SET(gy, NV, kRegY, 1.0f);
-    LF_RF_SET(gz, NV, kRegZ, 1.0f);
-    const Clause4* t4 = reinterpret_cast<const Clause4*>(t);
-    Clause4 cur = t4[0];
-    for (uint32_t k = 0; k < n; k += kFetch) {
-        const Clause4 nxt = t4[k / kFetch + 1];
-        LF_DERIV_STEP(cur.c[0]);
-        LF_DERIV_STEP(cur.c[1]);
-        LF_DERIV_STEP(cur.c[2]);
-        LF_DERIV_STEP(cur.c[3]);
-        cur = nxt;
-    }
+    const int axis = int(blockIdx.y);
+    const float x = ok ? xs[i] : 0.0f;
+    const float y = ok ? ys[i] : 0.0f;
+    const float z = ok ? zs[i] : 0.0f;
+    uint32_t flags = 0;
+    float g = 0.0f;
+    const float v = pe_deriv1<NV, FAST>(t, n, root, x, y, z, axis, &g, &flags);
     if (ok) {
-        ov_out[i] = LF_RF_GET(v, NV, root);
-        odx[i] = LF_RF_GET(gx, NV, root);
-        ody[i] = LF_RF_GET(gy, NV, root);
-        odz[i] = LF_RF_GET(gz, NV, root);
+        if (axis == 0) {
+            ov_out[i] = (FAST && (flags & PT_SLOW)) ? fmath::u2f(kRedoMark) : v;
+        }
+        float* o = (axis == 0) ? odx : (axis == 1) ? ody : odz;
+        o[i] = g;
     }
+}
+
+// The points k_eval_deriv1<NV, true> marked, all three components, with
+// the general rules (one lane reads the marker before it overwrites it).
+template <int NV>
+__global__ __launch_bounds__(256)
+void k_eval_deriv1_redo(const Clause* __restrict__ t, uint32_t n, uint32_t root,
+                        const float* __restrict__ xs, const float* __restrict__ ys,
+                        const float* __restrict__ zs, float* __restrict__ ov_out,
+                        float* __restrict__ odx, float* __restrict__ ody,
+                        float* __restrict__ odz, size_t npts)
+{
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= npts || fmath::f2u(ov_out[i]) != kRedoMark) return;
+    const float x = xs[i], y = ys[i], z = zs[i];
+    float v = 0.0f;
+#pragma unroll 1
+    for (int axis = 0; axis < 3; ++axis) {
+        uint32_t flags = 0;
+        float g = 0.0f;
+        v = pe_deriv1<NV, false>(t, n, root, x, y, z, axis, &g, &flags);
+        float* o = (axis == 0) ? odx : (axis == 1) ? ody : odz;
+        o[i] = g;
+    }
+    ov_out[i] = v;
 }
 
 ////////////////////////////////////////////////////////////////////////////////
@@ -360,20 +355,36 @@ hipError_t launch_eval_values_ppl(const Clause* t, uint32_t n, uint32_t root,
 }
 
 hipError_t launch_eval_derivs(const Clause* t, uint32_t n, uint32_t root,
-                              uint32_t nregs, const float* x, const float* y,
-                              const float* z, float* v, float* dx, float* dy,
-                              float* dz, size_t npts, hipStream_t s)
+                              uint32_t nregs, bool fast, const float* x,
+                              const float* y, const float* z, float* v,
+                              float* dx, float* dy, float* dz, size_t npts,
+                              hipStream_t s)
 {
     if (npts == 0) return hipSuccess;
+    const int nv = banks_for(nregs);
+    if (nv == 0) return hipErrorInvalidValue;
     const dim3 block(256);
-    const dim3 grid(unsigned((npts + 255) / 256));
-    switch (banks_for(nregs)) {
-        case 1: k_eval_derivs<1><<<grid, block, 0, s>>>(t, n, root, x, y, z, v, dx, dy, dz, npts); break;
-        case 2: k_eval_derivs<2><<<grid, block, 0, s>>>(t, n, root, x, y, z, v, dx, dy, dz, npts); break;
-        case 4: k_eval_derivs<4><<<grid, block, 0, s>>>(t, n, root, x, y, z, v, dx, dy, dz, npts); break;
-        case 8: k_eval_derivs<8><<<grid, block, 0, s>>>(t, n, root, x, y, z, v, dx, dy, dz, npts); break;
-        default: return hipErrorInvalidValue;
+    const dim3 grid(unsigned((npts + 255) / 256), 3);
+    const dim3 grid1(unsigned((npts + 255) / 256));
+#define LF_D1(NVV)                                                           \
+    do {                                                                     \
+        if (fast) {                                                          \
+            k_eval_deriv1<NVV, true><<<grid, block, 0, s>>>(                 \
+                t, n, root, x, y, z, v, dx, dy, dz, npts);                   \
+            k_eval_deriv1_redo<NVV><<<grid1, block, 0, s>>>(                 \
+                t, n, root, x, y, z, v, dx, dy, dz, npts);                   \
+        } else {                                                             \
+            k_eval_deriv1<NVV, false><<<grid, block, 0, s>>>(                \
+                t, n, root, x, y, z, v, dx, dy, dz, npts);                   \
+        }                                                                    \
+    } while (0)
+    switch (nv) {
+        case 1: LF_D1(1); break;
+        case 2: LF_D1(2); break;
+        case 4: LF_D1(4); break;
+        default: LF_D1(8); break;
     }
+#undef LF_D1
     return hipGetLastError();
 }
 

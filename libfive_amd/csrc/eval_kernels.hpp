@@ -29,10 +29,12 @@ std::vector<int> value_points_options(uint32_t nregs);
 int default_value_points(uint32_t nregs);
 bool value_points_forced();
 
+// fast: the tape has only tape::fast_opcode opcodes (tape_is_fast).
 hipError_t launch_eval_derivs(const tape::Clause* t, uint32_t n, uint32_t root,
-                              uint32_t nregs, const float* x, const float* y,
-                              const float* z, float* v, float* dx, float* dy,
-                              float* dz, size_t npts, hipStream_t s);
+                              uint32_t nregs, bool fast, const float* x,
+                              const float* y, const float* z, float* v,
+                              float* dx, float* dy, float* dz, size_t npts,
+                              hipStream_t s);
 
 // Per point: 1 where a MIN/MAX clause has equal operands (getAmbiguous), or
 // with derivs = true equal operands with different partials

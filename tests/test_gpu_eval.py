@@ -78,6 +78,11 @@ SHAPES = {
     "c2_dag": lambda: W.random_dag(),
     "dag_small": lambda: W.random_dag(120, seed=5),
     "spheres50": lambda: W.sphere_union(50, seed=3),
+    # common opcodes only, with sin / cos arguments on both sides of 2^17:
+    # the derivative interpreter's call-free pass and its redo of the
+    # points it cannot evaluate
+    "large_sincos": lambda: ((X() * 2e5).sin() * (Y() * 1e5 + 3e4).cos()
+                             + (Z() * 0.5).sin() + abs(X()).sqrt()),
     "all_ops": lambda: (Shape.binary("atan2", X(), Y() + 0.5)
                         + Shape.binary("mod", X() * 7, Y() + 2)
                         + Shape.binary("pow", X(), 3)

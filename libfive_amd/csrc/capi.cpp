@@ -471,6 +471,11 @@ int libfive_tape_eval_values_device(libfive_tape t, const float* x,
                   }, s);
             hip_check(launch_specialized_values(k, x, y, z, out, n, s, kBlocks[c % 3]),
                       "eval_values (specialised)");
+            if (k.redo) {
+                hip_check(launch_eval_redo(t->t->d.p, t->t->size(), h.root_reg, h.num_regs,
+                                           x, y, z, out, nullptr, nullptr, nullptr, n, s),
+                          "eval_values (redo)");
+            }
             return 0;
         }
         int ppl = default_value_points(h.num_regs);
@@ -514,6 +519,11 @@ int libfive_tape_eval_derivs_device(libfive_tape t, const float* x,
             hip_check(launch_specialized_derivs(k, x, y, z, value, dx, dy, dz, n, s,
                                                 kBlocks[c % 3]),
                       "eval_derivs (specialised)");
+            if (k.redo) {
+                hip_check(launch_eval_redo(t->t->d.p, t->t->size(), h.root_reg, h.num_regs,
+                                           x, y, z, value, dx, dy, dz, n, s),
+                          "eval_derivs (redo)");
+            }
             return 0;
         }
         hip_check(launch_eval_derivs(t->t->d.p, t->t->size(), h.root_reg,
@@ -577,6 +587,12 @@ int libfive_tape_eval_values_noalloc(libfive_tape t, const float* x,
         if (t->spec && t->use_spec) {
             hip_check(launch_specialized_values(*t->spec, px, py, pz, po, n, c.stream),
                       "eval_values (specialised)");
+            if (t->spec->redo) {
+                hip_check(launch_eval_redo(t->t->d.p, t->t->size(), h.root_reg, h.num_regs,
+                                           px, py, pz, po, nullptr, nullptr, nullptr, n,
+                                           c.stream),
+                          "eval_values (redo)");
+            }
         } else {
             hip_check(launch_eval_values(t->t->d.p, t->t->size(), h.root_reg,
                                          h.num_regs, px, py, pz, po, n, c.stream),

@@ -154,6 +154,36 @@ static __device__ __noinline__ ColdDeriv cold_deriv(uint32_t o, float av, float 
     return {ov, od[0], od[1], od[2]};
 }
 
+// SIN and COS without the out-of-line large-argument path, for kernels
+// that call no function: the results of apply_value / apply_deriv for
+// arguments below 2^17 in magnitude.  `amax` accumulates the largest
+// argument magnitude (NaN arguments are skipped by fmaxf, except that a
+// signalling NaN makes it NaN); the caller redoes the point with the
+// general rules unless amax < 2^17 at the end.
+__device__ __forceinline__ float value_sincos_fast(uint32_t o, float a,
+                                                   float& amax) {
+    float s, c;
+    fmath::sincos_core(a, &s, &c);
+    amax = fmaxf(amax, fabsf(a));
+    return (o == OC::OP_SIN) ? s : c;
+}
+
+__device__ __forceinline__ void deriv_sincos_fast(uint32_t o, float av,
+                                                  const float ad[3], float& ov,
+                                                  float od[3], float& amax) {
+    float s, c;
+    fmath::sincos_core(av, &s, &c);
+    amax = fmaxf(amax, fabsf(av));
+    if (o == OC::OP_SIN) {
+        ov = s;
+        for (int j = 0; j < 3; ++j) od[j] = ad[j] * c;
+    } else {
+        ov = c;
+        const float ns = -s;
+        for (int j = 0; j < 3; ++j) od[j] = ad[j] * ns;
+    }
+}
+
 // Value and partial derivatives of one clause (eval_deriv_array.cpp:
 // 119-256); the value is the same as apply_value's.
 __device__ __forceinline__ void apply_deriv(uint32_t o, float av,

@@ -127,12 +127,10 @@ void k_eval_values(const Clause* __restrict__ t, uint32_t n, uint32_t root,
 // FAST: the tape has only the common opcodes, and the kernel calls no
 // function (no stack, no registers held around calls).  For a point it
 // cannot evaluate (a sin / cos argument of 2^17 or above; every row sees
-// the same values, so every row agrees), row 0 writes kRedoMark as the
-// value, and k_eval_deriv1_redo evaluates the marked points again with
-// the general rules.  A point whose true value has the marker's bits is
-// evaluated twice, with the same result.  With two banks the fast form
-// fits in 168 VGPRs (three waves per SIMD) when asked to, without spilling.
-constexpr uint32_t kRedoMark = 0x7fa5a5a5u;    // a signalling NaN: no arithmetic result
+// the same values, so every row agrees), row 0 writes tape::kRedoMark as
+// the value, and k_eval_deriv1_redo evaluates the marked points again with
+// the general rules.  With two banks the fast form fits in 168 VGPRs
+// (three waves per SIMD) when asked to, without spilling.
 
 template <int NV, bool FAST>
 __global__ __launch_bounds__(256)
@@ -184,6 +182,21 @@ void k_eval_deriv1_redo(const Clause* __restrict__ t, uint32_t n, uint32_t root,
         o[i] = g;
     }
     ov_out[i] = v;
+}
+
+// Values of the points a call-free value kernel marked, with the general
+// rules.
+template <int NV>
+__global__ __launch_bounds__(256)
+void k_eval_values_redo(const Clause* __restrict__ t, uint32_t n, uint32_t root,
+                        const float* __restrict__ xs, const float* __restrict__ ys,
+                        const float* __restrict__ zs, float* __restrict__ out,
+                        size_t npts)
+{
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= npts || fmath::f2u(out[i]) != kRedoMark) return;
+    uint32_t flags = 0;
+    out[i] = pe_value<NV, false>(t, n, root, xs[i], ys[i], zs[i], &flags);
 }
 
 ////////////////////////////////////////////////////////////////////////////////
@@ -385,6 +398,33 @@ hipError_t launch_eval_derivs(const Clause* t, uint32_t n, uint32_t root,
         default: LF_D1(8); break;
     }
 #undef LF_D1
+    return hipGetLastError();
+}
+
+hipError_t launch_eval_redo(const Clause* t, uint32_t n, uint32_t root,
+                            uint32_t nregs, const float* x, const float* y,
+                            const float* z, float* v, float* dx, float* dy,
+                            float* dz, size_t npts, hipStream_t s)
+{
+    if (npts == 0) return hipSuccess;
+    const int nv = banks_for(nregs);
+    if (nv == 0) return hipErrorInvalidValue;
+    const dim3 block(256);
+    const dim3 grid(unsigned((npts + 255) / 256));
+#define LF_REDO(NVV)                                                         \
+    do {                                                                     \
+        if (dx) k_eval_deriv1_redo<NVV><<<grid, block, 0, s>>>(              \
+                    t, n, root, x, y, z, v, dx, dy, dz, npts);               \
+        else k_eval_values_redo<NVV><<<grid, block, 0, s>>>(                 \
+                    t, n, root, x, y, z, v, npts);                           \
+    } while (0)
+    switch (nv) {
+        case 1: LF_REDO(1); break;
+        case 2: LF_REDO(2); break;
+        case 4: LF_REDO(4); break;
+        default: LF_REDO(8); break;
+    }
+#undef LF_REDO
     return hipGetLastError();
 }
 

@@ -36,6 +36,15 @@ hipError_t launch_eval_derivs(const tape::Clause* t, uint32_t n, uint32_t root,
                               float* dx, float* dy, float* dz, size_t npts,
                               hipStream_t s);
 
+// Evaluates again, with the general rules, the points that a call-free
+// kernel (the fast derivative interpreter, or a specialised kernel of a
+// tape of common opcodes) marked with tape::kRedoMark in `v`: values only
+// when dx is null, else values and the three partials.
+hipError_t launch_eval_redo(const tape::Clause* t, uint32_t n, uint32_t root,
+                            uint32_t nregs, const float* x, const float* y,
+                            const float* z, float* v, float* dx, float* dy,
+                            float* dz, size_t npts, hipStream_t s);
+
 // Per point: 1 where a MIN/MAX clause has equal operands (getAmbiguous), or
 // with derivs = true equal operands with different partials
 // (getAmbiguousDerivs).

@@ -57,6 +57,14 @@ inline uint32_t padded_length(uint32_t n) {
 }
 
 constexpr uint32_t kRegX = 0, kRegY = 1, kRegZ = 2, kFirstFreeReg = 3;
+
+// Value written by the call-free evaluation kernels for a point they could
+// not evaluate (a sin / cos argument of 2^17 or above), which a second
+// kernel then evaluates with the general rules.  It is a signalling NaN,
+// which no arithmetic produces; a point whose true value has these bits
+// (an input NaN passed through unchanged) is evaluated twice, with the
+// same result.
+constexpr uint32_t kRedoMark = 0x7fa5a5a5u;
 constexpr uint32_t kMaxRegs = 0xffff;
 
 // Operand count of a device-tape opcode (Opcode::args for tree opcodes,

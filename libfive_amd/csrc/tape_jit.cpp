@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "deck.hpp"
 #include "runtime.hpp"
 #include "tape.hpp"
 
@@ -75,12 +76,23 @@ static void pin(std::string& out, const std::vector<std::string>& vars) {
     out += ");\n";
 }
 
-void value_body(const DeviceTape& t, std::string& out) {
+static bool is_sincos(uint32_t o) {
+    return o == Opcode::OP_SIN || o == Opcode::OP_COS;
+}
+
+// Call-free form (`fast`: the tape has only the common opcodes): sin and cos
+// take no out-of-line path, and a point with an argument of 2^17 or more
+// (or a NaN the accumulation cannot skip, or an infinity) gets
+// tape::kRedoMark as its value, for the interpreter to evaluate again.
+// Leaving out the calls takes the benchmark tape's derivative kernel from
+// 3.39 to 3.26 ms (with the occupancy request below).
+void value_body(const DeviceTape& t, bool fast, std::string& out) {
     Names n(t.num_regs);
     char buf[256];
     size_t k = 0;
     const size_t group = pin_group();
     std::vector<std::string> pending;
+    if (fast) out += "    float amax_ = 0.0f;\n";
     for (const Clause& c : t.clauses) {
         const uint32_t o = op(c);
         if (o == OP_NOP) continue;
@@ -89,10 +101,16 @@ void value_body(const DeviceTape& t, std::string& out) {
         const std::string a = (m == MODE_A_IMM) ? im : n.v[ra(c)];
         const std::string b = !is_binary(c) ? std::string("0.0f")
                               : (m == MODE_B_IMM) ? im : n.v[rb(c)];
-        std::snprintf(buf, sizeof(buf),
-                      "    float v%zu; { const float a_ = %s, b_ = %s; "
-                      "apply_value<1>(%uu, &a_, &b_, %s, &v%zu); }\n",
-                      k, a.c_str(), b.c_str(), o, im.c_str(), k);
+        if (fast && is_sincos(o)) {
+            std::snprintf(buf, sizeof(buf),
+                          "    float v%zu = value_sincos_fast(%uu, %s, amax_);\n",
+                          k, o, a.c_str());
+        } else {
+            std::snprintf(buf, sizeof(buf),
+                          "    float v%zu; { const float a_ = %s, b_ = %s; "
+                          "apply_value<1>(%uu, &a_, &b_, %s, &v%zu); }\n",
+                          k, a.c_str(), b.c_str(), o, im.c_str(), k);
+        }
         out += buf;
         pending.push_back("v" + std::to_string(k));
         if (pending.size() >= group) {
@@ -103,15 +121,21 @@ void value_body(const DeviceTape& t, std::string& out) {
         ++k;
     }
     pin(out, pending);
-    out += "    out[i] = " + n.v[t.root_reg] + ";\n";
+    if (fast) {
+        out += "    out[i] = (amax_ < fmath::kCwLimit) ? " + n.v[t.root_reg] +
+               " : fmath::u2f(tape::kRedoMark);\n";
+    } else {
+        out += "    out[i] = " + n.v[t.root_reg] + ";\n";
+    }
 }
 
-void deriv_body(const DeviceTape& t, std::string& out) {
+void deriv_body(const DeviceTape& t, bool fast, std::string& out) {
     Names n(t.num_regs);
     char buf[512];
     size_t k = 0;
     const size_t group = pin_group();
     std::vector<std::string> pending;
+    if (fast) out += "    float amax_ = 0.0f;\n";
     for (const Clause& c : t.clauses) {
         const uint32_t o = op(c);
         if (o == OP_NOP) continue;
@@ -131,19 +155,31 @@ void deriv_body(const DeviceTape& t, std::string& out) {
                 for (int j = 0; j < 3; ++j) bd[j] = n.g[j][rb(c)];
             }
         }
-        std::snprintf(buf, sizeof(buf),
-                      "    float v%zu, d%zu[3]; { const float ad_[3] = {%s, %s, %s}; "
-                      "const float bd_[3] = {%s, %s, %s}; "
-                      "apply_deriv(%uu, %s, ad_, %s, bd_, %s, v%zu, d%zu); }\n",
-                      k, k, ad[0].c_str(), ad[1].c_str(), ad[2].c_str(),
-                      bd[0].c_str(), bd[1].c_str(), bd[2].c_str(), o,
-                      av.c_str(), bv.c_str(), im.c_str(), k, k);
+        if (fast && is_sincos(o)) {
+            std::snprintf(buf, sizeof(buf),
+                          "    float v%zu, d%zu[3]; { const float ad_[3] = {%s, %s, %s}; "
+                          "deriv_sincos_fast(%uu, %s, ad_, v%zu, d%zu, amax_); }\n",
+                          k, k, ad[0].c_str(), ad[1].c_str(), ad[2].c_str(), o,
+                          av.c_str(), k, k);
+        } else {
+            std::snprintf(buf, sizeof(buf),
+                          "    float v%zu, d%zu[3]; { const float ad_[3] = {%s, %s, %s}; "
+                          "const float bd_[3] = {%s, %s, %s}; "
+                          "apply_deriv(%uu, %s, ad_, %s, bd_, %s, v%zu, d%zu); }\n",
+                          k, k, ad[0].c_str(), ad[1].c_str(), ad[2].c_str(),
+                          bd[0].c_str(), bd[1].c_str(), bd[2].c_str(), o,
+                          av.c_str(), bv.c_str(), im.c_str(), k, k);
+        }
         out += buf;
         const std::string ks = std::to_string(k);
         for (const char* sfx : {"", "[0]", "[1]", "[2]"}) {
             pending.push_back((*sfx ? "d" : "v") + ks + sfx);
         }
         if (pending.size() >= 4 * group) {
+            // the accumulator is pinned with every group: left free, it
+            // takes its operands' registers out of the allocation and
+            // costs the kernel 50 VGPRs on the benchmark tape
+            if (fast) pending.push_back("amax_");
             pin(out, pending);
             pending.clear();
         }
@@ -154,9 +190,15 @@ void deriv_body(const DeviceTape& t, std::string& out) {
         }
         ++k;
     }
+    if (fast && !pending.empty()) pending.push_back("amax_");
     pin(out, pending);
     const uint32_t r = t.root_reg;
-    out += "    ov[i] = " + n.v[r] + ";\n";
+    if (fast) {
+        out += "    ov[i] = (amax_ < fmath::kCwLimit) ? " + n.v[r] +
+               " : fmath::u2f(tape::kRedoMark);\n";
+    } else {
+        out += "    ov[i] = " + n.v[r] + ";\n";
+    }
     out += "    odx[i] = " + n.g[0][r] + ";\n";
     out += "    ody[i] = " + n.g[1][r] + ";\n";
     out += "    odz[i] = " + n.g[2][r] + ";\n";
@@ -164,7 +206,11 @@ void deriv_body(const DeviceTape& t, std::string& out) {
 
 }   // namespace
 
-std::string specialized_source(const DeviceTape& t) {
+std::string specialized_source(const DeviceTape& t, int deriv_waves) {
+    const bool fast = tape_is_fast(t);
+    const std::string occupancy = deriv_waves > 0
+        ? "__attribute__((amdgpu_waves_per_eu(" + std::to_string(deriv_waves) + ")))\n"
+        : std::string();
     std::string s =
         "#include \"clause_ops.hpp\"\n"
         "using namespace libfive;\n"
@@ -176,9 +222,9 @@ std::string specialized_source(const DeviceTape& t) {
         "    const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
         "    if (i >= n) return;\n"
         "    const float px = xs[i], py = ys[i], pz = zs[i];\n";
-    value_body(t, s);
+    value_body(t, fast, s);
     s += "}\n\n"
-         "extern \"C\" __global__ __launch_bounds__(256)\n"
+         "extern \"C\" __global__ __launch_bounds__(256)\n" + occupancy +
          "void lf_derivs(const float* __restrict__ xs, const float* __restrict__ ys,\n"
          "               const float* __restrict__ zs, float* __restrict__ ov,\n"
          "               float* __restrict__ odx, float* __restrict__ ody,\n"
@@ -186,7 +232,7 @@ std::string specialized_source(const DeviceTape& t) {
          "    const unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;\n"
          "    if (i >= n) return;\n"
          "    const float px = xs[i], py = ys[i], pz = zs[i];\n";
-    deriv_body(t, s);
+    deriv_body(t, fast, s);
     s += "}\n";
     return s;
 }
@@ -197,8 +243,8 @@ static void rtc_check(hiprtcResult r, const char* what) {
     }
 }
 
-std::vector<char> compile_specialized(const DeviceTape& t) {
-    const std::string src = specialized_source(t);
+std::vector<char> compile_specialized(const DeviceTape& t, int deriv_waves) {
+    const std::string src = specialized_source(t, deriv_waves);
     // LIBFIVE_AMD_JIT_DUMP=<path>: write the generated source there.
     if (const char* dump = std::getenv("LIBFIVE_AMD_JIT_DUMP")) {
         if (FILE* f = std::fopen(dump, "w")) {
@@ -231,19 +277,39 @@ std::vector<char> compile_specialized(const DeviceTape& t) {
     return code;
 }
 
-std::unique_ptr<SpecializedTape> specialize(const DeviceTape& t) {
-    context();     // device selected and initialised
-    const auto t0 = std::chrono::steady_clock::now();
-    const std::vector<char> code = compile_specialized(t);
+static std::unique_ptr<SpecializedTape> load_specialized(const std::vector<char>& code) {
     auto k = std::make_unique<SpecializedTape>();
     hip_check(hipModuleLoadData(&k->module, code.data()), "hipModuleLoadData");
     hip_check(hipModuleGetFunction(&k->values, k->module, "lf_values"),
               "hipModuleGetFunction(lf_values)");
     hip_check(hipModuleGetFunction(&k->derivs, k->module, "lf_derivs"),
               "hipModuleGetFunction(lf_derivs)");
+    k->code_bytes = code.size();
+    return k;
+}
+
+std::unique_ptr<SpecializedTape> specialize(const DeviceTape& t) {
+    context();     // device selected and initialised
+    const auto t0 = std::chrono::steady_clock::now();
+    auto k = load_specialized(compile_specialized(t, 0));
+    // A derivative kernel a few registers above the three-waves-per-SIMD
+    // line (168 VGPRs) runs at two waves; compiled again with a request for
+    // three, it usually fits with at most a few spilled registers, which
+    // costs less than the lost occupancy (benchmark tape: 3.82 -> 3.26 ms).
+    int regs = 0;
+    if (hipFuncGetAttribute(&regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, k->derivs) == hipSuccess &&
+        regs > 168 && regs <= 184) {
+        auto k3 = load_specialized(compile_specialized(t, 3));
+        int spill = 0;
+        if (hipFuncGetAttribute(&spill, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, k3->derivs) ==
+                hipSuccess && spill <= 64) {
+            k3->deriv_waves = 3;
+            k = std::move(k3);
+        }
+    }
     static std::atomic<uint64_t> serial{0};
     k->serial = ++serial;
-    k->code_bytes = code.size();
+    k->redo = tape_is_fast(t);
     k->compile_ms = std::chrono::duration<double, std::milli>(
                         std::chrono::steady_clock::now() - t0).count();
     return k;

@@ -30,6 +30,10 @@ struct SpecializedTape {
     hipModule_t module = nullptr;
     hipFunction_t values = nullptr;    // (x, y, z, out, n)
     hipFunction_t derivs = nullptr;    // (x, y, z, v, dx, dy, dz, n)
+    // The kernels are call-free and mark points they cannot evaluate with
+    // tape::kRedoMark: every launch is followed by launch_eval_redo.
+    bool redo = false;
+    int deriv_waves = 0;               // occupancy requested for lf_derivs (0: none)
     double compile_ms = 0;
     size_t code_bytes = 0;
     uint64_t serial = 0;               // distinct for every specialised tape
@@ -37,10 +41,12 @@ struct SpecializedTape {
 };
 
 // HIP source of the two kernels for tape `t` (exposed for tests).
-std::string specialized_source(const DeviceTape& t);
+// deriv_waves > 0 asks the compiler for that many waves per SIMD in the
+// derivative kernel.
+std::string specialized_source(const DeviceTape& t, int deriv_waves = 0);
 
 // Compiles the kernels for gfx950 (no device needed); throws Error.
-std::vector<char> compile_specialized(const DeviceTape& t);
+std::vector<char> compile_specialized(const DeviceTape& t, int deriv_waves = 0);
 
 // Compiles and loads the kernels on the current device; throws Error.
 std::unique_ptr<SpecializedTape> specialize(const DeviceTape& t);

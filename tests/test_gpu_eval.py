@@ -83,6 +83,9 @@ SHAPES = {
     # points it cannot evaluate
     "large_sincos": lambda: ((X() * 2e5).sin() * (Y() * 1e5 + 3e4).cos()
                              + (Z() * 0.5).sin() + abs(X()).sqrt()),
+    # sin / cos of infinities (x * 3e38 * 2 overflows for |x| > 0.57)
+    "inf_sincos": lambda: ((X() * 3e38 * 2).sin() + (Y() * 3e38 * 2).cos()
+                           + Z().sin()),
     "all_ops": lambda: (Shape.binary("atan2", X(), Y() + 0.5)
                         + Shape.binary("mod", X() * 7, Y() + 2)
                         + Shape.binary("pow", X(), 3)
@@ -238,7 +241,8 @@ def _device_eval(tape, p, derivs):
 
 
 @pytest.mark.parametrize("name", ["c1_csg", "c4_gyroid", "dag_small",
-                                  "spheres50", "all_ops"])
+                                  "spheres50", "all_ops", "large_sincos",
+                                  "inf_sincos"])
 def test_specialized_kernels_match_interpreter(gpu, name):
     """The hipRTC-specialised kernels give the interpreter's bits (values
     and partials), which are the oracle's."""

@@ -238,7 +238,7 @@ static float eval_point(libfive_tree t, libfive_vec3 p, float grad[3]) {
                   "eval_d");
     } else {
         hip_check(launch_eval_values(gt->d.p, gt->size(), h.root_reg,
-                                     h.num_regs, d, d + 1, d + 2, d + 4, 1,
+                                     h.num_regs, tape_is_fast(h), d, d + 1, d + 2, d + 4, 1,
                                      c.stream),
                   "eval_f");
     }
@@ -298,7 +298,7 @@ int libfive_tree_eval_array(libfive_tree t, const float* x, const float* y,
         hip_check(hipMemcpyAsync(buf.p + n, y, b, hipMemcpyHostToDevice, c.stream), "copy y");
         hip_check(hipMemcpyAsync(buf.p + 2 * n, z, b, hipMemcpyHostToDevice, c.stream), "copy z");
         hip_check(launch_eval_values(gt->d.p, gt->size(), gt->host.root_reg,
-                                     gt->host.num_regs, buf.p, buf.p + n,
+                                     gt->host.num_regs, tape_is_fast(gt->host), buf.p, buf.p + n,
                                      buf.p + 2 * n, buf.p + 3 * n, n, c.stream),
                   "eval_values");
         hip_check(hipMemcpyAsync(out, buf.p + 3 * n, b, hipMemcpyDeviceToHost, c.stream), "copy out");
@@ -485,13 +485,13 @@ int libfive_tape_eval_values_device(libfive_tape t, const float* x,
             for (int p : opts) names.push_back(std::to_string(p) + " points per lane");
             const std::string key = "eval_values/banks=" + std::to_string((h.num_regs + 31) / 32);
             const int c = n < kTuneMin ? tune::known(key) : tune::pick(key, names, [&](int k) {
-                return launch_eval_values_ppl(t->t->d.p, t->t->size(), h.root_reg, h.num_regs,
+                return launch_eval_values_ppl(t->t->d.p, t->t->size(), h.root_reg, h.num_regs, tape_is_fast(h),
                                               x, y, z, out, slice, s, opts[k]);
             }, s);
             if (c >= 0 && c < int(opts.size())) ppl = opts[c];
         }
         hip_check(launch_eval_values_ppl(t->t->d.p, t->t->size(), h.root_reg,
-                                         h.num_regs, x, y, z, out, n, s, ppl),
+                                         h.num_regs, tape_is_fast(h), x, y, z, out, n, s, ppl),
                   "eval_values");
         return 0;
     } catch (const std::exception& e) {
@@ -595,7 +595,7 @@ int libfive_tape_eval_values_noalloc(libfive_tape t, const float* x,
             }
         } else {
             hip_check(launch_eval_values(t->t->d.p, t->t->size(), h.root_reg,
-                                         h.num_regs, px, py, pz, po, n, c.stream),
+                                         h.num_regs, tape_is_fast(h), px, py, pz, po, n, c.stream),
                       "eval_values");
         }
         hip_check(hipMemcpyAsync(out, po, b, hipMemcpyDeviceToHost, c.stream), "copy out");

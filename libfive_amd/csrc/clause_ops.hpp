@@ -27,10 +27,43 @@ using tape::OP_LOADI;
 // Values of one clause for P points (b is ignored by unary opcodes).  The
 // common opcodes are inline; the rest go through the out-of-line scalar
 // table so that their register demand does not lower occupancy.
-template <int P>
+// SIN and COS without the out-of-line large-argument path, for kernels
+// that call no function: the results of apply_value / apply_deriv for
+// arguments below 2^17 in magnitude.  `amax` accumulates the largest
+// argument magnitude (NaN arguments are skipped by fmaxf, except that a
+// signalling NaN makes it NaN); the caller redoes the point with the
+// general rules unless amax < 2^17 at the end.
+__device__ __forceinline__ float value_sincos_fast(uint32_t o, float a,
+                                                   float& amax) {
+    float s, c;
+    fmath::sincos_core(a, &s, &c);
+    amax = fmaxf(amax, fabsf(a));
+    return (o == OC::OP_SIN) ? s : c;
+}
+
+__device__ __forceinline__ void deriv_sincos_fast(uint32_t o, float av,
+                                                  const float ad[3], float& ov,
+                                                  float od[3], float& amax) {
+    float s, c;
+    fmath::sincos_core(av, &s, &c);
+    amax = fmaxf(amax, fabsf(av));
+    if (o == OC::OP_SIN) {
+        ov = s;
+        for (int j = 0; j < 3; ++j) od[j] = ad[j] * c;
+    } else {
+        ov = c;
+        const float ns = -s;
+        for (int j = 0; j < 3; ++j) od[j] = ad[j] * ns;
+    }
+}
+
+// FAST: the tape has only the common opcodes (tape::fast_opcode); sin and
+// cos take only the single-precision path and accumulate the argument
+// magnitude in amax[p] (see value_sincos_fast), and nothing is called.
+template <int P, bool FAST = false>
 __device__ __forceinline__ void apply_value(uint32_t o, const float* a,
                                             const float* b, float im,
-                                            float* out) {
+                                            float* out, float* amax = nullptr) {
 #define LF_EACH for (int p = 0; p < P; ++p)
     switch (o) {
         case OC::OP_ADD: LF_EACH out[p] = a[p] + b[p]; break;
@@ -42,13 +75,27 @@ __device__ __forceinline__ void apply_value(uint32_t o, const float* a,
         case OC::OP_SQUARE: LF_EACH out[p] = a[p] * a[p]; break;
         case OC::OP_ABS: LF_EACH out[p] = fabsf(a[p]); break;
         case OC::OP_SQRT: LF_EACH out[p] = sqrtf(a[p]); break;
-        case OC::OP_SIN: LF_EACH out[p] = fmath::sin_f32(a[p]); break;
-        case OC::OP_COS: LF_EACH out[p] = fmath::cos_f32(a[p]); break;
+        case OC::OP_SIN:
+        case OC::OP_COS:
+            if (FAST) {
+                LF_EACH out[p] = value_sincos_fast(o, a[p], amax[p]);
+            } else if (o == OC::OP_SIN) {
+                LF_EACH out[p] = fmath::sin_f32(a[p]);
+            } else {
+                LF_EACH out[p] = fmath::cos_f32(a[p]);
+            }
+            break;
         case OC::OP_DIV: LF_EACH out[p] = a[p] / b[p]; break;
         case OP_LOADI: LF_EACH out[p] = im; break;
         case OP_COPY:
         case OC::CONST_VAR: LF_EACH out[p] = a[p]; break;
-        default: LF_EACH out[p] = fmath::eval_op(int(o), a[p], b[p]); break;
+        default:
+            if (FAST) {     // not reached: a fast tape has no other opcode
+                LF_EACH out[p] = 0.0f;
+            } else {
+                LF_EACH out[p] = fmath::eval_op(int(o), a[p], b[p]);
+            }
+            break;
     }
 #undef LF_EACH
 }
@@ -152,36 +199,6 @@ static __device__ __noinline__ ColdDeriv cold_deriv(uint32_t o, float av, float 
             break;
     }
     return {ov, od[0], od[1], od[2]};
-}
-
-// SIN and COS without the out-of-line large-argument path, for kernels
-// that call no function: the results of apply_value / apply_deriv for
-// arguments below 2^17 in magnitude.  `amax` accumulates the largest
-// argument magnitude (NaN arguments are skipped by fmaxf, except that a
-// signalling NaN makes it NaN); the caller redoes the point with the
-// general rules unless amax < 2^17 at the end.
-__device__ __forceinline__ float value_sincos_fast(uint32_t o, float a,
-                                                   float& amax) {
-    float s, c;
-    fmath::sincos_core(a, &s, &c);
-    amax = fmaxf(amax, fabsf(a));
-    return (o == OC::OP_SIN) ? s : c;
-}
-
-__device__ __forceinline__ void deriv_sincos_fast(uint32_t o, float av,
-                                                  const float ad[3], float& ov,
-                                                  float od[3], float& amax) {
-    float s, c;
-    fmath::sincos_core(av, &s, &c);
-    amax = fmaxf(amax, fabsf(av));
-    if (o == OC::OP_SIN) {
-        ov = s;
-        for (int j = 0; j < 3; ++j) od[j] = ad[j] * c;
-    } else {
-        ov = c;
-        const float ns = -s;
-        for (int j = 0; j < 3; ++j) od[j] = ad[j] * ns;
-    }
 }
 
 // Value and partial derivatives of one clause (eval_deriv_array.cpp:

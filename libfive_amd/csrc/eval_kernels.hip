@@ -65,7 +65,7 @@ namespace OC = Opcode;
                 _Pragma("unroll") for (int p = 0; p < P; ++p)                \
                     b_[p] = LF_PGET(p, NV, ib_);                             \
             }                                                                \
-            apply_value<P>(o_, a_, b_, im_, q_);                             \
+            apply_value<P, FAST>(o_, a_, b_, im_, q_, amax_);                \
             const uint32_t d_ = dst(c_);                                     \
             _Pragma("unroll") for (int p = 0; p < P; ++p)                    \
                 LF_PSET(p, NV, d_, q_[p]);                                   \
@@ -79,7 +79,10 @@ struct Clause4 {
 // P points per lane: lane l of block b handles points b*256*P + l + 256*j.
 // The tape is read kFetch clauses at a time, the next batch requested
 // before the current one is interpreted.
-template <int NV, int P>
+// FAST: the tape has only the common opcodes and the kernel calls no
+// function; a point with a sin / cos argument of 2^17 or more gets
+// tape::kRedoMark, for k_eval_values_redo.
+template <int NV, int P, bool FAST>
 __global__ __launch_bounds__(256)
 void k_eval_values(const Clause* __restrict__ t, uint32_t n, uint32_t root,
                    const float* __restrict__ xs, const float* __restrict__ ys,
@@ -87,6 +90,9 @@ void k_eval_values(const Clause* __restrict__ t, uint32_t n, uint32_t root,
                    size_t npts)
 {
     const size_t base = size_t(blockIdx.x) * (256u * P) + threadIdx.x;
+    float amax_[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) amax_[p] = 0.0f;
     LF_RF_DECL(r0);
     LF_RF_DECL(r1);
     LF_RF_DECL(r2);
@@ -113,7 +119,10 @@ void k_eval_values(const Clause* __restrict__ t, uint32_t n, uint32_t root,
     for (int p = 0; p < P; ++p) {
         const size_t i = base + 256u * p;
         const float v = LF_PGET(p, NV, root);
-        if (i < npts) out[i] = v;
+        if (i < npts) {
+            out[i] = (!FAST || amax_[p] < fmath::kCwLimit) ? v
+                                                            : fmath::u2f(kRedoMark);
+        }
     }
 }
 
@@ -322,10 +331,16 @@ static int points_for(int nv) {
 template <int NV, int P>
 static void launch_v(const Clause* t, uint32_t n, uint32_t root,
                      const float* x, const float* y, const float* z,
-                     float* out, size_t npts, hipStream_t s) {
+                     float* out, size_t npts, bool fast, hipStream_t s) {
     const dim3 block(256);
     const dim3 grid(unsigned((npts + 256u * P - 1) / (256u * P)));
-    k_eval_values<NV, P><<<grid, block, 0, s>>>(t, n, root, x, y, z, out, npts);
+    if (fast) {
+        k_eval_values<NV, P, true><<<grid, block, 0, s>>>(t, n, root, x, y, z, out, npts);
+        const dim3 grid1(unsigned((npts + 255) / 256));
+        k_eval_values_redo<NV><<<grid1, block, 0, s>>>(t, n, root, x, y, z, out, npts);
+    } else {
+        k_eval_values<NV, P, false><<<grid, block, 0, s>>>(t, n, root, x, y, z, out, npts);
+    }
 }
 
 std::vector<int> value_points_options(uint32_t nregs) {
@@ -340,29 +355,29 @@ int default_value_points(uint32_t nregs) { return points_for(banks_for(nregs)); 
 bool value_points_forced() { return g_points_per_lane > 0; }
 
 hipError_t launch_eval_values(const Clause* t, uint32_t n, uint32_t root,
-                              uint32_t nregs, const float* x, const float* y,
-                              const float* z, float* out, size_t npts,
-                              hipStream_t s)
+                              uint32_t nregs, bool fast, const float* x,
+                              const float* y, const float* z, float* out,
+                              size_t npts, hipStream_t s)
 {
-    return launch_eval_values_ppl(t, n, root, nregs, x, y, z, out, npts, s,
+    return launch_eval_values_ppl(t, n, root, nregs, fast, x, y, z, out, npts, s,
                                   points_for(banks_for(nregs)));
 }
 
 hipError_t launch_eval_values_ppl(const Clause* t, uint32_t n, uint32_t root,
-                                  uint32_t nregs, const float* x, const float* y,
-                                  const float* z, float* out, size_t npts,
-                                  hipStream_t s, int p)
+                                  uint32_t nregs, bool fast, const float* x,
+                                  const float* y, const float* z, float* out,
+                                  size_t npts, hipStream_t s, int p)
 {
     if (npts == 0) return hipSuccess;
     const int nv = banks_for(nregs);
     if (!(p == 1 || (p == 2 && nv <= 2) || (p == 4 && nv == 1))) return hipErrorInvalidValue;
-    if (nv == 1 && p == 1) launch_v<1, 1>(t, n, root, x, y, z, out, npts, s);
-    else if (nv == 1 && p == 2) launch_v<1, 2>(t, n, root, x, y, z, out, npts, s);
-    else if (nv == 1) launch_v<1, 4>(t, n, root, x, y, z, out, npts, s);
-    else if (nv == 2 && p == 1) launch_v<2, 1>(t, n, root, x, y, z, out, npts, s);
-    else if (nv == 2) launch_v<2, 2>(t, n, root, x, y, z, out, npts, s);
-    else if (nv == 4) launch_v<4, 1>(t, n, root, x, y, z, out, npts, s);
-    else if (nv == 8) launch_v<8, 1>(t, n, root, x, y, z, out, npts, s);
+    if (nv == 1 && p == 1) launch_v<1, 1>(t, n, root, x, y, z, out, npts, fast, s);
+    else if (nv == 1 && p == 2) launch_v<1, 2>(t, n, root, x, y, z, out, npts, fast, s);
+    else if (nv == 1) launch_v<1, 4>(t, n, root, x, y, z, out, npts, fast, s);
+    else if (nv == 2 && p == 1) launch_v<2, 1>(t, n, root, x, y, z, out, npts, fast, s);
+    else if (nv == 2) launch_v<2, 2>(t, n, root, x, y, z, out, npts, fast, s);
+    else if (nv == 4) launch_v<4, 1>(t, n, root, x, y, z, out, npts, fast, s);
+    else if (nv == 8) launch_v<8, 1>(t, n, root, x, y, z, out, npts, fast, s);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

@@ -12,19 +12,21 @@
 namespace libfive {
 namespace amd {
 
+// fast: the tape has only tape::fast_opcode opcodes (tape_is_fast): a
+// call-free kernel, then a redo launch for the points it marked.
 hipError_t launch_eval_values(const tape::Clause* t, uint32_t n, uint32_t root,
-                              uint32_t nregs, const float* x, const float* y,
-                              const float* z, float* out, size_t npts,
-                              hipStream_t s);
+                              uint32_t nregs, bool fast, const float* x,
+                              const float* y, const float* z, float* out,
+                              size_t npts, hipStream_t s);
 
 // The same with p points per lane (1, 2 or 4; 4 only for tapes of one
 // register bank, 2 for at most two); value_points_options() lists the
 // counts a tape allows, default_value_points() the one launch_eval_values
 // uses (libfive_set_points_per_lane's value when set, value_points_forced).
 hipError_t launch_eval_values_ppl(const tape::Clause* t, uint32_t n, uint32_t root,
-                                  uint32_t nregs, const float* x, const float* y,
-                                  const float* z, float* out, size_t npts,
-                                  hipStream_t s, int p);
+                                  uint32_t nregs, bool fast, const float* x,
+                                  const float* y, const float* z, float* out,
+                                  size_t npts, hipStream_t s, int p);
 std::vector<int> value_points_options(uint32_t nregs);
 int default_value_points(uint32_t nregs);
 bool value_points_forced();

@@ -59,11 +59,14 @@ struct Names {
 // towards its uses (which keeps all of its inputs alive; on the
 // 809-clause benchmark tape the derivative kernel otherwise needs all 512
 // VGPRs and spills, and with the barriers it needs 168).
-// Clauses per pinning statement (LIBFIVE_AMD_JIT_GROUP, default 1): the
-// results of a group are pinned together after its last clause.
+// Clauses per pinning statement (LIBFIVE_AMD_JIT_GROUP, default 2): the
+// results of a group are pinned together after its last clause.  On the
+// benchmark tape, groups of 2 give the derivative kernel 168 VGPRs (three
+// waves per SIMD) and 3.21 ms; 1 gives 170 (3.25 ms once recompiled for
+// three waves), 4 gives 174 (3.57 ms).
 static size_t pin_group() {
     const char* e = std::getenv("LIBFIVE_AMD_JIT_GROUP");
-    const long g = e ? std::atol(e) : 1;
+    const long g = e ? std::atol(e) : 2;
     return size_t(g < 1 ? 1 : g > 4 ? 4 : g);
 }
 

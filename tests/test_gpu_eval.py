@@ -286,6 +286,32 @@ def test_specialized_c2_grid_matches_oracle(gpu):
         assert_same_bits(got[k], ref[k], "c2:" + nm)
 
 
+@pytest.mark.parametrize("group", ["1", "4"])
+def test_specialized_pin_groups_match_oracle(gpu, group, monkeypatch):
+    """Other clause groupings of the specialised kernels (read from
+    LIBFIVE_AMD_JIT_GROUP when the source is generated): with one clause
+    per group the benchmark tape's derivative kernel needs 170 VGPRs and
+    is compiled a second time with a request for three waves per SIMD.
+    Values and partials stay bit-exact against the oracle."""
+    monkeypatch.setenv("LIBFIVE_AMD_JIT_GROUP", group)
+    s = W.random_dag()
+    x, y, z = W.grid_points(24)
+    p = np.stack([x, y, z])
+    L = ffi.lib()
+    tape = L.libfive_tape_new(s.ptr)
+    try:
+        assert L.libfive_tape_specialize(tape, None) == 0, ffi.last_error()
+        got = _device_eval(tape, p, True)
+        gv = _device_eval(tape, p, False)
+    finally:
+        L.libfive_tape_delete(tape)
+    o = oracle.OracleDeck(s.deck())
+    ref = o.derivs(x, y, z, threads=8)
+    assert_same_bits(gv[0], o.values(x, y, z), f"group {group}: values")
+    for k, nm in enumerate(("v", "dx", "dy", "dz")):
+        assert_same_bits(got[k], ref[k], f"group {group}: {nm}")
+
+
 def ctypes_double():
     import ctypes
     return ctypes.c_double()
